@@ -1,0 +1,316 @@
+/* Device contexts, the library stream, option side table and kernel timers.
+ *
+ * The reference keeps its fields in host memory owned by Discretization
+ * (reference src/discretization.c:85-94). Here every Discretization gets a
+ * NusifCtx holding the HBM copies; it is found again from the struct address
+ * or from any of its host array pointers (solve() only receives p and rhs,
+ * reference src/solver.h:35). */
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "nusif_internal.h"
+
+static hipStream_t gStream = nullptr;
+static std::vector<NusifCtx *> gCtx;
+static std::vector<NusifSolverCtx *> gSolvers;
+
+hipStream_t nusifStream(void)
+{
+  if (!gStream) {
+    int n = 0;
+    HIP_CHECK(hipGetDeviceCount(&n));
+    if (n < 1) NUSIF_FATAL("no GPU visible to the HIP runtime");
+    /* default priority on purpose: every stream of this library shares it */
+    HIP_CHECK(hipStreamCreateWithFlags(&gStream, hipStreamNonBlocking));
+  }
+  return gStream;
+}
+
+void nusifRegisterCtx(NusifCtx *c) { gCtx.push_back(c); }
+
+void nusifUnregisterCtx(NusifCtx *c)
+{
+  for (size_t q = 0; q < gCtx.size(); q++)
+    if (gCtx[q] == c) {
+      gCtx.erase(gCtx.begin() + q);
+      return;
+    }
+}
+
+NusifCtx *nusifCtxOf(const Discretization *d)
+{
+  for (NusifCtx *c : gCtx)
+    if (c->owner == d) return c;
+  /* a copied struct still carries the same host arrays */
+  for (NusifCtx *c : gCtx)
+    if (c->host[0] == d->p && c->host[5] == d->u) return c;
+  NUSIF_FATAL("Discretization %p was not initialised by this library", (const void *)d);
+  return nullptr;
+}
+
+NusifCtx *nusifCtxOfHost(const double *hostPtr, int *fieldIndex)
+{
+  for (NusifCtx *c : gCtx)
+    for (int q = 0; q < 8; q++)
+      if (c->host[q] == hostPtr) {
+        if (fieldIndex) *fieldIndex = q;
+        return c;
+      }
+  return nullptr;
+}
+
+void nusifRegisterSolver(NusifSolverCtx *s) { gSolvers.push_back(s); }
+
+void nusifUnregisterSolver(NusifSolverCtx *s)
+{
+  for (size_t q = 0; q < gSolvers.size(); q++)
+    if (gSolvers[q] == s) {
+      gSolvers.erase(gSolvers.begin() + q);
+      return;
+    }
+}
+
+NusifSolverCtx *nusifSolverCtxOf(const Solver *s)
+{
+  for (NusifSolverCtx *c : gSolvers)
+    if (c->owner == s) return c;
+  for (NusifSolverCtx *c : gSolvers)
+    if (&c->dctx->owner->grid == s->grid) return c;
+  NUSIF_FATAL("Solver %p was not initialised by this library", (const void *)s);
+  return nullptr;
+}
+
+/* ---- options ------------------------------------------------------------ */
+struct OptEntry {
+  const Parameter *prm;
+  NusifOptions opt;
+};
+static std::vector<OptEntry> gOpts;
+
+void nusifDefaultOptions(NusifOptions *o)
+{
+  memset(o, 0, sizeof(*o));
+  o->nsteps      = 0;
+  o->solver      = SOLVER_DEFAULT;
+  o->mixed       = 0;
+  o->mgTol       = 1e-6;
+  o->mgMaxCycles = 100;
+  o->mgOmega     = 1.0;
+  o->mgCoarse    = 64;
+  o->mgProject   = 0;
+  o->lid         = 1.0;
+  o->uInflow     = 2.0;
+  o->inflowNoise = 0.0;
+  o->seed        = 1;
+  o->warmstart   = 0;
+}
+
+static NusifOptions *optionsSlot(const Parameter *p)
+{
+  for (OptEntry &e : gOpts)
+    if (e.prm == p) return &e.opt;
+  OptEntry e;
+  e.prm = p;
+  nusifDefaultOptions(&e.opt);
+  gOpts.push_back(e);
+  return &gOpts.back().opt;
+}
+
+const NusifOptions *nusifOptionsFor(const Parameter *p)
+{
+  for (OptEntry &e : gOpts)
+    if (e.prm == p) return &e.opt;
+  static NusifOptions defaults;
+  static bool once = false;
+  if (!once) {
+    nusifDefaultOptions(&defaults);
+    once = true;
+  }
+  return &defaults;
+}
+
+static int parseInt(const char *v, int *out)
+{
+  char *end = nullptr;
+  long x    = strtol(v, &end, 10);
+  if (end == v) return -2;
+  *out = (int)x;
+  return 0;
+}
+
+static int parseReal(const char *v, double *out)
+{
+  char *end = nullptr;
+  double x  = strtod(v, &end);
+  if (end == v) return -2;
+  *out = x;
+  return 0;
+}
+
+/* value up to the first whitespace */
+static std::string word(const char *v)
+{
+  std::string s(v);
+  size_t b = s.find_first_not_of(" \t\r\n");
+  if (b == std::string::npos) return "";
+  size_t e = s.find_first_of(" \t\r\n", b);
+  return s.substr(b, e == std::string::npos ? std::string::npos : e - b);
+}
+
+extern "C" int nusifSetOption(Parameter *p, const char *key, const char *value)
+{
+  if (!p || !key || !value) return -2;
+  NusifOptions *o = optionsSlot(p);
+  std::string k(key), v = word(value);
+  if (k == "nsteps") return parseInt(v.c_str(), &o->nsteps);
+  if (k == "solver") {
+    if (v == "rb") o->solver = SOLVER_RB;
+    else if (v == "mg") o->solver = SOLVER_MG;
+    else if (v == "gmg") o->solver = SOLVER_GMG;
+    else return -2;
+    return 0;
+  }
+  if (k == "precision") {
+    if (v == "fp64") o->mixed = 0;
+    else if (v == "mixed") o->mixed = 1;
+    else return -2;
+    return 0;
+  }
+  if (k == "mg_tol") return parseReal(v.c_str(), &o->mgTol);
+  if (k == "mg_maxcycles") return parseInt(v.c_str(), &o->mgMaxCycles);
+  if (k == "mg_omega") return parseReal(v.c_str(), &o->mgOmega);
+  if (k == "mg_coarse") return parseInt(v.c_str(), &o->mgCoarse);
+  if (k == "mg_project") return parseInt(v.c_str(), &o->mgProject);
+  if (k == "lid") return parseReal(v.c_str(), &o->lid);
+  if (k == "u_inflow") return parseReal(v.c_str(), &o->uInflow);
+  if (k == "inflow_noise") return parseReal(v.c_str(), &o->inflowNoise);
+  if (k == "seed") return parseInt(v.c_str(), &o->seed);
+  if (k == "warmstart") {
+    if (v == "none") o->warmstart = 0;
+    else if (v == "unet") o->warmstart = 1;
+    else return -2;
+    return 0;
+  }
+  if (k == "unet_weights") {
+    if (v.size() >= sizeof(o->unetWeights)) return -2;
+    strcpy(o->unetWeights, v.c_str());
+    return 0;
+  }
+  return -1;
+}
+
+/* ---- device info ---------------------------------------------------------- */
+extern "C" int nusifDeviceName(char *buf, int len)
+{
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  if (buf && len > 0) {
+    buf[0] = '\0';
+    if (n > 0) {
+      hipDeviceProp_t prop;
+      HIP_CHECK(hipGetDeviceProperties(&prop, 0));
+      snprintf(buf, (size_t)len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName,
+          prop.multiProcessorCount);
+    }
+  }
+  return n;
+}
+
+extern "C" void nusifDeviceSynchronize(void) { HIP_CHECK(hipStreamSynchronize(nusifStream())); }
+
+/* ---- kernel timers -------------------------------------------------------- */
+struct KernelTimer {
+  std::string name;
+  std::vector<hipEvent_t> pool; /* start/stop pairs, recycled */
+  size_t used;
+  double ms;
+  int launches;
+};
+static std::vector<KernelTimer *> gTimers;
+int nusifTimerGeneration = 0;
+
+KernelTimer *nusifTimerFor(const char *name)
+{
+  for (KernelTimer *t : gTimers)
+    if (t->name == name) return t;
+  return nullptr;
+}
+
+extern "C" int nusifTimerEnable(const char *kernel)
+{
+  if (nusifTimerFor(kernel)) return 0;
+  KernelTimer *t = new KernelTimer();
+  t->name        = kernel;
+  t->used        = 0;
+  t->ms          = 0.0;
+  t->launches    = 0;
+  gTimers.push_back(t);
+  nusifTimerGeneration++;
+  return 0;
+}
+
+static void drain(KernelTimer *t)
+{
+  for (size_t q = 0; q + 1 < t->used; q += 2) {
+    float ms = 0.0f;
+    HIP_CHECK(hipEventSynchronize(t->pool[q + 1]));
+    HIP_CHECK(hipEventElapsedTime(&ms, t->pool[q], t->pool[q + 1]));
+    t->ms += ms;
+    t->launches++;
+  }
+  t->used = 0;
+}
+
+void nusifTimerBegin(KernelTimer *t)
+{
+  if (t->used + 2 > t->pool.size()) {
+    if (t->pool.size() >= 4096) drain(t); /* bound the outstanding events */
+    while (t->used + 2 > t->pool.size()) {
+      hipEvent_t e;
+      HIP_CHECK(hipEventCreate(&e));
+      t->pool.push_back(e);
+    }
+  }
+  HIP_CHECK(hipEventRecord(t->pool[t->used], nusifStream()));
+}
+
+void nusifTimerEnd(KernelTimer *t)
+{
+  HIP_CHECK(hipEventRecord(t->pool[t->used + 1], nusifStream()));
+  t->used += 2;
+}
+
+extern "C" double nusifTimerRead(const char *kernel, int *launches)
+{
+  KernelTimer *t = nusifTimerFor(kernel);
+  if (!t) {
+    if (launches) *launches = 0;
+    return 0.0;
+  }
+  drain(t);
+  if (launches) *launches = t->launches;
+  return t->ms;
+}
+
+extern "C" void nusifTimerReset(void)
+{
+  for (KernelTimer *t : gTimers) {
+    drain(t);
+    t->ms       = 0.0;
+    t->launches = 0;
+  }
+}
+
+/* Used by commCollectResult (host_api.c): make the host copy of a library
+ * field current before it is read on the host. */
+extern "C" int nusifSyncHostIfOwned(const double *hostPtr)
+{
+  NusifCtx *c = nusifCtxOfHost(hostPtr, nullptr);
+  if (!c) return 0;
+  nusifSyncHost(c->owner);
+  return 1;
+}
+
+extern "C" int nusifStepLimit(Parameter *p) { return nusifOptionsFor(p)->nsteps; }

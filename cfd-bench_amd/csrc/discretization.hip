@@ -1,0 +1,601 @@
+/* The eight step functions of discretization.h on the GPU.
+ *
+ * Reference: src/discretization.c. Fields keep the reference layout
+ * (i fastest, one ghost layer, src/util.h:19-34) so host staging copies are
+ * plain memcpy. Kernels are compiled with -ffp-contract=off and evaluate
+ * every expression in the reference's operand order, so each cell value is
+ * bit-identical to a strict IEEE build of the reference; only reductions
+ * (sums) differ in order.
+ *
+ * Launch shapes: 64 x 4 workgroups over (i, j); a 64-wide row keeps one
+ * wavefront on one contiguous 512 B stretch of a row (coalesced HBM
+ * streaming). */
+#include <float.h>
+#include <math.h>
+#include <string.h>
+
+#include "nusif_internal.h"
+#include "reduce.h"
+
+void nusifRegisterCtx(NusifCtx *c);
+void nusifUnregisterCtx(NusifCtx *c);
+
+#define TX 64
+#define TY 4
+
+/* ------------------------------------------------------------------------ */
+/* computeTimestep: max |u|, |v|, |w| over the padded arrays                */
+/* ------------------------------------------------------------------------ */
+/* src/discretization.c:391-401: every padded cell counts (ghost edges and
+ * corners included), seed DBL_MIN. */
+__global__ __launch_bounds__(256) void kAbsMax3(const double *__restrict__ u,
+    const double *__restrict__ v, const double *__restrict__ w, long long n,
+    double *__restrict__ partials, DevScalars *sc)
+{
+  __shared__ double lds[4];
+  double mu = DBL_MIN, mv = DBL_MIN, mw = DBL_MIN;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+    double a = fabs(u[q]), b = fabs(v[q]), c = fabs(w[q]);
+    mu       = mu > a ? mu : a;
+    mv       = mv > b ? mv : b;
+    mw       = mw > c ? mw : c;
+  }
+  mu = blockMax(mu, lds);
+  mv = blockMax(mv, lds);
+  mw = blockMax(mw, lds);
+  if (threadIdx.x == 0) {
+    partials[3 * blockIdx.x + 0] = mu;
+    partials[3 * blockIdx.x + 1] = mv;
+    partials[3 * blockIdx.x + 2] = mw;
+  }
+  if (lastBlock(&sc->counter[0], gridDim.x) && threadIdx.x < 3) {
+    double m = DBL_MIN;
+    for (unsigned b = 0; b < gridDim.x; b++) {
+      double x = loadAgent(&partials[3 * b + threadIdx.x]);
+      m        = m > x ? m : x;
+    }
+    sc->absmax[threadIdx.x] = (unsigned long long)__double_as_longlong(m);
+  }
+}
+
+static int reductionBlocks(long long n)
+{
+  long long b = (n + 256 * 8 - 1) / (256 * 8);
+  if (b > 2048) b = 2048;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+void launchAbsMax3(NusifCtx *c)
+{
+  int nb = reductionBlocks(c->geo.n);
+  NUSIF_TIMED("absmax", (kAbsMax3<<<nb, 256, 0, nusifStream()>>>(c->u, c->v, c->w, c->geo.n,
+                            c->partials, c->scal)));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Velocity boundary conditions (src/discretization.c:119-332)              */
+/* ------------------------------------------------------------------------ */
+enum { F_TOP = 0, F_BOTTOM, F_LEFT, F_RIGHT, F_FRONT, F_BACK };
+
+/* Applies one face's rule at face point (a, b) (1-based tangential indices).
+ * TOP/BOTTOM: (a, b) = (i, k); LEFT/RIGHT: (j, k); FRONT/BACK: (i, j). */
+__device__ inline void velocityFace(
+    double *u, double *v, double *w, const Geo &g, int face, int type, int a, int b)
+{
+  if (type != NOSLIP && type != SLIP && type != OUTFLOW) return; /* PERIODIC: no-op */
+  bool outflow = (type == OUTFLOW);
+  double s     = (type == NOSLIP) ? -1.0 : 1.0;
+  switch (face) {
+  case F_TOP:
+  case F_BOTTOM: {
+    if (a > g.ni || b > g.nk) return;
+    bool top = face == F_TOP;
+    int jg = top ? g.nj + 1 : 0, jin = top ? g.nj : 1, jw = top ? g.nj : 0;
+    int jn = top ? g.nj - 1 : 1;
+    double uu = u[g.at(a, jin, b)], ww = w[g.at(a, jin, b)];
+    u[g.at(a, jg, b)] = outflow ? uu : (s < 0.0 ? -uu : uu);
+    v[g.at(a, jw, b)] = outflow ? v[g.at(a, jn, b)] : 0.0;
+    w[g.at(a, jg, b)] = outflow ? ww : (s < 0.0 ? -ww : ww);
+  } break;
+  case F_LEFT:
+  case F_RIGHT: {
+    if (a > g.nj || b > g.nk) return;
+    bool right = face == F_RIGHT;
+    int ig = right ? g.ni + 1 : 0, iin = right ? g.ni : 1, iw = right ? g.ni : 0;
+    int in = right ? g.ni - 1 : 1;
+    double vv = v[g.at(iin, a, b)], ww = w[g.at(iin, a, b)];
+    double un = outflow ? u[g.at(in, a, b)] : 0.0;
+    u[g.at(iw, a, b)] = un;
+    v[g.at(ig, a, b)] = outflow ? vv : (s < 0.0 ? -vv : vv);
+    w[g.at(ig, a, b)] = outflow ? ww : (s < 0.0 ? -ww : ww);
+  } break;
+  default: { /* FRONT / BACK */
+    if (a > g.ni || b > g.nj) return;
+    bool back = face == F_BACK;
+    int kg = back ? g.nk + 1 : 0, kin = back ? g.nk : 1, kw = back ? g.nk : 0;
+    int kn = back ? g.nk - 1 : 1;
+    double uu = u[g.at(a, b, kin)], vv = v[g.at(a, b, kin)];
+    double wn = outflow ? w[g.at(a, b, kn)] : 0.0;
+    u[g.at(a, b, kg)] = outflow ? uu : (s < 0.0 ? -uu : uu);
+    v[g.at(a, b, kg)] = outflow ? vv : (s < 0.0 ? -vv : vv);
+    w[g.at(a, b, kw)] = wn;
+  } break;
+  }
+}
+
+/* Two opposite faces in one launch. They touch disjoint cells and read
+ * nothing the other writes as long as the normal extent is >= 2, so the pair
+ * is order-free; pairs themselves run in the reference's order. */
+__global__ __launch_bounds__(256) void kVelocityFaces(double *u, double *v, double *w, Geo g,
+    int faceA, int typeA, int faceB, int typeB)
+{
+  int a = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int b = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  velocityFace(u, v, w, g, faceA, typeA, a, b);
+  if (faceB >= 0) velocityFace(u, v, w, g, faceB, typeB, a, b);
+}
+
+static void faceExtent(const Geo &g, int face, int *ea, int *eb)
+{
+  if (face == F_TOP || face == F_BOTTOM) {
+    *ea = g.ni;
+    *eb = g.nk;
+  } else if (face == F_LEFT || face == F_RIGHT) {
+    *ea = g.nj;
+    *eb = g.nk;
+  } else {
+    *ea = g.ni;
+    *eb = g.nj;
+  }
+}
+
+static void launchFaces(NusifCtx *c, int fa, int ta, int fb, int tb)
+{
+  int ea, eb;
+  faceExtent(c->geo, fa, &ea, &eb);
+  dim3 blk(64, 4);
+  dim3 grd((ea + 63) / 64, (eb + 3) / 4);
+  kVelocityFaces<<<grd, blk, 0, nusifStream()>>>(c->u, c->v, c->w, c->geo, fa, ta, fb, tb);
+}
+
+void launchVelocityBC(NusifCtx *c, const Discretization *d)
+{
+  const Geo &g = c->geo;
+  /* reference order: TOP, BOTTOM, LEFT, RIGHT, FRONT, BACK */
+  if (g.nj >= 2) launchFaces(c, F_TOP, d->bcTop, F_BOTTOM, d->bcBottom);
+  else {
+    launchFaces(c, F_TOP, d->bcTop, -1, 0);
+    launchFaces(c, F_BOTTOM, d->bcBottom, -1, 0);
+  }
+  if (g.ni >= 2) launchFaces(c, F_LEFT, d->bcLeft, F_RIGHT, d->bcRight);
+  else {
+    launchFaces(c, F_LEFT, d->bcLeft, -1, 0);
+    launchFaces(c, F_RIGHT, d->bcRight, -1, 0);
+  }
+  if (g.nk >= 2) launchFaces(c, F_FRONT, d->bcFront, F_BACK, d->bcBack);
+  else {
+    launchFaces(c, F_FRONT, d->bcFront, -1, 0);
+    launchFaces(c, F_BACK, d->bcBack, -1, 0);
+  }
+}
+
+/* src/discretization.c:364-389. Lid: 1 <= i < imax, 1 <= k < kmax only. */
+__global__ __launch_bounds__(256) void kLid(double *u, Geo g, double twoLid)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  if (i >= g.ni || k >= g.nk) return;
+  u[g.at(i, g.nj + 1, k)] = twoLid - u[g.at(i, g.nj, k)];
+}
+
+__global__ __launch_bounds__(256) void kInflow(double *u, Geo g, double value, const double *profile)
+{
+  int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  if (j > g.nj || k > g.nk) return;
+  u[g.at(0, j, k)] = profile ? profile[(long long)k * (g.nj + 2) + j] : value;
+}
+
+void launchSpecialBC(NusifCtx *c, const Discretization *d)
+{
+  const Geo &g = c->geo;
+  dim3 blk(64, 4);
+  if (strcmp(d->problem, "dcavity") == 0) {
+    dim3 grd((g.ni + 63) / 64, (g.nk + 3) / 4);
+    kLid<<<grd, blk, 0, nusifStream()>>>(c->u, g, 2.0 * c->opt.lid);
+  } else if (strcmp(d->problem, "canal") == 0) {
+    dim3 grd((g.nj + 63) / 64, (g.nk + 3) / 4);
+    kInflow<<<grd, blk, 0, nusifStream()>>>(c->u, g, c->opt.uInflow, c->inflow);
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* computeFG (src/discretization.c:455-652)                                 */
+/* ------------------------------------------------------------------------ */
+/* c*((a1*b1) - (a0*b0)) + gamma*c*(|a1|*d1 + |a0|*d0), c = inv*0.25, in the
+ * reference's operand order */
+__device__ inline double donor(double inv, double gamma, double a1, double b1, double d1,
+    double a0, double b0, double d0)
+{
+  return inv * 0.25 * (a1 * b1 - a0 * b0) + gamma * inv * 0.25 * (fabs(a1) * d1 + fabs(a0) * d0);
+}
+
+__device__ inline double d2(double inv, double xp, double x, double xm)
+{
+  return inv * inv * (xp - 2.0 * x + xm);
+}
+
+struct FGCoef {
+  double gamma, dt, iRe, iDx, iDy, iDz, gx, gy, gz;
+};
+
+/* One thread per interior cell. The wall faces of F, G, H
+ * (src/discretization.c:599-651) are written by the threads next to them:
+ * F(imax) = U(imax) replaces the interior value there, F(0) = U(0) is written
+ * by i == 1, and likewise for G in j and H in k. dvwdz keeps the reference's
+ * V(i,j,k+1) in both halves of its k-1/2 flux (src/discretization.c:543-550). */
+__global__ __launch_bounds__(256) void kComputeFG(const double *__restrict__ u,
+    const double *__restrict__ v, const double *__restrict__ w, double *__restrict__ f,
+    double *__restrict__ g, double *__restrict__ h, Geo geo, FGCoef cf)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int k = blockIdx.z + 1;
+  if (i > geo.ni || j > geo.nj) return;
+  const long long sj = geo.sj, sk = geo.sk;
+  long long c = geo.at(i, j, k);
+  double U = u[c], V = v[c], W = w[c];
+  double gam = cf.gamma;
+
+  double fval, gval, hval;
+  if (i == geo.ni) {
+    fval = U;
+  } else {
+    double du2dx = donor(cf.iDx, gam, U + u[c + 1], U + u[c + 1], U - u[c + 1], U + u[c - 1],
+        U + u[c - 1], U - u[c - 1]);
+    double duvdy = donor(cf.iDy, gam, V + v[c + 1], U + u[c + sj], U - u[c + sj],
+        v[c - sj] + v[c - sj + 1], U + u[c - sj], U - u[c - sj]);
+    double duwdz = donor(cf.iDz, gam, W + w[c + 1], U + u[c + sk], U - u[c + sk],
+        w[c - sk] + w[c - sk + 1], U + u[c - sk], U - u[c - sk]);
+    double lapU = d2(cf.iDx, u[c + 1], U, u[c - 1]) + d2(cf.iDy, u[c + sj], U, u[c - sj]) +
+                  d2(cf.iDz, u[c + sk], U, u[c - sk]);
+    fval = U + cf.dt * (cf.iRe * lapU - du2dx - duvdy - duwdz + cf.gx);
+  }
+  if (j == geo.nj) {
+    gval = V;
+  } else {
+    double duvdx = donor(cf.iDx, gam, U + u[c + sj], V + v[c + 1], V - v[c + 1],
+        u[c - 1] + u[c - 1 + sj], V + v[c - 1], V - v[c - 1]);
+    double dv2dy = donor(cf.iDy, gam, V + v[c + sj], V + v[c + sj], V - v[c + sj],
+        V + v[c - sj], V + v[c - sj], V - v[c - sj]);
+    double dvwdz = donor(cf.iDz, gam, W + w[c + sj], V + v[c + sk], V - v[c + sk],
+        w[c - sk] + w[c - sk + sj], V + v[c + sk], V - v[c + sk]);
+    double lapV = d2(cf.iDx, v[c + 1], V, v[c - 1]) + d2(cf.iDy, v[c + sj], V, v[c - sj]) +
+                  d2(cf.iDz, v[c + sk], V, v[c - sk]);
+    gval = V + cf.dt * (cf.iRe * lapV - duvdx - dv2dy - dvwdz + cf.gy);
+  }
+  if (k == geo.nk) {
+    hval = W;
+  } else {
+    double duwdx = donor(cf.iDx, gam, U + u[c + sk], W + w[c + 1], W - w[c + 1],
+        u[c - 1] + u[c - 1 + sk], W + w[c - 1], W - w[c - 1]);
+    double dvwdy = donor(cf.iDy, gam, V + v[c + sk], W + w[c + sj], W - w[c + sj],
+        v[c - sj + sk] + v[c - sj], W + w[c - sj], W - w[c - sj]);
+    double dw2dz = donor(cf.iDz, gam, W + w[c + sk], W + w[c + sk], W - w[c + sk],
+        W + w[c - sk], W + w[c - sk], W - w[c - sk]);
+    double lapW = d2(cf.iDx, w[c + 1], W, w[c - 1]) + d2(cf.iDy, w[c + sj], W, w[c - sj]) +
+                  d2(cf.iDz, w[c + sk], W, w[c - sk]);
+    hval = W + cf.dt * (cf.iRe * lapW - duwdx - dvwdy - dw2dz + cf.gz);
+  }
+  f[c] = fval;
+  g[c] = gval;
+  h[c] = hval;
+  if (i == 1) f[c - 1] = u[c - 1];
+  if (j == 1) g[c - sj] = v[c - sj];
+  if (k == 1) h[c - sk] = w[c - sk];
+}
+
+void launchComputeFG(NusifCtx *c, const Discretization *d)
+{
+  FGCoef cf;
+  cf.gamma = d->gamma;
+  cf.dt    = d->dt;
+  cf.iRe   = 1.0 / d->re;
+  cf.iDx   = 1.0 / d->grid.dx;
+  cf.iDy   = 1.0 / d->grid.dy;
+  cf.iDz   = 1.0 / d->grid.dz;
+  cf.gx    = d->gx;
+  cf.gy    = d->gy;
+  cf.gz    = d->gz;
+  const Geo &g = c->geo;
+  dim3 blk(TX, TY);
+  dim3 grd((g.ni + TX - 1) / TX, (g.nj + TY - 1) / TY, g.nk);
+  NUSIF_TIMED("computeFG", (kComputeFG<<<grd, blk, 0, nusifStream()>>>(
+                                c->u, c->v, c->w, c->f, c->g, c->h, g, cf)));
+}
+
+/* ------------------------------------------------------------------------ */
+/* computeRHS (src/discretization.c:334-362)                                */
+/* ------------------------------------------------------------------------ */
+__global__ __launch_bounds__(256) void kComputeRHS(const double *__restrict__ f,
+    const double *__restrict__ g, const double *__restrict__ h, double *__restrict__ rhs,
+    Geo geo, double idx, double idy, double idz, double idt)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int k = blockIdx.z + 1;
+  if (i > geo.ni || j > geo.nj) return;
+  long long c = geo.at(i, j, k);
+  rhs[c] = ((f[c] - f[c - 1]) * idx + (g[c] - g[c - geo.sj]) * idy + (h[c] - h[c - geo.sk]) * idz) *
+           idt;
+}
+
+void launchComputeRHS(NusifCtx *c, const Discretization *d)
+{
+  const Geo &g = c->geo;
+  dim3 blk(TX, TY);
+  dim3 grd((g.ni + TX - 1) / TX, (g.nj + TY - 1) / TY, g.nk);
+  NUSIF_TIMED("computeRHS",
+      (kComputeRHS<<<grd, blk, 0, nusifStream()>>>(c->f, c->g, c->h, c->rhs, g, 1.0 / d->grid.dx,
+          1.0 / d->grid.dy, 1.0 / d->grid.dz, 1.0 / d->dt)));
+}
+
+/* ------------------------------------------------------------------------ */
+/* normalizePressure (src/discretization.c:403-429)                         */
+/* ------------------------------------------------------------------------ */
+/* Workgroup (64, 4) covers a 64 x 4 patch of (i, j) and marches all k. */
+__global__ __launch_bounds__(256) void kInteriorSum(
+    const double *__restrict__ p, Geo geo, double *partials, DevScalars *sc, double count)
+{
+  __shared__ double lds[4];
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  double acc = 0.0;
+  if (i <= geo.ni && j <= geo.nj)
+    for (int k = 1; k <= geo.nk; k++) acc += p[geo.at(i, j, k)];
+  acc = blockSum(acc, lds);
+  unsigned nb  = gridDim.x * gridDim.y;
+  unsigned bid = blockIdx.x + gridDim.x * blockIdx.y;
+  int tid      = threadIdx.x + blockDim.x * threadIdx.y;
+  if (tid == 0) partials[bid] = acc;
+  if (lastBlock(&sc->counter[1], nb)) {
+    double s = 0.0;
+    for (unsigned q = tid; q < nb; q += blockDim.x * blockDim.y) s += loadAgent(&partials[q]);
+    s = blockSum(s, lds);
+    if (tid == 0) sc->sum = s / count;
+  }
+}
+
+__global__ __launch_bounds__(256) void kSubtractMean(double *__restrict__ p, Geo geo, const DevScalars *sc)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int k = blockIdx.z + 1;
+  if (i > geo.ni || j > geo.nj) return;
+  double avg = sc->sum;
+  long long c = geo.at(i, j, k);
+  p[c]        = p[c] - avg;
+}
+
+void launchNormalize(NusifCtx *c)
+{
+  const Geo &g = c->geo;
+  dim3 blk(TX, TY);
+  dim3 g2((g.ni + TX - 1) / TX, (g.nj + TY - 1) / TY);
+  if ((size_t)g2.x * g2.y > c->nPartials) NUSIF_FATAL("partials buffer too small");
+  double count = (double)(c->owner->grid.imax * c->owner->grid.jmax * c->owner->grid.kmax);
+  kInteriorSum<<<g2, blk, 0, nusifStream()>>>(c->p, g, c->partials, c->scal, count);
+  dim3 g3(g2.x, g2.y, g.nk);
+  kSubtractMean<<<g3, blk, 0, nusifStream()>>>(c->p, g, c->scal);
+}
+
+/* ------------------------------------------------------------------------ */
+/* adaptUV (src/discretization.c:654-681)                                    */
+/* ------------------------------------------------------------------------ */
+__global__ __launch_bounds__(256) void kAdaptUV(const double *__restrict__ f,
+    const double *__restrict__ g, const double *__restrict__ h, const double *__restrict__ p,
+    double *__restrict__ u, double *__restrict__ v, double *__restrict__ w, Geo geo, double fx,
+    double fy, double fz)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int k = blockIdx.z + 1;
+  if (i > geo.ni || j > geo.nj) return;
+  long long c = geo.at(i, j, k);
+  double pc   = p[c];
+  u[c]        = f[c] - (p[c + 1] - pc) * fx;
+  v[c]        = g[c] - (p[c + geo.sj] - pc) * fy;
+  w[c]        = h[c] - (p[c + geo.sk] - pc) * fz;
+}
+
+void launchAdaptUV(NusifCtx *c, const Discretization *d)
+{
+  const Geo &g = c->geo;
+  dim3 blk(TX, TY);
+  dim3 grd((g.ni + TX - 1) / TX, (g.nj + TY - 1) / TY, g.nk);
+  NUSIF_TIMED("adaptUV", (kAdaptUV<<<grd, blk, 0, nusifStream()>>>(c->f, c->g, c->h, c->p, c->u,
+                             c->v, c->w, g, d->dt / d->grid.dx, d->dt / d->grid.dy,
+                             d->dt / d->grid.dz)));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Public API                                                                */
+/* ------------------------------------------------------------------------ */
+static double *hostField(size_t n, double value)
+{
+  double *a = nullptr;
+  HIP_CHECK(hipHostMalloc((void **)&a, n * sizeof(double), hipHostMallocDefault));
+  for (size_t q = 0; q < n; q++) a[q] = value;
+  return a;
+}
+
+static double *deviceField(size_t n)
+{
+  double *a = nullptr;
+  HIP_CHECK(hipMalloc((void **)&a, n * sizeof(double)));
+  return a;
+}
+
+/* Deterministic inflow perturbation: xorshift64* stream, uniform in [-1, 1]. */
+static void buildInflowProfile(NusifCtx *c)
+{
+  const Geo &g = c->geo;
+  size_t n     = (size_t)(g.nj + 2) * (g.nk + 2);
+  double *h    = (double *)malloc(n * sizeof(double));
+  unsigned long long x = 0x9E3779B97F4A7C15ull ^ (unsigned long long)(c->opt.seed + 1);
+  for (size_t q = 0; q < n; q++) {
+    x ^= x >> 12;
+    x ^= x << 25;
+    x ^= x >> 27;
+    unsigned long long r = x * 0x2545F4914F6CDD1Dull;
+    double xi            = (double)(r >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+    h[q]                 = c->opt.uInflow * (1.0 + c->opt.inflowNoise * xi);
+  }
+  HIP_CHECK(hipMalloc((void **)&c->inflow, n * sizeof(double)));
+  HIP_CHECK(hipMemcpy(c->inflow, h, n * sizeof(double), hipMemcpyHostToDevice));
+  free(h);
+}
+
+/* src/discretization.c:49-117 */
+extern "C" void initDiscretization(Discretization *s, Parameter *params)
+{
+  s->problem  = params->name;
+  s->bcLeft   = params->bcLeft;
+  s->bcRight  = params->bcRight;
+  s->bcBottom = params->bcBottom;
+  s->bcTop    = params->bcTop;
+  s->bcFront  = params->bcFront;
+  s->bcBack   = params->bcBack;
+
+  s->grid.imax    = params->imax;
+  s->grid.jmax    = params->jmax;
+  s->grid.kmax    = params->kmax;
+  s->grid.xlength = params->xlength;
+  s->grid.ylength = params->ylength;
+  s->grid.zlength = params->zlength;
+  s->grid.dx      = params->xlength / params->imax;
+  s->grid.dy      = params->ylength / params->jmax;
+  s->grid.dz      = params->zlength / params->kmax;
+
+  s->eps     = params->eps;
+  s->omega   = params->omg;
+  s->itermax = params->itermax;
+  s->re      = params->re;
+  s->gx      = params->gx;
+  s->gy      = params->gy;
+  s->gz      = params->gz;
+  s->dt      = params->dt;
+  s->te      = params->te;
+  s->tau     = params->tau;
+  s->gamma   = params->gamma;
+
+  if (s->comm.imaxLocal <= 0 || s->comm.jmaxLocal <= 0 || s->comm.kmaxLocal <= 0)
+    NUSIF_FATAL("call commPartition before initDiscretization (local sizes %d %d %d)",
+        s->comm.imaxLocal, s->comm.jmaxLocal, s->comm.kmaxLocal);
+
+  NusifCtx *c = new NusifCtx();
+  memset(c, 0, sizeof(*c));
+  c->owner = s;
+  c->geo   = makeGeo(s->comm.imaxLocal, s->comm.jmaxLocal, s->comm.kmaxLocal);
+  c->opt   = *nusifOptionsFor(params);
+  size_t n = (size_t)c->geo.n;
+
+  s->u   = hostField(n, params->u_init);
+  s->v   = hostField(n, params->v_init);
+  s->w   = hostField(n, params->w_init);
+  s->p   = hostField(n, params->p_init);
+  s->rhs = hostField(n, 0.0);
+  s->f   = hostField(n, 0.0);
+  s->g   = hostField(n, 0.0);
+  s->h   = hostField(n, 0.0);
+  double *hp[8] = {s->p, s->rhs, s->f, s->g, s->h, s->u, s->v, s->w};
+  memcpy(c->host, hp, sizeof(hp));
+
+  c->p    = deviceField(n);
+  c->rhs  = deviceField(n);
+  c->f    = deviceField(n);
+  c->g    = deviceField(n);
+  c->h    = deviceField(n);
+  c->u    = deviceField(n);
+  c->v    = deviceField(n);
+  c->w    = deviceField(n);
+  c->pAlt = deviceField(n);
+  c->nPartials = 65536;
+  HIP_CHECK(hipMalloc((void **)&c->partials, c->nPartials * sizeof(double)));
+  HIP_CHECK(hipMalloc((void **)&c->scal, sizeof(DevScalars)));
+  HIP_CHECK(hipHostMalloc((void **)&c->scalHost, sizeof(DevScalars), hipHostMallocDefault));
+  memset(c->scalHost, 0, sizeof(DevScalars));
+  HIP_CHECK(hipMemcpy(c->scal, c->scalHost, sizeof(DevScalars), hipMemcpyHostToDevice));
+  if (c->opt.inflowNoise != 0.0) buildInflowProfile(c);
+  nusifRegisterCtx(c);
+  nusifSyncDevice(s);
+  HIP_CHECK(hipMemcpyAsync(c->pAlt, c->p, n * sizeof(double), hipMemcpyDeviceToDevice, nusifStream()));
+
+  double dx = s->grid.dx, dy = s->grid.dy, dz = s->grid.dz;
+  double invSqrSum = 1.0 / (dx * dx) + 1.0 / (dy * dy) + 1.0 / (dz * dz);
+  s->dtBound       = 0.5 * s->re * 1.0 / invSqrSum;
+}
+
+extern "C" void nusifSyncHost(Discretization *d)
+{
+  NusifCtx *c   = nusifCtxOf(d);
+  size_t bytes  = (size_t)c->geo.n * sizeof(double);
+  double *dev[8] = {c->p, c->rhs, c->f, c->g, c->h, c->u, c->v, c->w};
+  for (int q = 0; q < 8; q++)
+    HIP_CHECK(hipMemcpyAsync(c->host[q], dev[q], bytes, hipMemcpyDeviceToHost, nusifStream()));
+  HIP_CHECK(hipStreamSynchronize(nusifStream()));
+}
+
+extern "C" void nusifSyncDevice(Discretization *d)
+{
+  NusifCtx *c   = nusifCtxOf(d);
+  size_t bytes  = (size_t)c->geo.n * sizeof(double);
+  double *dev[8] = {c->p, c->rhs, c->f, c->g, c->h, c->u, c->v, c->w};
+  for (int q = 0; q < 8; q++)
+    HIP_CHECK(hipMemcpyAsync(dev[q], c->host[q], bytes, hipMemcpyHostToDevice, nusifStream()));
+  HIP_CHECK(hipStreamSynchronize(nusifStream()));
+}
+
+extern "C" void nusifFreeDiscretization(Discretization *d)
+{
+  NusifCtx *c = nusifCtxOf(d);
+  HIP_CHECK(hipStreamSynchronize(nusifStream()));
+  double *dev[9] = {c->p, c->rhs, c->f, c->g, c->h, c->u, c->v, c->w, c->pAlt};
+  for (int q = 0; q < 9; q++) HIP_CHECK(hipFree(dev[q]));
+  HIP_CHECK(hipFree(c->partials));
+  HIP_CHECK(hipFree(c->scal));
+  if (c->inflow) HIP_CHECK(hipFree(c->inflow));
+  HIP_CHECK(hipHostFree(c->scalHost));
+  for (int q = 0; q < 8; q++) HIP_CHECK(hipHostFree(c->host[q]));
+  d->p = d->rhs = d->f = d->g = d->h = d->u = d->v = d->w = nullptr;
+  nusifUnregisterCtx(c);
+  delete c;
+}
+
+/* src/discretization.c:431-453. The maxima are reduced on the GPU; the dt
+ * arithmetic is the reference's, on the host, so d->dt is exact. */
+extern "C" void computeTimestep(Discretization *s)
+{
+  NusifCtx *c = nusifCtxOf(s);
+  launchAbsMax3(c);
+  HIP_CHECK(hipMemcpyAsync(c->scalHost->absmax, c->scal->absmax, sizeof(c->scal->absmax),
+      hipMemcpyDeviceToHost, nusifStream()));
+  HIP_CHECK(hipStreamSynchronize(nusifStream()));
+  double m[3];
+  for (int q = 0; q < 3; q++) memcpy(&m[q], &c->scalHost->absmax[q], sizeof(double));
+  double dt = s->dtBound;
+  double dx = s->grid.dx, dy = s->grid.dy, dz = s->grid.dz;
+  if (m[0] > 0) dt = (dt > dx / m[0]) ? dx / m[0] : dt;
+  if (m[1] > 0) dt = (dt > dy / m[1]) ? dy / m[1] : dt;
+  if (m[2] > 0) dt = (dt > dz / m[2]) ? dz / m[2] : dt;
+  s->dt = dt * s->tau;
+}
+
+extern "C" void setBoundaryConditions(Discretization *s) { launchVelocityBC(nusifCtxOf(s), s); }
+extern "C" void setSpecialBoundaryCondition(Discretization *s) { launchSpecialBC(nusifCtxOf(s), s); }
+extern "C" void computeFG(Discretization *s) { launchComputeFG(nusifCtxOf(s), s); }
+extern "C" void computeRHS(Discretization *s) { launchComputeRHS(nusifCtxOf(s), s); }
+extern "C" void normalizePressure(Discretization *s) { launchNormalize(nusifCtxOf(s)); }
+extern "C" void adaptUV(Discretization *s) { launchAdaptUV(nusifCtxOf(s), s); }

@@ -1,0 +1,170 @@
+/* Internal interfaces of the MI355X NuSiF library (host side + kernel
+ * launchers). Not installed; callers use include/nusif.h and the reference
+ * headers. */
+#ifndef NUSIF_INTERNAL_H
+#define NUSIF_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include "nusif.h"
+
+#define HIP_CHECK(call)                                                                \
+  do {                                                                                 \
+    hipError_t err_ = (call);                                                          \
+    if (err_ != hipSuccess) {                                                          \
+      fprintf(stderr, "nusif: HIP error %s at %s:%d: %s\n", hipGetErrorName(err_),     \
+          __FILE__, __LINE__, hipGetErrorString(err_));                                \
+      exit(EXIT_FAILURE);                                                              \
+    }                                                                                  \
+  } while (0)
+
+#define NUSIF_FATAL(...)                                                               \
+  do {                                                                                 \
+    fprintf(stderr, "nusif: " __VA_ARGS__);                                            \
+    fputc('\n', stderr);                                                               \
+    exit(EXIT_FAILURE);                                                                \
+  } while (0)
+
+/* Padded i-fastest grid with one ghost layer (reference src/util.h:19-34). */
+struct Geo {
+  int ni, nj, nk;   /* interior cells */
+  int sj;           /* ni + 2 */
+  long long sk;     /* (ni + 2) * (nj + 2) */
+  long long n;      /* padded cell count */
+  __host__ __device__ long long at(int i, int j, int k) const
+  {
+    return (long long)k * sk + (long long)j * sj + i;
+  }
+};
+
+static inline Geo makeGeo(int ni, int nj, int nk)
+{
+  Geo g;
+  g.ni = ni;
+  g.nj = nj;
+  g.nk = nk;
+  g.sj = ni + 2;
+  g.sk = (long long)(ni + 2) * (nj + 2);
+  g.n  = g.sk * (nk + 2);
+  return g;
+}
+
+/* Options carried next to a Parameter (nusifSetOption / extra .par keys). */
+enum SolverKind { SOLVER_DEFAULT = 0, SOLVER_RB, SOLVER_MG, SOLVER_GMG };
+struct NusifOptions {
+  int nsteps;
+  int solver;
+  int mixed;
+  double mgTol;
+  int mgMaxCycles;
+  double mgOmega;
+  int mgCoarse;
+  int mgProject;
+  double lid;
+  double uInflow;
+  double inflowNoise;
+  int seed;
+  int warmstart;
+  char unetWeights[512];
+};
+void nusifDefaultOptions(NusifOptions *o);
+/* Options registered for `p` (defaults when none were set). */
+const NusifOptions *nusifOptionsFor(const Parameter *p);
+
+/* Scalar state shared by the reduction kernels (device memory). */
+struct DevScalars {
+  unsigned long long absmax[3]; /* bit patterns of max |u|, |v|, |w| */
+  double sum;                   /* last sum reduction */
+  double res;                   /* solver residual (carried)      } RBState, */
+  int it;                       /* solver iterations completed    } same    */
+  int stopAt;                   /* iterations to run (stop test)  } layout  */
+  unsigned int counter[4];      /* last-block counters (self-resetting) */
+  double norm2[4];              /* gmg norms */
+};
+
+struct GmgState; /* gmg.hip */
+struct UnetModel; /* unet.hip */
+
+/* One Discretization's device state. */
+struct NusifCtx {
+  Discretization *owner;
+  double *host[8];     /* p rhs f g h u v w host arrays, identity for lookups */
+  Geo geo;
+  double *p, *rhs, *f, *g, *h, *u, *v, *w; /* device fields */
+  double *pAlt;        /* ping-pong partner of p for the fused RB kernel */
+  double *partials;    /* reduction partials */
+  size_t nPartials;
+  DevScalars *scal;    /* device */
+  DevScalars *scalHost;/* pinned mirror */
+  double *inflow;      /* optional per-cell inflow profile (jmax+2)*(kmax+2) */
+  NusifOptions opt;
+  double t;            /* nusifRunSteps clock */
+  int nt;
+};
+
+/* One Solver's device state. */
+struct NusifSolverCtx {
+  Solver *owner;
+  NusifCtx *dctx;
+  int kind;
+  int lastIterations;
+  double lastRel;
+  int warmUsed;
+  int predicted; /* iterations of the previous solve, sizes the launch batches */
+  GmgState *gmg;
+  UnetModel *unet;
+};
+
+hipStream_t nusifStream(void);
+NusifCtx *nusifCtxOf(const Discretization *d);
+NusifCtx *nusifCtxOfHost(const double *hostPtr, int *fieldIndex);
+NusifSolverCtx *nusifSolverCtxOf(const Solver *s);
+
+/* Kernel timing hooks (context.hip). */
+struct KernelTimer;
+KernelTimer *nusifTimerFor(const char *name);
+void nusifTimerBegin(KernelTimer *t);
+void nusifTimerEnd(KernelTimer *t);
+
+#define NUSIF_TIMED(name, launch)                                                     \
+  do {                                                                                \
+    static KernelTimer *tm_ = nullptr;                                                \
+    static int tmGen_       = -1;                                                     \
+    extern int nusifTimerGeneration;                                                  \
+    if (tmGen_ != nusifTimerGeneration) {                                             \
+      tm_    = nusifTimerFor(name);                                                   \
+      tmGen_ = nusifTimerGeneration;                                                  \
+    }                                                                                 \
+    if (tm_) nusifTimerBegin(tm_);                                                    \
+    launch;                                                                           \
+    if (tm_) nusifTimerEnd(tm_);                                                      \
+  } while (0)
+
+/* Launchers (step.hip) */
+void launchAbsMax3(NusifCtx *c);
+void launchVelocityBC(NusifCtx *c, const Discretization *d);
+void launchSpecialBC(NusifCtx *c, const Discretization *d);
+void launchComputeFG(NusifCtx *c, const Discretization *d);
+void launchComputeRHS(NusifCtx *c, const Discretization *d);
+void launchNormalize(NusifCtx *c);
+void launchAdaptUV(NusifCtx *c, const Discretization *d);
+
+/* Solvers */
+double rbSolve(NusifSolverCtx *sc, double *p, const double *rhs, double **pOut);
+double mgRefSolve(NusifSolverCtx *sc, double *p, const double *rhs);
+double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhs);
+void gmgFree(GmgState *g);
+void unetFree(UnetModel *m);
+
+/* Shared by rb / mg / gmg: colour sweep, pressure ghosts */
+struct SweepCoef {
+  double idx2, idy2, idz2, factor;
+};
+SweepCoef sweepCoef(const Grid *g, double omega);
+void launchPressureGhosts(const Geo &g, double *p, hipStream_t st);
+
+#endif

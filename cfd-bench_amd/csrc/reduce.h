@@ -1,0 +1,92 @@
+/* Wavefront / workgroup reductions and the "last workgroup finishes"
+ * pattern used by every reduction kernel: each workgroup writes one partial,
+ * the workgroup that arrives last (agent-scope counter, self-resetting) folds
+ * the partials in index order, so results do not depend on scheduling. */
+#ifndef NUSIF_REDUCE_H
+#define NUSIF_REDUCE_H
+
+#include <hip/hip_runtime.h>
+
+/* 64-lane butterfly sum / max over one wavefront */
+__device__ inline double waveSum(double x)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+__device__ inline double waveMax(double x)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    double y = __shfl_xor(x, o, 64);
+    x        = x > y ? x : y;
+  }
+  return x;
+}
+
+/* Sum over the workgroup; the result is valid in thread 0. `lds` holds one
+ * double per wavefront. */
+__device__ inline double blockSum(double x, double *lds)
+{
+  int tid   = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  int nthr  = blockDim.x * blockDim.y * blockDim.z;
+  int lane  = tid & 63, wave = tid >> 6, nwave = (nthr + 63) >> 6;
+  x         = waveSum(x);
+  if (lane == 0) lds[wave] = x;
+  __syncthreads();
+  double r = 0.0;
+  if (tid == 0)
+    for (int q = 0; q < nwave; q++) r += lds[q];
+  __syncthreads();
+  return r;
+}
+
+__device__ inline double blockMax(double x, double *lds)
+{
+  int tid   = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  int nthr  = blockDim.x * blockDim.y * blockDim.z;
+  int lane  = tid & 63, wave = tid >> 6, nwave = (nthr + 63) >> 6;
+  x         = waveMax(x);
+  if (lane == 0) lds[wave] = x;
+  __syncthreads();
+  double r = 0.0;
+  if (tid == 0) {
+    r = lds[0];
+    for (int q = 1; q < nwave; q++) r = r > lds[q] ? r : lds[q];
+  }
+  __syncthreads();
+  return r;
+}
+
+__device__ inline double loadAgent(const double *p)
+{
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline void storeAgent(double *p, double v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* Call with the workgroup's partial already stored (by thread 0). Returns
+ * true in every thread of the workgroup that arrived last; that workgroup
+ * then sees all partials. */
+__device__ inline bool lastBlock(unsigned int *counter, unsigned int nblocks)
+{
+  __shared__ int amLast;
+  int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  if (tid == 0) {
+    __threadfence();
+    unsigned int prev = atomicAdd(counter, 1u);
+    amLast            = (prev == nblocks - 1);
+    if (amLast) {
+      atomicExch(counter, 0u);
+      __threadfence();
+    }
+  }
+  __syncthreads();
+  return amLast != 0;
+}
+
+#endif

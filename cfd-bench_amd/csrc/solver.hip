@@ -1,0 +1,123 @@
+/* Solver plugin (reference src/solver.h:35-36) and the in-library step loop.
+ *
+ * The reference picks its solver at link time (Makefile:19-21, SOLVER=rb|mg).
+ * This library is built twice from the same sources, libnusif_rb.so and
+ * libnusif_mg.so, whose default solver is NUSIF_DEFAULT_SOLVER; the added
+ * option `solver` (rb | mg | gmg) overrides it per problem. */
+#include <string.h>
+
+#include "nusif_internal.h"
+
+#ifndef NUSIF_DEFAULT_SOLVER
+#define NUSIF_DEFAULT_SOLVER SOLVER_RB
+#endif
+
+void nusifRegisterSolver(NusifSolverCtx *s);
+void nusifUnregisterSolver(NusifSolverCtx *s);
+GmgState *gmgCreate(NusifSolverCtx *sc, const Parameter *prm);
+
+/* src/solver-rb.c:17-25 and src/solver-mg.c:356-389 */
+extern "C" void initSolver(Solver *s, Discretization *d, Parameter *p)
+{
+  memset(s, 0, sizeof(*s));
+  s->eps        = p->eps;
+  s->omega      = p->omg;
+  s->itermax    = p->itermax;
+  s->levels     = p->levels;
+  s->presmooth  = p->presmooth;
+  s->postsmooth = p->postsmooth;
+  s->grid       = &d->grid;
+  s->comm       = &d->comm;
+  s->problem    = p->name;
+  s->r = s->e = nullptr; /* device hierarchy lives in the solver context */
+
+  NusifSolverCtx *sc = new NusifSolverCtx();
+  memset(sc, 0, sizeof(*sc));
+  sc->owner    = s;
+  sc->dctx     = nusifCtxOf(d);
+  sc->kind     = sc->dctx->opt.solver != SOLVER_DEFAULT ? sc->dctx->opt.solver : NUSIF_DEFAULT_SOLVER;
+  sc->lastRel  = -1.0;
+  sc->warmUsed = -1;
+  if (sc->kind == SOLVER_MG && commIsMaster(s->comm))
+    printf("Using Multigrid solver with %d levels\n", s->levels);
+  if (sc->kind == SOLVER_GMG) sc->gmg = gmgCreate(sc, p);
+  nusifRegisterSolver(sc);
+}
+
+extern "C" void nusifFreeSolver(Solver *s)
+{
+  NusifSolverCtx *sc = nusifSolverCtxOf(s);
+  HIP_CHECK(hipStreamSynchronize(nusifStream()));
+  if (sc->gmg) gmgFree(sc->gmg);
+  if (sc->unet) unetFree(sc->unet);
+  nusifUnregisterSolver(sc);
+  delete sc;
+}
+
+/* p and rhs are host pointers (src/main.c:71). When they are the arrays of a
+ * Discretization of this library, its device copies are solved in place;
+ * other host arrays are staged through the owning context's device buffers. */
+extern "C" double solve(Solver *s, double *p, const double *rhs)
+{
+  NusifSolverCtx *sc = nusifSolverCtxOf(s);
+  NusifCtx *c        = sc->dctx;
+  int ip = -1, ir = -1;
+  NusifCtx *cp = nusifCtxOfHost(p, &ip);
+  NusifCtx *cr = nusifCtxOfHost(rhs, &ir);
+  bool staged  = !(cp == c && ip == 0 && cr == c && ir == 1);
+  size_t bytes = (size_t)c->geo.n * sizeof(double);
+  if (staged) {
+    /* foreign arrays: copy in, solve on the context's own p / rhs, copy out */
+    HIP_CHECK(hipMemcpyAsync(c->p, p, bytes, hipMemcpyHostToDevice, nusifStream()));
+    HIP_CHECK(hipMemcpyAsync(c->rhs, rhs, bytes, hipMemcpyHostToDevice, nusifStream()));
+  }
+  double res = 0.0;
+  switch (sc->kind) {
+  case SOLVER_RB: {
+    double *pOut = c->p;
+    res          = rbSolve(sc, c->p, c->rhs, &pOut);
+    if (pOut != c->p) { /* ping-pong ended on the partner buffer */
+      c->pAlt = c->p;
+      c->p    = pOut;
+    }
+  } break;
+  case SOLVER_MG: res = mgRefSolve(sc, c->p, c->rhs); break;
+  case SOLVER_GMG: res = gmgSolve(sc, c->p, c->rhs); break;
+  default: NUSIF_FATAL("unknown solver kind %d", sc->kind);
+  }
+  if (staged) {
+    HIP_CHECK(hipMemcpyAsync(p, c->p, bytes, hipMemcpyDeviceToHost, nusifStream()));
+    HIP_CHECK(hipStreamSynchronize(nusifStream()));
+  }
+  return res;
+}
+
+extern "C" int nusifSolverIterations(Solver *s) { return nusifSolverCtxOf(s)->lastIterations; }
+extern "C" double nusifSolverRelResidual(Solver *s) { return nusifSolverCtxOf(s)->lastRel; }
+extern "C" int nusifWarmstartUsed(Solver *s) { return nusifSolverCtxOf(s)->warmUsed; }
+
+/* The loop body of src/main.c:60-88, driven from inside the library. */
+extern "C" int nusifRunSteps(Discretization *d, Solver *s, int nsteps, double *trace)
+{
+  NusifCtx *c = nusifCtxOf(d);
+  for (int q = 0; q < nsteps; q++) {
+    if (d->tau > 0.0) computeTimestep(d);
+    setBoundaryConditions(d);
+    setSpecialBoundaryCondition(d);
+    computeFG(d);
+    computeRHS(d);
+    if (c->nt % 100 == 0) normalizePressure(d);
+    double res = solve(s, d->p, d->rhs);
+    adaptUV(d);
+    if (trace) {
+      trace[4 * q + 0] = c->t;
+      trace[4 * q + 1] = d->dt;
+      trace[4 * q + 2] = res;
+      trace[4 * q + 3] = (double)nusifSolverIterations(s);
+    }
+    c->t += d->dt;
+    c->nt++;
+  }
+  HIP_CHECK(hipStreamSynchronize(nusifStream()));
+  return nsteps;
+}

@@ -1,0 +1,305 @@
+/* Red-black SOR pressure solves on the GPU: the reference's RB solver
+ * (src/solver-rb.c:27-172) and the observable fine-grid behaviour of its MG
+ * solver (src/solver-mg.c:299-402).
+ *
+ * Colour passes: pass 0 updates cells with i+j+k odd, pass 1 even
+ * (src/solver-rb.c:55-79). A cell of one colour only reads cells of the other
+ * colour, so a pass is order-free and every thread can update its cell
+ * independently: p values are bit-identical to the sequential sweep; only the
+ * sum of r^2 is accumulated in a different order.
+ *
+ * Stop test on the device: the iteration loop runs as batches of launches;
+ * each kernel of iteration q returns at once when q >= stopAt, and the last
+ * workgroup of iteration q's final kernel folds the residual and lowers
+ * stopAt when `res >= eps^2 && it < itermax` fails (src/solver-rb.c:54). The
+ * host reads the state once per batch. */
+#include <limits.h>
+#include <math.h>
+
+#include "nusif_internal.h"
+#include "reduce.h"
+
+#define RB_TX 64
+#define RB_TY 4
+#define RB_KCH 8
+
+struct RBState {
+  double res;
+  int it;
+  int stopAt;
+};
+
+SweepCoef sweepCoef(const Grid *g, double omega)
+{
+  SweepCoef c;
+  double dx2 = g->dx * g->dx, dy2 = g->dy * g->dy, dz2 = g->dz * g->dz;
+  c.idx2   = 1.0 / dx2;
+  c.idy2   = 1.0 / dy2;
+  c.idz2   = 1.0 / dz2;
+  c.factor = omega * 0.5 * (dx2 * dy2 * dz2) / (dy2 * dz2 + dx2 * dz2 + dx2 * dy2);
+  return c;
+}
+
+/* r = rhs - Lap7(p), operand order of src/solver-rb.c:66-69 */
+__device__ inline double defect7(const double *p, double rhsc, long long c, long long sj,
+    long long sk, const SweepCoef &cf)
+{
+  double pc = p[c];
+  return rhsc - ((p[c + 1] - 2.0 * pc + p[c - 1]) * cf.idx2 +
+                    (p[c + sj] - 2.0 * pc + p[c - sj]) * cf.idy2 +
+                    (p[c + sk] - 2.0 * pc + p[c - sk]) * cf.idz2);
+}
+
+/* One colour pass. Thread x covers the x-th cell of its colour along a row,
+ * the workgroup covers RB_TY rows and RB_KCH planes. */
+__global__ __launch_bounds__(256) void kColourPass(double *__restrict__ p,
+    const double *__restrict__ rhs, Geo g, SweepCoef cf, int parity, int q,
+    const RBState *st, double *__restrict__ partials, int accumulate)
+{
+  __shared__ double lds[4];
+  if (q >= st->stopAt) return;
+  int x  = blockIdx.x * blockDim.x + threadIdx.x;
+  int j  = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int k0 = blockIdx.z * RB_KCH + 1;
+  double acc = 0.0;
+  if (j <= g.nj) {
+    int k1 = min(k0 + RB_KCH, g.nk + 1);
+    for (int k = k0; k < k1; k++) {
+      int i = 2 * x + (((1 + j + k) & 1) == parity ? 1 : 2);
+      if (i <= g.ni) {
+        long long c = g.at(i, j, k);
+        double r    = defect7(p, rhs[c], c, g.sj, g.sk, cf);
+        p[c] -= cf.factor * r;
+        acc += r * r;
+      }
+    }
+  }
+  if (accumulate) {
+    acc = blockSum(acc, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+      partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = acc;
+  }
+}
+
+/* Neumann ghost copy on the six faces over their interior ranges
+ * (src/solver-rb.c:129-148): thread (a, b) serves face points of all three
+ * face pairs that contain it. */
+__device__ inline void ghostsAt(double *p, const Geo &g, int a, int b)
+{
+  if (a <= g.ni && b <= g.nj) { /* FRONT / BACK: (i, j) */
+    p[g.at(a, b, 0)]        = p[g.at(a, b, 1)];
+    p[g.at(a, b, g.nk + 1)] = p[g.at(a, b, g.nk)];
+  }
+  if (a <= g.ni && b <= g.nk) { /* BOTTOM / TOP: (i, k) */
+    p[g.at(a, 0, b)]        = p[g.at(a, 1, b)];
+    p[g.at(a, g.nj + 1, b)] = p[g.at(a, g.nj, b)];
+  }
+  if (a <= g.nj && b <= g.nk) { /* LEFT / RIGHT: (j, k) */
+    p[g.at(0, a, b)]        = p[g.at(1, a, b)];
+    p[g.at(g.ni + 1, a, b)] = p[g.at(g.ni, a, b)];
+  }
+}
+
+__global__ __launch_bounds__(256) void kPressureGhosts(double *p, Geo g)
+{
+  int a = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int b = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  ghostsAt(p, g, a, b);
+}
+
+static dim3 ghostGrid(const Geo &g)
+{
+  int ea = g.ni > g.nj ? g.ni : g.nj;
+  int eb = g.nj > g.nk ? g.nj : g.nk;
+  return dim3((ea + 63) / 64, (eb + 3) / 4);
+}
+
+void launchPressureGhosts(const Geo &g, double *p, hipStream_t st)
+{
+  kPressureGhosts<<<ghostGrid(g), dim3(64, 4), 0, st>>>(p, g);
+}
+
+/* Iteration q's last kernel: ghosts, then the last workgroup folds both
+ * passes' partials (pass 0 first) into the carried residual
+ * (src/solver-rb.c:150-151) and applies the stop test. */
+__global__ __launch_bounds__(256) void kRBFinish(double *p, Geo g, int q, RBState *st,
+    const double *__restrict__ partials, int nPart, double n, double epssq, int itermax,
+    unsigned int *counter)
+{
+  __shared__ double lds[4];
+  if (q >= st->stopAt) return;
+  int a = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int b = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  ghostsAt(p, g, a, b);
+  if (lastBlock(counter, gridDim.x * gridDim.y)) {
+    int tid  = threadIdx.x + blockDim.x * threadIdx.y;
+    double s0 = 0.0, s1 = 0.0;
+    for (int x = tid; x < nPart; x += blockDim.x * blockDim.y) {
+      s0 += loadAgent(&partials[x]);
+      s1 += loadAgent(&partials[nPart + x]);
+    }
+    s0 = blockSum(s0, lds);
+    s1 = blockSum(s1, lds);
+    if (tid == 0) {
+      double res = (st->res + (s0 + s1)) / n;
+      int it     = q + 1;
+      st->res    = res;
+      st->it     = it;
+      if (!(res >= epssq && it < itermax)) st->stopAt = it;
+      __threadfence();
+    }
+  }
+}
+
+/* MG residual (src/solver-mg.c:226-289): mean square of rhs - Lap(p) over the
+ * interior, seeded with 1.0 (src/solver-mg.c:252). */
+__global__ __launch_bounds__(256) void kResidualMS(const double *__restrict__ p,
+    const double *__restrict__ rhs, Geo g, SweepCoef cf, double *partials, RBState *st,
+    double n, unsigned int *counter)
+{
+  __shared__ double lds[4];
+  int i  = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j  = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int k0 = blockIdx.z * RB_KCH + 1;
+  double acc = 0.0;
+  if (i <= g.ni && j <= g.nj) {
+    int k1 = min(k0 + RB_KCH, g.nk + 1);
+    for (int k = k0; k < k1; k++) {
+      long long c = g.at(i, j, k);
+      double r    = defect7(p, rhs[c], c, g.sj, g.sk, cf);
+      acc += r * r;
+    }
+  }
+  acc          = blockSum(acc, lds);
+  unsigned nb  = gridDim.x * gridDim.y * gridDim.z;
+  unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  int tid      = threadIdx.x + blockDim.x * threadIdx.y;
+  if (tid == 0) partials[bid] = acc;
+  if (lastBlock(counter, nb)) {
+    double s = 0.0;
+    for (unsigned x = tid; x < nb; x += blockDim.x * blockDim.y) s += loadAgent(&partials[x]);
+    s = blockSum(s, lds);
+    if (tid == 0) st->res = (1.0 + s) / n;
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+struct SorLaunch {
+  dim3 blk, grd;
+  int nPart;
+};
+
+static SorLaunch sorShape(const Geo &g)
+{
+  SorLaunch L;
+  L.blk   = dim3(RB_TX, RB_TY);
+  int half = (g.ni + 1) / 2;
+  L.grd   = dim3((half + RB_TX - 1) / RB_TX, (g.nj + RB_TY - 1) / RB_TY,
+      (g.nk + RB_KCH - 1) / RB_KCH);
+  L.nPart = (int)(L.grd.x * L.grd.y * L.grd.z);
+  return L;
+}
+
+static RBState *stateBuf(NusifCtx *c) { return (RBState *)&c->scal->res; }
+static RBState *stateHost(NusifCtx *c) { return (RBState *)&c->scalHost->res; }
+
+static void putState(NusifCtx *c, double res, int it, int stopAt)
+{
+  RBState *h = stateHost(c);
+  h->res     = res;
+  h->it      = it;
+  h->stopAt  = stopAt;
+  HIP_CHECK(hipMemcpyAsync(stateBuf(c), h, sizeof(RBState), hipMemcpyHostToDevice, nusifStream()));
+}
+
+static void getState(NusifCtx *c)
+{
+  HIP_CHECK(hipMemcpyAsync(stateHost(c), stateBuf(c), sizeof(RBState), hipMemcpyDeviceToHost,
+      nusifStream()));
+  HIP_CHECK(hipStreamSynchronize(nusifStream()));
+}
+
+static void launchPass(NusifCtx *c, const Geo &g, const SorLaunch &L, double *p, const double *rhs,
+    const SweepCoef &cf, int parity, int q, double *partials, int accumulate)
+{
+  NUSIF_TIMED("rb_pass", (kColourPass<<<L.grd, L.blk, 0, nusifStream()>>>(
+                             p, rhs, g, cf, parity, q, stateBuf(c), partials, accumulate)));
+}
+
+/* src/solver-rb.c:27-172 */
+double rbSolve(NusifSolverCtx *sc, double *p, const double *rhs, double **pOut)
+{
+  NusifCtx *c       = sc->dctx;
+  const Solver *s   = sc->owner;
+  const Geo &g      = c->geo;
+  SweepCoef cf      = sweepCoef(s->grid, s->omega);
+  SorLaunch L       = sorShape(g);
+  double epssq      = s->eps * s->eps;
+  double n          = (double)(s->grid->imax * s->grid->jmax * s->grid->kmax);
+  int itermax       = s->itermax;
+  if ((size_t)2 * L.nPart > c->nPartials) NUSIF_FATAL("partials buffer too small");
+  *pOut = p;
+
+  int stopAt = (1.0 >= epssq && 0 < itermax) ? itermax : 0;
+  putState(c, 1.0, 0, stopAt);
+  dim3 gg   = ghostGrid(g);
+  int launched = 0;
+  int batch    = sc->predicted > 0 ? sc->predicted : 8;
+  while (launched < stopAt) {
+    int end = launched + batch;
+    if (end > itermax) end = itermax;
+    for (int q = launched; q < end; q++) {
+      launchPass(c, g, L, p, rhs, cf, 1, q, c->partials, 1);
+      launchPass(c, g, L, p, rhs, cf, 0, q, c->partials + L.nPart, 1);
+      kRBFinish<<<gg, dim3(64, 4), 0, nusifStream()>>>(p, g, q, stateBuf(c), c->partials,
+          L.nPart, n, epssq, itermax, &c->scal->counter[2]);
+    }
+    launched = end;
+    getState(c);
+    stopAt = stateHost(c)->stopAt;
+    batch  = 8;
+  }
+  getState(c);
+  sc->lastIterations = stateHost(c)->it;
+  sc->predicted      = sc->lastIterations;
+  return stateHost(c)->res;
+}
+
+/* Observable behaviour of src/solver-mg.c:299-402 on p. The coarse levels'
+ * work never reaches p or the returned residual (`correct` adds s->e[0],
+ * which nothing writes; src/solver-mg.c:87,96,334), so it is not run. */
+double mgRefSolve(NusifSolverCtx *sc, double *p, const double *rhs)
+{
+  NusifCtx *c     = sc->dctx;
+  const Solver *s = sc->owner;
+  const Geo &g    = c->geo;
+  SweepCoef cf    = sweepCoef(s->grid, s->omega);
+  SorLaunch L     = sorShape(g);
+  double n        = (double)(s->grid->imax * s->grid->jmax * s->grid->kmax);
+  putState(c, 1.0, 0, INT_MAX);
+  bool coarsest = (s->levels - 1 == 0) || g.ni < 2 || g.nj < 2 || g.nk < 2;
+  auto smooth = [&](bool ghosts) {
+    launchPass(c, g, L, p, rhs, cf, 1, 0, c->partials, 0);
+    launchPass(c, g, L, p, rhs, cf, 0, 0, c->partials, 0);
+    if (ghosts) launchPressureGhosts(g, p, nusifStream());
+  };
+  dim3 rgrd((g.ni + 63) / 64, (g.nj + 3) / 4, (g.nk + RB_KCH - 1) / RB_KCH);
+  if ((size_t)rgrd.x * rgrd.y * rgrd.z > c->nPartials) NUSIF_FATAL("partials buffer too small");
+  auto residual = [&]() {
+    kResidualMS<<<rgrd, dim3(64, 4), 0, nusifStream()>>>(p, rhs, g, cf, c->partials, stateBuf(c),
+        n, &c->scal->counter[3]);
+  };
+  if (coarsest) {
+    for (int q = 0; q < s->presmooth; q++) smooth(false);
+    residual();
+    sc->lastIterations = s->presmooth;
+  } else {
+    for (int q = 0; q < s->presmooth; q++) smooth(true);
+    residual();
+    launchPressureGhosts(g, p, nusifStream());
+    for (int q = 0; q < s->postsmooth; q++) smooth(true);
+    sc->lastIterations = s->presmooth + s->postsmooth;
+  }
+  getState(c);
+  return stateHost(c)->res;
+}

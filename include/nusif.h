@@ -1,0 +1,94 @@
+/* nusif.h — what the MI355X library adds to the NuSiF C API.
+ *
+ * The reference API (discretization.h, solver.h, parameter.h, comm.h, ...) is
+ * exported unchanged. Its eight fields live in HBM of the one GPU; the host
+ * arrays in Discretization are staging copies:
+ *   - initDiscretization fills both copies;
+ *   - every step function works on the device copy only;
+ *   - nusifSyncHost() downloads the device copy (commCollectResult does this
+ *     by itself for the arrays it is given);
+ *   - nusifSyncDevice() uploads host arrays a caller has modified.
+ * Errors follow the reference convention: a message on stderr and
+ * exit(EXIT_FAILURE) (reference src/allocate.c:19-33).
+ *
+ * None of these functions has a counterpart in the reference; each comment
+ * names what it serves.
+ */
+#ifndef NUSIF_H
+#define NUSIF_H
+
+#include "discretization.h"
+#include "parameter.h"
+#include "solver.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Download all eight fields of `d` from the GPU into its host arrays. */
+void nusifSyncHost(Discretization *d);
+/* Upload all eight host arrays of `d` to the GPU. */
+void nusifSyncDevice(Discretization *d);
+/* Release the device and host memory of `d` / of a solver (the reference
+ * never frees; long-running callers and tests need it). */
+void nusifFreeDiscretization(Discretization *d);
+void nusifFreeSolver(Solver *s);
+
+/* Options beyond the reference's .par keys. readParameter() also accepts them
+ * as .par lines. Keys are chosen so that the reference's prefix matching
+ * (src/parameter.c:56-62) never mistakes them for one of its own keys.
+ *   nsteps        int     stop the driver loop after this many steps (0: off)
+ *   solver        rb | mg | gmg     pressure solver (default: library variant)
+ *   precision     fp64 | mixed      gmg only: mixed = FP32 V-cycles inside an
+ *                                   FP64 defect-correction loop
+ *   mg_tol        double  gmg: relative residual ||rhs-Ap||/||rhs|| (1e-6)
+ *   mg_maxcycles  int     gmg: cycle cap per solve (100)
+ *   mg_omega      double  gmg: smoother relaxation (1.0)
+ *   mg_coarse     int     gmg: sweeps of the coarsest-level solve (64)
+ *   mg_project    0 | 1   gmg: remove the rhs mean (all-Neumann compatibility)
+ *   lid           double  lid speed of "dcavity" (reference: 1.0)
+ *   u_inflow      double  inflow velocity of "canal" (reference: 2.0)
+ *   inflow_noise  double  relative uniform perturbation of the inflow (0)
+ *   seed          int     seed of inflow_noise (1)
+ *   warmstart     none | unet       initial guess of the gmg solve
+ *   unet_weights  path    trained U-Net weights (nusifUnetSave format)
+ * Returns 0 on success, -1 for an unknown key, -2 for a bad value. */
+int nusifSetOption(Parameter *p, const char *key, const char *value);
+
+/* The `nsteps` option of `p` (0 when unset). */
+int nusifStepLimit(Parameter *p);
+
+/* Iterations (rb), sweeps (mg) or V-cycles (gmg) of the last solve(). */
+int nusifSolverIterations(Solver *s);
+/* Relative residual ||rhs - A p||_2 / ||rhs||_2 after the last gmg solve
+ * (-1 for rb/mg, whose stop tests use the reference's mean-square residual). */
+double nusifSolverRelResidual(Solver *s);
+/* Outcome of the last warm start: 1 network guess used, 0 fell back to the
+ * previous-step pressure, -1 no network. */
+int nusifWarmstartUsed(Solver *s);
+
+/* Name of the GPU the library runs on; returns the device count. */
+int nusifDeviceName(char *buf, int len);
+void nusifDeviceSynchronize(void);
+
+/* Per-kernel HIP-event timers on the library's stream (for roofline
+ * reporting). nusifTimerEnable(name) starts timing every launch of that
+ * kernel; nusifTimerRead returns the summed milliseconds and the launch count
+ * (synchronising the stream). Names: see nusifKernelBytes. */
+int nusifTimerEnable(const char *kernel);
+double nusifTimerRead(const char *kernel, int *launches);
+void nusifTimerReset(void);
+/* Algorithmic HBM bytes of one launch of `kernel` on the grid of `d`
+ * (each array element read once and/or written once). */
+double nusifKernelBytes(Discretization *d, const char *kernel);
+
+/* Runs the reference time loop (src/main.c:60-88) for `nsteps` steps entirely
+ * inside the library: the same calls in the same order, without the Python
+ * or host-driver overhead between them. trace (may be NULL) receives
+ * (t, dt, res, iterations) per step. Returns the steps run. */
+int nusifRunSteps(Discretization *d, Solver *s, int nsteps, double *trace);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
