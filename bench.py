@@ -1,0 +1,180 @@
+"""Benchmark: time steps/s and MLUP/s of the 256^3 lid-driven cavity on one
+MI355X, pressure solved each step to a relative residual of 1e-6.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode M] [--size N]
+
+A step is the reference time step (computeTimestep, boundary conditions,
+computeFG, computeRHS, [normalizePressure], solve, adaptUV; reference
+src/main.c:60-88) on the GPU library; the K timed steps run inside one
+nusifRunSteps call with all fields resident in HBM. Modes:
+  learned  U-Net warm start + FP32 V-cycles in FP64 defect correction
+  mixed    FP32 V-cycles in FP64 defect correction (previous-step guess)
+  fp64     plain FP64 V(2,2) cycles (the comparison path)
+The headline line also times the fp64 path on the same workload and reports
+the speedup, the roofline of the dominant kernel (HIP events on the
+library's stream) and a CPU baseline (the oracle's FP64 multigrid on one host
+core, a bounded sample of steps of the same workload).
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cfd-bench_amd"))
+import nusif_api as A  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "time steps/s + MLUP/s, 256^3 cavity: learned+FP32-MG vs plain FP64 MG"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cavity_params(n):
+    return dict(name="dcavity", re=1000.0, imax=n, jmax=n, kmax=n, tau=0.5, omg=1.9, eps=1e-3,
+                itermax=1000, gamma=0.9, te=1e9, levels=6, presmooth=2, postsmooth=2)
+
+
+def mode_options(mode):
+    opts = {"solver": "gmg", "mg_tol": 1e-6}
+    if mode in ("mixed", "learned"):
+        opts["precision"] = "mixed"
+    if mode == "learned":
+        opts["warmstart"] = "unet"
+        opts["unet_weights"] = os.path.join(ROOT, "cfd-bench_amd", "weights", "unet_cavity.bin")
+    return opts
+
+
+DOMINANT = {"fp64": "gmg_smooth64_l0", "mixed": "gmg_smooth32_l0", "learned": "gmg_smooth32_l0"}
+TIMERS = ["gmg_smooth64_l0", "gmg_smooth32_l0", "gmg_restrict32_l0", "gmg_prolong32_l0",
+          "gmg_restrict64_l0", "gmg_prolong64_l0", "gmg_residual64", "gmg_correct", "computeFG",
+          "computeRHS", "adaptUV", "absmax"]
+
+
+def run_mode(lib, mode, n, warmup, steps):
+    case = A.Case(lib, options=mode_options(mode), **cavity_params(n))
+    trace = (C.c_double * (4 * max(warmup, steps, 1)))()
+    if warmup:
+        lib.nusifRunSteps(C.byref(case.d), C.byref(case.s), warmup, trace)
+    log("[bench] %s: %d warmup steps done" % (mode, warmup))
+    for t in TIMERS:
+        lib.nusifTimerEnable(t.encode())
+    lib.nusifTimerReset()
+    lib.nusifDeviceSynchronize()
+    t0 = time.perf_counter()
+    lib.nusifRunSteps(C.byref(case.d), C.byref(case.s), steps, trace)
+    lib.nusifDeviceSynchronize()
+    t1 = time.perf_counter()
+    cycles = [trace[4 * q + 3] for q in range(steps)]
+    rels = [trace[4 * q + 2] for q in range(steps)]
+    kern = {}
+    for t in TIMERS:
+        launches = C.c_int(0)
+        ms = lib.nusifTimerRead(t.encode(), C.byref(launches))
+        if launches.value:
+            kern[t] = dict(ms_total=ms, launches=launches.value,
+                           us_per_launch=1e3 * ms / launches.value,
+                           bytes_per_launch=lib.nusifKernelBytes(C.byref(case.d), t.encode()))
+    log("[bench] %s: %d steps in %.3f s" % (mode, steps, t1 - t0))
+    return case, dict(seconds=t1 - t0, cycles=cycles, rel=rels, kernels=kern)
+
+
+def cpu_baseline(case_gpu, n, budget_s=20.0):
+    """The oracle (one host core) on the same workload: the state after the
+    GPU warmup, then whole steps with its FP64 multigrid to the same 1e-6."""
+    orc = A.load(A.oracle_lib_path("rb_fast"))
+    gmg = A.load(A.oracle_lib_path("gmg_fast"))
+    prm = cavity_params(n)
+    o = A.Case(orc, **prm)
+    o.set_fields(**case_gpu.fields())
+    d = o.d
+    cells = n ** 3
+    npad = int(o.shape[0] * o.shape[1] * o.shape[2])
+    steps, t_total = 0, 0.0
+    while t_total < budget_s and steps < 50:
+        t0 = time.perf_counter()
+        orc.computeTimestep(C.byref(d))
+        for fn in ("setBoundaryConditions", "setSpecialBoundaryCondition", "computeFG",
+                   "computeRHS"):
+            getattr(orc, fn)(C.byref(d))
+        rel = C.c_double()
+        g = d.grid
+        gmg.orcGmgSolve(g.imax, g.jmax, g.kmax, g.dx, g.dy, g.dz, d.p, d.rhs, prm["levels"],
+                        prm["presmooth"], prm["postsmooth"], 1.0, 1e-6, 100, 0, 0,
+                        C.byref(rel))
+        orc.adaptUV(C.byref(d))
+        t_total += time.perf_counter() - t0
+        steps += 1
+        log("[bench] cpu baseline step %d: %.2f s" % (steps, t_total))
+    o.close()
+    del npad
+    return dict(value=steps / t_total, unit="time steps/s", cores=1, kind="port",
+                sample="%d step(s) of the %d^3 cavity from the GPU state after warmup; oracle "
+                       "C restatement (-O3, one host core), FP64 V(2,2) 6-level multigrid to "
+                       "1e-6 relative residual" % (steps, n),
+                mlups=cells * steps / t_total * 1e-6)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--mode", default="mixed", choices=["learned", "mixed", "fp64"])
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--no-compare", action="store_true", help="skip the fp64 comparison run")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        sys.exit("bench.py: this build runs on one GPU only (the solver is a single-domain "
+                 "MI355X path; --gpus %d is not supported)" % a.gpus)
+    lib = A.load(A.gpu_lib_path("rb"))
+    name = C.create_string_buffer(256)
+    if lib.nusifDeviceName(name, 256) < 1:
+        sys.exit("bench.py: no GPU visible to the HIP library")
+    log("[bench] device", name.value.decode())
+    n = a.size
+    cells = float(n) ** 3
+    case, res = run_mode(lib, a.mode, n, a.warmup, a.steps)
+    sps = a.steps / res["seconds"]
+    out = {
+        "metric": METRIC, "value": sps, "unit": "time steps/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * res["seconds"] / a.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if a.mode != "fp64" else "f64", "data": "synthetic",
+        "mlups": cells * sps * 1e-6,
+        "config": {"workload": "lid-driven cavity %d^3 cells, Re=1000, lid u=1, tau=0.5, "
+                               "pressure to 1e-6 relative residual per step, V(2,2) 6 levels; "
+                               "mode %s" % (n, a.mode),
+                   "grid": [n, n, n], "mode": a.mode, "device": name.value.decode()},
+        "cycles_per_step": sum(res["cycles"]) / len(res["cycles"]),
+        "max_rel_residual": max(res["rel"]),
+    }
+    dom = DOMINANT[a.mode]
+    k = res["kernels"].get(dom)
+    if k:
+        ach = k["bytes_per_launch"] / (k["us_per_launch"] * 1e-6) / 1e9
+        out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                           "us_per_launch": k["us_per_launch"]}
+    out["kernels"] = {t: {"us": round(v["us_per_launch"], 2), "launches": v["launches"],
+                          "GBs": round(v["bytes_per_launch"] / (v["us_per_launch"] * 1e-6) / 1e9, 1)}
+                      for t, v in res["kernels"].items()}
+    if not a.no_compare and a.mode != "fp64":
+        _, ref = run_mode(lib, "fp64", n, a.warmup, a.steps)
+        f64 = a.steps / ref["seconds"]
+        out["fp64_mg"] = {"value": f64, "mlups": cells * f64 * 1e-6,
+                          "cycles_per_step": sum(ref["cycles"]) / len(ref["cycles"])}
+        out["speedup_vs_fp64_mg"] = sps / f64
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(case, n)
+    case.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -83,8 +83,12 @@ NusifSolverCtx *nusifSolverCtxOf(const Solver *s)
 }
 
 /* ---- options ------------------------------------------------------------ */
+/* Options are kept per Parameter address until initDiscretization consumes
+ * them. A consumed entry is not applied again: a later Parameter that reuses
+ * the address starts from the defaults (initParameter also clears it). */
 struct OptEntry {
   const Parameter *prm;
+  bool consumed;
   NusifOptions opt;
 };
 static std::vector<OptEntry> gOpts;
@@ -98,7 +102,6 @@ void nusifDefaultOptions(NusifOptions *o)
   o->mgTol       = 1e-6;
   o->mgMaxCycles = 100;
   o->mgOmega     = 1.0;
-  o->mgCoarse    = 64;
   o->mgProject   = 0;
   o->lid         = 1.0;
   o->uInflow     = 2.0;
@@ -107,27 +110,56 @@ void nusifDefaultOptions(NusifOptions *o)
   o->warmstart   = 0;
 }
 
-static NusifOptions *optionsSlot(const Parameter *p)
+static OptEntry *findEntry(const Parameter *p)
 {
   for (OptEntry &e : gOpts)
-    if (e.prm == p) return &e.opt;
-  OptEntry e;
-  e.prm = p;
-  nusifDefaultOptions(&e.opt);
-  gOpts.push_back(e);
+    if (e.prm == p) return &e;
+  return nullptr;
+}
+
+static NusifOptions *optionsSlot(const Parameter *p)
+{
+  OptEntry *e = findEntry(p);
+  if (e && e->consumed) {
+    nusifDefaultOptions(&e->opt);
+    e->consumed = false;
+  }
+  if (e) return &e->opt;
+  OptEntry n;
+  n.prm      = p;
+  n.consumed = false;
+  nusifDefaultOptions(&n.opt);
+  gOpts.push_back(n);
   return &gOpts.back().opt;
+}
+
+extern "C" void nusifClearOptions(const Parameter *p)
+{
+  for (size_t q = 0; q < gOpts.size(); q++)
+    if (gOpts[q].prm == p) {
+      gOpts.erase(gOpts.begin() + q);
+      return;
+    }
+}
+
+/* Options for a new Discretization: the pending ones of `p`, or defaults. */
+void nusifConsumeOptions(const Parameter *p, NusifOptions *out)
+{
+  OptEntry *e = findEntry(p);
+  if (e && !e->consumed) {
+    *out        = e->opt;
+    e->consumed = true;
+  } else {
+    nusifDefaultOptions(out);
+  }
 }
 
 const NusifOptions *nusifOptionsFor(const Parameter *p)
 {
-  for (OptEntry &e : gOpts)
-    if (e.prm == p) return &e.opt;
+  OptEntry *e = findEntry(p);
+  if (e) return &e->opt;
   static NusifOptions defaults;
-  static bool once = false;
-  if (!once) {
-    nusifDefaultOptions(&defaults);
-    once = true;
-  }
+  nusifDefaultOptions(&defaults);
   return &defaults;
 }
 
@@ -181,7 +213,6 @@ extern "C" int nusifSetOption(Parameter *p, const char *key, const char *value)
   if (k == "mg_tol") return parseReal(v.c_str(), &o->mgTol);
   if (k == "mg_maxcycles") return parseInt(v.c_str(), &o->mgMaxCycles);
   if (k == "mg_omega") return parseReal(v.c_str(), &o->mgOmega);
-  if (k == "mg_coarse") return parseInt(v.c_str(), &o->mgCoarse);
   if (k == "mg_project") return parseInt(v.c_str(), &o->mgProject);
   if (k == "lid") return parseReal(v.c_str(), &o->lid);
   if (k == "u_inflow") return parseReal(v.c_str(), &o->uInflow);
@@ -314,3 +345,35 @@ extern "C" int nusifSyncHostIfOwned(const double *hostPtr)
 }
 
 extern "C" int nusifStepLimit(Parameter *p) { return nusifOptionsFor(p)->nsteps; }
+
+/* Algorithmic HBM bytes per launch: every array element the kernel must
+ * touch, read once and/or written once, on the finest grid (N interior
+ * cells, Npad padded cells). */
+extern "C" double nusifKernelBytes(Discretization *d, const char *kernel)
+{
+  double N    = (double)d->grid.imax * d->grid.jmax * d->grid.kmax;
+  double Npad = (double)(d->grid.imax + 2) * (d->grid.jmax + 2) * (d->grid.kmax + 2);
+  struct {
+    const char *name;
+    double perCell;
+    int padded;
+  } table[] = {
+      {"computeFG", 48.0, 0},        /* rd u v w, wr f g h */
+      {"computeRHS", 32.0, 0},       /* rd f g h, wr rhs */
+      {"adaptUV", 56.0, 0},          /* rd f g h p, wr u v w */
+      {"absmax", 24.0, 1},           /* rd u v w, padded */
+      {"rb_pass", 16.0, 0},          /* one colour: rd p (all) + rd rhs / wr p (half) */
+      {"rb_fused", 24.0, 0},         /* one full iteration: rd p, rd rhs, wr p */
+      {"gmg_smooth64_l0", 16.0, 0},  /* one colour pass, FP64 */
+      {"gmg_smooth32_l0", 8.0, 0},   /* one colour pass, FP32 */
+      {"gmg_restrict64_l0", 17.0, 0},/* rd x, b; wr b_coarse (1/8) */
+      {"gmg_restrict32_l0", 8.5, 0},
+      {"gmg_prolong64_l0", 17.0, 0}, /* rd/wr x, rd x_coarse (1/8) */
+      {"gmg_prolong32_l0", 8.5, 0},
+      {"gmg_residual64", 16.0, 0},   /* rd p, rhs (+4 when the FP32 copy is written) */
+      {"gmg_correct", 20.0, 0},      /* rd p, e32; wr p */
+  };
+  for (auto &e : table)
+    if (strcmp(kernel, e.name) == 0) return e.perCell * (e.padded ? Npad : N);
+  return 0.0;
+}

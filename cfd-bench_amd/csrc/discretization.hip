@@ -49,13 +49,14 @@ __global__ __launch_bounds__(256) void kAbsMax3(const double *__restrict__ u,
     partials[3 * blockIdx.x + 1] = mv;
     partials[3 * blockIdx.x + 2] = mw;
   }
-  if (lastBlock(&sc->counter[0], gridDim.x) && threadIdx.x < 3) {
-    double m = DBL_MIN;
-    for (unsigned b = 0; b < gridDim.x; b++) {
-      double x = loadAgent(&partials[3 * b + threadIdx.x]);
-      m        = m > x ? m : x;
-    }
-    sc->absmax[threadIdx.x] = (unsigned long long)__double_as_longlong(m);
+}
+
+__global__ __launch_bounds__(256) void kFoldMax3(const double *partials, int nb, DevScalars *sc)
+{
+  __shared__ double lds[4];
+  for (int q = 0; q < 3; q++) {
+    double m = foldMax(partials, nb, 3, q, DBL_MIN, lds);
+    if (threadIdx.x == 0) sc->absmax[q] = (unsigned long long)__double_as_longlong(m);
   }
 }
 
@@ -72,6 +73,7 @@ void launchAbsMax3(NusifCtx *c)
   int nb = reductionBlocks(c->geo.n);
   NUSIF_TIMED("absmax", (kAbsMax3<<<nb, 256, 0, nusifStream()>>>(c->u, c->v, c->w, c->geo.n,
                             c->partials, c->scal)));
+  kFoldMax3<<<1, 256, 0, nusifStream()>>>(c->partials, nb, c->scal);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -346,8 +348,7 @@ void launchComputeRHS(NusifCtx *c, const Discretization *d)
 /* normalizePressure (src/discretization.c:403-429)                         */
 /* ------------------------------------------------------------------------ */
 /* Workgroup (64, 4) covers a 64 x 4 patch of (i, j) and marches all k. */
-__global__ __launch_bounds__(256) void kInteriorSum(
-    const double *__restrict__ p, Geo geo, double *partials, DevScalars *sc, double count)
+__global__ __launch_bounds__(256) void kInteriorSum(const double *__restrict__ p, Geo geo, double *partials)
 {
   __shared__ double lds[4];
   int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -356,16 +357,15 @@ __global__ __launch_bounds__(256) void kInteriorSum(
   if (i <= geo.ni && j <= geo.nj)
     for (int k = 1; k <= geo.nk; k++) acc += p[geo.at(i, j, k)];
   acc = blockSum(acc, lds);
-  unsigned nb  = gridDim.x * gridDim.y;
-  unsigned bid = blockIdx.x + gridDim.x * blockIdx.y;
-  int tid      = threadIdx.x + blockDim.x * threadIdx.y;
-  if (tid == 0) partials[bid] = acc;
-  if (lastBlock(&sc->counter[1], nb)) {
-    double s = 0.0;
-    for (unsigned q = tid; q < nb; q += blockDim.x * blockDim.y) s += loadAgent(&partials[q]);
-    s = blockSum(s, lds);
-    if (tid == 0) sc->sum = s / count;
-  }
+  if (threadIdx.x == 0 && threadIdx.y == 0) partials[blockIdx.x + gridDim.x * blockIdx.y] = acc;
+}
+
+/* avg = sum / (imax * jmax * kmax) (src/discretization.c:420) */
+__global__ __launch_bounds__(256) void kFoldMean(const double *partials, int nb, double count, DevScalars *sc)
+{
+  __shared__ double lds[4];
+  double s = foldSum(partials, nb, 1, 0, lds);
+  if (threadIdx.x == 0) sc->sum = s / count;
 }
 
 __global__ __launch_bounds__(256) void kSubtractMean(double *__restrict__ p, Geo geo, const DevScalars *sc)
@@ -386,7 +386,8 @@ void launchNormalize(NusifCtx *c)
   dim3 g2((g.ni + TX - 1) / TX, (g.nj + TY - 1) / TY);
   if ((size_t)g2.x * g2.y > c->nPartials) NUSIF_FATAL("partials buffer too small");
   double count = (double)(c->owner->grid.imax * c->owner->grid.jmax * c->owner->grid.kmax);
-  kInteriorSum<<<g2, blk, 0, nusifStream()>>>(c->p, g, c->partials, c->scal, count);
+  kInteriorSum<<<g2, blk, 0, nusifStream()>>>(c->p, g, c->partials);
+  kFoldMean<<<1, 256, 0, nusifStream()>>>(c->partials, (int)(g2.x * g2.y), count, c->scal);
   dim3 g3(g2.x, g2.y, g.nk);
   kSubtractMean<<<g3, blk, 0, nusifStream()>>>(c->p, g, c->scal);
 }
@@ -499,7 +500,7 @@ extern "C" void initDiscretization(Discretization *s, Parameter *params)
   memset(c, 0, sizeof(*c));
   c->owner = s;
   c->geo   = makeGeo(s->comm.imaxLocal, s->comm.jmaxLocal, s->comm.kmaxLocal);
-  c->opt   = *nusifOptionsFor(params);
+  nusifConsumeOptions(params, &c->opt);
   size_t n = (size_t)c->geo.n;
 
   s->u   = hostField(n, params->u_init);

@@ -1,8 +1,629 @@
-/* placeholder: replaced by the geometric multigrid */
+/* Geometric multigrid pressure solve to a relative residual — the `gmg`
+ * solver. Added form: the reference's src/solver-mg.c has the V-cycle's
+ * skeleton (smooth / residual / restrict / prolongate / correct,
+ * src/solver-mg.c:299-354) but its coarse correction never reaches p; this is
+ * a working cell-centred V(nu1, nu2) cycle on the same discrete operator
+ * (7-point Laplacian with Neumann ghost copy, src/solver-rb.c:66-69,129-148).
+ *
+ *   smoother      red-black Gauss-Seidel/SOR, odd cells first (as the
+ *                 reference), relaxation mg_omega; face ghosts written by the
+ *                 second colour pass itself (no extra launch)
+ *   restriction   average of the 8 children of the fine residual, fused with
+ *                 the residual: b_{l+1} = (1/8) sum (b - A x)
+ *   prolongation  cell-centred trilinear (27/64, 9/64, 3/64, 1/64), coarse
+ *                 values read at clamped indices (Neumann extension)
+ *   coarsest      exact separable DCT solve in one workgroup (LDS); the
+ *                 constant mode is dropped (all-Neumann compatibility)
+ *   stop test     ||rhs - A p||_2 / ||rhs||_2 <= mg_tol in FP64 on the finest
+ *                 grid, checked after every cycle
+ *
+ * precision fp64:  V-cycles in FP64 directly on p.
+ * precision mixed: defect correction — FP64 residual r = rhs - A p (and its
+ *                  norm) on the GPU, cast to FP32, one FP32 V-cycle for
+ *                  A e = r from e = 0, then p += e in FP64.
+ *
+ * All level arrays use the padded layout of the fields (one ghost layer,
+ * i fastest); level l has h_l = 2^l h_0. */
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
 #include "nusif_internal.h"
-struct GmgState { int dummy; };
-GmgState *gmgCreate(NusifSolverCtx *sc, const Parameter *prm) { (void)sc; (void)prm; return new GmgState(); }
-double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhs) { (void)sc; (void)p; (void)rhs; NUSIF_FATAL("gmg not built"); return 0; }
-void gmgFree(GmgState *g) { delete g; }
-struct UnetModel { int dummy; };
+#include "reduce.h"
+
+#define MAXLEV 12
+#define GTX 64
+#define GTY 4
+#define GKCH 8
+#define COARSE_MAX_CELLS 2048 /* interior cells of the coarsest level (LDS solve) */
+
+template <typename T> struct Coef {
+  T idx2, idy2, idz2, factor;
+};
+
+template <typename T> struct LevelT {
+  Geo g;
+  T *x, *b;
+  Coef<T> cf;
+};
+
+struct DctTables {
+  int oCi, oCj, oCk, oDi, oDj, oDk, oLi, oLj, oLk, total;
+};
+
+static DctTables dctLayout(int ni, int nj, int nk)
+{
+  DctTables t;
+  int o = 0;
+  t.oCi = o; o += ni * ni;
+  t.oCj = o; o += nj * nj;
+  t.oCk = o; o += nk * nk;
+  t.oDi = o; o += ni * ni;
+  t.oDj = o; o += nj * nj;
+  t.oDk = o; o += nk * nk;
+  t.oLi = o; o += ni;
+  t.oLj = o; o += nj;
+  t.oLk = o; o += nk;
+  t.total = o;
+  return t;
+}
+
+struct GmgState {
+  DctTables dct;
+  double *tab64;
+  float *tab32;
+  int nl;
+  int nu1, nu2, maxCycles, project, mixed;
+  double tol, omega;
+  LevelT<double> L64[MAXLEV];
+  LevelT<float> L32[MAXLEV];
+  Coef<double> cf0; /* finest level, FP64 */
+};
+
+template <typename T> static Coef<T> coefFor(double dx, double dy, double dz, double omega)
+{
+  Coef<T> c;
+  double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
+  c.idx2   = (T)(1.0 / dx2);
+  c.idy2   = (T)(1.0 / dy2);
+  c.idz2   = (T)(1.0 / dz2);
+  c.factor = (T)(omega * 0.5 * (dx2 * dy2 * dz2) / (dy2 * dz2 + dx2 * dz2 + dx2 * dy2));
+  return c;
+}
+
+template <typename T>
+__device__ inline T lap7(const T *x, long long c, long long sj, long long sk, const Coef<T> &cf)
+{
+  T xc = x[c];
+  return (x[c + 1] - (T)2 * xc + x[c - 1]) * cf.idx2 + (x[c + sj] - (T)2 * xc + x[c - sj]) * cf.idy2 +
+         (x[c + sk] - (T)2 * xc + x[c - sk]) * cf.idz2;
+}
+
+/* Face ghosts of cell (i, j, k) from value v (reference ranges: each ghost
+ * plane over the interior of the other two directions). */
+template <typename T> __device__ inline void faceGhosts(T *x, const Geo &g, int i, int j, int k, long long c, T v)
+{
+  if (i == 1) x[c - 1] = v;
+  if (i == g.ni) x[c + 1] = v;
+  if (j == 1) x[c - g.sj] = v;
+  if (j == g.nj) x[c + g.sj] = v;
+  if (k == 1) x[c - g.sk] = v;
+  if (k == g.nk) x[c + g.sk] = v;
+}
+
+/* One colour pass. Thread x owns the cell pair (2x+1, 2x+2) of a row and
+ * updates the one of colour `parity`. With `ghosts` (second pass), it also
+ * writes the face ghosts of both cells of its pair: the other colour's
+ * values are final by then. */
+template <typename T>
+__global__ __launch_bounds__(256) void kSmooth(T *__restrict__ x, const T *__restrict__ b, Geo g,
+    Coef<T> cf, int parity, int ghosts)
+{
+  int tx = blockIdx.x * blockDim.x + threadIdx.x;
+  int j  = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int k0 = blockIdx.z * GKCH + 1;
+  if (j > g.nj) return;
+  int i1 = 2 * tx + 1;
+  if (i1 > g.ni) return;
+  int k1 = min(k0 + GKCH, g.nk + 1);
+  for (int k = k0; k < k1; k++) {
+    int i   = (((1 + j + k) & 1) == parity) ? i1 : i1 + 1;
+    int io  = (i == i1) ? i1 + 1 : i1; /* the pair's other cell */
+    bool in = i <= g.ni;
+    T v     = 0;
+    long long c = g.at(i, j, k);
+    if (in) {
+      T r = b[c] - lap7(x, c, g.sj, g.sk, cf);
+      v   = x[c] - cf.factor * r;
+      x[c] = v;
+    }
+    if (ghosts) {
+      if (in) faceGhosts(x, g, i, j, k, c, v);
+      if (io <= g.ni) {
+        long long co = g.at(io, j, k);
+        faceGhosts(x, g, io, j, k, co, x[co]);
+      }
+    }
+  }
+}
+
+/* b_c(I,J,K) = (1/8) sum over the 8 children of (b - A x); the coarse x is
+ * zeroed (interior and ghosts) by the caller. */
+template <typename T>
+__global__ __launch_bounds__(256) void kResidualRestrict(const T *__restrict__ x,
+    const T *__restrict__ b, Geo gf, Coef<T> cf, T *__restrict__ bc, Geo gc)
+{
+  int I = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int J = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int K = blockIdx.z + 1;
+  if (I > gc.ni || J > gc.nj) return;
+  T s = 0;
+#pragma unroll
+  for (int dk = 0; dk < 2; dk++)
+#pragma unroll
+    for (int dj = 0; dj < 2; dj++)
+#pragma unroll
+      for (int di = 0; di < 2; di++) {
+        long long c = gf.at(2 * I - 1 + di, 2 * J - 1 + dj, 2 * K - 1 + dk);
+        s += b[c] - lap7(x, c, gf.sj, gf.sk, cf);
+      }
+  bc[gc.at(I, J, K)] = (T)0.125 * s;
+}
+
+template <typename T> __global__ __launch_bounds__(256) void kFill(T *__restrict__ x, long long n, T v)
+{
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) x[q] = v;
+}
+
+/* x_f += P x_c, cell-centred trilinear; face ghosts of x_f refreshed. */
+template <typename T>
+__global__ __launch_bounds__(256) void kProlongCorrect(T *__restrict__ xf, Geo gf,
+    const T *__restrict__ xc, Geo gc)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int k = blockIdx.z + 1;
+  if (i > gf.ni || j > gf.nj) return;
+  int I = (i + 1) >> 1, J = (j + 1) >> 1, K = (k + 1) >> 1;
+  int I2 = min(max(I + ((i & 1) ? -1 : 1), 1), gc.ni);
+  int J2 = min(max(J + ((j & 1) ? -1 : 1), 1), gc.nj);
+  int K2 = min(max(K + ((k & 1) ? -1 : 1), 1), gc.nk);
+  const T a = (T)0.75, q = (T)0.25;
+  T c11 = a * xc[gc.at(I, J, K)] + q * xc[gc.at(I2, J, K)];
+  T c21 = a * xc[gc.at(I, J2, K)] + q * xc[gc.at(I2, J2, K)];
+  T c12 = a * xc[gc.at(I, J, K2)] + q * xc[gc.at(I2, J, K2)];
+  T c22 = a * xc[gc.at(I, J2, K2)] + q * xc[gc.at(I2, J2, K2)];
+  T e   = a * (a * c11 + q * c21) + q * (a * c12 + q * c22);
+  long long c = gf.at(i, j, k);
+  T v  = xf[c] + e;
+  xf[c] = v;
+  faceGhosts(xf, gf, i, j, k, c, v);
+}
+
+/* Coarsest level, exact: the Neumann (ghost-copy) Laplacian is diagonal in
+ * the separable DCT-II basis C_n[m][i] = cos(pi m (i + 1/2) / n) with
+ * eigenvalues -4/h^2 sin^2(pi m / (2n)), so x = A^+ b is three 1-D forward
+ * transforms, a division by the eigenvalue sums (the constant mode, the
+ * null space, set to 0 — this is the rhs-mean projection), and three inverse
+ * transforms. One workgroup, both work arrays in LDS; every output element
+ * is one thread's ascending-index sum. Tables (host-built):
+ *   fwd  Ci[ni*ni] Cj[nj*nj] Ck[nk*nk]          C[m][i]
+ *   inv  Di Dj Dk (same shapes)                  s_m C[m][i], s_0 = 1/n, s_m = 2/n
+ *   lam  Li[ni] Lj[nj] Lk[nk]                    eigenvalues */
+template <typename T>
+__global__ __launch_bounds__(1024) void kCoarseDct(T *__restrict__ x, const T *__restrict__ b, Geo g,
+    const T *__restrict__ tab, DctTables L)
+{
+  extern __shared__ unsigned char smem[];
+  const int ni = g.ni, nj = g.nj, nk = g.nk, N = ni * nj * nk;
+  T *W1 = (T *)smem;
+  T *W2 = W1 + N;
+  const T *Ci = tab + L.oCi, *Cj = tab + L.oCj, *Ck = tab + L.oCk;
+  const T *Di = tab + L.oDi, *Dj = tab + L.oDj, *Dk = tab + L.oDk;
+  const T *Li = tab + L.oLi, *Lj = tab + L.oLj, *Lk = tab + L.oLk;
+  int tid = threadIdx.x, nt = blockDim.x;
+  for (int q = tid; q < N; q += nt) {
+    int i = q % ni, j = (q / ni) % nj, k = q / (ni * nj);
+    W1[q] = b[g.at(i + 1, j + 1, k + 1)];
+  }
+  __syncthreads();
+  for (int q = tid; q < N; q += nt) { /* along i */
+    int m = q % ni, row = q - m;
+    T s = 0;
+    for (int i = 0; i < ni; i++) s += Ci[m * ni + i] * W1[row + i];
+    W2[q] = s;
+  }
+  __syncthreads();
+  for (int q = tid; q < N; q += nt) { /* along j */
+    int mi = q % ni, m = (q / ni) % nj, k = q / (ni * nj);
+    T s = 0;
+    for (int j = 0; j < nj; j++) s += Cj[m * nj + j] * W2[(k * nj + j) * ni + mi];
+    W1[q] = s;
+  }
+  __syncthreads();
+  for (int q = tid; q < N; q += nt) { /* along k, then divide */
+    int mi = q % ni, mj = (q / ni) % nj, m = q / (ni * nj);
+    T s = 0;
+    for (int k = 0; k < nk; k++) s += Ck[m * nk + k] * W1[(k * nj + mj) * ni + mi];
+    T lam = Li[mi] + Lj[mj] + Lk[m];
+    W2[q] = (q == 0) ? (T)0 : s / lam;
+  }
+  __syncthreads();
+  for (int q = tid; q < N; q += nt) { /* inverse along k */
+    int mi = q % ni, mj = (q / ni) % nj, k = q / (ni * nj);
+    T s = 0;
+    for (int m = 0; m < nk; m++) s += Dk[m * nk + k] * W2[(m * nj + mj) * ni + mi];
+    W1[q] = s;
+  }
+  __syncthreads();
+  for (int q = tid; q < N; q += nt) { /* inverse along j */
+    int mi = q % ni, j = (q / ni) % nj, k = q / (ni * nj);
+    T s = 0;
+    for (int m = 0; m < nj; m++) s += Dj[m * nj + j] * W1[(k * nj + m) * ni + mi];
+    W2[q] = s;
+  }
+  __syncthreads();
+  for (int q = tid; q < N; q += nt) { /* inverse along i, store with face ghosts */
+    int i = q % ni, row = q - i, j = (q / ni) % nj, k = q / (ni * nj);
+    T s = 0;
+    for (int m = 0; m < ni; m++) s += Di[m * ni + i] * W2[row + m];
+    long long c = g.at(i + 1, j + 1, k + 1);
+    x[c]        = s;
+    faceGhosts(x, g, i + 1, j + 1, k + 1, c, s);
+  }
+}
+
+/* ---- FP64 finest-level kernels: norms, defect correction ---------------- */
+/* slot 0: sum r^2 (r = rhs - A p), slot 1: sum rhs^2; optional FP32 copy of r */
+__global__ __launch_bounds__(256) void kResidualNorm(const double *__restrict__ p,
+    const double *__restrict__ rhs, Geo g, Coef<double> cf, float *__restrict__ r32,
+    double *partials, DevScalars *sc)
+{
+  __shared__ double lds[4];
+  int i  = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j  = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int k0 = blockIdx.z * GKCH + 1;
+  double a0 = 0.0, a1 = 0.0;
+  if (i <= g.ni && j <= g.nj) {
+    int k1 = min(k0 + GKCH, g.nk + 1);
+    for (int k = k0; k < k1; k++) {
+      long long c = g.at(i, j, k);
+      double bb   = rhs[c];
+      double r    = bb - lap7(p, c, g.sj, g.sk, cf);
+      if (r32) r32[c] = (float)r;
+      a0 += r * r;
+      a1 += bb * bb;
+    }
+  }
+  a0 = blockSum(a0, lds);
+  a1 = blockSum(a1, lds);
+  unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    partials[2 * bid]     = a0;
+    partials[2 * bid + 1] = a1;
+  }
+}
+
+__global__ __launch_bounds__(256) void kFoldNorms(const double *partials, int nb, DevScalars *sc)
+{
+  __shared__ double lds[4];
+  double s0 = foldSum(partials, nb, 2, 0, lds);
+  double s1 = foldSum(partials, nb, 2, 1, lds);
+  if (threadIdx.x == 0) {
+    sc->norm2[0] = s0;
+    sc->norm2[1] = s1;
+  }
+}
+
+/* sum of rhs over the interior -> norm2[2] */
+__global__ __launch_bounds__(256) void kInteriorSumRhs(const double *__restrict__ rhs, Geo g, double *partials)
+{
+  __shared__ double lds[4];
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  double acc = 0.0;
+  if (i <= g.ni && j <= g.nj)
+    for (int k = 1; k <= g.nk; k++) acc += rhs[g.at(i, j, k)];
+  acc = blockSum(acc, lds);
+  if (threadIdx.x == 0 && threadIdx.y == 0) partials[blockIdx.x + gridDim.x * blockIdx.y] = acc;
+}
+
+__global__ __launch_bounds__(256) void kFoldRhsSum(const double *partials, int nb, DevScalars *sc)
+{
+  __shared__ double lds[4];
+  double s = foldSum(partials, nb, 1, 0, lds);
+  if (threadIdx.x == 0) sc->norm2[2] = s;
+}
+
+__global__ __launch_bounds__(256) void kSubtractRhsMean(double *__restrict__ rhs, Geo g,
+    const DevScalars *sc, double count)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int k = blockIdx.z + 1;
+  if (i > g.ni || j > g.nj) return;
+  double m = sc->norm2[2] / count;
+  long long c = g.at(i, j, k);
+  rhs[c]      = rhs[c] - m;
+}
+
+/* p += e (FP32 correction), face ghosts of p refreshed */
+__global__ __launch_bounds__(256) void kCorrect64(double *__restrict__ p, const float *__restrict__ e, Geo g)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int k = blockIdx.z + 1;
+  if (i > g.ni || j > g.nj) return;
+  long long c = g.at(i, j, k);
+  double v    = p[c] + (double)e[c];
+  p[c]        = v;
+  faceGhosts(p, g, i, j, k, c, v);
+}
+
+/* ---- host orchestration -------------------------------------------------- */
+static dim3 cellGrid(const Geo &g) { return dim3((g.ni + GTX - 1) / GTX, (g.nj + GTY - 1) / GTY, g.nk); }
+static dim3 sweepGrid(const Geo &g)
+{
+  return dim3(((g.ni + 1) / 2 + GTX - 1) / GTX, (g.nj + GTY - 1) / GTY, (g.nk + GKCH - 1) / GKCH);
+}
+static dim3 normGrid(const Geo &g)
+{
+  return dim3((g.ni + GTX - 1) / GTX, (g.nj + GTY - 1) / GTY, (g.nk + GKCH - 1) / GKCH);
+}
+
+template <typename T> static void fill(T *x, long long n, T v)
+{
+  long long b = (n + 255) / 256;
+  if (b > 4096) b = 4096;
+  kFill<T><<<(unsigned)b, 256, 0, nusifStream()>>>(x, n, v);
+}
+
+/* Timers cover the finest level only ("..._l0"), whose algorithmic bytes
+ * nusifKernelBytes reports. */
+template <typename T> static void smoothPass(LevelT<T> &L, int parity, bool fine)
+{
+  dim3 grd = sweepGrid(L.g);
+  int gh   = parity == 0;
+  if (fine)
+    NUSIF_TIMED(sizeof(T) == 8 ? "gmg_smooth64_l0" : "gmg_smooth32_l0",
+        (kSmooth<T><<<grd, dim3(GTX, GTY), 0, nusifStream()>>>(L.x, L.b, L.g, L.cf, parity, gh)));
+  else
+    kSmooth<T><<<grd, dim3(GTX, GTY), 0, nusifStream()>>>(L.x, L.b, L.g, L.cf, parity, gh);
+}
+
+template <typename T> static void smooth(LevelT<T> &L, int sweeps, bool fine)
+{
+  for (int s = 0; s < sweeps; s++) {
+    smoothPass(L, 1, fine);
+    smoothPass(L, 0, fine);
+  }
+}
+
+static const double *tables(GmgState *G, double *) { return G->tab64; }
+static const float *tables(GmgState *G, float *) { return G->tab32; }
+
+template <typename T> static void coarseSolve(GmgState *G, LevelT<T> &L)
+{
+  size_t lds = 2 * (size_t)L.g.ni * L.g.nj * L.g.nk * sizeof(T);
+  kCoarseDct<T><<<1, 1024, lds, nusifStream()>>>(L.x, L.b, L.g, tables(G, (T *)nullptr), G->dct);
+}
+
+/* DCT-II tables of the coarsest level (closed forms; host libm) */
+static void buildDctTables(GmgState *G, const Geo &g, double hx, double hy, double hz)
+{
+  G->dct = dctLayout(g.ni, g.nj, g.nk);
+  std::vector<double> t((size_t)G->dct.total);
+  const int n[3] = {g.ni, g.nj, g.nk};
+  const double h[3] = {hx, hy, hz};
+  const int oC[3] = {G->dct.oCi, G->dct.oCj, G->dct.oCk};
+  const int oD[3] = {G->dct.oDi, G->dct.oDj, G->dct.oDk};
+  const int oL[3] = {G->dct.oLi, G->dct.oLj, G->dct.oLk};
+  for (int d = 0; d < 3; d++) {
+    int nn = n[d];
+    for (int m = 0; m < nn; m++) {
+      double sm = (m == 0 ? 1.0 : 2.0) / nn;
+      for (int i = 0; i < nn; i++) {
+        double cv = cos(M_PI * m * (i + 0.5) / nn);
+        t[(size_t)oC[d] + (size_t)m * nn + i] = cv;
+        t[(size_t)oD[d] + (size_t)m * nn + i] = sm * cv;
+      }
+      double sn = sin(M_PI * m / (2.0 * nn));
+      t[(size_t)oL[d] + m] = -4.0 / (h[d] * h[d]) * sn * sn;
+    }
+  }
+  std::vector<float> f(t.size());
+  for (size_t q = 0; q < t.size(); q++) f[q] = (float)t[q];
+  HIP_CHECK(hipMalloc((void **)&G->tab64, t.size() * sizeof(double)));
+  HIP_CHECK(hipMalloc((void **)&G->tab32, f.size() * sizeof(float)));
+  HIP_CHECK(hipMemcpy(G->tab64, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(G->tab32, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+}
+
+template <typename T> static void vcycle(GmgState *G, LevelT<T> *L, int l)
+{
+  if (l == G->nl - 1) {
+    coarseSolve(G, L[l]);
+    return;
+  }
+  bool fine = l == 0;
+  smooth(L[l], G->nu1, fine);
+  LevelT<T> &C = L[l + 1];
+  fill<T>(C.x, C.g.n, (T)0);
+  if (fine)
+    NUSIF_TIMED(sizeof(T) == 8 ? "gmg_restrict64_l0" : "gmg_restrict32_l0",
+        (kResidualRestrict<T><<<cellGrid(C.g), dim3(GTX, GTY), 0, nusifStream()>>>(
+            L[l].x, L[l].b, L[l].g, L[l].cf, C.b, C.g)));
+  else
+    kResidualRestrict<T><<<cellGrid(C.g), dim3(GTX, GTY), 0, nusifStream()>>>(
+        L[l].x, L[l].b, L[l].g, L[l].cf, C.b, C.g);
+  vcycle(G, L, l + 1);
+  if (fine)
+    NUSIF_TIMED(sizeof(T) == 8 ? "gmg_prolong64_l0" : "gmg_prolong32_l0",
+        (kProlongCorrect<T><<<cellGrid(L[l].g), dim3(GTX, GTY), 0, nusifStream()>>>(
+            L[l].x, L[l].g, C.x, C.g)));
+  else
+    kProlongCorrect<T><<<cellGrid(L[l].g), dim3(GTX, GTY), 0, nusifStream()>>>(
+        L[l].x, L[l].g, C.x, C.g);
+  smooth(L[l], G->nu2, fine);
+}
+
+template <typename T> static T *devAlloc(long long n)
+{
+  T *a = nullptr;
+  HIP_CHECK(hipMalloc((void **)&a, (size_t)n * sizeof(T)));
+  return a;
+}
+
+GmgState *gmgCreate(NusifSolverCtx *sc, const Parameter *prm)
+{
+  NusifCtx *c  = sc->dctx;
+  const Grid *gr = &c->owner->grid;
+  GmgState *G  = new GmgState();
+  memset(G, 0, sizeof(*G));
+  G->nu1       = prm->presmooth;
+  G->nu2       = prm->postsmooth;
+  G->maxCycles = c->opt.mgMaxCycles;
+  G->project   = c->opt.mgProject;
+  G->mixed     = c->opt.mixed;
+  G->tol       = c->opt.mgTol;
+  G->omega     = c->opt.mgOmega;
+  /* levels: halve while every extent is even and the coarse one stays >= 2 */
+  int want = prm->levels < 1 ? 1 : prm->levels;
+  if (want > MAXLEV) want = MAXLEV;
+  int ni = c->geo.ni, nj = c->geo.nj, nk = c->geo.nk, nl = 1;
+  while (nl < want && ni % 2 == 0 && nj % 2 == 0 && nk % 2 == 0 && ni >= 4 && nj >= 4 && nk >= 4) {
+    ni /= 2;
+    nj /= 2;
+    nk /= 2;
+    nl++;
+  }
+  /* the coarsest level must fit the LDS solve; add levels if it does not */
+  while ((long long)ni * nj * nk > COARSE_MAX_CELLS) {
+    if (ni % 2 || nj % 2 || nk % 2 || ni < 4 || nj < 4 || nk < 4 || nl >= MAXLEV)
+      NUSIF_FATAL("gmg: grid %dx%dx%d cannot be coarsened to <= %d cells "
+                  "(extents must stay even)", c->geo.ni, c->geo.nj, c->geo.nk, COARSE_MAX_CELLS);
+    ni /= 2;
+    nj /= 2;
+    nk /= 2;
+    nl++;
+  }
+  G->nl  = nl;
+  G->cf0 = coefFor<double>(gr->dx, gr->dy, gr->dz, G->omega);
+  double dx = gr->dx, dy = gr->dy, dz = gr->dz;
+  ni = c->geo.ni;
+  nj = c->geo.nj;
+  nk = c->geo.nk;
+  for (int l = 0; l < nl; l++) {
+    Geo g = makeGeo(ni, nj, nk);
+    if (G->mixed) {
+      LevelT<float> &L = G->L32[l];
+      L.g              = g;
+      L.x              = devAlloc<float>(g.n);
+      L.b              = devAlloc<float>(g.n);
+      L.cf             = coefFor<float>(dx, dy, dz, G->omega);
+      fill<float>(L.x, g.n, 0.0f);
+      fill<float>(L.b, g.n, 0.0f);
+    } else {
+      LevelT<double> &L = G->L64[l];
+      L.g               = g;
+      L.cf              = coefFor<double>(dx, dy, dz, G->omega);
+      if (l > 0) {
+        L.x = devAlloc<double>(g.n);
+        L.b = devAlloc<double>(g.n);
+        fill<double>(L.x, g.n, 0.0);
+        fill<double>(L.b, g.n, 0.0);
+      }
+    }
+    if (l == nl - 1) buildDctTables(G, g, dx, dy, dz);
+    ni /= 2;
+    nj /= 2;
+    nk /= 2;
+    dx *= 2;
+    dy *= 2;
+    dz *= 2;
+  }
+  dim3 ng = normGrid(c->geo);
+  if ((size_t)2 * ng.x * ng.y * ng.z > c->nPartials) NUSIF_FATAL("partials buffer too small");
+  HIP_CHECK(hipStreamSynchronize(nusifStream()));
+  return G;
+}
+
+void gmgFree(GmgState *G)
+{
+  if (G->tab64) HIP_CHECK(hipFree(G->tab64));
+  if (G->tab32) HIP_CHECK(hipFree(G->tab32));
+  for (int l = 0; l < G->nl; l++) {
+    if (G->L32[l].x) HIP_CHECK(hipFree(G->L32[l].x));
+    if (G->L32[l].b) HIP_CHECK(hipFree(G->L32[l].b));
+    if (l > 0 && G->L64[l].x) HIP_CHECK(hipFree(G->L64[l].x));
+    if (l > 0 && G->L64[l].b) HIP_CHECK(hipFree(G->L64[l].b));
+  }
+  delete G;
+}
+
+/* residual norms on the finest level: returns ||r||, sets *nb = ||rhs|| */
+static double residualNorm(NusifCtx *c, GmgState *G, double *p, const double *rhs, float *r32, double *nb)
+{
+  dim3 ng = normGrid(c->geo);
+  NUSIF_TIMED("gmg_residual64", (kResidualNorm<<<ng, dim3(GTX, GTY), 0, nusifStream()>>>(
+                                    p, rhs, c->geo, G->cf0, r32, c->partials, c->scal)));
+  kFoldNorms<<<1, 256, 0, nusifStream()>>>(c->partials, (int)(ng.x * ng.y * ng.z), c->scal);
+  HIP_CHECK(hipMemcpyAsync(c->scalHost->norm2, c->scal->norm2, 2 * sizeof(double),
+      hipMemcpyDeviceToHost, nusifStream()));
+  HIP_CHECK(hipStreamSynchronize(nusifStream()));
+  *nb = sqrt(c->scalHost->norm2[1]);
+  return sqrt(c->scalHost->norm2[0]);
+}
+
+double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
+{
+  NusifCtx *c  = sc->dctx;
+  GmgState *G  = sc->gmg;
+  const Geo &g = c->geo;
+  double *rhs  = (double *)rhsIn;
+  if (G->project) {
+    dim3 g2((g.ni + GTX - 1) / GTX, (g.nj + GTY - 1) / GTY);
+    kInteriorSumRhs<<<g2, dim3(GTX, GTY), 0, nusifStream()>>>(rhs, g, c->partials);
+    kFoldRhsSum<<<1, 256, 0, nusifStream()>>>(c->partials, (int)(g2.x * g2.y), c->scal);
+    kSubtractRhsMean<<<cellGrid(g), dim3(GTX, GTY), 0, nusifStream()>>>(
+        rhs, g, c->scal, (double)g.ni * g.nj * g.nk);
+  }
+  launchPressureGhosts(g, p, nusifStream());
+  double nb = 0.0;
+  int cycles = 0;
+  double rel = 0.0;
+  if (!G->mixed) {
+    G->L64[0].x = p;
+    G->L64[0].b = rhs;
+    double rn = residualNorm(c, G, p, rhs, nullptr, &nb);
+    rel       = nb > 0.0 ? rn / nb : 0.0;
+    while (nb > 0.0 && rel > G->tol && cycles < G->maxCycles) {
+      vcycle<double>(G, G->L64, 0);
+      cycles++;
+      rn  = residualNorm(c, G, p, rhs, nullptr, &nb);
+      rel = rn / nb;
+    }
+  } else {
+    LevelT<float> &L0 = G->L32[0];
+    double rn = residualNorm(c, G, p, rhs, L0.b, &nb);
+    rel       = nb > 0.0 ? rn / nb : 0.0;
+    while (nb > 0.0 && rel > G->tol && cycles < G->maxCycles) {
+      fill<float>(L0.x, g.n, 0.0f);
+      vcycle<float>(G, G->L32, 0);
+      NUSIF_TIMED("gmg_correct", (kCorrect64<<<cellGrid(g), dim3(GTX, GTY), 0, nusifStream()>>>(p, L0.x, g)));
+      cycles++;
+      rn  = residualNorm(c, G, p, rhs, L0.b, &nb);
+      rel = rn / nb;
+    }
+  }
+  sc->lastIterations = cycles;
+  sc->lastRel        = rel;
+  return rel;
+}
+
+struct UnetModel {
+  int dummy;
+};
 void unetFree(UnetModel *m) { delete m; }

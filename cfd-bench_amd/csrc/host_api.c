@@ -129,8 +129,11 @@ void commCollectResult(CommType *c, double *ug, double *vg, double *wg, double *
 }
 
 /* ---- parameters (src/parameter.c:13-133) ------------------------------- */
+extern void nusifClearOptions(const Parameter *p);
+
 void initParameter(Parameter *param)
 {
+  nusifClearOptions(param);
   param->xlength    = 1.0;
   param->ylength    = 1.0;
   param->zlength    = 1.0;
@@ -190,7 +193,7 @@ static const KeySpec kKeys[] = {
 /* Keys this library adds; matched exactly. None of the reference keys is a
  * prefix of any of them, so the reference's own keys never fire on them. */
 static const char *kExtraKeys[] = {"nsteps", "solver", "precision", "mg_tol", "mg_maxcycles",
-    "mg_omega", "mg_coarse", "mg_project", "lid", "u_inflow", "inflow_noise", "seed",
+    "mg_omega", "mg_project", "lid", "u_inflow", "inflow_noise", "seed",
     "warmstart", "unet_weights"};
 
 void readParameter(Parameter *param, const char *filename)

@@ -62,7 +62,6 @@ struct NusifOptions {
   double mgTol;
   int mgMaxCycles;
   double mgOmega;
-  int mgCoarse;
   int mgProject;
   double lid;
   double uInflow;
@@ -74,6 +73,7 @@ struct NusifOptions {
 void nusifDefaultOptions(NusifOptions *o);
 /* Options registered for `p` (defaults when none were set). */
 const NusifOptions *nusifOptionsFor(const Parameter *p);
+void nusifConsumeOptions(const Parameter *p, NusifOptions *out);
 
 /* Scalar state shared by the reduction kernels (device memory). */
 struct DevScalars {

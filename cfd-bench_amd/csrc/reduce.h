@@ -59,34 +59,26 @@ __device__ inline double blockMax(double x, double *lds)
   return r;
 }
 
-__device__ inline double loadAgent(const double *p)
+/* Second stage of a two-kernel reduction: one workgroup folds `n` partials
+ * (stride `stride`, component `comp`) in a fixed order, so the result does
+ * not depend on scheduling. Kernel boundaries order the partial writes
+ * before these reads; no device-scope atomics or fences are needed. */
+__device__ inline double foldSum(const double *part, int n, int stride, int comp, double *lds)
 {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  double s = 0.0;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) s += part[(long long)q * stride + comp];
+  return blockSum(s, lds);
 }
 
-__device__ inline void storeAgent(double *p, double v)
+__device__ inline double foldMax(const double *part, int n, int stride, int comp, double init,
+    double *lds)
 {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-/* Call with the workgroup's partial already stored (by thread 0). Returns
- * true in every thread of the workgroup that arrived last; that workgroup
- * then sees all partials. */
-__device__ inline bool lastBlock(unsigned int *counter, unsigned int nblocks)
-{
-  __shared__ int amLast;
-  int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
-  if (tid == 0) {
-    __threadfence();
-    unsigned int prev = atomicAdd(counter, 1u);
-    amLast            = (prev == nblocks - 1);
-    if (amLast) {
-      atomicExch(counter, 0u);
-      __threadfence();
-    }
+  double m = init;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    double x = part[(long long)q * stride + comp];
+    m        = m > x ? m : x;
   }
-  __syncthreads();
-  return amLast != 0;
+  return blockMax(m, lds);
 }
 
 #endif

@@ -119,43 +119,37 @@ void launchPressureGhosts(const Geo &g, double *p, hipStream_t st)
   kPressureGhosts<<<ghostGrid(g), dim3(64, 4), 0, st>>>(p, g);
 }
 
-/* Iteration q's last kernel: ghosts, then the last workgroup folds both
- * passes' partials (pass 0 first) into the carried residual
- * (src/solver-rb.c:150-151) and applies the stop test. */
-__global__ __launch_bounds__(256) void kRBFinish(double *p, Geo g, int q, RBState *st,
-    const double *__restrict__ partials, int nPart, double n, double epssq, int itermax,
-    unsigned int *counter)
+/* Iteration q's ghost fill (src/solver-rb.c:129-148) */
+__global__ __launch_bounds__(256) void kRBGhosts(double *p, Geo g, int q, const RBState *st)
 {
-  __shared__ double lds[4];
   if (q >= st->stopAt) return;
   int a = blockIdx.x * blockDim.x + threadIdx.x + 1;
   int b = blockIdx.y * blockDim.y + threadIdx.y + 1;
   ghostsAt(p, g, a, b);
-  if (lastBlock(counter, gridDim.x * gridDim.y)) {
-    int tid  = threadIdx.x + blockDim.x * threadIdx.y;
-    double s0 = 0.0, s1 = 0.0;
-    for (int x = tid; x < nPart; x += blockDim.x * blockDim.y) {
-      s0 += loadAgent(&partials[x]);
-      s1 += loadAgent(&partials[nPart + x]);
-    }
-    s0 = blockSum(s0, lds);
-    s1 = blockSum(s1, lds);
-    if (tid == 0) {
-      double res = (st->res + (s0 + s1)) / n;
-      int it     = q + 1;
-      st->res    = res;
-      st->it     = it;
-      if (!(res >= epssq && it < itermax)) st->stopAt = it;
-      __threadfence();
-    }
+}
+
+/* Iteration q's fold: both passes' partials (pass 0 first) into the carried
+ * residual (src/solver-rb.c:150-151), then the stop test (src/solver-rb.c:54). */
+__global__ __launch_bounds__(256) void kRBFold(int q, RBState *st, const double *__restrict__ partials,
+    int nPart, double n, double epssq, int itermax)
+{
+  __shared__ double lds[4];
+  if (q >= st->stopAt) return;
+  double s0 = foldSum(partials, nPart, 1, 0, lds);
+  double s1 = foldSum(partials + nPart, nPart, 1, 0, lds);
+  if (threadIdx.x == 0) {
+    double res = (st->res + (s0 + s1)) / n;
+    int it     = q + 1;
+    st->res    = res;
+    st->it     = it;
+    if (!(res >= epssq && it < itermax)) st->stopAt = it;
   }
 }
 
 /* MG residual (src/solver-mg.c:226-289): mean square of rhs - Lap(p) over the
  * interior, seeded with 1.0 (src/solver-mg.c:252). */
 __global__ __launch_bounds__(256) void kResidualMS(const double *__restrict__ p,
-    const double *__restrict__ rhs, Geo g, SweepCoef cf, double *partials, RBState *st,
-    double n, unsigned int *counter)
+    const double *__restrict__ rhs, Geo g, SweepCoef cf, double *partials)
 {
   __shared__ double lds[4];
   int i  = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -170,17 +164,16 @@ __global__ __launch_bounds__(256) void kResidualMS(const double *__restrict__ p,
       acc += r * r;
     }
   }
-  acc          = blockSum(acc, lds);
-  unsigned nb  = gridDim.x * gridDim.y * gridDim.z;
-  unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-  int tid      = threadIdx.x + blockDim.x * threadIdx.y;
-  if (tid == 0) partials[bid] = acc;
-  if (lastBlock(counter, nb)) {
-    double s = 0.0;
-    for (unsigned x = tid; x < nb; x += blockDim.x * blockDim.y) s += loadAgent(&partials[x]);
-    s = blockSum(s, lds);
-    if (tid == 0) st->res = (1.0 + s) / n;
-  }
+  acc = blockSum(acc, lds);
+  if (threadIdx.x == 0 && threadIdx.y == 0)
+    partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = acc;
+}
+
+__global__ __launch_bounds__(256) void kFoldMS(const double *partials, int nb, double n, RBState *st)
+{
+  __shared__ double lds[4];
+  double s = foldSum(partials, nb, 1, 0, lds);
+  if (threadIdx.x == 0) st->res = (1.0 + s) / n;
 }
 
 /* ---------------------------------------------------------------------- */
@@ -251,8 +244,8 @@ double rbSolve(NusifSolverCtx *sc, double *p, const double *rhs, double **pOut)
     for (int q = launched; q < end; q++) {
       launchPass(c, g, L, p, rhs, cf, 1, q, c->partials, 1);
       launchPass(c, g, L, p, rhs, cf, 0, q, c->partials + L.nPart, 1);
-      kRBFinish<<<gg, dim3(64, 4), 0, nusifStream()>>>(p, g, q, stateBuf(c), c->partials,
-          L.nPart, n, epssq, itermax, &c->scal->counter[2]);
+      kRBGhosts<<<gg, dim3(64, 4), 0, nusifStream()>>>(p, g, q, stateBuf(c));
+      kRBFold<<<1, 256, 0, nusifStream()>>>(q, stateBuf(c), c->partials, L.nPart, n, epssq, itermax);
     }
     launched = end;
     getState(c);
@@ -286,8 +279,8 @@ double mgRefSolve(NusifSolverCtx *sc, double *p, const double *rhs)
   dim3 rgrd((g.ni + 63) / 64, (g.nj + 3) / 4, (g.nk + RB_KCH - 1) / RB_KCH);
   if ((size_t)rgrd.x * rgrd.y * rgrd.z > c->nPartials) NUSIF_FATAL("partials buffer too small");
   auto residual = [&]() {
-    kResidualMS<<<rgrd, dim3(64, 4), 0, nusifStream()>>>(p, rhs, g, cf, c->partials, stateBuf(c),
-        n, &c->scal->counter[3]);
+    kResidualMS<<<rgrd, dim3(64, 4), 0, nusifStream()>>>(p, rhs, g, cf, c->partials);
+    kFoldMS<<<1, 256, 0, nusifStream()>>>(c->partials, (int)(rgrd.x * rgrd.y * rgrd.z), n, stateBuf(c));
   };
   if (coarsest) {
     for (int q = 0; q < s->presmooth; q++) smooth(false);

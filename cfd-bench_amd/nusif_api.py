@@ -95,6 +95,9 @@ DEFAULTS = dict(xlength=1.0, ylength=1.0, zlength=1.0, imax=100, jmax=100, kmax=
 def load(path):
     """Open a library exporting the NuSiF API and declare its prototypes."""
     lib = C.CDLL(path)
+    if not hasattr(lib, "initDiscretization"):  # helper library (oracle gmg / unet)
+        _declare_optional(lib)
+        return lib
     for fn in STEP_FUNCS + ("initDiscretization",):
         getattr(lib, fn).restype = None
     lib.initDiscretization.argtypes = [C.POINTER(Discretization), C.POINTER(Parameter)]
@@ -133,6 +136,12 @@ def _declare_optional(lib):
         "orcSetProblemExtras": ([C.c_double, C.c_double], None),
         "orcLastIterations": ([], C.c_int),
         "orcFreeDiscretization": ([C.POINTER(Discretization)], None),
+        "orcGmgLevels": ([C.c_int] * 4, C.c_int),
+        "orcGmgSolve": ([C.c_int] * 3 + [C.c_double] * 3 + [_DP, _DP] + [C.c_int] * 3
+                        + [C.c_double] * 2 + [C.c_int] * 3 + [_DP], C.c_int),
+        "orcGmgCoarseSolve": ([C.c_int] * 3 + [C.c_double] * 3 + [_DP, _DP], None),
+        "orcGmgProlong": ([C.c_int] * 3 + [_DP, _DP], None),
+        "orcGmgRestrict": ([C.c_int] * 3 + [C.c_double] * 3 + [_DP] * 3, None),
     }
     for name, (args, res) in opt.items():
         if hasattr(lib, name):

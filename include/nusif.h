@@ -44,7 +44,6 @@ void nusifFreeSolver(Solver *s);
  *   mg_tol        double  gmg: relative residual ||rhs-Ap||/||rhs|| (1e-6)
  *   mg_maxcycles  int     gmg: cycle cap per solve (100)
  *   mg_omega      double  gmg: smoother relaxation (1.0)
- *   mg_coarse     int     gmg: sweeps of the coarsest-level solve (64)
  *   mg_project    0 | 1   gmg: remove the rhs mean (all-Neumann compatibility)
  *   lid           double  lid speed of "dcavity" (reference: 1.0)
  *   u_inflow      double  inflow velocity of "canal" (reference: 2.0)

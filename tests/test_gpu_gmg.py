@@ -1,0 +1,93 @@
+"""The GPU geometric multigrid (FP64 and FP32-in-FP64 defect correction)
+against the oracle's restatement, through the C ABI."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import make_golden as G
+import nusif_api as A
+from conftest import lib
+
+pytestmark = pytest.mark.gpu
+DP = A._DP
+
+
+def oracle_solve(p, rhs, prm, opts):
+    prm = dict(A.DEFAULTS, **prm)
+    rel = C.c_double()
+    cyc = lib("orc_gmg").orcGmgSolve(
+        prm["imax"], prm["jmax"], prm["kmax"], prm["xlength"] / prm["imax"],
+        prm["ylength"] / prm["jmax"], prm["zlength"] / prm["kmax"], p.ctypes.data_as(DP),
+        rhs.ctypes.data_as(DP), prm["levels"], prm["presmooth"], prm["postsmooth"],
+        float(opts.get("mg_omega", 1.0)),
+        float(opts["mg_tol"]), int(opts.get("mg_maxcycles", 100)),
+        int(opts.get("mg_project", 0)), 1 if opts.get("precision") == "mixed" else 0,
+        C.byref(rel))
+    return cyc, rel.value
+
+
+CASES = [
+    ((32, 32, 32), 4, "fp64"), ((32, 32, 32), 4, "mixed"),
+    ((48, 24, 16), 3, "fp64"), ((48, 24, 16), 3, "mixed"),
+    ((64, 64, 64), 6, "mixed"),
+]
+
+
+@pytest.mark.parametrize("shape,levels,precision", CASES)
+def test_gpu_gmg_solve_matches_oracle(shape, levels, precision, gpu_ok):
+    print("gmg", shape, levels, precision, flush=True)
+    ni, nj, nk = shape
+    prm = dict(G.KAT_PARAMS, imax=ni, jmax=nj, kmax=nk, xlength=ni / 32.0, ylength=nj / 32.0,
+               zlength=nk / 32.0, levels=levels, presmooth=2, postsmooth=2)
+    opts = {"solver": "gmg", "precision": precision, "mg_tol": 1e-10, "mg_project": 1}
+    c = A.Case(lib("gpu_rb"), options=opts, **prm)
+    f = G.random_fields(c.shape, 31)
+    p0 = np.zeros(c.shape)
+    c.set_fields(p=p0, rhs=f["rhs"])
+    res = c.lib.solve(C.byref(c.s), c.d.p, c.d.rhs)
+    out = c.fields()
+    cyc_g = c.lib.nusifSolverIterations(C.byref(c.s))
+    p = p0.copy()
+    rhs = np.ascontiguousarray(f["rhs"]).copy()
+    cyc_o, rel_o = oracle_solve(p, rhs, prm, opts)
+    assert cyc_g == cyc_o
+    assert res == pytest.approx(rel_o, rel=1e-6)
+    assert res <= 1e-10
+    a = out["p"][1:-1, 1:-1, 1:-1]
+    b = p[1:-1, 1:-1, 1:-1]
+    tol = 1e-12 if precision == "fp64" else 1e-9
+    assert np.max(np.abs(a - b)) <= tol * np.max(np.abs(b))
+    c.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gpu_gmg_cavity_steps(precision, gpu_ok):
+    """Whole steps of the 32^3 cavity with the gmg solver: GPU vs the oracle's
+    step functions + oracle gmg; per-field error stated against FP64."""
+    prm = dict(G.TRAJ["c1_rb"][1], re=1000.0, levels=4, presmooth=2, postsmooth=2)
+    opts = {"solver": "gmg", "precision": precision, "mg_tol": 1e-8}
+    g = A.Case(lib("gpu_rb"), options=opts, **prm)
+    o = A.Case(lib("orc_rb"), **prm)
+    for step in range(5):
+        g.step()
+        d = o.d
+        if d.tau > 0:
+            o.lib.computeTimestep(C.byref(d))
+        for fn in ("setBoundaryConditions", "setSpecialBoundaryCondition", "computeFG",
+                   "computeRHS"):
+            o.call(fn)
+        if step == 0:
+            o.call("normalizePressure")
+        n = int(np.prod(o.shape))
+        pv = np.ctypeslib.as_array(d.p, shape=(n,))
+        rv = np.ctypeslib.as_array(d.rhs, shape=(n,))
+        oracle_solve(pv, rv, prm, opts)
+        o.call("adaptUV")
+    fg, fo = g.fields(), o.fields()
+    for k in ("u", "v", "w", "p"):
+        err = A.rel_maxnorm(fg[k], fo[k])
+        print(precision, k, "rel err vs oracle", err)
+        assert err <= (1e-10 if precision == "fp64" else 1e-6), k
+    g.close()
+    o.close()
