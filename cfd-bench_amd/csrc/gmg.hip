@@ -45,6 +45,7 @@ template <typename T> struct Coef {
 template <typename T> struct LevelT {
   Geo g;
   T *x, *b;
+  T *xAlt; /* ping-pong partner of x for the fused sweep */
   Coef<T> cf;
 };
 
@@ -146,6 +147,154 @@ __global__ __launch_bounds__(256) void kSmooth(T *__restrict__ x, const T *__res
       }
     }
   }
+}
+
+/* One full red-black sweep in one launch (ping-pong: reads xo, writes xn).
+ * A workgroup owns a 64 x 4 (i, j) tile and marches a chunk of FT_KC planes
+ * in k. Plane k's odd ("red") cells are updated from the old values of
+ * planes k-1..k+1 held in LDS (X), over the tile plus a one-cell ring, into
+ * LDS (Y); then plane k-1's even ("black") cells are updated from Y. The ring
+ * and the planes k0-1 and k1 are computed redundantly, so no workgroup reads
+ * another's new values. Every cell gets exactly the values of the two-pass
+ * sweep (same operands, same order); face ghosts are written with the new
+ * values. X and Y rotate over 4 plane slots: loads of plane k+2 never touch
+ * a slot still being read. */
+#define FT_I 64
+#define FT_J 4
+#define FT_KC 32
+#define FUSED_MIN_EXTENT 128 /* levels narrower than this use the two-pass sweep */
+#define XW (FT_I + 4)
+#define XH (FT_J + 4)
+#define YW (FT_I + 2)
+#define YH (FT_J + 2)
+
+template <typename T>
+__global__ __launch_bounds__(256) void kSweepFused(const T *__restrict__ xo, T *__restrict__ xn,
+    const T *__restrict__ b, Geo g, Coef<T> cf)
+{
+  __shared__ T X[4][XH * XW];
+  __shared__ T Y[4][YH * YW];
+  constexpr int NX = (XH * XW + 255) / 256; /* X elements per thread */
+  constexpr int NY = (YH * YW + 255) / 256; /* Y elements per thread */
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.x * FT_I + 1, j0 = blockIdx.y * FT_J + 1;
+  const int k0 = blockIdx.z * FT_KC + 1;
+  const int k1 = min(k0 + FT_KC, g.nk + 1); /* black planes [k0, k1) */
+  const int tx = tid & (FT_I - 1), ty = tid / FT_I;
+  const int bi = i0 + tx, bj = j0 + ty;
+  const bool bIn = bi <= g.ni && bj <= g.nj;
+
+  /* Software pipeline: the global loads of plane k+2 (x) and of plane k+1
+   * (b of its red cells, b of this thread's cell) are issued into registers
+   * before the barriers of iteration k and consumed one iteration later. */
+  T xr[NX];
+  auto fetchX = [&](int k) {
+#pragma unroll
+    for (int q = 0; q < NX; q++) {
+      int e = tid + 256 * q;
+      int i = i0 - 2 + e % XW, j = j0 - 2 + e / XW;
+      bool in = e < XH * XW && k >= 0 && k <= g.nk + 1 && i >= 0 && i <= g.ni + 1 && j >= 0 &&
+                j <= g.nj + 1;
+      xr[q] = in ? xo[g.at(i, j, k)] : (T)0;
+    }
+  };
+  auto storeX = [&](int k) {
+#pragma unroll
+    for (int q = 0; q < NX; q++) {
+      int e = tid + 256 * q;
+      if (e < XH * XW) X[k & 3][e] = xr[q];
+    }
+  };
+  auto isRed = [&](int e, int k) {
+    int i = i0 - 1 + e % YW, j = j0 - 1 + e / YW;
+    return e < YH * YW && k >= 1 && k <= g.nk && i >= 1 && i <= g.ni && j >= 1 && j <= g.nj &&
+           ((i + j + k) & 1) == 1;
+  };
+  auto fetchB = [&](int k, T *out) { /* b at the red cells of plane k (Y tile) */
+#pragma unroll
+    for (int q = 0; q < NY; q++) {
+      int e = tid + 256 * q;
+      out[q] = isRed(e, k) ? b[g.at(i0 - 1 + e % YW, j0 - 1 + e / YW, k)] : (T)0;
+    }
+  };
+  auto redPlane = [&](int k, const T *bred) {
+    const T *Xm = X[(k + 3) & 3], *Xc = X[k & 3], *Xp = X[(k + 1) & 3];
+    T *Yc = Y[k & 3];
+#pragma unroll
+    for (int q = 0; q < NY; q++) {
+      int e = tid + 256 * q;
+      if (e < YH * YW) {
+        int xe = (e / YW + 1) * XW + (e % YW + 1);
+        T v    = Xc[xe];
+        if (isRed(e, k)) {
+          T lap = (Xc[xe + 1] - (T)2 * v + Xc[xe - 1]) * cf.idx2 +
+                  (Xc[xe + XW] - (T)2 * v + Xc[xe - XW]) * cf.idy2 +
+                  (Xp[xe] - (T)2 * v + Xm[xe]) * cf.idz2;
+          T r = bred[q] - lap;
+          v   = v - cf.factor * r;
+        }
+        Yc[e] = v;
+      }
+    }
+  };
+  auto blackPlane = [&](int k, T bc) {
+    if (!bIn) return;
+    int ye = (ty + 1) * YW + (tx + 1);
+    const T *Yc = Y[k & 3], *Ym = Y[(k + 3) & 3], *Yp = Y[(k + 1) & 3];
+    long long c = g.at(bi, bj, k);
+    T v;
+    if (((bi + bj + k) & 1) == 0) {
+      T xc  = X[k & 3][(ty + 2) * XW + (tx + 2)];
+      T lap = (Yc[ye + 1] - (T)2 * xc + Yc[ye - 1]) * cf.idx2 +
+              (Yc[ye + YW] - (T)2 * xc + Yc[ye - YW]) * cf.idy2 +
+              (Yp[ye] - (T)2 * xc + Ym[ye]) * cf.idz2;
+      T r = bc - lap;
+      v   = xc - cf.factor * r;
+    } else {
+      v = Yc[ye];
+    }
+    xn[c] = v;
+    faceGhosts(xn, g, bi, bj, k, c, v);
+  };
+
+  /* prologue: planes k0-2, k0-1 straight to LDS; registers for iteration k0-1 */
+  if (k0 - 2 >= 0) {
+    fetchX(k0 - 2);
+    storeX(k0 - 2);
+  }
+  fetchX(k0 - 1);
+  storeX(k0 - 1);
+  fetchX(k0);
+  T brN[NY];
+  fetchB(k0 - 1, brN);
+  T bbN = 0;
+  for (int kk = k0 - 1; kk <= k1; kk++) {
+    storeX(kk + 1); /* xr holds plane kk+1 */
+    T brUse[NY];
+#pragma unroll
+    for (int q = 0; q < NY; q++) brUse[q] = brN[q];
+    T bbUse = bbN;  /* b of this thread's cell at plane kk-1 */
+    fetchX(kk + 2);
+    fetchB(kk + 1, brN);
+    bbN = (bIn && kk >= 1 && kk <= g.nk) ? b[g.at(bi, bj, kk)] : (T)0;
+    __syncthreads();
+    if (kk == 0 || kk == g.nk + 1) { /* ghost plane: old values */
+#pragma unroll
+      for (int q = 0; q < NY; q++) {
+        int e = tid + 256 * q;
+        if (e < YH * YW) Y[kk & 3][e] = X[kk & 3][(e / YW + 1) * XW + (e % YW + 1)];
+      }
+    } else {
+      redPlane(kk, brUse);
+    }
+    __syncthreads();
+    if (kk - 1 >= k0) blackPlane(kk - 1, bbUse);
+  }
+}
+
+static dim3 fusedGrid(const Geo &g)
+{
+  return dim3((g.ni + FT_I - 1) / FT_I, (g.nj + FT_J - 1) / FT_J, (g.nk + FT_KC - 1) / FT_KC);
 }
 
 /* b_c(I,J,K) = (1/8) sum over the 8 children of (b - A x); the coarse x is
@@ -381,23 +530,27 @@ template <typename T> static void fill(T *x, long long n, T v)
 }
 
 /* Timers cover the finest level only ("..._l0"), whose algorithmic bytes
- * nusifKernelBytes reports. */
-template <typename T> static void smoothPass(LevelT<T> &L, int parity, bool fine)
-{
-  dim3 grd = sweepGrid(L.g);
-  int gh   = parity == 0;
-  if (fine)
-    NUSIF_TIMED(sizeof(T) == 8 ? "gmg_smooth64_l0" : "gmg_smooth32_l0",
-        (kSmooth<T><<<grd, dim3(GTX, GTY), 0, nusifStream()>>>(L.x, L.b, L.g, L.cf, parity, gh)));
-  else
-    kSmooth<T><<<grd, dim3(GTX, GTY), 0, nusifStream()>>>(L.x, L.b, L.g, L.cf, parity, gh);
-}
-
+ * nusifKernelBytes reports. Each sweep swaps x and xAlt. */
 template <typename T> static void smooth(LevelT<T> &L, int sweeps, bool fine)
 {
+  if (L.g.ni < FUSED_MIN_EXTENT || L.g.nk < FT_KC) { /* small levels: two colour passes */
+    dim3 grd = sweepGrid(L.g);
+    for (int s = 0; s < sweeps; s++) {
+      kSmooth<T><<<grd, dim3(GTX, GTY), 0, nusifStream()>>>(L.x, L.b, L.g, L.cf, 1, 0);
+      kSmooth<T><<<grd, dim3(GTX, GTY), 0, nusifStream()>>>(L.x, L.b, L.g, L.cf, 0, 1);
+    }
+    return;
+  }
+  dim3 grd = fusedGrid(L.g);
   for (int s = 0; s < sweeps; s++) {
-    smoothPass(L, 1, fine);
-    smoothPass(L, 0, fine);
+    if (fine)
+      NUSIF_TIMED(sizeof(T) == 8 ? "gmg_smooth64_l0" : "gmg_smooth32_l0",
+          (kSweepFused<T><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf)));
+    else
+      kSweepFused<T><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf);
+    T *t   = L.x;
+    L.x    = L.xAlt;
+    L.xAlt = t;
   }
 }
 
@@ -521,18 +674,22 @@ GmgState *gmgCreate(NusifSolverCtx *sc, const Parameter *prm)
       LevelT<float> &L = G->L32[l];
       L.g              = g;
       L.x              = devAlloc<float>(g.n);
+      L.xAlt           = devAlloc<float>(g.n);
       L.b              = devAlloc<float>(g.n);
       L.cf             = coefFor<float>(dx, dy, dz, G->omega);
       fill<float>(L.x, g.n, 0.0f);
+      fill<float>(L.xAlt, g.n, 0.0f);
       fill<float>(L.b, g.n, 0.0f);
     } else {
       LevelT<double> &L = G->L64[l];
       L.g               = g;
       L.cf              = coefFor<double>(dx, dy, dz, G->omega);
       if (l > 0) {
-        L.x = devAlloc<double>(g.n);
-        L.b = devAlloc<double>(g.n);
+        L.x    = devAlloc<double>(g.n);
+        L.xAlt = devAlloc<double>(g.n);
+        L.b    = devAlloc<double>(g.n);
         fill<double>(L.x, g.n, 0.0);
+        fill<double>(L.xAlt, g.n, 0.0);
         fill<double>(L.b, g.n, 0.0);
       }
     }
@@ -556,7 +713,9 @@ void gmgFree(GmgState *G)
   if (G->tab32) HIP_CHECK(hipFree(G->tab32));
   for (int l = 0; l < G->nl; l++) {
     if (G->L32[l].x) HIP_CHECK(hipFree(G->L32[l].x));
+    if (G->L32[l].xAlt) HIP_CHECK(hipFree(G->L32[l].xAlt));
     if (G->L32[l].b) HIP_CHECK(hipFree(G->L32[l].b));
+    if (l > 0 && G->L64[l].xAlt) HIP_CHECK(hipFree(G->L64[l].xAlt));
     if (l > 0 && G->L64[l].x) HIP_CHECK(hipFree(G->L64[l].x));
     if (l > 0 && G->L64[l].b) HIP_CHECK(hipFree(G->L64[l].b));
   }
@@ -595,15 +754,21 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
   int cycles = 0;
   double rel = 0.0;
   if (!G->mixed) {
-    G->L64[0].x = p;
-    G->L64[0].b = rhs;
-    double rn = residualNorm(c, G, p, rhs, nullptr, &nb);
+    LevelT<double> &L0 = G->L64[0];
+    L0.x    = p;
+    L0.xAlt = c->pAlt;
+    L0.b    = rhs;
+    double rn = residualNorm(c, G, L0.x, rhs, nullptr, &nb);
     rel       = nb > 0.0 ? rn / nb : 0.0;
     while (nb > 0.0 && rel > G->tol && cycles < G->maxCycles) {
       vcycle<double>(G, G->L64, 0);
       cycles++;
-      rn  = residualNorm(c, G, p, rhs, nullptr, &nb);
+      rn  = residualNorm(c, G, L0.x, rhs, nullptr, &nb);
       rel = rn / nb;
+    }
+    if (L0.x != c->p) { /* the result sits in the partner buffer */
+      c->pAlt = c->p;
+      c->p    = L0.x;
     }
   } else {
     LevelT<float> &L0 = G->L32[0];

@@ -31,6 +31,8 @@ CASES = [
     ((32, 32, 32), 4, "fp64"), ((32, 32, 32), 4, "mixed"),
     ((48, 24, 16), 3, "fp64"), ((48, 24, 16), 3, "mixed"),
     ((64, 64, 64), 6, "mixed"),
+    # finest extent >= 128 and nk >= 32: the fused one-launch sweep (partial k chunk)
+    ((128, 12, 72), 3, "fp64"), ((128, 12, 72), 3, "mixed"),
 ]
 
 
