@@ -323,11 +323,13 @@ void launchComputeFG(NusifCtx *c, const Discretization *d)
 /* ------------------------------------------------------------------------ */
 __global__ __launch_bounds__(256) void kComputeRHS(const double *__restrict__ f,
     const double *__restrict__ g, const double *__restrict__ h, double *__restrict__ rhs,
-    Geo geo, double idx, double idy, double idz, double idt)
+    Geo geo, double idx, double idy, double idz, double idt, int gridGx, int gridGy)
 {
-  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  int k = blockIdx.z + 1;
+  int bx, by, bz;
+  if (!xcdBlock(gridGx, gridGy, geo.nk, bx, by, bz)) return;
+  int i = bx * blockDim.x + threadIdx.x + 1;
+  int j = by * blockDim.y + threadIdx.y + 1;
+  int k = bz + 1;
   if (i > geo.ni || j > geo.nj) return;
   long long c = geo.at(i, j, k);
   rhs[c] = ((f[c] - f[c - 1]) * idx + (g[c] - g[c - geo.sj]) * idy + (h[c] - h[c - geo.sk]) * idz) *
@@ -340,8 +342,8 @@ void launchComputeRHS(NusifCtx *c, const Discretization *d)
   dim3 blk(TX, TY);
   dim3 grd((g.ni + TX - 1) / TX, (g.nj + TY - 1) / TY, g.nk);
   NUSIF_TIMED("computeRHS",
-      (kComputeRHS<<<grd, blk, 0, nusifStream()>>>(c->f, c->g, c->h, c->rhs, g, 1.0 / d->grid.dx,
-          1.0 / d->grid.dy, 1.0 / d->grid.dz, 1.0 / d->dt)));
+      (kComputeRHS<<<xcdGrid(grd), blk, 0, nusifStream()>>>(c->f, c->g, c->h, c->rhs, g,
+          1.0 / d->grid.dx, 1.0 / d->grid.dy, 1.0 / d->grid.dz, 1.0 / d->dt, grd.x, grd.y)));
 }
 
 /* ------------------------------------------------------------------------ */
@@ -368,11 +370,13 @@ __global__ __launch_bounds__(256) void kFoldMean(const double *partials, int nb,
   if (threadIdx.x == 0) sc->sum = s / count;
 }
 
-__global__ __launch_bounds__(256) void kSubtractMean(double *__restrict__ p, Geo geo, const DevScalars *sc)
+__global__ __launch_bounds__(256) void kSubtractMean(double *__restrict__ p, Geo geo, const DevScalars *sc, int gridGx, int gridGy)
 {
-  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  int k = blockIdx.z + 1;
+  int bx, by, bz;
+  if (!xcdBlock(gridGx, gridGy, geo.nk, bx, by, bz)) return;
+  int i = bx * blockDim.x + threadIdx.x + 1;
+  int j = by * blockDim.y + threadIdx.y + 1;
+  int k = bz + 1;
   if (i > geo.ni || j > geo.nj) return;
   double avg = sc->sum;
   long long c = geo.at(i, j, k);
@@ -389,7 +393,7 @@ void launchNormalize(NusifCtx *c)
   kInteriorSum<<<g2, blk, 0, nusifStream()>>>(c->p, g, c->partials);
   kFoldMean<<<1, 256, 0, nusifStream()>>>(c->partials, (int)(g2.x * g2.y), count, c->scal);
   dim3 g3(g2.x, g2.y, g.nk);
-  kSubtractMean<<<g3, blk, 0, nusifStream()>>>(c->p, g, c->scal);
+  kSubtractMean<<<xcdGrid(g3), blk, 0, nusifStream()>>>(c->p, g, c->scal, g3.x, g3.y);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -398,11 +402,13 @@ void launchNormalize(NusifCtx *c)
 __global__ __launch_bounds__(256) void kAdaptUV(const double *__restrict__ f,
     const double *__restrict__ g, const double *__restrict__ h, const double *__restrict__ p,
     double *__restrict__ u, double *__restrict__ v, double *__restrict__ w, Geo geo, double fx,
-    double fy, double fz)
+    double fy, double fz, int gridGx, int gridGy)
 {
-  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  int k = blockIdx.z + 1;
+  int bx, by, bz;
+  if (!xcdBlock(gridGx, gridGy, geo.nk, bx, by, bz)) return;
+  int i = bx * blockDim.x + threadIdx.x + 1;
+  int j = by * blockDim.y + threadIdx.y + 1;
+  int k = bz + 1;
   if (i > geo.ni || j > geo.nj) return;
   long long c = geo.at(i, j, k);
   double pc   = p[c];
@@ -416,9 +422,9 @@ void launchAdaptUV(NusifCtx *c, const Discretization *d)
   const Geo &g = c->geo;
   dim3 blk(TX, TY);
   dim3 grd((g.ni + TX - 1) / TX, (g.nj + TY - 1) / TY, g.nk);
-  NUSIF_TIMED("adaptUV", (kAdaptUV<<<grd, blk, 0, nusifStream()>>>(c->f, c->g, c->h, c->p, c->u,
-                             c->v, c->w, g, d->dt / d->grid.dx, d->dt / d->grid.dy,
-                             d->dt / d->grid.dz)));
+  NUSIF_TIMED("adaptUV", (kAdaptUV<<<xcdGrid(grd), blk, 0, nusifStream()>>>(c->f, c->g, c->h,
+                             c->p, c->u, c->v, c->w, g, d->dt / d->grid.dx, d->dt / d->grid.dy,
+                             d->dt / d->grid.dz, grd.x, grd.y)));
 }
 
 /* ------------------------------------------------------------------------ */

@@ -159,6 +159,14 @@ __global__ __launch_bounds__(256) void kSmooth(T *__restrict__ x, const T *__res
  * sweep (same operands, same order); face ghosts are written with the new
  * values. X and Y rotate over 4 plane slots: loads of plane k+2 never touch
  * a slot still being read. */
+/* Workgroup barrier that orders LDS traffic only: waits for this wave's
+ * LDS operations, then s_barrier. Unlike __syncthreads() it does not wait
+ * for outstanding global loads, so the prefetches below stay in flight
+ * across it. The "memory" clobber keeps the compiler from moving memory
+ * accesses across the barrier. */
+__device__ inline void ldsBarrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+#define NUM_XCD 8 /* MI355X: 8 XCDs, each with its own L2 */
 #define FT_I 64
 #define FT_J 4
 #define FT_KC 32
@@ -169,143 +177,181 @@ __global__ __launch_bounds__(256) void kSmooth(T *__restrict__ x, const T *__res
 #define YH (FT_J + 2)
 
 template <typename T>
-__global__ __launch_bounds__(256) void kSweepFused(const T *__restrict__ xo, T *__restrict__ xn,
+__global__ __launch_bounds__(256, 8) void kSweepFused(const T *__restrict__ xo, T *__restrict__ xn,
     const T *__restrict__ b, Geo g, Coef<T> cf)
 {
-  __shared__ T X[4][XH * XW];
-  __shared__ T Y[4][YH * YW];
   constexpr int NX = (XH * XW + 255) / 256; /* X elements per thread */
   constexpr int NY = (YH * YW + 255) / 256; /* Y elements per thread */
+  /* slots padded to whole thread multiples so every LDS access is unconditional */
+  __shared__ T X[4][NX * 256];
+  __shared__ T Y[4][NY * 256];
   const int tid = threadIdx.x;
-  const int i0 = blockIdx.x * FT_I + 1, j0 = blockIdx.y * FT_J + 1;
-  const int k0 = blockIdx.z * FT_KC + 1;
+  /* XCD-aware tile order: workgroup ids are dealt round-robin over the 8
+   * XCDs, so id w runs on XCD w % 8. Giving each XCD a contiguous run of
+   * tiles (x fastest, then y, then the k chunk) keeps the halos that
+   * neighbouring tiles share inside one L2. */
+  const int gxT = (g.ni + FT_I - 1) / FT_I, gyT = (g.nj + FT_J - 1) / FT_J;
+  const int nTiles = gxT * gyT * ((g.nk + FT_KC - 1) / FT_KC);
+  const int perXcd = (nTiles + NUM_XCD - 1) / NUM_XCD;
+  const int t = (blockIdx.x % NUM_XCD) * perXcd + blockIdx.x / NUM_XCD;
+  if (t >= nTiles) return;
+  const int i0 = (t % gxT) * FT_I + 1, j0 = ((t / gxT) % gyT) * FT_J + 1;
+  const int k0 = (t / (gxT * gyT)) * FT_KC + 1;
   const int k1 = min(k0 + FT_KC, g.nk + 1); /* black planes [k0, k1) */
+  const int nk = g.nk, sj = g.sj;
+  const long long sk = g.sk;
+
+  /* Plane-independent per-thread addressing, computed once. Positions
+   * outside the padded box load from a clamped (valid) address: those X
+   * entries are only neighbours of ghost or out-of-box Y entries, which are
+   * never updated, so their values are never used. */
+  int xOff[NX];  /* j*sj + i of this thread's X elements (clamped) */
+  int yXe[NY];   /* X index under this thread's Y elements */
+  int yOff[NY];  /* j*sj + i of the Y element (clamped) */
+  int yPar[NY];  /* (i + j) & 1, or 2 when the element is not an interior cell */
+#pragma unroll
+  for (int q = 0; q < NX; q++) {
+    int e = min(tid + 256 * q, XH * XW - 1);
+    int i = min(max(i0 - 2 + e % XW, 0), g.ni + 1);
+    int j = min(max(j0 - 2 + e / XW, 0), g.nj + 1);
+    xOff[q] = j * sj + i;
+  }
+#pragma unroll
+  for (int q = 0; q < NY; q++) {
+    int e  = tid + 256 * q;
+    int ec = min(e, YH * YW - 1);
+    int yi = ec % YW, yj = ec / YW;
+    int i = i0 - 1 + yi, j = j0 - 1 + yj;
+    bool in = e < YH * YW && i >= 1 && i <= g.ni && j >= 1 && j <= g.nj;
+    yXe[q]  = (yj + 1) * XW + (yi + 1);
+    yOff[q] = min(j, g.nj + 1) * sj + min(i, g.ni + 1);
+    yPar[q] = in ? ((i + j) & 1) : 2;
+  }
   const int tx = tid & (FT_I - 1), ty = tid / FT_I;
   const int bi = i0 + tx, bj = j0 + ty;
   const bool bIn = bi <= g.ni && bj <= g.nj;
+  const int bOff = min(bj, g.nj + 1) * sj + min(bi, g.ni + 1);
+  const int bPar = (bi + bj) & 1;
+  const int bYe  = (ty + 1) * YW + (tx + 1);
+  const int bXe  = (ty + 2) * XW + (tx + 2);
+  const bool fIlo = bi == 1, fIhi = bi == g.ni, fJlo = bj == 1, fJhi = bj == g.nj;
 
-  /* Software pipeline: the global loads of plane k+2 (x) and of plane k+1
-   * (b of its red cells, b of this thread's cell) are issued into registers
-   * before the barriers of iteration k and consumed one iteration later. */
   T xr[NX];
   auto fetchX = [&](int k) {
+    long long base = (long long)min(max(k, 0), nk + 1) * sk;
 #pragma unroll
-    for (int q = 0; q < NX; q++) {
-      int e = tid + 256 * q;
-      int i = i0 - 2 + e % XW, j = j0 - 2 + e / XW;
-      bool in = e < XH * XW && k >= 0 && k <= g.nk + 1 && i >= 0 && i <= g.ni + 1 && j >= 0 &&
-                j <= g.nj + 1;
-      xr[q] = in ? xo[g.at(i, j, k)] : (T)0;
-    }
+    for (int q = 0; q < NX; q++) xr[q] = xo[base + xOff[q]];
   };
   auto storeX = [&](int k) {
 #pragma unroll
-    for (int q = 0; q < NX; q++) {
-      int e = tid + 256 * q;
-      if (e < XH * XW) X[k & 3][e] = xr[q];
-    }
+    for (int q = 0; q < NX; q++) X[k & 3][tid + 256 * q] = xr[q];
   };
-  auto isRed = [&](int e, int k) {
-    int i = i0 - 1 + e % YW, j = j0 - 1 + e / YW;
-    return e < YH * YW && k >= 1 && k <= g.nk && i >= 1 && i <= g.ni && j >= 1 && j <= g.nj &&
-           ((i + j + k) & 1) == 1;
-  };
-  auto fetchB = [&](int k, T *out) { /* b at the red cells of plane k (Y tile) */
+  auto fetchB = [&](int k, T *out) { /* b under this thread's Y elements, plane k */
+    long long base = (long long)min(max(k, 1), nk) * sk;
 #pragma unroll
-    for (int q = 0; q < NY; q++) {
-      int e = tid + 256 * q;
-      out[q] = isRed(e, k) ? b[g.at(i0 - 1 + e % YW, j0 - 1 + e / YW, k)] : (T)0;
-    }
+    for (int q = 0; q < NY; q++) out[q] = b[base + yOff[q]];
   };
   auto redPlane = [&](int k, const T *bred) {
     const T *Xm = X[(k + 3) & 3], *Xc = X[k & 3], *Xp = X[(k + 1) & 3];
     T *Yc = Y[k & 3];
+    bool kin = k >= 1 && k <= nk;
 #pragma unroll
     for (int q = 0; q < NY; q++) {
-      int e = tid + 256 * q;
-      if (e < YH * YW) {
-        int xe = (e / YW + 1) * XW + (e % YW + 1);
-        T v    = Xc[xe];
-        if (isRed(e, k)) {
-          T lap = (Xc[xe + 1] - (T)2 * v + Xc[xe - 1]) * cf.idx2 +
-                  (Xc[xe + XW] - (T)2 * v + Xc[xe - XW]) * cf.idy2 +
-                  (Xp[xe] - (T)2 * v + Xm[xe]) * cf.idz2;
-          T r = bred[q] - lap;
-          v   = v - cf.factor * r;
-        }
-        Yc[e] = v;
-      }
+      int xe = yXe[q];
+      T v    = Xc[xe];
+      T lap  = (Xc[xe + 1] - (T)2 * v + Xc[xe - 1]) * cf.idx2 +
+              (Xc[xe + XW] - (T)2 * v + Xc[xe - XW]) * cf.idy2 +
+              (Xp[xe] - (T)2 * v + Xm[xe]) * cf.idz2;
+      T r    = bred[q] - lap;
+      T vNew = v - cf.factor * r;
+      bool red = kin && ((yPar[q] + k) & 1) == 1 && yPar[q] < 2;
+      Yc[tid + 256 * q] = red ? vNew : v;
     }
   };
   auto blackPlane = [&](int k, T bc) {
-    if (!bIn) return;
-    int ye = (ty + 1) * YW + (tx + 1);
     const T *Yc = Y[k & 3], *Ym = Y[(k + 3) & 3], *Yp = Y[(k + 1) & 3];
-    long long c = g.at(bi, bj, k);
-    T v;
-    if (((bi + bj + k) & 1) == 0) {
-      T xc  = X[k & 3][(ty + 2) * XW + (tx + 2)];
-      T lap = (Yc[ye + 1] - (T)2 * xc + Yc[ye - 1]) * cf.idx2 +
-              (Yc[ye + YW] - (T)2 * xc + Yc[ye - YW]) * cf.idy2 +
-              (Yp[ye] - (T)2 * xc + Ym[ye]) * cf.idz2;
-      T r = bc - lap;
-      v   = xc - cf.factor * r;
-    } else {
-      v = Yc[ye];
+    T xc  = X[k & 3][bXe];
+    T lap = (Yc[bYe + 1] - (T)2 * xc + Yc[bYe - 1]) * cf.idx2 +
+            (Yc[bYe + YW] - (T)2 * xc + Yc[bYe - YW]) * cf.idy2 +
+            (Yp[bYe] - (T)2 * xc + Ym[bYe]) * cf.idz2;
+    T r    = bc - lap;
+    T vNew = xc - cf.factor * r;
+    T v    = (((bPar + k) & 1) == 0) ? vNew : Yc[bYe];
+    if (bIn) {
+      long long c = k * sk + bOff;
+      xn[c]       = v;
+      if (fIlo) xn[c - 1] = v; /* face ghosts (reference ranges) */
+      if (fIhi) xn[c + 1] = v;
+      if (fJlo) xn[c - sj] = v;
+      if (fJhi) xn[c + sj] = v;
+      if (k == 1) xn[c - sk] = v;
+      if (k == nk) xn[c + sk] = v;
     }
-    xn[c] = v;
-    faceGhosts(xn, g, bi, bj, k, c, v);
   };
 
-  /* prologue: planes k0-2, k0-1 straight to LDS; registers for iteration k0-1 */
-  if (k0 - 2 >= 0) {
-    fetchX(k0 - 2);
-    storeX(k0 - 2);
-  }
+  /* One iteration of the plane loop with register set (xs, bs, bbs): store
+   * plane kk+1 to LDS, issue the loads of plane kk+3 (x), kk+2 (red b) and
+   * kk+1 (own b) two iterations ahead, then red(kk) and black(kk-1). */
+  auto iterate = [&](int kk, T(&xs)[NX], T(&bs)[NY], T &bbs) {
+#pragma unroll
+    for (int q = 0; q < NX; q++) xr[q] = xs[q];
+    storeX(kk + 1);
+    T bUse[NY];
+#pragma unroll
+    for (int q = 0; q < NY; q++) bUse[q] = bs[q];
+    T bbUse = bbs;
+    fetchX(kk + 3);
+#pragma unroll
+    for (int q = 0; q < NX; q++) xs[q] = xr[q];
+    fetchB(kk + 2, bs);
+    bbs = b[(long long)min(max(kk + 1, 1), nk) * sk + bOff];
+    ldsBarrier();
+    redPlane(kk, bUse); /* ghost planes (kk = 0, nk+1) copy old values */
+    ldsBarrier();
+    if (kk - 1 >= k0) blackPlane(kk - 1, bbUse);
+  };
+
+  /* prologue: planes k0-2, k0-1 straight to LDS; two register sets ahead */
+  fetchX(k0 - 2); /* clamped to plane 0 when k0 = 1 (then never read) */
+  storeX(k0 - 2 + 4);
   fetchX(k0 - 1);
   storeX(k0 - 1);
+  T xA[NX], xB[NX], bA[NY], bB[NY], bbA = 0, bbB = 0;
   fetchX(k0);
-  T brN[NY];
-  fetchB(k0 - 1, brN);
-  T bbN = 0;
-  for (int kk = k0 - 1; kk <= k1; kk++) {
-    storeX(kk + 1); /* xr holds plane kk+1 */
-    T brUse[NY];
 #pragma unroll
-    for (int q = 0; q < NY; q++) brUse[q] = brN[q];
-    T bbUse = bbN;  /* b of this thread's cell at plane kk-1 */
-    fetchX(kk + 2);
-    fetchB(kk + 1, brN);
-    bbN = (bIn && kk >= 1 && kk <= g.nk) ? b[g.at(bi, bj, kk)] : (T)0;
-    __syncthreads();
-    if (kk == 0 || kk == g.nk + 1) { /* ghost plane: old values */
+  for (int q = 0; q < NX; q++) xA[q] = xr[q];
+  fetchX(k0 + 1);
 #pragma unroll
-      for (int q = 0; q < NY; q++) {
-        int e = tid + 256 * q;
-        if (e < YH * YW) Y[kk & 3][e] = X[kk & 3][(e / YW + 1) * XW + (e % YW + 1)];
-      }
-    } else {
-      redPlane(kk, brUse);
-    }
-    __syncthreads();
-    if (kk - 1 >= k0) blackPlane(kk - 1, bbUse);
+  for (int q = 0; q < NX; q++) xB[q] = xr[q];
+  fetchB(k0 - 1, bA);
+  fetchB(k0, bB);
+  /* the own-b slots of both sets stand for planes k0-2 and k0-1 at their
+   * first use, where no black plane is updated */
+  iterate(k0 - 1, xA, bA, bbA);
+  for (int kk = k0; kk <= k1; kk += 2) {
+    iterate(kk, xB, bB, bbB);
+    if (kk + 1 <= k1) iterate(kk + 1, xA, bA, bbA);
   }
 }
 
+/* 1-D grid: NUM_XCD * ceil(tiles / NUM_XCD) workgroups (surplus ids exit) */
 static dim3 fusedGrid(const Geo &g)
 {
-  return dim3((g.ni + FT_I - 1) / FT_I, (g.nj + FT_J - 1) / FT_J, (g.nk + FT_KC - 1) / FT_KC);
+  int n = ((g.ni + FT_I - 1) / FT_I) * ((g.nj + FT_J - 1) / FT_J) * ((g.nk + FT_KC - 1) / FT_KC);
+  return dim3(((n + NUM_XCD - 1) / NUM_XCD) * NUM_XCD);
 }
 
 /* b_c(I,J,K) = (1/8) sum over the 8 children of (b - A x); the coarse x is
  * zeroed (interior and ghosts) by the caller. */
 template <typename T>
 __global__ __launch_bounds__(256) void kResidualRestrict(const T *__restrict__ x,
-    const T *__restrict__ b, Geo gf, Coef<T> cf, T *__restrict__ bc, Geo gc)
+    const T *__restrict__ b, Geo gf, Coef<T> cf, T *__restrict__ bc, Geo gc, int gridGx, int gridGy)
 {
-  int I = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int J = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  int K = blockIdx.z + 1;
+  int bx, by, bz;
+  if (!xcdBlock(gridGx, gridGy, gc.nk, bx, by, bz)) return;
+  int I = bx * blockDim.x + threadIdx.x + 1;
+  int J = by * blockDim.y + threadIdx.y + 1;
+  int K = bz + 1;
   if (I > gc.ni || J > gc.nj) return;
   T s = 0;
 #pragma unroll
@@ -428,12 +474,14 @@ __global__ __launch_bounds__(1024) void kCoarseDct(T *__restrict__ x, const T *_
 /* slot 0: sum r^2 (r = rhs - A p), slot 1: sum rhs^2; optional FP32 copy of r */
 __global__ __launch_bounds__(256) void kResidualNorm(const double *__restrict__ p,
     const double *__restrict__ rhs, Geo g, Coef<double> cf, float *__restrict__ r32,
-    double *partials, DevScalars *sc)
+    double *partials, int gridGx, int gridGy, int gridGz)
 {
   __shared__ double lds[4];
-  int i  = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int j  = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  int k0 = blockIdx.z * GKCH + 1;
+  int bx, by, bz;
+  if (!xcdBlock(gridGx, gridGy, gridGz, bx, by, bz)) return;
+  int i  = bx * blockDim.x + threadIdx.x + 1;
+  int j  = by * blockDim.y + threadIdx.y + 1;
+  int k0 = bz * GKCH + 1;
   double a0 = 0.0, a1 = 0.0;
   if (i <= g.ni && j <= g.nj) {
     int k1 = min(k0 + GKCH, g.nk + 1);
@@ -448,7 +496,7 @@ __global__ __launch_bounds__(256) void kResidualNorm(const double *__restrict__ 
   }
   a0 = blockSum(a0, lds);
   a1 = blockSum(a1, lds);
-  unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  unsigned bid = bx + gridGx * (by + gridGy * bz);
   if (threadIdx.x == 0 && threadIdx.y == 0) {
     partials[2 * bid]     = a0;
     partials[2 * bid + 1] = a1;
@@ -499,11 +547,14 @@ __global__ __launch_bounds__(256) void kSubtractRhsMean(double *__restrict__ rhs
 }
 
 /* p += e (FP32 correction), face ghosts of p refreshed */
-__global__ __launch_bounds__(256) void kCorrect64(double *__restrict__ p, const float *__restrict__ e, Geo g)
+__global__ __launch_bounds__(256) void kCorrect64(double *__restrict__ p, const float *__restrict__ e, Geo g,
+    int gridGx, int gridGy)
 {
-  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  int k = blockIdx.z + 1;
+  int bx, by, bz;
+  if (!xcdBlock(gridGx, gridGy, g.nk, bx, by, bz)) return;
+  int i = bx * blockDim.x + threadIdx.x + 1;
+  int j = by * blockDim.y + threadIdx.y + 1;
+  int k = bz + 1;
   if (i > g.ni || j > g.nj) return;
   long long c = g.at(i, j, k);
   double v    = p[c] + (double)e[c];
@@ -606,11 +657,11 @@ template <typename T> static void vcycle(GmgState *G, LevelT<T> *L, int l)
   fill<T>(C.x, C.g.n, (T)0);
   if (fine)
     NUSIF_TIMED(sizeof(T) == 8 ? "gmg_restrict64_l0" : "gmg_restrict32_l0",
-        (kResidualRestrict<T><<<cellGrid(C.g), dim3(GTX, GTY), 0, nusifStream()>>>(
-            L[l].x, L[l].b, L[l].g, L[l].cf, C.b, C.g)));
+        (kResidualRestrict<T><<<xcdGrid(cellGrid(C.g)), dim3(GTX, GTY), 0, nusifStream()>>>(
+            L[l].x, L[l].b, L[l].g, L[l].cf, C.b, C.g, cellGrid(C.g).x, cellGrid(C.g).y)));
   else
-    kResidualRestrict<T><<<cellGrid(C.g), dim3(GTX, GTY), 0, nusifStream()>>>(
-        L[l].x, L[l].b, L[l].g, L[l].cf, C.b, C.g);
+    kResidualRestrict<T><<<xcdGrid(cellGrid(C.g)), dim3(GTX, GTY), 0, nusifStream()>>>(
+        L[l].x, L[l].b, L[l].g, L[l].cf, C.b, C.g, cellGrid(C.g).x, cellGrid(C.g).y);
   vcycle(G, L, l + 1);
   if (fine)
     NUSIF_TIMED(sizeof(T) == 8 ? "gmg_prolong64_l0" : "gmg_prolong32_l0",
@@ -726,8 +777,8 @@ void gmgFree(GmgState *G)
 static double residualNorm(NusifCtx *c, GmgState *G, double *p, const double *rhs, float *r32, double *nb)
 {
   dim3 ng = normGrid(c->geo);
-  NUSIF_TIMED("gmg_residual64", (kResidualNorm<<<ng, dim3(GTX, GTY), 0, nusifStream()>>>(
-                                    p, rhs, c->geo, G->cf0, r32, c->partials, c->scal)));
+  NUSIF_TIMED("gmg_residual64", (kResidualNorm<<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(
+                                    p, rhs, c->geo, G->cf0, r32, c->partials, ng.x, ng.y, ng.z)));
   kFoldNorms<<<1, 256, 0, nusifStream()>>>(c->partials, (int)(ng.x * ng.y * ng.z), c->scal);
   HIP_CHECK(hipMemcpyAsync(c->scalHost->norm2, c->scal->norm2, 2 * sizeof(double),
       hipMemcpyDeviceToHost, nusifStream()));
@@ -777,7 +828,8 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
     while (nb > 0.0 && rel > G->tol && cycles < G->maxCycles) {
       fill<float>(L0.x, g.n, 0.0f);
       vcycle<float>(G, G->L32, 0);
-      NUSIF_TIMED("gmg_correct", (kCorrect64<<<cellGrid(g), dim3(GTX, GTY), 0, nusifStream()>>>(p, L0.x, g)));
+      NUSIF_TIMED("gmg_correct", (kCorrect64<<<xcdGrid(cellGrid(g)), dim3(GTX, GTY), 0, nusifStream()>>>(
+                                     p, L0.x, g, cellGrid(g).x, cellGrid(g).y)));
       cycles++;
       rn  = residualNorm(c, G, p, rhs, L0.b, &nb);
       rel = rn / nb;

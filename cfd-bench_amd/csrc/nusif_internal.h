@@ -53,6 +53,31 @@ static inline Geo makeGeo(int ni, int nj, int nk)
   return g;
 }
 
+/* XCD-aware workgroup order. MI355X deals workgroup ids round-robin over its
+ * 8 XCDs (each with its own L2). Kernels over a (gx, gy, gz) block grid are
+ * launched 1-D and give XCD x the contiguous block range
+ * [x * ceil(n/8), (x+1) * ceil(n/8)), x fastest: blocks that share stencil
+ * halos then share an L2. */
+#define NUM_XCDS 8
+static inline dim3 xcdGrid(unsigned gx, unsigned gy, unsigned gz)
+{
+  unsigned n = gx * gy * gz;
+  return dim3(((n + NUM_XCDS - 1) / NUM_XCDS) * NUM_XCDS);
+}
+static inline dim3 xcdGrid(dim3 g) { return xcdGrid(g.x, g.y, g.z); }
+/* Logical block (bx, by, bz) of this workgroup; false for surplus ids. */
+__device__ inline bool xcdBlock(int gx, int gy, int gz, int &bx, int &by, int &bz)
+{
+  int n   = gx * gy * gz;
+  int per = (n + NUM_XCDS - 1) / NUM_XCDS;
+  int t   = (int)(blockIdx.x % NUM_XCDS) * per + (int)(blockIdx.x / NUM_XCDS);
+  if (t >= n) return false;
+  bx = t % gx;
+  by = (t / gx) % gy;
+  bz = t / (gx * gy);
+  return true;
+}
+
 /* Options carried next to a Parameter (nusifSetOption / extra .par keys). */
 enum SolverKind { SOLVER_DEFAULT = 0, SOLVER_RB, SOLVER_MG, SOLVER_GMG };
 struct NusifOptions {
