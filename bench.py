@@ -55,6 +55,19 @@ TIMERS = ["gmg_smooth64_l0", "gmg_smooth32_l0", "gmg_restrict32_l0", "gmg_prolon
           "computeRHS", "adaptUV", "absmax"]
 
 
+def measured_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` at n^3 from the committed rocprofv3
+    counter summary (profiles/), or None when it was not measured."""
+    path = os.path.join(ROOT, "profiles", "sweep32_cavity256_hbm_traffic.md")
+    if n != 256 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+            return float(parts[2])
+    return None
+
+
 def run_mode(lib, mode, n, warmup, steps):
     case = A.Case(lib, options=mode_options(mode), **cavity_params(n))
     trace = (C.c_double * (4 * max(warmup, steps, 1)))()
@@ -128,6 +141,8 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-compare", action="store_true", help="skip the fp64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
+                    help="only time a device-to-device copy of MB megabytes (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         sys.exit("bench.py: this build runs on one GPU only (the solver is a single-domain "
@@ -137,6 +152,12 @@ def main():
     if lib.nusifDeviceName(name, 256) < 1:
         sys.exit("bench.py: no GPU visible to the HIP library")
     log("[bench] device", name.value.decode())
+    if a.copy_probe:
+        nbytes = a.copy_probe * 1 << 20
+        ms = lib.nusifCopyProbe(nbytes, 20)
+        print(json.dumps({"copy_bytes": nbytes, "ms_per_copy": ms,
+                          "GBs_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+        return
     n = a.size
     cells = float(n) ** 3
     case, res = run_mode(lib, a.mode, n, a.warmup, a.steps)
@@ -159,7 +180,8 @@ def main():
     if k:
         ach = k["bytes_per_launch"] / (k["us_per_launch"] * 1e-6) / 1e9
         out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
-                           "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                           "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                           "traffic": measured_traffic(dom, n),
                            "us_per_launch": k["us_per_launch"]}
     out["kernels"] = {t: {"us": round(v["us_per_launch"], 2), "launches": v["launches"],
                           "GBs": round(v["bytes_per_launch"] / (v["us_per_launch"] * 1e-6) / 1e9, 1)}

@@ -377,3 +377,30 @@ extern "C" double nusifKernelBytes(Discretization *d, const char *kernel)
     if (strcmp(kernel, e.name) == 0) return e.perCell * (e.padded ? Npad : N);
   return 0.0;
 }
+
+/* Device-to-device copy of `bytes` bytes, `reps` times, on the library
+ * stream: a plain copy of known size, used to calibrate HBM byte counters
+ * and as the achievable-bandwidth reference. Returns ms per copy. */
+extern "C" double nusifCopyProbe(size_t bytes, int reps)
+{
+  void *a = nullptr, *b = nullptr;
+  HIP_CHECK(hipMalloc(&a, bytes));
+  HIP_CHECK(hipMalloc(&b, bytes));
+  HIP_CHECK(hipMemsetAsync(a, 1, bytes, nusifStream()));
+  HIP_CHECK(hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, nusifStream()));
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  HIP_CHECK(hipEventRecord(e0, nusifStream()));
+  for (int r = 0; r < reps; r++)
+    HIP_CHECK(hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, nusifStream()));
+  HIP_CHECK(hipEventRecord(e1, nusifStream()));
+  HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  HIP_CHECK(hipEventDestroy(e0));
+  HIP_CHECK(hipEventDestroy(e1));
+  HIP_CHECK(hipFree(a));
+  HIP_CHECK(hipFree(b));
+  return reps > 0 ? ms / reps : 0.0;
+}

@@ -130,6 +130,7 @@ def _declare_optional(lib):
         "nusifTimerReset": ([], None),
         "nusifKernelBytes": ([C.POINTER(Discretization), C.c_char_p], C.c_double),
         "nusifDeviceSynchronize": ([], None),
+        "nusifCopyProbe": ([C.c_size_t, C.c_int], C.c_double),
         "nusifRunSteps": ([C.POINTER(Discretization), C.POINTER(Solver), C.c_int,
                            C.POINTER(C.c_double)], C.c_int),
         # oracle extras

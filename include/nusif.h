@@ -81,6 +81,11 @@ void nusifTimerReset(void);
  * (each array element read once and/or written once). */
 double nusifKernelBytes(Discretization *d, const char *kernel);
 
+/* Device-to-device copy of `bytes` bytes repeated `reps` times on the library
+ * stream; returns ms per copy (calibration of HBM byte counters and the
+ * achievable-bandwidth reference). */
+double nusifCopyProbe(size_t bytes, int reps);
+
 /* Runs the reference time loop (src/main.c:60-88) for `nsteps` steps entirely
  * inside the library: the same calls in the same order, without the Python
  * or host-driver overhead between them. trace (may be NULL) receives
