@@ -28,7 +28,7 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-ffp-contr
              "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
 C_FLAGS = ["-O2", "-fPIC", "-std=gnu99", "-Wall", "-Wno-unused-result"]
 
-HIP_SOURCES = ["context.hip", "discretization.hip", "sor.hip", "gmg.hip"]
+HIP_SOURCES = ["context.hip", "discretization.hip", "sor.hip", "gmg.hip", "unet.hip", "warmstart.hip"]
 SOLVER_VARIANTS = {"rb": "SOLVER_RB", "mg": "SOLVER_MG"}
 
 

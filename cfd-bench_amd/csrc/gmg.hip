@@ -31,6 +31,7 @@
 
 #include "nusif_internal.h"
 #include "reduce.h"
+#include "warmstart.h"
 
 #define MAXLEV 12
 #define GTX 64
@@ -804,6 +805,25 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
   double nb = 0.0;
   int cycles = 0;
   double rel = 0.0;
+  WarmStart *ws = sc->ws;
+  if (ws) {
+    /* r0 = rhs - A p_prev (FP32 copy for the network input) */
+    double nb0 = 0.0;
+    double rn0 = residualNorm(c, G, p, rhs, warmstartResidualBuffer(ws), &nb0);
+    warmstartCaptureBegin(ws, p, rhs, g);
+    if (warmstartNet(ws)) {
+      warmstartGuess(ws, p, rhs, c->pAlt);
+      double nbg = 0.0;
+      double rng = residualNorm(c, G, c->pAlt, rhs, nullptr, &nbg);
+      sc->warmUsed = rng < rn0 ? 1 : 0;
+      if (sc->warmUsed) { /* start from the network's guess */
+        double *t = c->p;
+        c->p      = c->pAlt;
+        c->pAlt   = t;
+        p         = c->p;
+      }
+    }
+  }
   if (!G->mixed) {
     LevelT<double> &L0 = G->L64[0];
     L0.x    = p;
@@ -835,12 +855,10 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
       rel = rn / nb;
     }
   }
+  if (ws) warmstartCaptureEnd(ws, c->p);
   sc->lastIterations = cycles;
   sc->lastRel        = rel;
   return rel;
 }
 
-struct UnetModel {
-  int dummy;
-};
-void unetFree(UnetModel *m) { delete m; }
+

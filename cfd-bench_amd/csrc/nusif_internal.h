@@ -111,8 +111,9 @@ struct DevScalars {
   double norm2[4];              /* gmg norms */
 };
 
-struct GmgState; /* gmg.hip */
+struct GmgState;  /* gmg.hip */
 struct UnetModel; /* unet.hip */
+struct WarmStart; /* warmstart.hip */
 
 /* One Discretization's device state. */
 struct NusifCtx {
@@ -141,7 +142,7 @@ struct NusifSolverCtx {
   int warmUsed;
   int predicted; /* iterations of the previous solve, sizes the launch batches */
   GmgState *gmg;
-  UnetModel *unet;
+  WarmStart *ws; /* learned warm start / training-pair capture (gmg only) */
 };
 
 hipStream_t nusifStream(void);
@@ -183,7 +184,6 @@ double rbSolve(NusifSolverCtx *sc, double *p, const double *rhs, double **pOut);
 double mgRefSolve(NusifSolverCtx *sc, double *p, const double *rhs);
 double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhs);
 void gmgFree(GmgState *g);
-void unetFree(UnetModel *m);
 
 /* Shared by rb / mg / gmg: colour sweep, pressure ghosts */
 struct SweepCoef {

@@ -7,6 +7,8 @@
 #include <string.h>
 
 #include "nusif_internal.h"
+#include "unet.h"
+#include "warmstart.h"
 
 #ifndef NUSIF_DEFAULT_SOLVER
 #define NUSIF_DEFAULT_SOLVER SOLVER_RB
@@ -41,6 +43,14 @@ extern "C" void initSolver(Solver *s, Discretization *d, Parameter *p)
   if (sc->kind == SOLVER_MG && commIsMaster(s->comm))
     printf("Using Multigrid solver with %d levels\n", s->levels);
   if (sc->kind == SOLVER_GMG) sc->gmg = gmgCreate(sc, p);
+  if (sc->dctx->opt.warmstart) {
+    if (sc->kind != SOLVER_GMG) NUSIF_FATAL("warmstart unet needs solver gmg");
+    const char *path = sc->dctx->opt.unetWeights;
+    sc->ws           = warmstartCreate(sc->dctx, nullptr, false);
+    UnetModel *net   = path[0] ? unetLoad(path, warmstartExtent(sc->ws)) : nullptr;
+    if (!net) NUSIF_FATAL("warmstart unet: cannot read weights '%s' (option unet_weights)", path);
+    warmstartSetNet(sc->ws, net);
+  }
   nusifRegisterSolver(sc);
 }
 
@@ -49,7 +59,10 @@ extern "C" void nusifFreeSolver(Solver *s)
   NusifSolverCtx *sc = nusifSolverCtxOf(s);
   HIP_CHECK(hipStreamSynchronize(nusifStream()));
   if (sc->gmg) gmgFree(sc->gmg);
-  if (sc->unet) unetFree(sc->unet);
+  if (sc->ws) {
+    unetFree(warmstartNet(sc->ws));
+    warmstartFree(sc->ws);
+  }
   nusifUnregisterSolver(sc);
   delete sc;
 }

@@ -1,0 +1,30 @@
+/* Host interface of the warm-start U-Net (unet.hip). */
+#ifndef NUSIF_UNET_H
+#define NUSIF_UNET_H
+#include <stddef.h>
+#include <stdint.h>
+
+#define UNET_MAXL 4
+
+struct UnetModel;
+UnetModel *unetNew(int L, const int *ch, int cin, int n);
+void unetFree(UnetModel *m);
+void unetInit(UnetModel *m, unsigned seed);
+size_t unetParamCount(const UnetModel *m);
+void unetSetParams(UnetModel *m, const float *hostParams);
+void unetGetParams(UnetModel *m, float *hostParams);
+void unetGetGrads(UnetModel *m, float *hostGrads);
+void unetZeroGrads(UnetModel *m);
+void unetSyncWeights(UnetModel *m);
+/* x: n^3 x cin bf16 (channels-last); the prediction stays on the device */
+void unetForward(UnetModel *m, const uint16_t *x);
+const float *unetOutput(UnetModel *m);
+/* after unetForward: loss against t (n^3 FP32), gradients added into G */
+double unetBackward(UnetModel *m, const uint16_t *x, const float *t, bool wantLoss);
+double unetLastLoss(UnetModel *m);
+void unetAdam(UnetModel *m, float lr, int batch);
+int unetSave(UnetModel *m, const char *path);
+UnetModel *unetLoad(const char *path, int n);
+int unetLevels(const UnetModel *m);
+int unetExtent(const UnetModel *m);
+#endif

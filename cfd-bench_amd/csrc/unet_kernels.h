@@ -1,0 +1,207 @@
+/* Kernels of the warm-start U-Net: BF16 implicit-GEMM 3x3x3 convolutions on
+ * gfx950 MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulation), their data and
+ * weight gradients, and the elementwise layers. Tensors are channels-last
+ * ([z][y][x][c], c a multiple of 8), cubes of extent n, 'same' zero padding.
+ *
+ * MFMA 16x16x32 lane layout (checked on the device): lane l holds
+ *   A[row l%16][k 8*(l/16) .. +7],  B[k 8*(l/16) .. +7][col l%16],
+ *   D[rows 4*(l/16) .. +3][col l%16].
+ * Convolutions map rows to output channels and columns to voxels, so each
+ * lane's four results are four consecutive channels of one voxel (one 8 B
+ * store), and both operands are 16 B contiguous loads: weights [cout][27][cin]
+ * along cin, activations along c. */
+#ifndef NUSIF_UNET_KERNELS_H
+#define NUSIF_UNET_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__device__ __host__ inline uint16_t f2bf(float f)
+{
+  uint32_t u = __builtin_bit_cast(uint32_t, f);
+  if ((u & 0x7f800000u) != 0x7f800000u) u += 0x7fffu + ((u >> 16) & 1u); /* nearest even */
+  return (uint16_t)(u >> 16);
+}
+
+__device__ __host__ inline float bf2f(uint16_t h) { return __builtin_bit_cast(float, (uint32_t)h << 16); }
+
+/* 8 bf16 from 16 contiguous bytes (16 B aligned) */
+__device__ inline bf16x8 ld8(const uint16_t *p) { return __builtin_bit_cast(bf16x8, *(const uint4 *)p); }
+
+/* 8 floats (two 16 B loads) rounded to bf16 */
+__device__ inline bf16x8 ld8(const float *p)
+{
+  float4 a = *(const float4 *)p, b = *(const float4 *)(p + 4);
+  uint4 r;
+  r.x = (uint32_t)f2bf(a.x) | ((uint32_t)f2bf(a.y) << 16);
+  r.y = (uint32_t)f2bf(a.z) | ((uint32_t)f2bf(a.w) << 16);
+  r.z = (uint32_t)f2bf(b.x) | ((uint32_t)f2bf(b.y) << 16);
+  r.w = (uint32_t)f2bf(b.z) | ((uint32_t)f2bf(b.w) << 16);
+  return __builtin_bit_cast(bf16x8, r);
+}
+
+__device__ inline bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0)); }
+
+/* One convolution as C[rows][vox] = sum_k A[rows][k] B[k][vox]:
+ *   forward:  rows = cout, A = W[cout][27][cin], B = in[vox + off(t)][cin]
+ *   dgrad:    rows = cin,  A = Wt[cin][27][cout], B = dOut[vox - off(t)][cout]
+ * The B operand may be the concatenation of two tensors (channels c0 + c1);
+ * the result rows may be split over two outputs (r0 + r1 rows). */
+template <typename InT, typename OutT> struct ConvArgs {
+  const InT *in0, *in1;   /* B operand tensors, c0 and c1 channels (c1 = 0: none) */
+  int c0, c1;
+  const uint16_t *A;      /* [rows][27][c0 + c1] bf16 */
+  const float *bias;      /* [rows] or null */
+  OutT *out0, *out1;      /* [vox][r0], [vox][r1] */
+  int r0, r1;
+  int n;                  /* cube extent */
+  int tapSign;            /* +1 forward, -1 data gradient */
+  int relu;               /* ReLU on the result */
+  int accumulate;         /* out += result (FP32 outputs) */
+};
+
+template <typename T> __device__ inline void storeOut(T *p, float v, int accumulate);
+template <> __device__ inline void storeOut<float>(float *p, float v, int accumulate)
+{
+  *p = accumulate ? *p + v : v;
+}
+template <> __device__ inline void storeOut<uint16_t>(uint16_t *p, float v, int) { *p = f2bf(v); }
+
+/* Workgroup: 4 waves; each wave owns MB blocks of 16 voxels and NB blocks of
+ * 16 rows (grid.y walks the row tiles). */
+template <typename InT, typename OutT, int NB, int MB>
+__global__ __launch_bounds__(256) void kConvMfma(ConvArgs<InT, OutT> a)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n = a.n, nvox = n * n * n;
+  const int cin = a.c0 + a.c1, g8 = cin >> 3, G = 27 * g8;
+  const int rows = a.r0 + a.r1;
+  const int row0 = blockIdx.y * (16 * NB);
+  const int vox0 = (blockIdx.x * 4 + wave) * (16 * MB);
+  const int lr = lane & 15, lg = lane >> 4;
+
+  int vz[MB], vy[MB], vx[MB];
+  bool vin[MB];
+#pragma unroll
+  for (int mb = 0; mb < MB; mb++) {
+    int v   = vox0 + mb * 16 + lr;
+    vin[mb] = v < nvox;
+    int vc  = vin[mb] ? v : 0;
+    vx[mb]  = vc % n;
+    vy[mb]  = (vc / n) % n;
+    vz[mb]  = vc / (n * n);
+  }
+  floatx4 acc[NB][MB];
+#pragma unroll
+  for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++) acc[nb][mb] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  for (int s = 0; s < G; s += 4) {
+    int gi  = s + lg;
+    bool gv = gi < G;
+    int t = gv ? gi / g8 : 0, c8 = gv ? gi - t * g8 : 0;
+    int dz = (t / 9 - 1) * a.tapSign, dy = ((t / 3) % 3 - 1) * a.tapSign, dx = (t % 3 - 1) * a.tapSign;
+    bf16x8 fa[NB], fb[MB];
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) {
+      int r  = row0 + nb * 16 + lr;
+      fa[nb] = (gv && r < rows) ? ld8(a.A + ((size_t)r * 27 + t) * cin + c8 * 8) : zero8();
+    }
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++) {
+      int z = vz[mb] + dz, y = vy[mb] + dy, x = vx[mb] + dx;
+      bool ok = gv && vin[mb] && z >= 0 && z < n && y >= 0 && y < n && x >= 0 && x < n;
+      if (ok) {
+        size_t vv = ((size_t)z * n + y) * n + x;
+        int cc    = c8 * 8;
+        fb[mb]    = cc < a.c0 ? ld8(a.in0 + vv * a.c0 + cc) : ld8(a.in1 + vv * a.c1 + (cc - a.c0));
+      } else {
+        fb[mb] = zero8();
+      }
+    }
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+      for (int mb = 0; mb < MB; mb++)
+        acc[nb][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[nb], fb[mb], acc[nb][mb], 0, 0, 0);
+  }
+#pragma unroll
+  for (int mb = 0; mb < MB; mb++) {
+    int v = vox0 + mb * 16 + lr;
+    if (!vin[mb]) continue;
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        int r = row0 + nb * 16 + 4 * lg + q;
+        if (r >= rows) continue;
+        float val = acc[nb][mb][q] + (a.bias ? a.bias[r] : 0.f);
+        if (a.relu) val = val > 0.f ? val : 0.f;
+        if (r < a.r0) storeOut<OutT>(a.out0 + (size_t)v * a.r0 + r, val, a.accumulate);
+        else storeOut<OutT>(a.out1 + (size_t)v * a.r1 + (r - a.r0), val, a.accumulate);
+      }
+  }
+}
+
+/* Weight gradient of one tap: dW[n][t][c] += sum_v dOut[v][n] * in[v + off(t)][c].
+ * Workgroup: 32 output channels x 32 input channels of one tap over a chunk
+ * of voxels; 32-voxel slices of both operands are staged transposed in LDS
+ * so each MFMA fragment is one 16 B LDS read. Each wave owns one 16 x 16
+ * block; partial sums go to dW with FP32 atomics (split over voxel chunks). */
+template <typename GT>
+__global__ __launch_bounds__(256) void kConvWgrad(const GT *__restrict__ dOut, int cout,
+    const uint16_t *__restrict__ in0, const uint16_t *__restrict__ in1, int c0, int c1, int n,
+    float *__restrict__ dW, int voxPerChunk)
+{
+  __shared__ __attribute__((aligned(16))) uint16_t sA[32][40]; /* [cout][vox], padded rows */
+  __shared__ __attribute__((aligned(16))) uint16_t sB[32][40]; /* [cin][vox] */
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int cin = c0 + c1, nvox = n * n * n;
+  const int t   = blockIdx.y % 27;
+  const int nT  = blockIdx.y / 27;          /* cout tile */
+  const int cT  = blockIdx.x;               /* cin tile */
+  const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+  const int v0 = blockIdx.z * voxPerChunk, v1 = min(v0 + voxPerChunk, nvox);
+  const int nb = wave >> 1, cb = wave & 1;
+  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (int vs = v0; vs < v1; vs += 32) {
+    /* stage: 32 vox x 32 channels of each operand, 4 elements per thread */
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int e = tid + 256 * q;       /* 0..1023 */
+      int vv = e >> 5, ch = e & 31; /* voxel in slice, channel in tile */
+      int v  = vs + vv;
+      float ga = 0.f;
+      int co = nT * 32 + ch;
+      if (v < v1 && co < cout) ga = (float)dOut[(size_t)v * cout + co];
+      sA[ch][vv] = f2bf(ga);
+      uint16_t gb = 0;
+      int ci = cT * 32 + ch;
+      if (v < v1 && ci < cin) {
+        int x = v % n + dx, y = (v / n) % n + dy, z = v / (n * n) + dz;
+        if (z >= 0 && z < n && y >= 0 && y < n && x >= 0 && x < n) {
+          size_t w = ((size_t)z * n + y) * n + x;
+          gb       = ci < c0 ? in0[w * c0 + ci] : in1[w * c1 + (ci - c0)];
+        }
+      }
+      sB[ch][vv] = gb;
+    }
+    __syncthreads();
+    bf16x8 fa = ld8(&sA[nb * 16 + (lane & 15)][8 * (lane >> 4)]);
+    bf16x8 fb = ld8(&sB[cb * 16 + (lane & 15)][8 * (lane >> 4)]);
+    acc       = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb, acc, 0, 0, 0);
+    __syncthreads();
+  }
+  int ci = cT * 32 + cb * 16 + (lane & 15);
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    int co = nT * 32 + nb * 16 + 4 * (lane >> 4) + q;
+    if (co < cout && ci < cin) atomicAdd(&dW[((size_t)co * 27 + t) * cin + ci], acc[q]);
+  }
+}
+
+#endif

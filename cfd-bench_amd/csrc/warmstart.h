@@ -1,0 +1,26 @@
+/* Learned warm start of the gmg solve (warmstart.hip). */
+#ifndef NUSIF_WARMSTART_H
+#define NUSIF_WARMSTART_H
+#include <stdint.h>
+
+#include <vector>
+
+#include "nusif_internal.h"
+
+struct WarmStart;
+struct UnetModel;
+WarmStart *warmstartCreate(NusifCtx *c, UnetModel *net, bool capture);
+void warmstartFree(WarmStart *w);
+int warmstartLevel(const WarmStart *w);
+int warmstartExtent(const WarmStart *w);
+void warmstartSetNet(WarmStart *w, UnetModel *net);
+UnetModel *warmstartNet(WarmStart *w);
+bool warmstartCapturing(const WarmStart *w);
+/* FP32 level-0 buffer the caller fills with r0 = rhs - A p before the guess */
+float *warmstartResidualBuffer(WarmStart *w);
+void warmstartGuess(WarmStart *w, const double *p, const double *rhs, double *guess);
+void warmstartCaptureBegin(WarmStart *w, const double *p, const double *rhs, const Geo &g0);
+void warmstartCaptureEnd(WarmStart *w, const double *p);
+size_t warmstartSamples(const WarmStart *w);
+void warmstartTakeSamples(WarmStart *w, std::vector<uint16_t *> &xs, std::vector<float *> &ts);
+#endif

@@ -1,0 +1,481 @@
+/* Learned warm start of the gmg pressure solve (no counterpart in the
+ * reference, whose initial guess is the previous step's p: p persists
+ * across steps, reference src/main.c:71).
+ *
+ * Inputs, on the network grid (multigrid level Ln, the first whose extent is
+ * <= 64; restriction = average of the 8 children):
+ *   channel 0: r0 / s_r,  r0 = rhs - A p_prev (the previous pressure against
+ *              the new right-hand side), s_r = rms(r0)
+ *   channel 1: rhs / s_b, s_b = rms(rhs)
+ * Output y, rescaled: dp = y * s_r * (n h_Ln / pi)^2 (the inverse Laplacian of
+ * the smoothest mode on the network grid of n cells), prolongated (trilinear) to the
+ * finest grid; guess = p_prev + dp. The guess replaces p_prev only when its
+ * residual ||rhs - A guess|| is smaller than ||r0||; otherwise the solve
+ * starts from p_prev. Either way the multigrid iterates to the same
+ * tolerance, so the answer's accuracy does not depend on the network.
+ *
+ * Training pairs come from the solver itself: with capture on, each solve
+ * stores the input above and the target (p_solved - p_prev) restricted to
+ * level Ln, divided by the same scale. */
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "nusif_internal.h"
+#include "reduce.h"
+#include "unet.h"
+#include "unet_kernels.h"
+#include "warmstart.h"
+
+#define WTX 64
+#define WTY 4
+
+struct WsLevel {
+  Geo g;
+  float *r, *b, *d; /* residual, rhs, correction (padded, FP32) */
+};
+
+struct WarmStart {
+  UnetModel *net;
+  int Ln;                 /* network level */
+  int n;                  /* network extent */
+  double h2;              /* output scale (n h_Ln / pi)^2, x spacing */
+  WsLevel lv[12];
+  uint16_t *x;            /* network input n^3 x 8 */
+  double *pPrev;          /* FP64 copy of p for capture */
+  double *scal;           /* [0] sum r^2, [1] sum b^2 on level Ln; [2] used flag */
+  double *partials;
+  /* capture */
+  bool capture;
+  std::vector<uint16_t *> dsX;
+  std::vector<float *> dsT;
+};
+
+/* ---- kernels --------------------------------------------------------------- */
+template <typename Tin>
+__global__ __launch_bounds__(256) void kRestrictAvg(const Tin *__restrict__ f, Geo gf, float *__restrict__ c, Geo gc)
+{
+  int I = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int J = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int K = blockIdx.z + 1;
+  if (I > gc.ni || J > gc.nj) return;
+  double s = 0.0;
+  for (int e = 0; e < 8; e++)
+    s += (double)f[gf.at(2 * I - 1 + (e & 1), 2 * J - 1 + ((e >> 1) & 1), 2 * K - 1 + (e >> 2))];
+  c[gc.at(I, J, K)] = (float)(0.125 * s);
+}
+
+__global__ __launch_bounds__(256) void kCast(const double *__restrict__ a, float *__restrict__ b, Geo g)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int k = blockIdx.z + 1;
+  if (i > g.ni || j > g.nj) return;
+  b[g.at(i, j, k)] = (float)a[g.at(i, j, k)];
+}
+
+/* per-block partial sums of r^2 and b^2 over the interior of level Ln */
+__global__ __launch_bounds__(256) void kSumSq2(const float *r, const float *b, Geo g, double *partials)
+{
+  __shared__ double lds[4];
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  double a0 = 0.0, a1 = 0.0;
+  if (i <= g.ni && j <= g.nj)
+    for (int k = 1; k <= g.nk; k++) {
+      long long c = g.at(i, j, k);
+      a0 += (double)r[c] * r[c];
+      a1 += (double)b[c] * b[c];
+    }
+  a0 = blockSum(a0, lds);
+  a1 = blockSum(a1, lds);
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    unsigned bid             = blockIdx.x + gridDim.x * blockIdx.y;
+    partials[2 * bid]     = a0;
+    partials[2 * bid + 1] = a1;
+  }
+}
+
+__global__ __launch_bounds__(256) void kFoldSumSq2(const double *partials, int nb, double *out)
+{
+  __shared__ double lds[4];
+  double s0 = foldSum(partials, nb, 2, 0, lds), s1 = foldSum(partials, nb, 2, 1, lds);
+  if (threadIdx.x == 0) {
+    out[0] = s0;
+    out[1] = s1;
+  }
+}
+
+/* network input: channels 0, 1 as described above, 2..7 zero */
+__global__ void kPack(const float *r, const float *b, Geo g, const double *scal, uint16_t *x)
+{
+  size_t nv = (size_t)g.ni * g.nj * g.nk;
+  double cnt = (double)nv;
+  float sr = (float)sqrt(scal[0] / cnt), sb = (float)sqrt(scal[1] / cnt);
+  float ir = sr > 0.f ? 1.f / sr : 0.f, ib = sb > 0.f ? 1.f / sb : 0.f;
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (size_t)gridDim.x * blockDim.x) {
+    int i = (int)(v % g.ni) + 1, j = (int)((v / g.ni) % g.nj) + 1, k = (int)(v / ((size_t)g.ni * g.nj)) + 1;
+    long long c = g.at(i, j, k);
+    uint4 o;
+    o.x = (uint32_t)f2bf(r[c] * ir) | ((uint32_t)f2bf(b[c] * ib) << 16);
+    o.y = o.z = o.w = 0;
+    *(uint4 *)(x + v * 8) = o;
+  }
+}
+
+/* network output -> correction on level Ln (interior + face ghosts) */
+__global__ void kUnpack(const float *y, Geo g, const double *scal, double h2, float *d)
+{
+  size_t nv = (size_t)g.ni * g.nj * g.nk;
+  float sr  = (float)sqrt(scal[0] / (double)nv);
+  float f   = (float)(sr * h2);
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (size_t)gridDim.x * blockDim.x) {
+    int i = (int)(v % g.ni) + 1, j = (int)((v / g.ni) % g.nj) + 1, k = (int)(v / ((size_t)g.ni * g.nj)) + 1;
+    d[g.at(i, j, k)] = y[v] * f;
+  }
+}
+
+/* training target: (p - pPrev) restricted, / (s_r h2); level-0 difference */
+__global__ __launch_bounds__(256) void kDiff(const double *p, const double *q, Geo g, float *d)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int k = blockIdx.z + 1;
+  if (i > g.ni || j > g.nj) return;
+  long long c = g.at(i, j, k);
+  d[c]        = (float)(p[c] - q[c]);
+}
+
+__global__ void kTarget(const float *d, Geo g, const double *scal, double h2, float *t)
+{
+  size_t nv = (size_t)g.ni * g.nj * g.nk;
+  float sr  = (float)sqrt(scal[0] / (double)nv);
+  float f   = sr > 0.f ? (float)(1.0 / (sr * h2)) : 0.f;
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (size_t)gridDim.x * blockDim.x) {
+    int i = (int)(v % g.ni) + 1, j = (int)((v / g.ni) % g.nj) + 1, k = (int)(v / ((size_t)g.ni * g.nj)) + 1;
+    t[v] = d[g.at(i, j, k)] * f;
+  }
+}
+
+/* cell-centred trilinear x_f = P x_c (overwrite), face ghosts written */
+__global__ __launch_bounds__(256) void kProlongSet(float *xf, Geo gf, const float *xc, Geo gc)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int k = blockIdx.z + 1;
+  if (i > gf.ni || j > gf.nj) return;
+  int I = (i + 1) >> 1, J = (j + 1) >> 1, K = (k + 1) >> 1;
+  int I2 = min(max(I + ((i & 1) ? -1 : 1), 1), gc.ni);
+  int J2 = min(max(J + ((j & 1) ? -1 : 1), 1), gc.nj);
+  int K2 = min(max(K + ((k & 1) ? -1 : 1), 1), gc.nk);
+  const float a = 0.75f, q = 0.25f;
+  float c11 = a * xc[gc.at(I, J, K)] + q * xc[gc.at(I2, J, K)];
+  float c21 = a * xc[gc.at(I, J2, K)] + q * xc[gc.at(I2, J2, K)];
+  float c12 = a * xc[gc.at(I, J, K2)] + q * xc[gc.at(I2, J, K2)];
+  float c22 = a * xc[gc.at(I, J2, K2)] + q * xc[gc.at(I2, J2, K2)];
+  xf[gf.at(i, j, k)] = a * (a * c11 + q * c21) + q * (a * c12 + q * c22);
+}
+
+/* guess = p + d (FP64), face ghosts of the guess written */
+__global__ __launch_bounds__(256) void kGuess(const double *p, const float *d, Geo g, double *out)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int k = blockIdx.z + 1;
+  if (i > g.ni || j > g.nj) return;
+  long long c = g.at(i, j, k);
+  double v    = p[c] + (double)d[c];
+  out[c]      = v;
+  if (i == 1) out[c - 1] = v;
+  if (i == g.ni) out[c + 1] = v;
+  if (j == 1) out[c - g.sj] = v;
+  if (j == g.nj) out[c + g.sj] = v;
+  if (k == 1) out[c - g.sk] = v;
+  if (k == g.nk) out[c + g.sk] = v;
+}
+
+/* ---- host ------------------------------------------------------------------ */
+static dim3 cells(const Geo &g) { return dim3((g.ni + WTX - 1) / WTX, (g.nj + WTY - 1) / WTY, g.nk); }
+
+static float *falloc(long long n)
+{
+  float *p = nullptr;
+  HIP_CHECK(hipMalloc((void **)&p, (size_t)n * sizeof(float)));
+  HIP_CHECK(hipMemsetAsync(p, 0, (size_t)n * sizeof(float), nusifStream()));
+  return p;
+}
+
+/* first level whose largest extent is <= 64 (extents must stay even) */
+static int networkLevel(const Geo &g0)
+{
+  int ni = g0.ni, nj = g0.nj, nk = g0.nk, l = 0;
+  while ((ni > 64 || nj > 64 || nk > 64) && ni % 2 == 0 && nj % 2 == 0 && nk % 2 == 0) {
+    ni /= 2;
+    nj /= 2;
+    nk /= 2;
+    l++;
+  }
+  return l;
+}
+
+WarmStart *warmstartCreate(NusifCtx *c, UnetModel *net, bool capture)
+{
+  WarmStart *w = new WarmStart();
+  w->net       = net;
+  w->capture   = capture;
+  w->Ln        = networkLevel(c->geo);
+  Geo g        = c->geo;
+  double h     = c->owner->grid.dx;
+  for (int l = 0; l <= w->Ln; l++) {
+    w->lv[l].g = g;
+    w->lv[l].r = falloc(g.n);
+    w->lv[l].b = falloc(g.n);
+    w->lv[l].d = falloc(g.n);
+    if (l < w->Ln) {
+      g = makeGeo(g.ni / 2, g.nj / 2, g.nk / 2);
+      h *= 2.0;
+    }
+  }
+  const Geo &gn = w->lv[w->Ln].g;
+  if (gn.ni != gn.nj || gn.nj != gn.nk)
+    NUSIF_FATAL("warm start: the network needs a cubic grid on its level (got %dx%dx%d)", gn.ni, gn.nj, gn.nk);
+  w->n  = gn.ni;
+  w->h2 = (gn.ni * h / M_PI) * (gn.ni * h / M_PI);
+  HIP_CHECK(hipMalloc((void **)&w->x, (size_t)w->n * w->n * w->n * 8 * sizeof(uint16_t)));
+  HIP_CHECK(hipMalloc((void **)&w->scal, 4 * sizeof(double)));
+  HIP_CHECK(hipMalloc((void **)&w->partials, 65536 * sizeof(double)));
+  HIP_CHECK(hipMalloc((void **)&w->pPrev, (size_t)c->geo.n * sizeof(double)));
+  return w;
+}
+
+void warmstartFree(WarmStart *w)
+{
+  if (!w) return;
+  HIP_CHECK(hipStreamSynchronize(nusifStream()));
+  for (int l = 0; l <= w->Ln; l++) {
+    HIP_CHECK(hipFree(w->lv[l].r));
+    HIP_CHECK(hipFree(w->lv[l].b));
+    HIP_CHECK(hipFree(w->lv[l].d));
+  }
+  HIP_CHECK(hipFree(w->x));
+  HIP_CHECK(hipFree(w->scal));
+  HIP_CHECK(hipFree(w->partials));
+  HIP_CHECK(hipFree(w->pPrev));
+  for (auto p : w->dsX) HIP_CHECK(hipFree(p));
+  for (auto p : w->dsT) HIP_CHECK(hipFree(p));
+  delete w;
+}
+
+int warmstartLevel(const WarmStart *w) { return w->Ln; }
+int warmstartExtent(const WarmStart *w) { return w->n; }
+void warmstartSetNet(WarmStart *w, UnetModel *net) { w->net = net; }
+UnetModel *warmstartNet(WarmStart *w) { return w->net; }
+bool warmstartCapturing(const WarmStart *w) { return w->capture; }
+
+/* level-0 FP32 residual r0 (already in lv[0].r) and the rhs -> level Ln,
+ * norms on Ln, network input x */
+static void buildInput(WarmStart *w, const double *rhs)
+{
+  hipStream_t st = nusifStream();
+  if (w->Ln == 0) {
+    kCast<<<cells(w->lv[0].g), dim3(WTX, WTY), 0, st>>>(rhs, w->lv[0].b, w->lv[0].g);
+  } else {
+    kRestrictAvg<double><<<cells(w->lv[1].g), dim3(WTX, WTY), 0, st>>>(rhs, w->lv[0].g, w->lv[1].b, w->lv[1].g);
+    kRestrictAvg<float><<<cells(w->lv[1].g), dim3(WTX, WTY), 0, st>>>(w->lv[0].r, w->lv[0].g, w->lv[1].r, w->lv[1].g);
+    for (int l = 2; l <= w->Ln; l++) {
+      kRestrictAvg<float><<<cells(w->lv[l].g), dim3(WTX, WTY), 0, st>>>(w->lv[l - 1].b, w->lv[l - 1].g, w->lv[l].b, w->lv[l].g);
+      kRestrictAvg<float><<<cells(w->lv[l].g), dim3(WTX, WTY), 0, st>>>(w->lv[l - 1].r, w->lv[l - 1].g, w->lv[l].r, w->lv[l].g);
+    }
+  }
+  const WsLevel &L = w->lv[w->Ln];
+  dim3 g2((L.g.ni + WTX - 1) / WTX, (L.g.nj + WTY - 1) / WTY);
+  kSumSq2<<<g2, dim3(WTX, WTY), 0, st>>>(L.r, L.b, L.g, w->partials);
+  kFoldSumSq2<<<1, 256, 0, st>>>(w->partials, (int)(g2.x * g2.y), w->scal);
+  kPack<<<1024, 256, 0, st>>>(L.r, L.b, L.g, w->scal, w->x);
+}
+
+/* Network guess into `guess` (FP64, face ghosts set), from p and the r0
+ * already in lv[0].r. */
+void warmstartGuess(WarmStart *w, const double *p, const double *rhs, double *guess)
+{
+  hipStream_t st = nusifStream();
+  buildInput(w, rhs);
+  NUSIF_TIMED("unet_forward", unetForward(w->net, w->x));
+  const WsLevel &L = w->lv[w->Ln];
+  kUnpack<<<1024, 256, 0, st>>>(unetOutput(w->net), L.g, w->scal, w->h2, L.d);
+  for (int l = w->Ln; l > 0; l--)
+    kProlongSet<<<cells(w->lv[l - 1].g), dim3(WTX, WTY), 0, st>>>(w->lv[l - 1].d, w->lv[l - 1].g, w->lv[l].d, w->lv[l].g);
+  kGuess<<<cells(w->lv[0].g), dim3(WTX, WTY), 0, st>>>(p, w->lv[0].d, w->lv[0].g, guess);
+}
+
+float *warmstartResidualBuffer(WarmStart *w) { return w->lv[0].r; }
+
+/* capture, part 1 (before the solve): the input and a copy of p */
+void warmstartCaptureBegin(WarmStart *w, const double *p, const double *rhs, const Geo &g0)
+{
+  if (!w->capture) return;
+  HIP_CHECK(hipMemcpyAsync(w->pPrev, p, (size_t)g0.n * sizeof(double), hipMemcpyDeviceToDevice, nusifStream()));
+  buildInput(w, rhs);
+  uint16_t *x = nullptr;
+  size_t nv   = (size_t)w->n * w->n * w->n;
+  HIP_CHECK(hipMalloc((void **)&x, nv * 8 * sizeof(uint16_t)));
+  HIP_CHECK(hipMemcpyAsync(x, w->x, nv * 8 * sizeof(uint16_t), hipMemcpyDeviceToDevice, nusifStream()));
+  w->dsX.push_back(x);
+}
+
+/* capture, part 2 (after the solve): target from p - pPrev */
+void warmstartCaptureEnd(WarmStart *w, const double *p)
+{
+  if (!w->capture) return;
+  hipStream_t st = nusifStream();
+  kDiff<<<cells(w->lv[0].g), dim3(WTX, WTY), 0, st>>>(p, w->pPrev, w->lv[0].g, w->lv[0].d);
+  for (int l = 1; l <= w->Ln; l++)
+    kRestrictAvg<float><<<cells(w->lv[l].g), dim3(WTX, WTY), 0, st>>>(w->lv[l - 1].d, w->lv[l - 1].g, w->lv[l].d, w->lv[l].g);
+  size_t nv = (size_t)w->n * w->n * w->n;
+  float *t  = nullptr;
+  HIP_CHECK(hipMalloc((void **)&t, nv * sizeof(float)));
+  kTarget<<<1024, 256, 0, st>>>(w->lv[w->Ln].d, w->lv[w->Ln].g, w->scal, w->h2, t);
+  w->dsT.push_back(t);
+}
+
+size_t warmstartSamples(const WarmStart *w) { return w->dsT.size(); }
+
+/* hand the captured pairs over (ownership moves to the caller) */
+void warmstartTakeSamples(WarmStart *w, std::vector<uint16_t *> &xs, std::vector<float *> &ts)
+{
+  xs.insert(xs.end(), w->dsX.begin(), w->dsX.end());
+  ts.insert(ts.end(), w->dsT.begin(), w->dsT.end());
+  w->dsX.clear();
+  w->dsT.clear();
+}
+
+/* ---- C ABI: training driver ------------------------------------------------ */
+static const int kUnetLevels = 4;
+static const int kUnetChannels[4] = {16, 32, 64, 128};
+
+extern "C" int nusifUnetTrainCavity(const char *weightsOut, int grid, int runs, int stepsPerRun,
+    int skipSteps, int epochs, int batch, double lr, int seed, double *lossPerEpoch)
+{
+  std::vector<uint16_t *> xs;
+  std::vector<float *> ts;
+  unsigned long long rng = 0x243F6A8885A308D3ull ^ ((unsigned long long)seed * 0x9E3779B97F4A7C15ull);
+  auto uni = [&]() {
+    rng ^= rng >> 12;
+    rng ^= rng << 25;
+    rng ^= rng >> 27;
+    return ((rng * 0x2545F4914F6CDD1Dull) >> 11) * (1.0 / 9007199254740992.0);
+  };
+  int netN = 0;
+  for (int r = 0; r < runs; r++) {
+    double re = 10.0 + 990.0 * uni(), lid = 0.5 + 1.5 * uni();
+    Parameter prm;
+    initParameter(&prm);
+    prm.imax = prm.jmax = prm.kmax = grid;
+    prm.xlength = prm.ylength = prm.zlength = 1.0;
+    prm.re = re;
+    prm.tau = 0.5;
+    prm.te = 1e9;
+    prm.dt = 0.02;
+    prm.gx = prm.gy = prm.gz = 0.0;
+    prm.name = (char *)"dcavity";
+    prm.bcLeft = prm.bcRight = prm.bcBottom = prm.bcTop = prm.bcFront = prm.bcBack = NOSLIP;
+    prm.u_init = prm.v_init = prm.w_init = prm.p_init = 0.0;
+    prm.levels = 6;
+    prm.presmooth = prm.postsmooth = 2;
+    char buf[64];
+    nusifSetOption(&prm, "solver", "gmg");
+    nusifSetOption(&prm, "precision", "mixed");
+    nusifSetOption(&prm, "mg_tol", "1e-6");
+    snprintf(buf, sizeof buf, "%.17g", lid);
+    nusifSetOption(&prm, "lid", buf);
+    Discretization d;
+    Solver s;
+    memset(&d, 0, sizeof d);
+    commInit(&d.comm, 0, nullptr);
+    commPartition(&d.comm, grid, grid, grid);
+    initDiscretization(&d, &prm);
+    initSolver(&s, &d, &prm);
+    NusifSolverCtx *sc = nusifSolverCtxOf(&s);
+    if (skipSteps > 0) nusifRunSteps(&d, &s, skipSteps, nullptr);
+    sc->ws = warmstartCreate(sc->dctx, nullptr, true);
+    netN   = sc->ws->n;
+    nusifRunSteps(&d, &s, stepsPerRun, nullptr);
+    warmstartTakeSamples(sc->ws, xs, ts);
+    nusifFreeSolver(&s);
+    nusifFreeDiscretization(&d);
+    printf("[unet] run %d/%d: Re %.1f lid %.2f, %zu pairs so far\n", r + 1, runs, re, lid, ts.size());
+    fflush(stdout);
+  }
+  if (ts.empty()) return 0;
+  UnetModel *m = unetNew(kUnetLevels, kUnetChannels, 8, netN);
+  unetInit(m, (unsigned)seed);
+  unetSyncWeights(m);
+  unetZeroGrads(m);
+  std::vector<int> order(ts.size());
+  for (size_t q = 0; q < order.size(); q++) order[q] = (int)q;
+  for (int e = 0; e < epochs; e++) {
+    for (size_t q = order.size() - 1; q > 0; q--) std::swap(order[q], order[(size_t)(uni() * (q + 1)) % (q + 1)]);
+    double sum = 0.0;
+    float lrE  = (float)(lr * 0.5 * (1.0 + cos(M_PI * e / (double)epochs))); /* cosine decay */
+    for (size_t q = 0; q < order.size(); q++) {
+      unetForward(m, xs[order[q]]);
+      bool last = (q + 1) % batch == 0 || q + 1 == order.size();
+      sum += unetBackward(m, xs[order[q]], ts[order[q]], true);
+      if (last) unetAdam(m, lrE, batch);
+    }
+    if (lossPerEpoch) lossPerEpoch[e] = sum / order.size();
+    printf("[unet] epoch %d/%d: mean loss %.5g (lr %.3g)\n", e + 1, epochs, sum / order.size(), lrE);
+    fflush(stdout);
+  }
+  int rc = unetSave(m, weightsOut);
+  unetFree(m);
+  for (auto p : xs) HIP_CHECK(hipFree(p));
+  for (auto p : ts) HIP_CHECK(hipFree(p));
+  if (rc != 0) NUSIF_FATAL("cannot write %s", weightsOut);
+  return (int)ts.size();
+}
+
+/* ---- C ABI: one forward/backward for gradient checks ------------------------ */
+/* cfg = {L, ch0, ch1, ch2, ch3, cin}; x: n^3 x cin (channels-last), target n^3;
+ * params in the oracle's flat layout. Returns the loss; grad and pred (may be
+ * NULL) receive the gradient and the prediction. */
+extern "C" double nusifUnetLossGrad(const int *cfg, int n, const double *params, const double *x,
+    const double *target, double *grad, double *pred)
+{
+  UnetModel *m = unetNew(cfg[0], cfg + 1, cfg[5], n);
+  size_t np    = unetParamCount(m);
+  std::vector<float> hp(np);
+  for (size_t q = 0; q < np; q++) hp[q] = (float)params[q];
+  unetSetParams(m, hp.data());
+  unetSyncWeights(m);
+  unetZeroGrads(m);
+  size_t nv = (size_t)n * n * n;
+  std::vector<uint16_t> hx(nv * cfg[5]);
+  for (size_t q = 0; q < hx.size(); q++) hx[q] = f2bf((float)x[q]);
+  std::vector<float> ht(nv);
+  for (size_t q = 0; q < nv; q++) ht[q] = (float)target[q];
+  uint16_t *dx = nullptr;
+  float *dt    = nullptr;
+  HIP_CHECK(hipMalloc((void **)&dx, hx.size() * sizeof(uint16_t)));
+  HIP_CHECK(hipMalloc((void **)&dt, nv * sizeof(float)));
+  HIP_CHECK(hipMemcpy(dx, hx.data(), hx.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(dt, ht.data(), nv * sizeof(float), hipMemcpyHostToDevice));
+  unetForward(m, dx);
+  double loss = unetBackward(m, dx, dt, true);
+  if (pred) {
+    std::vector<float> hy(nv);
+    HIP_CHECK(hipMemcpy(hy.data(), unetOutput(m), nv * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < nv; q++) pred[q] = hy[q];
+  }
+  if (grad) {
+    std::vector<float> hg(np);
+    unetGetGrads(m, hg.data());
+    for (size_t q = 0; q < np; q++) grad[q] = hg[q];
+  }
+  HIP_CHECK(hipFree(dx));
+  HIP_CHECK(hipFree(dt));
+  unetFree(m);
+  return loss;
+}

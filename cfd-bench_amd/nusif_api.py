@@ -131,6 +131,9 @@ def _declare_optional(lib):
         "nusifKernelBytes": ([C.POINTER(Discretization), C.c_char_p], C.c_double),
         "nusifDeviceSynchronize": ([], None),
         "nusifCopyProbe": ([C.c_size_t, C.c_int], C.c_double),
+        "nusifWarmstartUsed": ([C.POINTER(Solver)], C.c_int),
+        "nusifUnetTrainCavity": ([C.c_char_p] + [C.c_int] * 6 + [C.c_double, C.c_int, _DP], C.c_int),
+        "nusifUnetLossGrad": ([C.POINTER(C.c_int), C.c_int, _DP, _DP, _DP, _DP, _DP], C.c_double),
         "nusifRunSteps": ([C.POINTER(Discretization), C.POINTER(Solver), C.c_int,
                            C.POINTER(C.c_double)], C.c_int),
         # oracle extras
@@ -141,6 +144,9 @@ def _declare_optional(lib):
         "orcGmgSolve": ([C.c_int] * 3 + [C.c_double] * 3 + [_DP, _DP] + [C.c_int] * 3
                         + [C.c_double] * 2 + [C.c_int] * 3 + [_DP], C.c_int),
         "orcGmgCoarseSolve": ([C.c_int] * 3 + [C.c_double] * 3 + [_DP, _DP], None),
+        "orcUnetParamCount": ([C.POINTER(C.c_int)], C.c_long),
+        "orcUnetLoss": ([C.POINTER(C.c_int), C.c_int, _DP, _DP, _DP, _DP, _DP, C.c_int], C.c_double),
+        "orcBf16": ([C.c_double], C.c_double),
         "orcGmgProlong": ([C.c_int] * 3 + [_DP, _DP], None),
         "orcGmgRestrict": ([C.c_int] * 3 + [C.c_double] * 3 + [_DP] * 3, None),
     }

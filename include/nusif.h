@@ -86,6 +86,24 @@ double nusifKernelBytes(Discretization *d, const char *kernel);
  * achievable-bandwidth reference). */
 double nusifCopyProbe(size_t bytes, int reps);
 
+/* Trains the warm-start U-Net on the GPU from pressure-solve pairs the
+ * solver generates itself: `runs` lid-driven cavities of grid^3 cells with
+ * seeded Re ~ U[10, 1000] and lid speed ~ U[0.5, 2], `skipSteps` steps of
+ * spin-up then `stepsPerRun` captured steps each (gmg, mixed precision,
+ * 1e-6); then `epochs` of Adam (cosine-decayed `lr`, `batch` samples per
+ * update). Writes the weights to `weightsOut` (option unet_weights reads
+ * them); lossPerEpoch (may be NULL) receives the mean training loss per
+ * epoch. Returns the number of training pairs. */
+int nusifUnetTrainCavity(const char *weightsOut, int grid, int runs, int stepsPerRun,
+    int skipSteps, int epochs, int batch, double lr, int seed, double *lossPerEpoch);
+
+/* One forward and backward pass of the U-Net on the GPU, for gradient
+ * checks: cfg = {levels, ch0, ch1, ch2, ch3, input channels}, x is n^3 x
+ * cin channels-last, params in the flat layout of the weights file. Returns
+ * the loss; grad and pred (may be NULL) receive the results. */
+double nusifUnetLossGrad(const int *cfg, int n, const double *params, const double *x,
+    const double *target, double *grad, double *pred);
+
 /* Runs the reference time loop (src/main.c:60-88) for `nsteps` steps entirely
  * inside the library: the same calls in the same order, without the Python
  * or host-driver overhead between them. trace (may be NULL) receives

@@ -1,0 +1,83 @@
+"""The warm-start U-Net on the GPU (BF16 MFMA convolutions, FP32
+accumulation) against the CPU oracle (oracle/orc_unet.c run with bfloat16
+rounding of weights and stored activations), and the learned warm start end
+to end: solver-generated training pairs, GPU training, guess-or-fall-back.
+
+Tolerances: forward outputs and the loss within 2e-2 relative (bf16 has an
+8-bit mantissa; the oracle rounds at the same points, only FP32 accumulation
+order differs); gradients within 5e-2 relative L2 error with cosine >= 0.998
+(the GPU also rounds the backward operands to bf16 for MFMA, the oracle's
+backward is exact FP64)."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import nusif_api as A
+from conftest import lib
+
+pytestmark = pytest.mark.gpu
+DP = A._DP
+
+
+def cfg_arr(L, ch, cin):
+    return (C.c_int * 6)(L, *(list(ch) + [0] * (4 - len(ch))), cin)
+
+
+@pytest.mark.parametrize("L,ch,n", [(2, (16, 32), 8), (4, (16, 32, 64, 128), 16)])
+def test_gpu_unet_forward_and_gradient_match_oracle(L, ch, n, gpu_ok):
+    print("unet", L, ch, n, flush=True)
+    cin = 8
+    cfg = cfg_arr(L, ch, cin)
+    total = lib("orc_unet").orcUnetParamCount(cfg)
+    rng = np.random.default_rng(11)
+    params = np.zeros(total)
+    # He-like init per conv, small head
+    params[:] = rng.normal(0, 0.08, total)
+    x = np.zeros((n, n, n, cin))
+    x[..., :2] = rng.uniform(-1, 1, (n, n, n, 2))
+    t = rng.uniform(-1, 1, (n, n, n))
+    g_gpu = np.zeros(total)
+    p_gpu = np.zeros((n, n, n))
+    loss_gpu = lib("gpu_rb").nusifUnetLossGrad(cfg, n, params.ctypes.data_as(DP), x.ctypes.data_as(DP),
+                                               t.ctypes.data_as(DP), g_gpu.ctypes.data_as(DP),
+                                               p_gpu.ctypes.data_as(DP))
+    g_orc = np.zeros(total)
+    p_orc = np.zeros((n, n, n))
+    loss_orc = lib("orc_unet").orcUnetLoss(cfg, n, params.ctypes.data_as(DP), x.ctypes.data_as(DP),
+                                          t.ctypes.data_as(DP), g_orc.ctypes.data_as(DP),
+                                          p_orc.ctypes.data_as(DP), 1)
+    perr = np.linalg.norm(p_gpu - p_orc) / np.linalg.norm(p_orc)
+    gerr = np.linalg.norm(g_gpu - g_orc) / np.linalg.norm(g_orc)
+    cos = float(g_gpu @ g_orc / (np.linalg.norm(g_gpu) * np.linalg.norm(g_orc)))
+    print("pred rel err %.3e loss %.6g vs %.6g grad rel err %.3e cos %.6f" % (perr, loss_gpu, loss_orc, gerr, cos))
+    assert perr <= 2e-2
+    assert loss_gpu == pytest.approx(loss_orc, rel=2e-2)
+    assert gerr <= 5e-2 and cos >= 0.998
+
+
+def test_gpu_unet_train_and_warm_start(tmp_path, gpu_ok):
+    """Short training on solver-generated pairs (32^3 cavities), then a
+    warm-started gmg run: the loss must fall, every solve must still reach
+    the tolerance, and the network's guess is used or rejected per step."""
+    w = str(tmp_path / "unet.bin")
+    losses = (C.c_double * 4)()
+    npairs = lib("gpu_rb").nusifUnetTrainCavity(w.encode(), 32, 3, 8, 4, 4, 4, 2e-3, 5, losses)
+    print("pairs", npairs, "loss per epoch", list(losses))
+    assert npairs == 24
+    assert losses[3] < losses[0]
+    assert os.path.getsize(w) > 1000
+    prm = dict(name="dcavity", re=400.0, imax=32, jmax=32, kmax=32, tau=0.5, te=1e9, levels=4,
+               presmooth=2, postsmooth=2)
+    opts = {"solver": "gmg", "precision": "mixed", "mg_tol": 1e-6, "warmstart": "unet",
+            "unet_weights": w}
+    c = A.Case(lib("gpu_rb"), options=opts, **prm)
+    used = []
+    for _ in range(12):
+        c.step()
+        assert c.lib.nusifSolverRelResidual(C.byref(c.s)) <= 1e-6
+        used.append(c.lib.nusifWarmstartUsed(C.byref(c.s)))
+    print("warm start used per step:", used)
+    assert set(used) <= {0, 1}
+    c.close()
