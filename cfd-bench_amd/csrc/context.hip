@@ -221,6 +221,7 @@ extern "C" int nusifSetOption(Parameter *p, const char *key, const char *value)
   if (k == "warmstart") {
     if (v == "none") o->warmstart = 0;
     else if (v == "unet") o->warmstart = 1;
+    else if (v == "unet_force") o->warmstart = 2; /* diagnostic: always take the guess */
     else return -2;
     return 0;
   }

@@ -813,9 +813,19 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
     warmstartCaptureBegin(ws, p, rhs, g);
     if (warmstartNet(ws)) {
       warmstartGuess(ws, p, rhs, c->pAlt);
+      double rc0 = warmstartInputNorm(ws);
       double nbg = 0.0;
-      double rng = residualNorm(c, G, c->pAlt, rhs, nullptr, &nbg);
-      sc->warmUsed = rng < rn0 ? 1 : 0;
+      double rng = residualNorm(c, G, c->pAlt, rhs, warmstartResidualBuffer(ws), &nbg);
+      double rcg = warmstartCoarseNorm(ws);
+      /* the starting residuals are compared on the network grid (restricted
+       * fine residuals): a correction interpolated from that grid carries
+       * interpolation error whose fine-grid residual the first smoothing
+       * sweep removes, and which would otherwise hide any gain */
+      sc->wsRes[0] = rn0;
+      sc->wsRes[1] = rng;
+      sc->wsRes[2] = rc0;
+      sc->wsRes[3] = rcg;
+      sc->warmUsed = (c->opt.warmstart == 2 || rcg < rc0) ? 1 : 0;
       if (sc->warmUsed) { /* start from the network's guess */
         double *t = c->p;
         c->p      = c->pAlt;

@@ -143,6 +143,7 @@ struct NusifSolverCtx {
   int predicted; /* iterations of the previous solve, sizes the launch batches */
   GmgState *gmg;
   WarmStart *ws; /* learned warm start / training-pair capture (gmg only) */
+  double wsRes[4]; /* last warm start: fine ||r0||, fine ||r_guess||, restricted both */
 };
 
 hipStream_t nusifStream(void);

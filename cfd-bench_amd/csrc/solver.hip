@@ -109,6 +109,12 @@ extern "C" int nusifSolverIterations(Solver *s) { return nusifSolverCtxOf(s)->la
 extern "C" double nusifSolverRelResidual(Solver *s) { return nusifSolverCtxOf(s)->lastRel; }
 extern "C" int nusifWarmstartUsed(Solver *s) { return nusifSolverCtxOf(s)->warmUsed; }
 
+extern "C" void nusifWarmstartResiduals(Solver *s, double *out4)
+{
+  NusifSolverCtx *sc = nusifSolverCtxOf(s);
+  for (int q = 0; q < 4; q++) out4[q] = sc->wsRes[q];
+}
+
 /* The loop body of src/main.c:60-88, driven from inside the library. */
 extern "C" int nusifRunSteps(Discretization *d, Solver *s, int nsteps, double *trace)
 {

@@ -45,7 +45,7 @@ struct WarmStart {
   WsLevel lv[12];
   uint16_t *x;            /* network input n^3 x 8 */
   double *pPrev;          /* FP64 copy of p for capture */
-  double *scal;           /* [0] sum r^2, [1] sum b^2 on level Ln; [2] used flag */
+  double *scal;           /* [0] sum r0^2, [1] sum b^2 on level Ln; [2..3] scratch sums */
   double *partials;
   /* capture */
   bool capture;
@@ -312,6 +312,33 @@ void warmstartGuess(WarmStart *w, const double *p, const double *rhs, double *gu
 
 float *warmstartResidualBuffer(WarmStart *w) { return w->lv[0].r; }
 
+/* ||R r||_2 on the network level of the level-0 residual in lv[0].r
+ * (R = repeated 8-child averaging); the r0 input's value was computed by
+ * buildInput, this is for another residual (the guess's). Clobbers lv[l].r. */
+double warmstartCoarseNorm(WarmStart *w)
+{
+  hipStream_t st = nusifStream();
+  for (int l = 1; l <= w->Ln; l++)
+    kRestrictAvg<float><<<cells(w->lv[l].g), dim3(WTX, WTY), 0, st>>>(w->lv[l - 1].r, w->lv[l - 1].g, w->lv[l].r, w->lv[l].g);
+  const WsLevel &L = w->lv[w->Ln];
+  dim3 g2((L.g.ni + WTX - 1) / WTX, (L.g.nj + WTY - 1) / WTY);
+  kSumSq2<<<g2, dim3(WTX, WTY), 0, st>>>(L.r, L.r, L.g, w->partials);
+  kFoldSumSq2<<<1, 256, 0, st>>>(w->partials, (int)(g2.x * g2.y), w->scal + 2);
+  double h[2];
+  HIP_CHECK(hipMemcpyAsync(h, w->scal + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipStreamSynchronize(st));
+  return sqrt(h[0]);
+}
+
+/* ||R r0||_2 from the last input build */
+double warmstartInputNorm(WarmStart *w)
+{
+  double h;
+  HIP_CHECK(hipMemcpyAsync(&h, w->scal, sizeof(double), hipMemcpyDeviceToHost, nusifStream()));
+  HIP_CHECK(hipStreamSynchronize(nusifStream()));
+  return sqrt(h);
+}
+
 /* capture, part 1 (before the solve): the input and a copy of p */
 void warmstartCaptureBegin(WarmStart *w, const double *p, const double *rhs, const Geo &g0)
 {
@@ -424,6 +451,10 @@ extern "C" int nusifUnetTrainCavity(const char *weightsOut, int grid, int runs, 
       bool last = (q + 1) % batch == 0 || q + 1 == order.size();
       sum += unetBackward(m, xs[order[q]], ts[order[q]], true);
       if (last) unetAdam(m, lrE, batch);
+      if ((q + 1) % 256 == 0) {
+        printf("[unet] epoch %d: %zu/%zu samples, running loss %.5g\n", e + 1, q + 1, order.size(), sum / (q + 1));
+        fflush(stdout);
+      }
     }
     if (lossPerEpoch) lossPerEpoch[e] = sum / order.size();
     printf("[unet] epoch %d/%d: mean loss %.5g (lr %.3g)\n", e + 1, epochs, sum / order.size(), lrE);

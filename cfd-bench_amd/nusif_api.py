@@ -132,6 +132,7 @@ def _declare_optional(lib):
         "nusifDeviceSynchronize": ([], None),
         "nusifCopyProbe": ([C.c_size_t, C.c_int], C.c_double),
         "nusifWarmstartUsed": ([C.POINTER(Solver)], C.c_int),
+        "nusifWarmstartResiduals": ([C.POINTER(Solver), _DP], None),
         "nusifUnetTrainCavity": ([C.c_char_p] + [C.c_int] * 6 + [C.c_double, C.c_int, _DP], C.c_int),
         "nusifUnetLossGrad": ([C.POINTER(C.c_int), C.c_int, _DP, _DP, _DP, _DP, _DP], C.c_double),
         "nusifRunSteps": ([C.POINTER(Discretization), C.POINTER(Solver), C.c_int,

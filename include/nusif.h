@@ -49,7 +49,8 @@ void nusifFreeSolver(Solver *s);
  *   u_inflow      double  inflow velocity of "canal" (reference: 2.0)
  *   inflow_noise  double  relative uniform perturbation of the inflow (0)
  *   seed          int     seed of inflow_noise (1)
- *   warmstart     none | unet       initial guess of the gmg solve
+ *   warmstart     none | unet | unet_force   initial guess of the gmg solve
+ *                 (unet_force: always take the network's guess, diagnostic)
  *   unet_weights  path    trained U-Net weights (nusifUnetSave format)
  * Returns 0 on success, -1 for an unknown key, -2 for a bad value. */
 int nusifSetOption(Parameter *p, const char *key, const char *value);
@@ -65,6 +66,10 @@ double nusifSolverRelResidual(Solver *s);
 /* Outcome of the last warm start: 1 network guess used, 0 fell back to the
  * previous-step pressure, -1 no network. */
 int nusifWarmstartUsed(Solver *s);
+/* Residual norms of the last warm start: fine ||r0|| (previous-step p),
+ * fine ||r_guess||, and both restricted to the network grid (the acceptance
+ * test compares the restricted pair). */
+void nusifWarmstartResiduals(Solver *s, double *out4);
 
 /* Name of the GPU the library runs on; returns the device count. */
 int nusifDeviceName(char *buf, int len);
