@@ -103,6 +103,7 @@ void nusifDefaultOptions(NusifOptions *o)
   o->mgMaxCycles = 100;
   o->mgOmega     = 1.0;
   o->mgProject   = 0;
+  o->mgTail      = 1;
   o->lid         = 1.0;
   o->uInflow     = 2.0;
   o->inflowNoise = 0.0;
@@ -214,6 +215,7 @@ extern "C" int nusifSetOption(Parameter *p, const char *key, const char *value)
   if (k == "mg_maxcycles") return parseInt(v.c_str(), &o->mgMaxCycles);
   if (k == "mg_omega") return parseReal(v.c_str(), &o->mgOmega);
   if (k == "mg_project") return parseInt(v.c_str(), &o->mgProject);
+  if (k == "mg_tail") return parseInt(v.c_str(), &o->mgTail);
   if (k == "lid") return parseReal(v.c_str(), &o->lid);
   if (k == "u_inflow") return parseReal(v.c_str(), &o->uInflow);
   if (k == "inflow_noise") return parseReal(v.c_str(), &o->inflowNoise);
@@ -314,6 +316,17 @@ void nusifTimerEnd(KernelTimer *t)
   t->used += 2;
 }
 
+extern "C" void nusifTimerDisableAll(void)
+{
+  for (KernelTimer *t : gTimers) {
+    drain(t);
+    for (hipEvent_t e : t->pool) HIP_CHECK(hipEventDestroy(e));
+    delete t;
+  }
+  gTimers.clear();
+  nusifTimerGeneration++;
+}
+
 extern "C" double nusifTimerRead(const char *kernel, int *launches)
 {
   KernelTimer *t = nusifTimerFor(kernel);
@@ -373,6 +386,7 @@ extern "C" double nusifKernelBytes(Discretization *d, const char *kernel)
       {"gmg_prolong32_l0", 8.5, 0},
       {"gmg_residual64", 16.0, 0},   /* rd p, rhs (+4 when the FP32 copy is written) */
       {"gmg_correct", 20.0, 0},      /* rd p, e32; wr p */
+      {"gmg_correct_residual", 36.0, 0}, /* rd p, e32, rhs; wr p', r32, e32 = 0 */
   };
   for (auto &e : table)
     if (strcmp(kernel, e.name) == 0) return e.perCell * (e.padded ? Npad : N);

@@ -77,6 +77,7 @@ struct GmgState {
   float *tab32;
   int nl;
   int nu1, nu2, maxCycles, project, mixed;
+  int tail; /* first level of the one-workgroup tail cycle, -1: none */
   double tol, omega;
   LevelT<double> L64[MAXLEV];
   LevelT<float> L32[MAXLEV];
@@ -343,10 +344,11 @@ static dim3 fusedGrid(const Geo &g)
 }
 
 /* b_c(I,J,K) = (1/8) sum over the 8 children of (b - A x); the coarse x is
- * zeroed (interior and ghosts) by the caller. */
+ * zeroed (interior and face ghosts: the coarse cycle's initial guess). */
 template <typename T>
 __global__ __launch_bounds__(256) void kResidualRestrict(const T *__restrict__ x,
-    const T *__restrict__ b, Geo gf, Coef<T> cf, T *__restrict__ bc, Geo gc, int gridGx, int gridGy)
+    const T *__restrict__ b, Geo gf, Coef<T> cf, T *__restrict__ bc, T *__restrict__ xc, Geo gc,
+    int gridGx, int gridGy)
 {
   int bx, by, bz;
   if (!xcdBlock(gridGx, gridGy, gc.nk, bx, by, bz)) return;
@@ -364,7 +366,10 @@ __global__ __launch_bounds__(256) void kResidualRestrict(const T *__restrict__ x
         long long c = gf.at(2 * I - 1 + di, 2 * J - 1 + dj, 2 * K - 1 + dk);
         s += b[c] - lap7(x, c, gf.sj, gf.sk, cf);
       }
-  bc[gc.at(I, J, K)] = (T)0.125 * s;
+  long long cc = gc.at(I, J, K);
+  bc[cc]       = (T)0.125 * s;
+  xc[cc]       = (T)0;
+  faceGhosts(xc, gc, I, J, K, cc, (T)0);
 }
 
 template <typename T> __global__ __launch_bounds__(256) void kFill(T *__restrict__ x, long long n, T v)
@@ -373,29 +378,57 @@ template <typename T> __global__ __launch_bounds__(256) void kFill(T *__restrict
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) x[q] = v;
 }
 
-/* x_f += P x_c, cell-centred trilinear; face ghosts of x_f refreshed. */
+/* x_f += P x_c, cell-centred trilinear (weights 3/4, 1/4 per direction;
+ * coarse indices clamped to the interior, i.e. a Neumann extension); face
+ * ghosts of x_f refreshed. Fine cell (i,j,k) lies in coarse cell
+ * (I,J,K) = ((i+1)/2, ...) and leans towards I2 = I -/+ 1 for odd/even i.
+ * With a = 3/4, q = 1/4 and, per coarse plane P,
+ *   c1(P) = a x(I,J,P) + q x(I2,J,P),  c2(P) = a x(I,J2,P) + q x(I2,J2,P),
+ *   S(P)  = a c1(P) + q c2(P),
+ * the correction is e = a S(K) + q S(K2). A thread owns one (i, j) column
+ * and marches PKC coarse planes (2 PKC fine planes): each S(P) is computed
+ * once and used by the two fine planes next to it (odd plane 2K-1 takes
+ * K2 = max(K-1, 1), even plane 2K takes K2 = min(K+1, nkc)), so coarse values
+ * are read once per column rather than eight times per fine cell. The tail
+ * cycle's tailProlong evaluates the same expression per cell. */
+#define PKC 4
 template <typename T>
-__global__ __launch_bounds__(256) void kProlongCorrect(T *__restrict__ xf, Geo gf,
-    const T *__restrict__ xc, Geo gc)
+__global__ __launch_bounds__(256) void kProlongMarch(T *__restrict__ xf, Geo gf, const T *__restrict__ xc,
+    Geo gc, int gridGx, int gridGy, int gridGz)
 {
-  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  int k = blockIdx.z + 1;
+  int bx, by, bz;
+  if (!xcdBlock(gridGx, gridGy, gridGz, bx, by, bz)) return;
+  int i = bx * blockDim.x + threadIdx.x + 1;
+  int j = by * blockDim.y + threadIdx.y + 1;
   if (i > gf.ni || j > gf.nj) return;
-  int I = (i + 1) >> 1, J = (j + 1) >> 1, K = (k + 1) >> 1;
-  int I2 = min(max(I + ((i & 1) ? -1 : 1), 1), gc.ni);
-  int J2 = min(max(J + ((j & 1) ? -1 : 1), 1), gc.nj);
-  int K2 = min(max(K + ((k & 1) ? -1 : 1), 1), gc.nk);
+  const int I = (i + 1) >> 1, J = (j + 1) >> 1;
+  const int I2 = min(max(I + ((i & 1) ? -1 : 1), 1), gc.ni);
+  const int J2 = min(max(J + ((j & 1) ? -1 : 1), 1), gc.nj);
   const T a = (T)0.75, q = (T)0.25;
-  T c11 = a * xc[gc.at(I, J, K)] + q * xc[gc.at(I2, J, K)];
-  T c21 = a * xc[gc.at(I, J2, K)] + q * xc[gc.at(I2, J2, K)];
-  T c12 = a * xc[gc.at(I, J, K2)] + q * xc[gc.at(I2, J, K2)];
-  T c22 = a * xc[gc.at(I, J2, K2)] + q * xc[gc.at(I2, J2, K2)];
-  T e   = a * (a * c11 + q * c21) + q * (a * c12 + q * c22);
-  long long c = gf.at(i, j, k);
-  T v  = xf[c] + e;
-  xf[c] = v;
-  faceGhosts(xf, gf, i, j, k, c, v);
+  const long long o11 = (long long)J * gc.sj + I, o21 = (long long)J * gc.sj + I2;
+  const long long o12 = (long long)J2 * gc.sj + I, o22 = (long long)J2 * gc.sj + I2;
+  auto S = [&](int P) {
+    const T *pl = xc + (long long)P * gc.sk;
+    T c1 = a * pl[o11] + q * pl[o21];
+    T c2 = a * pl[o12] + q * pl[o22];
+    return a * c1 + q * c2;
+  };
+  const int K0 = bz * PKC + 1, K1 = min(K0 + PKC, gc.nk + 1);
+  T sm = S(max(K0 - 1, 1)), s0 = S(K0);
+  for (int K = K0; K < K1; K++) {
+    T sp = S(min(K + 1, gc.nk));
+    int k = 2 * K - 1;
+    long long c = gf.at(i, j, k);
+    T v = xf[c] + (a * s0 + q * sm);
+    xf[c] = v;
+    faceGhosts(xf, gf, i, j, k, c, v);
+    c += gf.sk;
+    v     = xf[c] + (a * s0 + q * sp);
+    xf[c] = v;
+    faceGhosts(xf, gf, i, j, k + 1, c, v);
+    sm = s0;
+    s0 = sp;
+  }
 }
 
 /* Coarsest level, exact: the Neumann (ghost-copy) Laplacian is diagonal in
@@ -409,10 +442,9 @@ __global__ __launch_bounds__(256) void kProlongCorrect(T *__restrict__ xf, Geo g
  *   inv  Di Dj Dk (same shapes)                  s_m C[m][i], s_0 = 1/n, s_m = 2/n
  *   lam  Li[ni] Lj[nj] Lk[nk]                    eigenvalues */
 template <typename T>
-__global__ __launch_bounds__(1024) void kCoarseDct(T *__restrict__ x, const T *__restrict__ b, Geo g,
-    const T *__restrict__ tab, DctTables L)
+__device__ void coarseDct(T *__restrict__ x, const T *__restrict__ b, const Geo &g,
+    const T *__restrict__ tab, const DctTables &L, unsigned char *smem)
 {
-  extern __shared__ unsigned char smem[];
   const int ni = g.ni, nj = g.nj, nk = g.nk, N = ni * nj * nk;
   T *W1 = (T *)smem;
   T *W2 = W1 + N;
@@ -471,11 +503,206 @@ __global__ __launch_bounds__(1024) void kCoarseDct(T *__restrict__ x, const T *_
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(1024) void kCoarseDct(T *__restrict__ x, const T *__restrict__ b, Geo g,
+    const T *__restrict__ tab, DctTables L)
+{
+  extern __shared__ unsigned char smem[];
+  coarseDct(x, b, g, tab, L, smem);
+}
+
+/* ---- the coarse tail of the V-cycle in one workgroup ---------------------
+ * Levels of at most TAIL_MAX_CELLS cells are too small to fill the chip;
+ * launched one kernel per colour pass they cost a launch gap each. One
+ * workgroup of 1024 threads runs the whole V-cycle below the first such
+ * level instead: the same smoother passes, residual restriction, trilinear
+ * prolongation and DCT coarse solve as the per-level kernels (same operands,
+ * same order, so the same values), separated by workgroup barriers. The
+ * level arrays stay in this XCD's L2. Loads of B cells are issued before
+ * any of their stores: within a colour pass (and within a prolongation) no
+ * cell reads a value another cell of the pass writes, so this is safe and
+ * keeps B x 8 loads in flight per thread. */
+#define TAIL_MAX_CELLS 32768
+#define MAXTAIL 8
+#define TAIL_THREADS 1024
+/* cells per thread with loads in flight: 4 (FP32) or 2 (FP64) fit 128 VGPRs */
+template <typename T> struct TailB { static constexpr int value = sizeof(T) == 8 ? 2 : 4; };
+
+template <typename T> struct TailArgs {
+  int n;          /* levels in the tail; level 0 of the tail is the finest of them */
+  int nu1, nu2;
+  T *x[MAXTAIL];
+  T *b[MAXTAIL];
+  Geo g[MAXTAIL];
+  Coef<T> cf[MAXTAIL];
+  const T *tab;   /* DCT tables of the coarsest level */
+  DctTables dct;
+};
+
+__device__ inline void tailSync() { __syncthreads(); }
+
+/* one colour pass over all cell pairs (the kSmooth work of every thread) */
+template <typename T>
+__device__ void tailPass(T *__restrict__ x, const T *__restrict__ b, const Geo &g, const Coef<T> &cf,
+    int parity, int ghosts)
+{
+  constexpr int TAIL_B = TailB<T>::value;
+  const int np = (g.ni + 1) >> 1, total = np * g.nj * g.nk;
+  for (int base = threadIdx.x; base < total; base += TAIL_B * TAIL_THREADS) {
+    T xc[TAIL_B], xm[TAIL_B], xp[TAIL_B], ym[TAIL_B], yp[TAIL_B], zm[TAIL_B], zp[TAIL_B], bb[TAIL_B];
+    T xo[TAIL_B];
+    long long cc[TAIL_B], co[TAIL_B];
+    int ii[TAIL_B], io[TAIL_B], jj[TAIL_B], kk[TAIL_B];
+    bool live[TAIL_B];
+#pragma unroll
+    for (int u = 0; u < TAIL_B; u++) {
+      int q   = base + u * TAIL_THREADS;
+      live[u] = q < total;
+      int qq  = live[u] ? q : 0;
+      int tx = qq % np, r = qq / np;
+      int j = r % g.nj + 1, k = r / g.nj + 1, i1 = 2 * tx + 1;
+      int i  = (((1 + j + k) & 1) == parity) ? i1 : i1 + 1;
+      ii[u]  = i;
+      io[u]  = (i == i1) ? i1 + 1 : i1;
+      jj[u]  = j;
+      kk[u]  = k;
+      /* a pair's missing second cell (odd ni) reads a valid dummy cell */
+      long long c = g.at(min(i, g.ni), j, k);
+      cc[u] = c;
+      co[u] = g.at(min(io[u], g.ni), j, k);
+      xc[u] = x[c];
+      xm[u] = x[c - 1];
+      xp[u] = x[c + 1];
+      ym[u] = x[c - g.sj];
+      yp[u] = x[c + g.sj];
+      zm[u] = x[c - g.sk];
+      zp[u] = x[c + g.sk];
+      bb[u] = b[c];
+      xo[u] = x[co[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < TAIL_B; u++) {
+      if (!live[u]) continue;
+      bool in = ii[u] <= g.ni;
+      T v     = 0;
+      if (in) {
+        T lap = (xp[u] - (T)2 * xc[u] + xm[u]) * cf.idx2 + (yp[u] - (T)2 * xc[u] + ym[u]) * cf.idy2 +
+                (zp[u] - (T)2 * xc[u] + zm[u]) * cf.idz2;
+        T r   = bb[u] - lap;
+        v     = xc[u] - cf.factor * r;
+        x[cc[u]] = v;
+      }
+      if (ghosts) {
+        if (in) faceGhosts(x, g, ii[u], jj[u], kk[u], cc[u], v);
+        if (io[u] <= g.ni) faceGhosts(x, g, io[u], jj[u], kk[u], co[u], xo[u]);
+      }
+    }
+  }
+}
+
+template <typename T>
+__device__ void tailRestrict(const T *__restrict__ x, const T *__restrict__ b, const Geo &gf,
+    const Coef<T> &cf, T *__restrict__ bc, const Geo &gc)
+{
+  const int total = gc.ni * gc.nj * gc.nk;
+  for (int q = threadIdx.x; q < total; q += TAIL_THREADS) {
+    int I = q % gc.ni + 1, J = (q / gc.ni) % gc.nj + 1, K = q / (gc.ni * gc.nj) + 1;
+    T s = 0;
+#pragma unroll
+    for (int dk = 0; dk < 2; dk++)
+#pragma unroll
+      for (int dj = 0; dj < 2; dj++)
+#pragma unroll
+        for (int di = 0; di < 2; di++) {
+          long long c = gf.at(2 * I - 1 + di, 2 * J - 1 + dj, 2 * K - 1 + dk);
+          s += b[c] - lap7(x, c, gf.sj, gf.sk, cf);
+        }
+    bc[gc.at(I, J, K)] = (T)0.125 * s;
+  }
+}
+
+template <typename T>
+__device__ void tailProlong(T *__restrict__ xf, const Geo &gf, const T *__restrict__ xc, const Geo &gc)
+{
+  constexpr int TAIL_B = TailB<T>::value;
+  const int total = gf.ni * gf.nj * gf.nk;
+  const T a = (T)0.75, qq = (T)0.25;
+  for (int base = threadIdx.x; base < total; base += TAIL_B * TAIL_THREADS) {
+    T e[TAIL_B], old[TAIL_B];
+    long long cf[TAIL_B];
+    int ii[TAIL_B], jj[TAIL_B], kk[TAIL_B];
+    bool live[TAIL_B];
+#pragma unroll
+    for (int u = 0; u < TAIL_B; u++) {
+      int q   = base + u * TAIL_THREADS;
+      live[u] = q < total;
+      int qv  = live[u] ? q : 0;
+      int i = qv % gf.ni + 1, j = (qv / gf.ni) % gf.nj + 1, k = qv / (gf.ni * gf.nj) + 1;
+      ii[u] = i;
+      jj[u] = j;
+      kk[u] = k;
+      int I = (i + 1) >> 1, J = (j + 1) >> 1, K = (k + 1) >> 1;
+      int I2 = min(max(I + ((i & 1) ? -1 : 1), 1), gc.ni);
+      int J2 = min(max(J + ((j & 1) ? -1 : 1), 1), gc.nj);
+      int K2 = min(max(K + ((k & 1) ? -1 : 1), 1), gc.nk);
+      T c11 = a * xc[gc.at(I, J, K)] + qq * xc[gc.at(I2, J, K)];
+      T c21 = a * xc[gc.at(I, J2, K)] + qq * xc[gc.at(I2, J2, K)];
+      T c12 = a * xc[gc.at(I, J, K2)] + qq * xc[gc.at(I2, J, K2)];
+      T c22 = a * xc[gc.at(I, J2, K2)] + qq * xc[gc.at(I2, J2, K2)];
+      e[u]  = a * (a * c11 + qq * c21) + qq * (a * c12 + qq * c22);
+      cf[u] = gf.at(i, j, k);
+      old[u] = xf[cf[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < TAIL_B; u++) {
+      if (!live[u]) continue;
+      T v = old[u] + e[u];
+      xf[cf[u]] = v;
+      faceGhosts(xf, gf, ii[u], jj[u], kk[u], cf[u], v);
+    }
+  }
+}
+
+template <typename T> __device__ void tailZero(T *__restrict__ x, long long n)
+{
+  for (long long q = threadIdx.x; q < n; q += TAIL_THREADS) x[q] = (T)0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(TAIL_THREADS) void kTailCycle(TailArgs<T> a)
+{
+  extern __shared__ unsigned char smem[];
+  for (int l = 0; l < a.n; l++) tailZero(a.x[l], a.g[l].n);
+  tailSync();
+  for (int l = 0; l < a.n - 1; l++) {
+    for (int s = 0; s < a.nu1; s++) {
+      tailPass(a.x[l], a.b[l], a.g[l], a.cf[l], 1, 0);
+      tailSync();
+      tailPass(a.x[l], a.b[l], a.g[l], a.cf[l], 0, 1);
+      tailSync();
+    }
+    tailRestrict(a.x[l], a.b[l], a.g[l], a.cf[l], a.b[l + 1], a.g[l + 1]);
+    tailSync();
+  }
+  coarseDct(a.x[a.n - 1], a.b[a.n - 1], a.g[a.n - 1], a.tab, a.dct, smem);
+  tailSync();
+  for (int l = a.n - 2; l >= 0; l--) {
+    tailProlong(a.x[l], a.g[l], a.x[l + 1], a.g[l + 1]);
+    tailSync();
+    for (int s = 0; s < a.nu2; s++) {
+      tailPass(a.x[l], a.b[l], a.g[l], a.cf[l], 1, 0);
+      tailSync();
+      tailPass(a.x[l], a.b[l], a.g[l], a.cf[l], 0, 1);
+      tailSync();
+    }
+  }
+}
+
 /* ---- FP64 finest-level kernels: norms, defect correction ---------------- */
 /* slot 0: sum r^2 (r = rhs - A p), slot 1: sum rhs^2; optional FP32 copy of r */
 __global__ __launch_bounds__(256) void kResidualNorm(const double *__restrict__ p,
     const double *__restrict__ rhs, Geo g, Coef<double> cf, float *__restrict__ r32,
-    double *partials, int gridGx, int gridGy, int gridGz)
+    float *__restrict__ eZero, double *partials, int gridGx, int gridGy, int gridGz)
 {
   __shared__ double lds[4];
   int bx, by, bz;
@@ -491,6 +718,10 @@ __global__ __launch_bounds__(256) void kResidualNorm(const double *__restrict__ 
       double bb   = rhs[c];
       double r    = bb - lap7(p, c, g.sj, g.sk, cf);
       if (r32) r32[c] = (float)r;
+      if (eZero) { /* the next FP32 cycle's initial guess */
+        eZero[c] = 0.0f;
+        faceGhosts(eZero, g, i, j, k, c, 0.0f);
+      }
       a0 += r * r;
       a1 += bb * bb;
     }
@@ -504,14 +735,14 @@ __global__ __launch_bounds__(256) void kResidualNorm(const double *__restrict__ 
   }
 }
 
-__global__ __launch_bounds__(256) void kFoldNorms(const double *partials, int nb, DevScalars *sc)
+__global__ __launch_bounds__(256) void kFoldNorms(const double *partials, int nb, double *out)
 {
   __shared__ double lds[4];
   double s0 = foldSum(partials, nb, 2, 0, lds);
   double s1 = foldSum(partials, nb, 2, 1, lds);
   if (threadIdx.x == 0) {
-    sc->norm2[0] = s0;
-    sc->norm2[1] = s1;
+    out[0] = s0;
+    out[1] = s1;
   }
 }
 
@@ -545,6 +776,56 @@ __global__ __launch_bounds__(256) void kSubtractRhsMean(double *__restrict__ rhs
   double m = sc->norm2[2] / count;
   long long c = g.at(i, j, k);
   rhs[c]      = rhs[c] - m;
+}
+
+/* Defect correction and the next residual in one pass (mixed precision):
+ *   pn = p + e (FP64, face ghosts refreshed), r = rhs - A pn,
+ *   r32 = (float) r, eNext = 0 (interior and face ghosts), norms of r, rhs.
+ * A pn at a neighbour is evaluated as p + e there: both hold Neumann face
+ * ghosts (ghost = adjacent cell), so at a ghost this is the value kCorrect64
+ * would have written, and pn equals kCorrect64's result bit for bit.
+ * pn and eNext must not alias p or e (neighbours are read). */
+__global__ __launch_bounds__(256) void kCorrectResidual(const double *__restrict__ p,
+    const float *__restrict__ e, const double *__restrict__ rhs, Geo g, Coef<double> cf,
+    double *__restrict__ pn, float *__restrict__ r32, float *__restrict__ eNext, double *partials,
+    int gridGx, int gridGy, int gridGz)
+{
+  __shared__ double lds[4];
+  int bx, by, bz;
+  if (!xcdBlock(gridGx, gridGy, gridGz, bx, by, bz)) return;
+  int i  = bx * blockDim.x + threadIdx.x + 1;
+  int j  = by * blockDim.y + threadIdx.y + 1;
+  int k0 = bz * GKCH + 1;
+  double a0 = 0.0, a1 = 0.0;
+  if (i <= g.ni && j <= g.nj) {
+    int k1 = min(k0 + GKCH, g.nk + 1);
+    const long long sj = g.sj, sk = g.sk;
+    for (int k = k0; k < k1; k++) {
+      long long c = g.at(i, j, k);
+      double vc   = p[c] + (double)e[c];
+      double xm = p[c - 1] + (double)e[c - 1], xp = p[c + 1] + (double)e[c + 1];
+      double ym = p[c - sj] + (double)e[c - sj], yp = p[c + sj] + (double)e[c + sj];
+      double zm = p[c - sk] + (double)e[c - sk], zp = p[c + sk] + (double)e[c + sk];
+      double lap = (xp - 2.0 * vc + xm) * cf.idx2 + (yp - 2.0 * vc + ym) * cf.idy2 +
+                   (zp - 2.0 * vc + zm) * cf.idz2;
+      double bb = rhs[c];
+      double r  = bb - lap;
+      pn[c]     = vc;
+      faceGhosts(pn, g, i, j, k, c, vc);
+      r32[c]   = (float)r;
+      eNext[c] = 0.0f;
+      faceGhosts(eNext, g, i, j, k, c, 0.0f);
+      a0 += r * r;
+      a1 += bb * bb;
+    }
+  }
+  a0 = blockSum(a0, lds);
+  a1 = blockSum(a1, lds);
+  unsigned bid = bx + gridGx * (by + gridGy * bz);
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    partials[2 * bid]     = a0;
+    partials[2 * bid + 1] = a1;
+  }
 }
 
 /* p += e (FP32 correction), face ghosts of p refreshed */
@@ -646,8 +927,33 @@ static void buildDctTables(GmgState *G, const Geo &g, double hx, double hy, doub
   HIP_CHECK(hipMemcpy(G->tab32, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
 }
 
+/* levels G->tail .. nl-1 in one workgroup */
+template <typename T> static void tailCycle(GmgState *G, LevelT<T> *L)
+{
+  TailArgs<T> a;
+  memset(&a, 0, sizeof(a));
+  a.n   = G->nl - G->tail;
+  a.nu1 = G->nu1;
+  a.nu2 = G->nu2;
+  for (int q = 0; q < a.n; q++) {
+    a.x[q]  = L[G->tail + q].x;
+    a.b[q]  = L[G->tail + q].b;
+    a.g[q]  = L[G->tail + q].g;
+    a.cf[q] = L[G->tail + q].cf;
+  }
+  a.tab = tables(G, (T *)nullptr);
+  a.dct = G->dct;
+  const Geo &gc = L[G->nl - 1].g;
+  size_t lds    = 2 * (size_t)gc.ni * gc.nj * gc.nk * sizeof(T);
+  kTailCycle<T><<<1, TAIL_THREADS, lds, nusifStream()>>>(a);
+}
+
 template <typename T> static void vcycle(GmgState *G, LevelT<T> *L, int l)
 {
+  if (l == G->tail) {
+    tailCycle(G, L);
+    return;
+  }
   if (l == G->nl - 1) {
     coarseSolve(G, L[l]);
     return;
@@ -655,22 +961,22 @@ template <typename T> static void vcycle(GmgState *G, LevelT<T> *L, int l)
   bool fine = l == 0;
   smooth(L[l], G->nu1, fine);
   LevelT<T> &C = L[l + 1];
-  fill<T>(C.x, C.g.n, (T)0);
   if (fine)
     NUSIF_TIMED(sizeof(T) == 8 ? "gmg_restrict64_l0" : "gmg_restrict32_l0",
         (kResidualRestrict<T><<<xcdGrid(cellGrid(C.g)), dim3(GTX, GTY), 0, nusifStream()>>>(
-            L[l].x, L[l].b, L[l].g, L[l].cf, C.b, C.g, cellGrid(C.g).x, cellGrid(C.g).y)));
+            L[l].x, L[l].b, L[l].g, L[l].cf, C.b, C.x, C.g, cellGrid(C.g).x, cellGrid(C.g).y)));
   else
     kResidualRestrict<T><<<xcdGrid(cellGrid(C.g)), dim3(GTX, GTY), 0, nusifStream()>>>(
-        L[l].x, L[l].b, L[l].g, L[l].cf, C.b, C.g, cellGrid(C.g).x, cellGrid(C.g).y);
+        L[l].x, L[l].b, L[l].g, L[l].cf, C.b, C.x, C.g, cellGrid(C.g).x, cellGrid(C.g).y);
   vcycle(G, L, l + 1);
+  dim3 pg((L[l].g.ni + GTX - 1) / GTX, (L[l].g.nj + GTY - 1) / GTY, (C.g.nk + PKC - 1) / PKC);
   if (fine)
     NUSIF_TIMED(sizeof(T) == 8 ? "gmg_prolong64_l0" : "gmg_prolong32_l0",
-        (kProlongCorrect<T><<<cellGrid(L[l].g), dim3(GTX, GTY), 0, nusifStream()>>>(
-            L[l].x, L[l].g, C.x, C.g)));
+        (kProlongMarch<T><<<xcdGrid(pg), dim3(GTX, GTY), 0, nusifStream()>>>(
+            L[l].x, L[l].g, C.x, C.g, pg.x, pg.y, pg.z)));
   else
-    kProlongCorrect<T><<<cellGrid(L[l].g), dim3(GTX, GTY), 0, nusifStream()>>>(
-        L[l].x, L[l].g, C.x, C.g);
+    kProlongMarch<T><<<xcdGrid(pg), dim3(GTX, GTY), 0, nusifStream()>>>(
+        L[l].x, L[l].g, C.x, C.g, pg.x, pg.y, pg.z);
   smooth(L[l], G->nu2, fine);
 }
 
@@ -716,6 +1022,22 @@ GmgState *gmgCreate(NusifSolverCtx *sc, const Parameter *prm)
   }
   G->nl  = nl;
   G->cf0 = coefFor<double>(gr->dx, gr->dy, gr->dz, G->omega);
+  /* the tail starts at the first level below the finest that is small
+   * enough, when it covers at least one level above the coarsest */
+  G->tail = -1;
+  {
+    int a = c->geo.ni, b2 = c->geo.nj, c3 = c->geo.nk;
+    for (int l = 1; l < nl; l++) {
+      a /= 2;
+      b2 /= 2;
+      c3 /= 2;
+      if ((long long)a * b2 * c3 <= TAIL_MAX_CELLS) {
+        if (nl - l >= 2 && nl - l <= MAXTAIL) G->tail = l;
+        break;
+      }
+    }
+    if (!c->opt.mgTail) G->tail = -1;
+  }
   double dx = gr->dx, dy = gr->dy, dz = gr->dz;
   ni = c->geo.ni;
   nj = c->geo.nj;
@@ -774,18 +1096,43 @@ void gmgFree(GmgState *G)
   delete G;
 }
 
-/* residual norms on the finest level: returns ||r||, sets *nb = ||rhs|| */
-static double residualNorm(NusifCtx *c, GmgState *G, double *p, const double *rhs, float *r32, double *nb)
+/* Residual of p on the finest level (FP64): partial sums of r^2 and rhs^2
+ * folded into the device pair out[0..1]; optionally r as FP32 (r32) and an
+ * FP32 array zeroed (eZero). No host synchronisation. */
+static void launchResidual(NusifCtx *c, GmgState *G, const double *p, const double *rhs, float *r32,
+    float *eZero, double *out)
 {
   dim3 ng = normGrid(c->geo);
   NUSIF_TIMED("gmg_residual64", (kResidualNorm<<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(
-                                    p, rhs, c->geo, G->cf0, r32, c->partials, ng.x, ng.y, ng.z)));
-  kFoldNorms<<<1, 256, 0, nusifStream()>>>(c->partials, (int)(ng.x * ng.y * ng.z), c->scal);
-  HIP_CHECK(hipMemcpyAsync(c->scalHost->norm2, c->scal->norm2, 2 * sizeof(double),
+                                    p, rhs, c->geo, G->cf0, r32, eZero, c->partials, ng.x, ng.y, ng.z)));
+  kFoldNorms<<<1, 256, 0, nusifStream()>>>(c->partials, (int)(ng.x * ng.y * ng.z), out);
+}
+
+/* read back norm2[0..1] (||r||^2, ||rhs||^2): returns ||r||, sets *nb */
+static double readNorms(NusifCtx *c, double *nb)
+{
+  HIP_CHECK(hipMemcpyAsync(c->scalHost->norm2, c->scal->norm2, 4 * sizeof(double),
       hipMemcpyDeviceToHost, nusifStream()));
   HIP_CHECK(hipStreamSynchronize(nusifStream()));
   *nb = sqrt(c->scalHost->norm2[1]);
   return sqrt(c->scalHost->norm2[0]);
+}
+
+/* fold the partials of a normGrid launch into norm2[0..1] and read them */
+static double foldNorms(NusifCtx *c, double *nb)
+{
+  dim3 ng = normGrid(c->geo);
+  kFoldNorms<<<1, 256, 0, nusifStream()>>>(c->partials, (int)(ng.x * ng.y * ng.z), c->scal->norm2);
+  return readNorms(c, nb);
+}
+
+/* residual norms on the finest level: returns ||r||, sets *nb = ||rhs||;
+ * optionally writes r as FP32 (r32) and zeroes an FP32 array (eZero) */
+static double residualNorm(NusifCtx *c, GmgState *G, double *p, const double *rhs, float *r32, double *nb,
+    float *eZero = nullptr)
+{
+  launchResidual(c, G, p, rhs, r32, eZero, c->scal->norm2);
+  return readNorms(c, nb);
 }
 
 double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
@@ -806,22 +1153,31 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
   int cycles = 0;
   double rel = 0.0;
   WarmStart *ws = sc->ws;
+  bool haveInitial = false; /* mixed: residual of c->p already in L0.b, its norms in rnInit/nb */
+  double rnInit = 0.0;
   if (ws) {
-    /* r0 = rhs - A p_prev (FP32 copy for the network input) */
-    double nb0 = 0.0;
-    double rn0 = residualNorm(c, G, p, rhs, warmstartResidualBuffer(ws), &nb0);
+    /* r0 = rhs - A p_prev as FP32 (network input); norms into norm2[2..3] */
+    launchResidual(c, G, p, rhs, warmstartResidualBuffer(ws), nullptr, c->scal->norm2 + 2);
     warmstartCaptureBegin(ws, p, rhs, g);
     if (warmstartNet(ws)) {
       warmstartGuess(ws, p, rhs, c->pAlt);
-      double rc0 = warmstartInputNorm(ws);
-      double nbg = 0.0;
-      double rng = residualNorm(c, G, c->pAlt, rhs, warmstartResidualBuffer(ws), &nbg);
-      double rcg = warmstartCoarseNorm(ws);
+      /* the guess's residual, in mixed mode straight into the first defect
+       * buffer (L0.b) with the first correction zeroed, so an accepted guess
+       * costs no second residual pass; then its restriction to the network
+       * grid, and one read-back for the decision */
+      float *rg  = G->mixed ? G->L32[0].b : warmstartResidualBuffer(ws);
+      float *e0  = G->mixed ? G->L32[0].x : nullptr;
+      launchResidual(c, G, c->pAlt, rhs, rg, e0, c->scal->norm2);
+      warmstartCoarseNormLaunch(ws, rg);
+      const double *wn = warmstartNormsAsync(ws);
+      double nbg       = 0.0;
+      double rng       = readNorms(c, &nbg);
+      double rc0 = sqrt(wn[0]), rcg = sqrt(wn[2]);
       /* the starting residuals are compared on the network grid (restricted
        * fine residuals): a correction interpolated from that grid carries
        * interpolation error whose fine-grid residual the first smoothing
        * sweep removes, and which would otherwise hide any gain */
-      sc->wsRes[0] = rn0;
+      sc->wsRes[0] = sqrt(c->scalHost->norm2[2]);
       sc->wsRes[1] = rng;
       sc->wsRes[2] = rc0;
       sc->wsRes[3] = rcg;
@@ -831,6 +1187,11 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
         c->p      = c->pAlt;
         c->pAlt   = t;
         p         = c->p;
+        if (G->mixed) {
+          haveInitial = true;
+          rnInit      = rng;
+          nb          = nbg;
+        }
       }
     }
   }
@@ -852,16 +1213,27 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
       c->p    = L0.x;
     }
   } else {
+    /* r (FP32) into L0.b, e = 0 into L0.x; each cycle then ends with one
+     * pass that applies e to p (into the partner buffer), forms the next r
+     * and zeroes the partner of e for the next cycle */
     LevelT<float> &L0 = G->L32[0];
-    double rn = residualNorm(c, G, p, rhs, L0.b, &nb);
+    double rn = haveInitial ? rnInit : residualNorm(c, G, p, rhs, L0.b, &nb, L0.x);
     rel       = nb > 0.0 ? rn / nb : 0.0;
+    dim3 ng   = normGrid(g);
     while (nb > 0.0 && rel > G->tol && cycles < G->maxCycles) {
-      fill<float>(L0.x, g.n, 0.0f);
       vcycle<float>(G, G->L32, 0);
-      NUSIF_TIMED("gmg_correct", (kCorrect64<<<xcdGrid(cellGrid(g)), dim3(GTX, GTY), 0, nusifStream()>>>(
-                                     p, L0.x, g, cellGrid(g).x, cellGrid(g).y)));
+      NUSIF_TIMED("gmg_correct_residual",
+          (kCorrectResidual<<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(p, L0.x, rhs, g, G->cf0,
+              c->pAlt, L0.b, L0.xAlt, c->partials, ng.x, ng.y, ng.z)));
+      float *te = L0.x;
+      L0.x      = L0.xAlt;
+      L0.xAlt   = te;
+      double *tp = c->p; /* == p: the new values are in c->pAlt */
+      c->p       = c->pAlt;
+      c->pAlt    = tp;
+      p          = c->p;
       cycles++;
-      rn  = residualNorm(c, G, p, rhs, L0.b, &nb);
+      rn  = foldNorms(c, &nb);
       rel = rn / nb;
     }
   }

@@ -88,6 +88,7 @@ struct NusifOptions {
   int mgMaxCycles;
   double mgOmega;
   int mgProject;
+  int mgTail;      /* gmg: coarse levels in one workgroup (1) or per-level kernels (0) */
   double lid;
   double uInflow;
   double inflowNoise;

@@ -409,6 +409,21 @@ template <typename InT, typename OutT> static void launchConv(const ConvArgs<InT
     NUSIF_TIMED("unet_conv", (kConvMfma<InT, OutT, 1, MB><<<grid(1), blk, 0, nusifStream()>>>(a)));
 }
 
+/* forward convolutions of cubes of extent >= 32: LDS-staged bricks, row
+ * tiles chosen so the grid has enough workgroups */
+static void launchConvLds(const ConvArgs<uint16_t, uint16_t> &a)
+{
+  int rows = a.r0 + a.r1, n = a.n;
+  unsigned bricks = (unsigned)(((n + CL_BX - 1) / CL_BX) * ((n + CL_BY - 1) / CL_BY) * ((n + CL_BZ - 1) / CL_BZ));
+  auto grid = [&](int NB) { return dim3(bricks, (rows + 16 * NB - 1) / (16 * NB)); };
+  if (rows >= 64 && bricks >= 512)
+    NUSIF_TIMED("unet_conv", (kConvLds<uint16_t, 4><<<grid(4), 256, 0, nusifStream()>>>(a)));
+  else if (rows >= 32 && bricks >= 512)
+    NUSIF_TIMED("unet_conv", (kConvLds<uint16_t, 2><<<grid(2), 256, 0, nusifStream()>>>(a)));
+  else
+    NUSIF_TIMED("unet_conv", (kConvLds<uint16_t, 1><<<grid(1), 256, 0, nusifStream()>>>(a)));
+}
+
 /* forward conv k: out = relu(conv(in0 ++ in1) + b) */
 static void convForward(UnetModel *m, int k, const uint16_t *in0, int c0, const uint16_t *in1, int c1,
     uint16_t *out, int n)
@@ -429,7 +444,10 @@ static void convForward(UnetModel *m, int k, const uint16_t *in0, int c0, const 
   a.tapSign = 1;
   a.relu = 1;
   a.accumulate = 0;
-  launchConv(a);
+  if (n >= 32 && (c0 + c1) % 8 == 0)
+    launchConvLds(a);
+  else
+    launchConv(a);
 }
 
 /* backward of conv k given dOut (already ReLU-masked): dIn0 (= or +=),

@@ -147,6 +147,107 @@ __global__ __launch_bounds__(256) void kConvMfma(ConvArgs<InT, OutT> a)
   }
 }
 
+/* Forward convolution with the input brick staged in LDS. A workgroup owns a
+ * brick of 16 (x) x 4 (y) x 4 (z) output voxels; wave w owns z-slice w and
+ * its four 16-voxel blocks are the brick's four y-rows. The input brick plus
+ * its one-voxel halo (18 x 6 x 6 voxels) is loaded once per chunk of up to
+ * 32 channels, and all 27 taps read their B fragments from LDS (one 16 B LDS
+ * read per lane and block), instead of 27 global reads per voxel. K is
+ * ordered (tap, 8-channel group) within a chunk; the A fragments (weights,
+ * [cout][27][cin]) come from global memory through the caches. Voxel rows in
+ * LDS are padded to 40 bf16 (80 B) to spread the 16 B reads over the banks. */
+#define CL_BX 16
+#define CL_BY 4
+#define CL_BZ 4
+#define CL_HX (CL_BX + 2)
+#define CL_HY (CL_BY + 2)
+#define CL_HZ (CL_BZ + 2)
+#define CL_HV (CL_HX * CL_HY * CL_HZ)
+#define CL_CC 32
+#define CL_CSTR 40
+
+template <typename OutT, int NB>
+__global__ __launch_bounds__(256) void kConvLds(ConvArgs<uint16_t, OutT> a)
+{
+  __shared__ __attribute__((aligned(16))) uint16_t sIn[CL_HV * CL_CSTR];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int n = a.n;
+  const int nbx = (n + CL_BX - 1) / CL_BX, nby = (n + CL_BY - 1) / CL_BY;
+  const int x0 = (blockIdx.x % nbx) * CL_BX;
+  const int y0 = ((blockIdx.x / nbx) % nby) * CL_BY;
+  const int z0 = (blockIdx.x / (nbx * nby)) * CL_BZ;
+  const int cin = a.c0 + a.c1, rows = a.r0 + a.r1;
+  const int row0 = blockIdx.y * (16 * NB);
+  const int lr = lane & 15, lg = lane >> 4;
+
+  floatx4 acc[NB][4];
+#pragma unroll
+  for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++) acc[nb][mb] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  for (int cc0 = 0; cc0 < cin; cc0 += CL_CC) {
+    const int g8 = min(CL_CC, cin - cc0) >> 3; /* 8-channel groups in this chunk */
+    for (int u = tid; u < CL_HV * g8; u += 256) {
+      int v = u / g8, q = u - v * g8;
+      int hx = v % CL_HX, hy = (v / CL_HX) % CL_HY, hz = v / (CL_HX * CL_HY);
+      int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
+      uint4 val = make_uint4(0, 0, 0, 0);
+      if (x >= 0 && x < n && y >= 0 && y < n && z >= 0 && z < n) {
+        size_t vv = ((size_t)z * n + y) * n + x;
+        int c     = cc0 + q * 8;
+        val = c < a.c0 ? *(const uint4 *)(a.in0 + vv * a.c0 + c) : *(const uint4 *)(a.in1 + vv * a.c1 + (c - a.c0));
+      }
+      *(uint4 *)&sIn[v * CL_CSTR + q * 8] = val;
+    }
+    __syncthreads();
+    const int G = 27 * g8;
+    for (int s = 0; s < G; s += 4) {
+      int gi  = s + lg;
+      bool gv = gi < G;
+      int t = gv ? gi / g8 : 0, c8 = gv ? gi - t * g8 : 0;
+      int dz = t / 9, dy = (t / 3) % 3, dx = t % 3; /* tap offset + 1 = halo offset */
+      bf16x8 fa[NB], fb[4];
+#pragma unroll
+      for (int nb = 0; nb < NB; nb++) {
+        int r  = row0 + nb * 16 + lr;
+        fa[nb] = (gv && r < rows) ? ld8(a.A + ((size_t)r * 27 + t) * cin + cc0 + c8 * 8) : zero8();
+      }
+#pragma unroll
+      for (int mb = 0; mb < 4; mb++) {
+        int hv = ((wave + dz) * CL_HY + (mb + dy)) * CL_HX + (lr + dx);
+        uint4 raw = *(const uint4 *)&sIn[hv * CL_CSTR + c8 * 8];
+        fb[mb]    = gv ? __builtin_bit_cast(bf16x8, raw) : zero8();
+      }
+#pragma unroll
+      for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++)
+          acc[nb][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[nb], fb[mb], acc[nb][mb], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  const int x = x0 + lr, z = z0 + wave;
+  if (x >= n || z >= n) return;
+#pragma unroll
+  for (int mb = 0; mb < 4; mb++) {
+    int y = y0 + mb;
+    if (y >= n) continue;
+    size_t v = ((size_t)z * n + y) * n + x;
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        int r = row0 + nb * 16 + 4 * lg + q;
+        if (r >= rows) continue;
+        float val = acc[nb][mb][q] + (a.bias ? a.bias[r] : 0.f);
+        if (a.relu) val = val > 0.f ? val : 0.f;
+        if (r < a.r0) storeOut<OutT>(a.out0 + v * a.r0 + r, val, a.accumulate);
+        else storeOut<OutT>(a.out1 + v * a.r1 + (r - a.r0), val, a.accumulate);
+      }
+  }
+}
+
 /* Weight gradient of one tap: dW[n][t][c] += sum_v dOut[v][n] * in[v + off(t)][c].
  * Workgroup: 32 output channels x 32 input channels of one tap over a chunk
  * of voxels; 32-voxel slices of both operands are staged transposed in LDS

@@ -18,8 +18,8 @@ UnetModel *warmstartNet(WarmStart *w);
 bool warmstartCapturing(const WarmStart *w);
 /* FP32 level-0 buffer the caller fills with r0 = rhs - A p before the guess */
 float *warmstartResidualBuffer(WarmStart *w);
-double warmstartCoarseNorm(WarmStart *w);
-double warmstartInputNorm(WarmStart *w);
+void warmstartCoarseNormLaunch(WarmStart *w, const float *rFine);
+const double *warmstartNormsAsync(WarmStart *w);
 void warmstartGuess(WarmStart *w, const double *p, const double *rhs, double *guess);
 void warmstartCaptureBegin(WarmStart *w, const double *p, const double *rhs, const Geo &g0);
 void warmstartCaptureEnd(WarmStart *w, const double *p);

@@ -46,6 +46,7 @@ struct WarmStart {
   uint16_t *x;            /* network input n^3 x 8 */
   double *pPrev;          /* FP64 copy of p for capture */
   double *scal;           /* [0] sum r0^2, [1] sum b^2 on level Ln; [2..3] scratch sums */
+  double *scalHost;       /* pinned mirror of scal */
   double *partials;
   /* capture */
   bool capture;
@@ -245,6 +246,7 @@ WarmStart *warmstartCreate(NusifCtx *c, UnetModel *net, bool capture)
   w->h2 = (gn.ni * h / M_PI) * (gn.ni * h / M_PI);
   HIP_CHECK(hipMalloc((void **)&w->x, (size_t)w->n * w->n * w->n * 8 * sizeof(uint16_t)));
   HIP_CHECK(hipMalloc((void **)&w->scal, 4 * sizeof(double)));
+  HIP_CHECK(hipHostMalloc((void **)&w->scalHost, 4 * sizeof(double), hipHostMallocDefault));
   HIP_CHECK(hipMalloc((void **)&w->partials, 65536 * sizeof(double)));
   HIP_CHECK(hipMalloc((void **)&w->pPrev, (size_t)c->geo.n * sizeof(double)));
   return w;
@@ -261,6 +263,7 @@ void warmstartFree(WarmStart *w)
   }
   HIP_CHECK(hipFree(w->x));
   HIP_CHECK(hipFree(w->scal));
+  HIP_CHECK(hipHostFree(w->scalHost));
   HIP_CHECK(hipFree(w->partials));
   HIP_CHECK(hipFree(w->pPrev));
   for (auto p : w->dsX) HIP_CHECK(hipFree(p));
@@ -312,31 +315,31 @@ void warmstartGuess(WarmStart *w, const double *p, const double *rhs, double *gu
 
 float *warmstartResidualBuffer(WarmStart *w) { return w->lv[0].r; }
 
-/* ||R r||_2 on the network level of the level-0 residual in lv[0].r
- * (R = repeated 8-child averaging); the r0 input's value was computed by
- * buildInput, this is for another residual (the guess's). Clobbers lv[l].r. */
-double warmstartCoarseNorm(WarmStart *w)
+/* Sum of squares of R r on the network level, R = repeated 8-child
+ * averaging, of a level-0 FP32 residual (any padded buffer on the finest
+ * grid), into the device slot scal[2]; no host synchronisation. lv[l].r
+ * (l >= 1) are reused as scratch. */
+void warmstartCoarseNormLaunch(WarmStart *w, const float *rFine)
 {
   hipStream_t st = nusifStream();
-  for (int l = 1; l <= w->Ln; l++)
-    kRestrictAvg<float><<<cells(w->lv[l].g), dim3(WTX, WTY), 0, st>>>(w->lv[l - 1].r, w->lv[l - 1].g, w->lv[l].r, w->lv[l].g);
+  const float *src = rFine;
+  for (int l = 1; l <= w->Ln; l++) {
+    kRestrictAvg<float><<<cells(w->lv[l].g), dim3(WTX, WTY), 0, st>>>(src, w->lv[l - 1].g, w->lv[l].r, w->lv[l].g);
+    src = w->lv[l].r;
+  }
   const WsLevel &L = w->lv[w->Ln];
   dim3 g2((L.g.ni + WTX - 1) / WTX, (L.g.nj + WTY - 1) / WTY);
-  kSumSq2<<<g2, dim3(WTX, WTY), 0, st>>>(L.r, L.r, L.g, w->partials);
+  kSumSq2<<<g2, dim3(WTX, WTY), 0, st>>>(src, src, L.g, w->partials);
   kFoldSumSq2<<<1, 256, 0, st>>>(w->partials, (int)(g2.x * g2.y), w->scal + 2);
-  double h[2];
-  HIP_CHECK(hipMemcpyAsync(h, w->scal + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
-  HIP_CHECK(hipStreamSynchronize(st));
-  return sqrt(h[0]);
 }
 
-/* ||R r0||_2 from the last input build */
-double warmstartInputNorm(WarmStart *w)
+/* enqueue the copy of scal[0..3] ([0] = sum (R r0)^2, [2] = the last
+ * warmstartCoarseNormLaunch) into pinned host memory; valid after the
+ * stream synchronises */
+const double *warmstartNormsAsync(WarmStart *w)
 {
-  double h;
-  HIP_CHECK(hipMemcpyAsync(&h, w->scal, sizeof(double), hipMemcpyDeviceToHost, nusifStream()));
-  HIP_CHECK(hipStreamSynchronize(nusifStream()));
-  return sqrt(h);
+  HIP_CHECK(hipMemcpyAsync(w->scalHost, w->scal, 4 * sizeof(double), hipMemcpyDeviceToHost, nusifStream()));
+  return w->scalHost;
 }
 
 /* capture, part 1 (before the solve): the input and a copy of p */

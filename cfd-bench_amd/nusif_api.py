@@ -126,6 +126,7 @@ def _declare_optional(lib):
         "nusifSolverRelResidual": ([C.POINTER(Solver)], C.c_double),
         "nusifDeviceName": ([C.c_char_p, C.c_int], C.c_int),
         "nusifTimerEnable": ([C.c_char_p], C.c_int),
+        "nusifTimerDisableAll": ([], None),
         "nusifTimerRead": ([C.c_char_p, C.POINTER(C.c_int)], C.c_double),
         "nusifTimerReset": ([], None),
         "nusifKernelBytes": ([C.POINTER(Discretization), C.c_char_p], C.c_double),

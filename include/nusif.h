@@ -45,6 +45,8 @@ void nusifFreeSolver(Solver *s);
  *   mg_maxcycles  int     gmg: cycle cap per solve (100)
  *   mg_omega      double  gmg: smoother relaxation (1.0)
  *   mg_project    0 | 1   gmg: remove the rhs mean (all-Neumann compatibility)
+ *   mg_tail       0 | 1   gmg: run the levels of <= 32768 cells in one workgroup
+ *                         (1, default) or with one kernel per pass (0); same values
  *   lid           double  lid speed of "dcavity" (reference: 1.0)
  *   u_inflow      double  inflow velocity of "canal" (reference: 2.0)
  *   inflow_noise  double  relative uniform perturbation of the inflow (0)
@@ -81,6 +83,8 @@ void nusifDeviceSynchronize(void);
  * (synchronising the stream). Names: see nusifKernelBytes. */
 int nusifTimerEnable(const char *kernel);
 double nusifTimerRead(const char *kernel, int *launches);
+/* Stops and removes every timer (read them first). */
+void nusifTimerDisableAll(void);
 void nusifTimerReset(void);
 /* Algorithmic HBM bytes of one launch of `kernel` on the grid of `d`
  * (each array element read once and/or written once). */

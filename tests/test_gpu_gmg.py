@@ -93,3 +93,28 @@ def test_gpu_gmg_cavity_steps(precision, gpu_ok):
         assert err <= (1e-10 if precision == "fp64" else 1e-6), k
     g.close()
     o.close()
+
+
+@pytest.mark.parametrize("shape,levels,precision", [((64, 64, 64), 6, "fp64"),
+                                                   ((64, 64, 64), 6, "mixed"),
+                                                   ((128, 16, 32), 4, "mixed")])
+def test_gpu_gmg_tail_cycle_equals_per_level_kernels(shape, levels, precision, gpu_ok):
+    """The one-workgroup coarse tail (mg_tail 1) runs the same passes in the
+    same operand order as the per-level kernels (mg_tail 0): same p, bit for
+    bit, and the same cycle count."""
+    print("gmg tail", shape, levels, precision, flush=True)
+    ni, nj, nk = shape
+    prm = dict(G.KAT_PARAMS, imax=ni, jmax=nj, kmax=nk, xlength=ni / 32.0, ylength=nj / 32.0,
+               zlength=nk / 32.0, levels=levels, presmooth=2, postsmooth=2)
+    out = []
+    for tail in (0, 1):
+        opts = {"solver": "gmg", "precision": precision, "mg_tol": 1e-9, "mg_project": 1,
+                "mg_tail": tail}
+        c = A.Case(lib("gpu_rb"), options=opts, **prm)
+        f = G.random_fields(c.shape, 7)
+        c.set_fields(p=np.zeros(c.shape), rhs=f["rhs"])
+        c.lib.solve(C.byref(c.s), c.d.p, c.d.rhs)
+        out.append((c.fields()["p"], c.lib.nusifSolverIterations(C.byref(c.s))))
+        c.close()
+    assert out[0][1] == out[1][1]
+    np.testing.assert_array_equal(out[0][0], out[1][0])

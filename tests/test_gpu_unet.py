@@ -25,7 +25,9 @@ def cfg_arr(L, ch, cin):
     return (C.c_int * 6)(L, *(list(ch) + [0] * (4 - len(ch))), cin)
 
 
-@pytest.mark.parametrize("L,ch,n", [(2, (16, 32), 8), (4, (16, 32, 64, 128), 16)])
+# n >= 32 runs the forward convolutions on the LDS-staged brick kernel
+@pytest.mark.parametrize("L,ch,n", [(2, (16, 32), 8), (4, (16, 32, 64, 128), 16),
+                                    (2, (16, 32), 32), (3, (16, 32, 64), 48)])
 def test_gpu_unet_forward_and_gradient_match_oracle(L, ch, n, gpu_ok):
     print("unet", L, ch, n, flush=True)
     cin = 8
