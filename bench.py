@@ -39,8 +39,12 @@ def cavity_params(n):
                 itermax=1000, gamma=0.9, te=1e9, levels=6, presmooth=2, postsmooth=2)
 
 
+EXTRA_OPTS = {}
+
+
 def mode_options(mode):
     opts = {"solver": "gmg", "mg_tol": 1e-6}
+    opts.update(EXTRA_OPTS)
     if mode in ("mixed", "learned"):
         opts["precision"] = "mixed"
     if mode == "learned":
@@ -151,7 +155,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only time a device-to-device copy of MB megabytes (counter calibration)")
+    ap.add_argument("--opt", action="append", default=[], metavar="KEY=VALUE",
+                    help="extra solver option (nusifSetOption), e.g. mg_graph=0")
     a = ap.parse_args()
+    for kv in a.opt:
+        k, v = kv.split("=", 1)
+        EXTRA_OPTS[k] = v
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         sys.exit("bench.py: this build runs on one GPU only (the solver is a single-domain "
                  "MI355X path; --gpus %d is not supported)" % a.gpus)
@@ -179,7 +188,8 @@ def main():
         "config": {"workload": "lid-driven cavity %d^3 cells, Re=1000, lid u=1, tau=0.5, "
                                "pressure to 1e-6 relative residual per step, V(2,2) 6 levels; "
                                "mode %s" % (n, a.mode),
-                   "grid": [n, n, n], "mode": a.mode, "device": name.value.decode()},
+                   "grid": [n, n, n], "mode": a.mode, "device": name.value.decode(),
+                   "options": dict(EXTRA_OPTS)},
         "cycles_per_step": sum(res["cycles"]) / len(res["cycles"]),
         "max_rel_residual": max(res["rel"]),
     }

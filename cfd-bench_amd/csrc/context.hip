@@ -104,6 +104,7 @@ void nusifDefaultOptions(NusifOptions *o)
   o->mgOmega     = 1.0;
   o->mgProject   = 0;
   o->mgTail      = 1;
+  o->mgGraph     = 1;
   o->lid         = 1.0;
   o->uInflow     = 2.0;
   o->inflowNoise = 0.0;
@@ -216,6 +217,7 @@ extern "C" int nusifSetOption(Parameter *p, const char *key, const char *value)
   if (k == "mg_omega") return parseReal(v.c_str(), &o->mgOmega);
   if (k == "mg_project") return parseInt(v.c_str(), &o->mgProject);
   if (k == "mg_tail") return parseInt(v.c_str(), &o->mgTail);
+  if (k == "mg_graph") return parseInt(v.c_str(), &o->mgGraph);
   if (k == "lid") return parseReal(v.c_str(), &o->lid);
   if (k == "u_inflow") return parseReal(v.c_str(), &o->uInflow);
   if (k == "inflow_noise") return parseReal(v.c_str(), &o->inflowNoise);
@@ -245,8 +247,10 @@ extern "C" int nusifDeviceName(char *buf, int len)
     if (n > 0) {
       hipDeviceProp_t prop;
       HIP_CHECK(hipGetDeviceProperties(&prop, 0));
-      snprintf(buf, (size_t)len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName,
-          prop.multiProcessorCount);
+      if (prop.name[0])
+        snprintf(buf, (size_t)len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+      else
+        snprintf(buf, (size_t)len, "%s, %d CUs", prop.gcnArchName, prop.multiProcessorCount);
     }
   }
   return n;
@@ -315,6 +319,8 @@ void nusifTimerEnd(KernelTimer *t)
   HIP_CHECK(hipEventRecord(t->pool[t->used + 1], nusifStream()));
   t->used += 2;
 }
+
+bool nusifTimersActive() { return !gTimers.empty(); }
 
 extern "C" void nusifTimerDisableAll(void)
 {

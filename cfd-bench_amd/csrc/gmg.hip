@@ -71,7 +71,16 @@ static DctTables dctLayout(int ni, int nj, int nk)
   return t;
 }
 
+/* One captured cycle: the buffer pointers it was captured with (the key)
+ * and the pointer state the host-side swaps leave behind. */
+struct CycleGraph {
+  std::vector<void *> before, after;
+  hipGraphExec_t exec;
+};
+
 struct GmgState {
+  std::vector<CycleGraph> graphs; /* replayable cycles, at most GRAPH_CACHE */
+  int useGraphs;
   DctTables dct;
   double *tab64;
   float *tab32;
@@ -172,7 +181,7 @@ __device__ inline void ldsBarrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_ba
 #define FT_I 64
 #define FT_J 4
 #define FT_KC 32
-#define FUSED_MIN_EXTENT 128 /* levels narrower than this use the two-pass sweep */
+#define FUSED_MIN_EXTENT 32 /* levels narrower than this use the two-pass sweep */
 #define XW (FT_I + 4)
 #define XH (FT_J + 4)
 #define YW (FT_I + 2)
@@ -180,7 +189,7 @@ __device__ inline void ldsBarrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_ba
 
 template <typename T>
 __global__ __launch_bounds__(256, 8) void kSweepFused(const T *__restrict__ xo, T *__restrict__ xn,
-    const T *__restrict__ b, Geo g, Coef<T> cf)
+    const T *__restrict__ b, Geo g, Coef<T> cf, int kc)
 {
   constexpr int NX = (XH * XW + 255) / 256; /* X elements per thread */
   constexpr int NY = (YH * YW + 255) / 256; /* Y elements per thread */
@@ -193,13 +202,13 @@ __global__ __launch_bounds__(256, 8) void kSweepFused(const T *__restrict__ xo, 
    * tiles (x fastest, then y, then the k chunk) keeps the halos that
    * neighbouring tiles share inside one L2. */
   const int gxT = (g.ni + FT_I - 1) / FT_I, gyT = (g.nj + FT_J - 1) / FT_J;
-  const int nTiles = gxT * gyT * ((g.nk + FT_KC - 1) / FT_KC);
+  const int nTiles = gxT * gyT * ((g.nk + kc - 1) / kc);
   const int perXcd = (nTiles + NUM_XCD - 1) / NUM_XCD;
   const int t = (blockIdx.x % NUM_XCD) * perXcd + blockIdx.x / NUM_XCD;
   if (t >= nTiles) return;
   const int i0 = (t % gxT) * FT_I + 1, j0 = ((t / gxT) % gyT) * FT_J + 1;
-  const int k0 = (t / (gxT * gyT)) * FT_KC + 1;
-  const int k1 = min(k0 + FT_KC, g.nk + 1); /* black planes [k0, k1) */
+  const int k0 = (t / (gxT * gyT)) * kc + 1;
+  const int k1 = min(k0 + kc, g.nk + 1); /* black planes [k0, k1) */
   const int nk = g.nk, sj = g.sj;
   const long long sk = g.sk;
 
@@ -336,10 +345,21 @@ __global__ __launch_bounds__(256, 8) void kSweepFused(const T *__restrict__ xo, 
   }
 }
 
-/* 1-D grid: NUM_XCD * ceil(tiles / NUM_XCD) workgroups (surplus ids exit) */
-static dim3 fusedGrid(const Geo &g)
+/* k-chunk of the fused sweep: the longest (<= FT_KC planes, a power of two
+ * >= 4) that still gives >= 2048 workgroups (8 per CU), so small levels
+ * trade some halo re-loading for enough waves to hide latency */
+static int fusedChunk(const Geo &g)
 {
-  int n = ((g.ni + FT_I - 1) / FT_I) * ((g.nj + FT_J - 1) / FT_J) * ((g.nk + FT_KC - 1) / FT_KC);
+  long long tiles = (long long)((g.ni + FT_I - 1) / FT_I) * ((g.nj + FT_J - 1) / FT_J);
+  int kc          = FT_KC;
+  while (kc > 4 && tiles * ((g.nk + kc - 1) / kc) < 2048) kc /= 2;
+  return kc;
+}
+
+/* 1-D grid: NUM_XCD * ceil(tiles / NUM_XCD) workgroups (surplus ids exit) */
+static dim3 fusedGrid(const Geo &g, int kc)
+{
+  int n = ((g.ni + FT_I - 1) / FT_I) * ((g.nj + FT_J - 1) / FT_J) * ((g.nk + kc - 1) / kc);
   return dim3(((n + NUM_XCD - 1) / NUM_XCD) * NUM_XCD);
 }
 
@@ -522,7 +542,7 @@ __global__ __launch_bounds__(1024) void kCoarseDct(T *__restrict__ x, const T *_
  * any of their stores: within a colour pass (and within a prolongation) no
  * cell reads a value another cell of the pass writes, so this is safe and
  * keeps B x 8 loads in flight per thread. */
-#define TAIL_MAX_CELLS 32768
+#define TAIL_MAX_CELLS 4096
 #define MAXTAIL 8
 #define TAIL_THREADS 1024
 /* cells per thread with loads in flight: 4 (FP32) or 2 (FP64) fit 128 VGPRs */
@@ -699,37 +719,85 @@ __global__ __launch_bounds__(TAIL_THREADS) void kTailCycle(TailArgs<T> a)
 }
 
 /* ---- FP64 finest-level kernels: norms, defect correction ---------------- */
-/* slot 0: sum r^2 (r = rhs - A p), slot 1: sum rhs^2; optional FP32 copy of r */
-__global__ __launch_bounds__(256) void kResidualNorm(const double *__restrict__ p,
-    const double *__restrict__ rhs, Geo g, Coef<double> cf, float *__restrict__ r32,
+/* Residual of the finest level with its norms, one workgroup per 64 x 4
+ * (i, j) tile marching NKC planes: the tile's values plus a one-cell ring
+ * are staged in LDS per plane (three planes rotate), so the 7-point stencil
+ * reads LDS instead of seven global loads per cell.
+ *   CORR = false: x = p;      r = rhs - A x
+ *   CORR = true:  x = p + e;  pn = x (face ghosts refreshed), r = rhs - A x
+ *                 (the mixed-precision defect correction and the next
+ *                 residual in one pass; A x at a neighbour is p + e there:
+ *                 both hold Neumann face ghosts, so at a ghost this is the
+ *                 value the corrected p gets, and pn is bit-identical to
+ *                 correcting first)
+ * Optional outputs: r as FP32 (r32), an FP32 array zeroed with its face
+ * ghosts (eZero: the next FP32 cycle's initial guess). Partial sums of r^2
+ * and rhs^2 per workgroup (slots 2 b, 2 b + 1). pn, eZero must not alias
+ * p or e. */
+#define NKC 16
+template <bool CORR>
+__global__ __launch_bounds__(256) void kResidualLds(const double *__restrict__ p, const float *__restrict__ e,
+    const double *__restrict__ rhs, Geo g, Coef<double> cf, double *__restrict__ pn, float *__restrict__ r32,
     float *__restrict__ eZero, double *partials, int gridGx, int gridGy, int gridGz)
 {
+  constexpr int HW = GTX + 2, HN = HW * (GTY + 2);
   __shared__ double lds[4];
+  __shared__ double X[3][GTY + 2][HW];
   int bx, by, bz;
   if (!xcdBlock(gridGx, gridGy, gridGz, bx, by, bz)) return;
-  int i  = bx * blockDim.x + threadIdx.x + 1;
-  int j  = by * blockDim.y + threadIdx.y + 1;
-  int k0 = bz * GKCH + 1;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + GTX * ty;
+  const int i = bx * GTX + tx + 1, j = by * GTY + ty + 1;
+  const int k0 = bz * NKC + 1, k1 = min(k0 + NKC, g.nk + 1);
+  /* this thread's ring elements (two at most: HN <= 512); positions past the
+   * padded box load a clamped cell, used only by cells outside the interior */
+  const int e1 = tid + 256;
+  const bool has1 = e1 < HN;
+  const int hi0 = tid % HW, hj0 = tid / HW, hi1 = e1 % HW, hj1 = has1 ? e1 / HW : 0;
+  const long long o0 = (long long)min(by * GTY + hj0, g.nj + 1) * g.sj + min(bx * GTX + hi0, g.ni + 1);
+  const long long o1 = (long long)min(by * GTY + hj1, g.nj + 1) * g.sj + min(bx * GTX + hi1, g.ni + 1);
+  auto value = [&](long long q) {
+    if constexpr (CORR) return p[q] + (double)e[q];
+    else return p[q];
+  };
+  auto stage = [&](int k, int slot) {
+    long long base = (long long)k * g.sk;
+    X[slot][hj0][hi0] = value(base + o0);
+    if (has1) X[slot][hj1][hi1] = value(base + o1);
+  };
+  stage(k0 - 1, (k0 - 1) % 3);
+  stage(k0, k0 % 3);
+  const bool in = i <= g.ni && j <= g.nj;
   double a0 = 0.0, a1 = 0.0;
-  if (i <= g.ni && j <= g.nj) {
-    int k1 = min(k0 + GKCH, g.nk + 1);
-    for (int k = k0; k < k1; k++) {
-      long long c = g.at(i, j, k);
-      double bb   = rhs[c];
-      double r    = bb - lap7(p, c, g.sj, g.sk, cf);
+  for (int k = k0; k < k1; k++) {
+    stage(k + 1, (k + 1) % 3);
+    __syncthreads();
+    if (in) {
+      const int sm = (k + 2) % 3, sc = k % 3, sp = (k + 1) % 3;
+      const long long c = g.at(i, j, k);
+      double xc  = X[sc][ty + 1][tx + 1];
+      double lap = (X[sc][ty + 1][tx + 2] - 2.0 * xc + X[sc][ty + 1][tx]) * cf.idx2 +
+                   (X[sc][ty + 2][tx + 1] - 2.0 * xc + X[sc][ty][tx + 1]) * cf.idy2 +
+                   (X[sp][ty + 1][tx + 1] - 2.0 * xc + X[sm][ty + 1][tx + 1]) * cf.idz2;
+      double bb = rhs[c];
+      double r  = bb - lap;
+      if constexpr (CORR) {
+        pn[c] = xc;
+        faceGhosts(pn, g, i, j, k, c, xc);
+      }
       if (r32) r32[c] = (float)r;
-      if (eZero) { /* the next FP32 cycle's initial guess */
+      if (eZero) {
         eZero[c] = 0.0f;
         faceGhosts(eZero, g, i, j, k, c, 0.0f);
       }
       a0 += r * r;
       a1 += bb * bb;
     }
+    __syncthreads();
   }
   a0 = blockSum(a0, lds);
   a1 = blockSum(a1, lds);
   unsigned bid = bx + gridGx * (by + gridGy * bz);
-  if (threadIdx.x == 0 && threadIdx.y == 0) {
+  if (tid == 0) {
     partials[2 * bid]     = a0;
     partials[2 * bid + 1] = a1;
   }
@@ -738,8 +806,8 @@ __global__ __launch_bounds__(256) void kResidualNorm(const double *__restrict__ 
 __global__ __launch_bounds__(256) void kFoldNorms(const double *partials, int nb, double *out)
 {
   __shared__ double lds[4];
-  double s0 = foldSum(partials, nb, 2, 0, lds);
-  double s1 = foldSum(partials, nb, 2, 1, lds);
+  double s0, s1;
+  foldSum2(partials, nb, lds, s0, s1);
   if (threadIdx.x == 0) {
     out[0] = s0;
     out[1] = s1;
@@ -778,72 +846,6 @@ __global__ __launch_bounds__(256) void kSubtractRhsMean(double *__restrict__ rhs
   rhs[c]      = rhs[c] - m;
 }
 
-/* Defect correction and the next residual in one pass (mixed precision):
- *   pn = p + e (FP64, face ghosts refreshed), r = rhs - A pn,
- *   r32 = (float) r, eNext = 0 (interior and face ghosts), norms of r, rhs.
- * A pn at a neighbour is evaluated as p + e there: both hold Neumann face
- * ghosts (ghost = adjacent cell), so at a ghost this is the value kCorrect64
- * would have written, and pn equals kCorrect64's result bit for bit.
- * pn and eNext must not alias p or e (neighbours are read). */
-__global__ __launch_bounds__(256) void kCorrectResidual(const double *__restrict__ p,
-    const float *__restrict__ e, const double *__restrict__ rhs, Geo g, Coef<double> cf,
-    double *__restrict__ pn, float *__restrict__ r32, float *__restrict__ eNext, double *partials,
-    int gridGx, int gridGy, int gridGz)
-{
-  __shared__ double lds[4];
-  int bx, by, bz;
-  if (!xcdBlock(gridGx, gridGy, gridGz, bx, by, bz)) return;
-  int i  = bx * blockDim.x + threadIdx.x + 1;
-  int j  = by * blockDim.y + threadIdx.y + 1;
-  int k0 = bz * GKCH + 1;
-  double a0 = 0.0, a1 = 0.0;
-  if (i <= g.ni && j <= g.nj) {
-    int k1 = min(k0 + GKCH, g.nk + 1);
-    const long long sj = g.sj, sk = g.sk;
-    for (int k = k0; k < k1; k++) {
-      long long c = g.at(i, j, k);
-      double vc   = p[c] + (double)e[c];
-      double xm = p[c - 1] + (double)e[c - 1], xp = p[c + 1] + (double)e[c + 1];
-      double ym = p[c - sj] + (double)e[c - sj], yp = p[c + sj] + (double)e[c + sj];
-      double zm = p[c - sk] + (double)e[c - sk], zp = p[c + sk] + (double)e[c + sk];
-      double lap = (xp - 2.0 * vc + xm) * cf.idx2 + (yp - 2.0 * vc + ym) * cf.idy2 +
-                   (zp - 2.0 * vc + zm) * cf.idz2;
-      double bb = rhs[c];
-      double r  = bb - lap;
-      pn[c]     = vc;
-      faceGhosts(pn, g, i, j, k, c, vc);
-      r32[c]   = (float)r;
-      eNext[c] = 0.0f;
-      faceGhosts(eNext, g, i, j, k, c, 0.0f);
-      a0 += r * r;
-      a1 += bb * bb;
-    }
-  }
-  a0 = blockSum(a0, lds);
-  a1 = blockSum(a1, lds);
-  unsigned bid = bx + gridGx * (by + gridGy * bz);
-  if (threadIdx.x == 0 && threadIdx.y == 0) {
-    partials[2 * bid]     = a0;
-    partials[2 * bid + 1] = a1;
-  }
-}
-
-/* p += e (FP32 correction), face ghosts of p refreshed */
-__global__ __launch_bounds__(256) void kCorrect64(double *__restrict__ p, const float *__restrict__ e, Geo g,
-    int gridGx, int gridGy)
-{
-  int bx, by, bz;
-  if (!xcdBlock(gridGx, gridGy, g.nk, bx, by, bz)) return;
-  int i = bx * blockDim.x + threadIdx.x + 1;
-  int j = by * blockDim.y + threadIdx.y + 1;
-  int k = bz + 1;
-  if (i > g.ni || j > g.nj) return;
-  long long c = g.at(i, j, k);
-  double v    = p[c] + (double)e[c];
-  p[c]        = v;
-  faceGhosts(p, g, i, j, k, c, v);
-}
-
 /* ---- host orchestration -------------------------------------------------- */
 static dim3 cellGrid(const Geo &g) { return dim3((g.ni + GTX - 1) / GTX, (g.nj + GTY - 1) / GTY, g.nk); }
 static dim3 sweepGrid(const Geo &g)
@@ -852,7 +854,7 @@ static dim3 sweepGrid(const Geo &g)
 }
 static dim3 normGrid(const Geo &g)
 {
-  return dim3((g.ni + GTX - 1) / GTX, (g.nj + GTY - 1) / GTY, (g.nk + GKCH - 1) / GKCH);
+  return dim3((g.ni + GTX - 1) / GTX, (g.nj + GTY - 1) / GTY, (g.nk + NKC - 1) / NKC);
 }
 
 template <typename T> static void fill(T *x, long long n, T v)
@@ -866,7 +868,7 @@ template <typename T> static void fill(T *x, long long n, T v)
  * nusifKernelBytes reports. Each sweep swaps x and xAlt. */
 template <typename T> static void smooth(LevelT<T> &L, int sweeps, bool fine)
 {
-  if (L.g.ni < FUSED_MIN_EXTENT || L.g.nk < FT_KC) { /* small levels: two colour passes */
+  if (L.g.ni < FUSED_MIN_EXTENT || L.g.nk < 4) { /* small levels: two colour passes */
     dim3 grd = sweepGrid(L.g);
     for (int s = 0; s < sweeps; s++) {
       kSmooth<T><<<grd, dim3(GTX, GTY), 0, nusifStream()>>>(L.x, L.b, L.g, L.cf, 1, 0);
@@ -874,13 +876,14 @@ template <typename T> static void smooth(LevelT<T> &L, int sweeps, bool fine)
     }
     return;
   }
-  dim3 grd = fusedGrid(L.g);
+  const int kc = fusedChunk(L.g);
+  dim3 grd     = fusedGrid(L.g, kc);
   for (int s = 0; s < sweeps; s++) {
     if (fine)
       NUSIF_TIMED(sizeof(T) == 8 ? "gmg_smooth64_l0" : "gmg_smooth32_l0",
-          (kSweepFused<T><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf)));
+          (kSweepFused<T><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc)));
     else
-      kSweepFused<T><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf);
+      kSweepFused<T><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc);
     T *t   = L.x;
     L.x    = L.xAlt;
     L.xAlt = t;
@@ -992,7 +995,15 @@ GmgState *gmgCreate(NusifSolverCtx *sc, const Parameter *prm)
   NusifCtx *c  = sc->dctx;
   const Grid *gr = &c->owner->grid;
   GmgState *G  = new GmgState();
-  memset(G, 0, sizeof(*G));
+  G->dct       = DctTables();
+  G->tab64     = nullptr;
+  G->tab32     = nullptr;
+  G->nl        = 0;
+  for (int l = 0; l < MAXLEV; l++) {
+    G->L64[l] = LevelT<double>();
+    G->L32[l] = LevelT<float>();
+  }
+  G->useGraphs = c->opt.mgGraph;
   G->nu1       = prm->presmooth;
   G->nu2       = prm->postsmooth;
   G->maxCycles = c->opt.mgMaxCycles;
@@ -1083,6 +1094,8 @@ GmgState *gmgCreate(NusifSolverCtx *sc, const Parameter *prm)
 
 void gmgFree(GmgState *G)
 {
+  for (CycleGraph &e : G->graphs) HIP_CHECK(hipGraphExecDestroy(e.exec));
+  G->graphs.clear();
   if (G->tab64) HIP_CHECK(hipFree(G->tab64));
   if (G->tab32) HIP_CHECK(hipFree(G->tab32));
   for (int l = 0; l < G->nl; l++) {
@@ -1096,6 +1109,82 @@ void gmgFree(GmgState *G)
   delete G;
 }
 
+/* ---- cycle replay through HIP graphs ---------------------------------------
+ * A cycle is ~25 launches, many of them a few microseconds long on the
+ * coarse levels; launched one by one, the host's launch rate sets the pace
+ * there. The cycle body is therefore captured once per buffer arrangement
+ * (the fused sweeps and the defect correction ping-pong their buffers, so a
+ * cycle alternates between a few arrangements) and replayed with one
+ * hipGraphLaunch. The body contains kernels only (no memset or copy nodes).
+ * Timers put events around launches, so cycles run eagerly while any timer
+ * is enabled. */
+#define GRAPH_CACHE 8
+
+static std::vector<void *> cycleState(NusifCtx *c, GmgState *G)
+{
+  std::vector<void *> v;
+  v.push_back(c->p);
+  v.push_back(c->pAlt);
+  for (int l = 0; l < G->nl; l++) {
+    if (G->mixed) {
+      v.push_back(G->L32[l].x);
+      v.push_back(G->L32[l].xAlt);
+    } else {
+      v.push_back(G->L64[l].x);
+      v.push_back(G->L64[l].xAlt);
+    }
+  }
+  return v;
+}
+
+static void setCycleState(NusifCtx *c, GmgState *G, const std::vector<void *> &v)
+{
+  size_t q = 0;
+  c->p    = (double *)v[q++];
+  c->pAlt = (double *)v[q++];
+  for (int l = 0; l < G->nl; l++) {
+    if (G->mixed) {
+      G->L32[l].x    = (float *)v[q++];
+      G->L32[l].xAlt = (float *)v[q++];
+    } else {
+      G->L64[l].x    = (double *)v[q++];
+      G->L64[l].xAlt = (double *)v[q++];
+    }
+  }
+}
+
+extern bool nusifTimersActive();
+
+template <typename Body> static void runCycle(NusifCtx *c, GmgState *G, Body body)
+{
+  if (!G->useGraphs || nusifTimersActive()) {
+    body();
+    return;
+  }
+  std::vector<void *> before = cycleState(c, G);
+  for (CycleGraph &e : G->graphs)
+    if (e.before == before) {
+      HIP_CHECK(hipGraphLaunch(e.exec, nusifStream()));
+      setCycleState(c, G, e.after);
+      return;
+    }
+  if (G->graphs.size() >= GRAPH_CACHE) {
+    HIP_CHECK(hipGraphExecDestroy(G->graphs.front().exec));
+    G->graphs.erase(G->graphs.begin());
+  }
+  hipGraph_t graph;
+  HIP_CHECK(hipStreamBeginCapture(nusifStream(), hipStreamCaptureModeThreadLocal));
+  body();
+  HIP_CHECK(hipStreamEndCapture(nusifStream(), &graph));
+  CycleGraph e;
+  e.before = before;
+  e.after  = cycleState(c, G);
+  HIP_CHECK(hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0));
+  HIP_CHECK(hipGraphDestroy(graph));
+  HIP_CHECK(hipGraphLaunch(e.exec, nusifStream()));
+  G->graphs.push_back(e);
+}
+
 /* Residual of p on the finest level (FP64): partial sums of r^2 and rhs^2
  * folded into the device pair out[0..1]; optionally r as FP32 (r32) and an
  * FP32 array zeroed (eZero). No host synchronisation. */
@@ -1103,8 +1192,9 @@ static void launchResidual(NusifCtx *c, GmgState *G, const double *p, const doub
     float *eZero, double *out)
 {
   dim3 ng = normGrid(c->geo);
-  NUSIF_TIMED("gmg_residual64", (kResidualNorm<<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(
-                                    p, rhs, c->geo, G->cf0, r32, eZero, c->partials, ng.x, ng.y, ng.z)));
+  NUSIF_TIMED("gmg_residual64", (kResidualLds<false><<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(
+                                    p, nullptr, rhs, c->geo, G->cf0, nullptr, r32, eZero, c->partials, ng.x,
+                                    ng.y, ng.z)));
   kFoldNorms<<<1, 256, 0, nusifStream()>>>(c->partials, (int)(ng.x * ng.y * ng.z), out);
 }
 
@@ -1203,9 +1293,12 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
     double rn = residualNorm(c, G, L0.x, rhs, nullptr, &nb);
     rel       = nb > 0.0 ? rn / nb : 0.0;
     while (nb > 0.0 && rel > G->tol && cycles < G->maxCycles) {
-      vcycle<double>(G, G->L64, 0);
+      runCycle(c, G, [&] {
+        vcycle<double>(G, G->L64, 0);
+        launchResidual(c, G, L0.x, rhs, nullptr, nullptr, c->scal->norm2);
+      });
       cycles++;
-      rn  = residualNorm(c, G, L0.x, rhs, nullptr, &nb);
+      rn  = readNorms(c, &nb);
       rel = rn / nb;
     }
     if (L0.x != c->p) { /* the result sits in the partner buffer */
@@ -1221,19 +1314,22 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
     rel       = nb > 0.0 ? rn / nb : 0.0;
     dim3 ng   = normGrid(g);
     while (nb > 0.0 && rel > G->tol && cycles < G->maxCycles) {
-      vcycle<float>(G, G->L32, 0);
-      NUSIF_TIMED("gmg_correct_residual",
-          (kCorrectResidual<<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(p, L0.x, rhs, g, G->cf0,
-              c->pAlt, L0.b, L0.xAlt, c->partials, ng.x, ng.y, ng.z)));
-      float *te = L0.x;
-      L0.x      = L0.xAlt;
-      L0.xAlt   = te;
-      double *tp = c->p; /* == p: the new values are in c->pAlt */
-      c->p       = c->pAlt;
-      c->pAlt    = tp;
-      p          = c->p;
+      runCycle(c, G, [&] {
+        vcycle<float>(G, G->L32, 0);
+        NUSIF_TIMED("gmg_correct_residual",
+            (kResidualLds<true><<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(c->p, L0.x, rhs, g,
+                G->cf0, c->pAlt, L0.b, L0.xAlt, c->partials, ng.x, ng.y, ng.z)));
+        float *te  = L0.x; /* the next cycle's zeroed e */
+        L0.x       = L0.xAlt;
+        L0.xAlt    = te;
+        double *tp = c->p; /* the corrected p is in c->pAlt */
+        c->p       = c->pAlt;
+        c->pAlt    = tp;
+        kFoldNorms<<<1, 256, 0, nusifStream()>>>(c->partials, (int)(ng.x * ng.y * ng.z), c->scal->norm2);
+      });
+      p = c->p;
       cycles++;
-      rn  = foldNorms(c, &nb);
+      rn  = readNorms(c, &nb);
       rel = rn / nb;
     }
   }

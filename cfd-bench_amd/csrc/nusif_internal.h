@@ -89,6 +89,7 @@ struct NusifOptions {
   double mgOmega;
   int mgProject;
   int mgTail;      /* gmg: coarse levels in one workgroup (1) or per-level kernels (0) */
+  int mgGraph;     /* gmg: replay each cycle from a captured HIP graph (1) or launch eagerly (0) */
   double lid;
   double uInflow;
   double inflowNoise;

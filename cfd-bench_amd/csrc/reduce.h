@@ -1,7 +1,7 @@
-/* Wavefront / workgroup reductions and the "last workgroup finishes"
- * pattern used by every reduction kernel: each workgroup writes one partial,
- * the workgroup that arrives last (agent-scope counter, self-resetting) folds
- * the partials in index order, so results do not depend on scheduling. */
+/* Wavefront / workgroup reductions for the two-kernel reduction pattern used
+ * by every reduction: each workgroup writes one partial, then a second
+ * one-workgroup kernel folds the partials in a fixed order, so results do
+ * not depend on scheduling. */
 #ifndef NUSIF_REDUCE_H
 #define NUSIF_REDUCE_H
 
@@ -68,6 +68,37 @@ __device__ inline double foldSum(const double *part, int n, int stride, int comp
   double s = 0.0;
   for (int q = threadIdx.x; q < n; q += blockDim.x) s += part[(long long)q * stride + comp];
   return blockSum(s, lds);
+}
+
+/* Two interleaved components (stride 2) at once. Each thread adds its
+ * elements in ascending index order, exactly as foldSum does, but issues the
+ * loads of four elements before adding them, so the fold is not one memory
+ * latency per element. */
+__device__ inline void foldSum2(const double *part, int n, double *lds, double &s0, double &s1)
+{
+  double a = 0.0, b = 0.0;
+  const int st = blockDim.x;
+  int q        = threadIdx.x;
+  for (; q + 3 * st < n; q += 4 * st) {
+    double a0 = part[2LL * q], b0 = part[2LL * q + 1];
+    double a1 = part[2LL * (q + st)], b1 = part[2LL * (q + st) + 1];
+    double a2 = part[2LL * (q + 2 * st)], b2 = part[2LL * (q + 2 * st) + 1];
+    double a3 = part[2LL * (q + 3 * st)], b3 = part[2LL * (q + 3 * st) + 1];
+    a += a0;
+    a += a1;
+    a += a2;
+    a += a3;
+    b += b0;
+    b += b1;
+    b += b2;
+    b += b3;
+  }
+  for (; q < n; q += st) {
+    a += part[2LL * q];
+    b += part[2LL * q + 1];
+  }
+  s0 = blockSum(a, lds);
+  s1 = blockSum(b, lds);
 }
 
 __device__ inline double foldMax(const double *part, int n, int stride, int comp, double init,

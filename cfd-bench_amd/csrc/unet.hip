@@ -230,6 +230,8 @@ struct UnetModel {
   float *y, *dy;
   double *partials, *lossDev, *lossHost;
   int step;
+  float *split;    /* split-K partial sums of the small-cube forward convs */
+  size_t splitCap; /* floats */
 };
 
 static size_t cube(int n) { return (size_t)n * n * n; }
@@ -318,6 +320,27 @@ UnetModel *unetNew(int L, const int *ch, int cin, int n)
   kFillF<<<blocksFor(m->nParams), 256, 0, st>>>(m->M, m->nParams, 0.f);
   kFillF<<<blocksFor(m->nParams), 256, 0, st>>>(m->V, m->nParams, 0.f);
   allocBuffers(m);
+  /* split-K scratch: the largest chunks x voxels x cout over the convs whose
+   * cube has fewer than 256 bricks (same order as layout()) */
+  {
+    size_t cap = 0;
+    int q      = 0;
+    auto need  = [&](int k, int nl) {
+      int bricks = ((nl + CL_BX - 1) / CL_BX) * ((nl + CL_BY - 1) / CL_BY) * ((nl + CL_BZ - 1) / CL_BZ);
+      int chunks = (m->conv[k].cin + CL_CC - 1) / CL_CC;
+      if (bricks < 256 && chunks > 1) cap = std::max(cap, (size_t)chunks * cube(nl) * m->conv[k].cout);
+    };
+    for (int l = 0; l < L; l++) {
+      need(q++, n >> l);
+      need(q++, n >> l);
+    }
+    for (int l = L - 2; l >= 0; l--) {
+      need(q++, n >> l);
+      need(q++, n >> l);
+    }
+    m->splitCap = cap;
+    m->split    = cap ? dalloc<float>(cap) : nullptr;
+  }
   return m;
 }
 
@@ -325,8 +348,9 @@ void unetFree(UnetModel *m)
 {
   if (!m) return;
   HIP_CHECK(hipStreamSynchronize(nusifStream()));
-  void *ptrs[] = {m->P, m->G, m->M, m->V, m->Wb, m->Wt, m->y, m->dy, m->partials, m->lossDev};
-  for (void *p : ptrs) HIP_CHECK(hipFree(p));
+  void *ptrs[] = {m->P, m->G, m->M, m->V, m->Wb, m->Wt, m->y, m->dy, m->partials, m->lossDev, m->split};
+  for (void *p : ptrs)
+    if (p) HIP_CHECK(hipFree(p));
   for (int l = 0; l < m->L; l++) {
     LevelBuf &b = m->lv[l];
     void *q[] = {b.a, b.s, b.pl, b.up, b.da, b.d, b.gA, b.gS, b.gD, b.gU, b.gP};
@@ -409,19 +433,31 @@ template <typename InT, typename OutT> static void launchConv(const ConvArgs<InT
     NUSIF_TIMED("unet_conv", (kConvMfma<InT, OutT, 1, MB><<<grid(1), blk, 0, nusifStream()>>>(a)));
 }
 
-/* forward convolutions of cubes of extent >= 32: LDS-staged bricks, row
- * tiles chosen so the grid has enough workgroups */
-static void launchConvLds(const ConvArgs<uint16_t, uint16_t> &a)
+/* Forward convolutions on LDS-staged bricks; row tiles chosen so the grid
+ * has enough workgroups. Small cubes (few bricks) split K over the 32-channel
+ * chunks (grid.z) into the model's FP32 scratch, summed in chunk order by
+ * kConvSplitFinish, so the result does not depend on scheduling. */
+static void launchConvLds(UnetModel *m, const ConvArgs<uint16_t, uint16_t> &a)
 {
-  int rows = a.r0 + a.r1, n = a.n;
+  int rows = a.r0 + a.r1, n = a.n, cin = a.c0 + a.c1;
   unsigned bricks = (unsigned)(((n + CL_BX - 1) / CL_BX) * ((n + CL_BY - 1) / CL_BY) * ((n + CL_BZ - 1) / CL_BZ));
-  auto grid = [&](int NB) { return dim3(bricks, (rows + 16 * NB - 1) / (16 * NB)); };
+  int chunks      = (cin + CL_CC - 1) / CL_CC;
+  int split       = 1;
+  if (bricks < 256 && chunks > 1 &&
+      (size_t)chunks * n * n * n * rows <= m->splitCap) split = chunks;
+  auto grid = [&](int NB) { return dim3(bricks, (rows + 16 * NB - 1) / (16 * NB), split); };
+  hipStream_t st = nusifStream();
   if (rows >= 64 && bricks >= 512)
-    NUSIF_TIMED("unet_conv", (kConvLds<uint16_t, 4><<<grid(4), 256, 0, nusifStream()>>>(a)));
+    NUSIF_TIMED("unet_conv", (kConvLds<uint16_t, 4><<<grid(4), 256, 0, st>>>(a, m->split)));
   else if (rows >= 32 && bricks >= 512)
-    NUSIF_TIMED("unet_conv", (kConvLds<uint16_t, 2><<<grid(2), 256, 0, nusifStream()>>>(a)));
+    NUSIF_TIMED("unet_conv", (kConvLds<uint16_t, 2><<<grid(2), 256, 0, st>>>(a, m->split)));
   else
-    NUSIF_TIMED("unet_conv", (kConvLds<uint16_t, 1><<<grid(1), 256, 0, nusifStream()>>>(a)));
+    NUSIF_TIMED("unet_conv", (kConvLds<uint16_t, 1><<<grid(1), 256, 0, st>>>(a, m->split)));
+  if (split > 1) {
+    size_t total = (size_t)n * n * n * rows;
+    unsigned nb  = (unsigned)std::min<size_t>((total + 255) / 256, 4096);
+    kConvSplitFinish<uint16_t><<<nb, 256, 0, st>>>(a, m->split, split);
+  }
 }
 
 /* forward conv k: out = relu(conv(in0 ++ in1) + b) */
@@ -444,8 +480,8 @@ static void convForward(UnetModel *m, int k, const uint16_t *in0, int c0, const 
   a.tapSign = 1;
   a.relu = 1;
   a.accumulate = 0;
-  if (n >= 32 && (c0 + c1) % 8 == 0)
-    launchConvLds(a);
+  if ((c0 + c1) % 8 == 0)
+    launchConvLds(m, a);
   else
     launchConv(a);
 }

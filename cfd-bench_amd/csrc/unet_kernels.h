@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256) void kConvMfma(ConvArgs<InT, OutT> a)
 #define CL_CSTR 40
 
 template <typename OutT, int NB>
-__global__ __launch_bounds__(256) void kConvLds(ConvArgs<uint16_t, OutT> a)
+__global__ __launch_bounds__(256) void kConvLds(ConvArgs<uint16_t, OutT> a, float *__restrict__ split)
 {
   __shared__ __attribute__((aligned(16))) uint16_t sIn[CL_HV * CL_CSTR];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -186,7 +186,12 @@ __global__ __launch_bounds__(256) void kConvLds(ConvArgs<uint16_t, OutT> a)
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) acc[nb][mb] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  for (int cc0 = 0; cc0 < cin; cc0 += CL_CC) {
+  /* split-K: with gridDim.z > 1, workgroup z takes channel chunk z only and
+   * writes its FP32 partial sums to split[z][vox][rows] (kConvSplitFinish
+   * adds the chunks in order) */
+  const int cBeg = gridDim.z > 1 ? blockIdx.z * CL_CC : 0;
+  const int cEnd = gridDim.z > 1 ? min(cBeg + CL_CC, cin) : cin;
+  for (int cc0 = cBeg; cc0 < cEnd; cc0 += CL_CC) {
     const int g8 = min(CL_CC, cin - cc0) >> 3; /* 8-channel groups in this chunk */
     for (int u = tid; u < CL_HV * g8; u += 256) {
       int v = u / g8, q = u - v * g8;
@@ -229,6 +234,23 @@ __global__ __launch_bounds__(256) void kConvLds(ConvArgs<uint16_t, OutT> a)
   }
   const int x = x0 + lr, z = z0 + wave;
   if (x >= n || z >= n) return;
+  if (gridDim.z > 1) {
+    float *sp = split + (size_t)blockIdx.z * n * n * n * rows;
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++) {
+      int y = y0 + mb;
+      if (y >= n) continue;
+      size_t v = ((size_t)z * n + y) * n + x;
+#pragma unroll
+      for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          int r = row0 + nb * 16 + 4 * lg + q;
+          if (r < rows) sp[v * rows + r] = acc[nb][mb][q];
+        }
+    }
+    return;
+  }
 #pragma unroll
   for (int mb = 0; mb < 4; mb++) {
     int y = y0 + mb;
@@ -245,6 +267,25 @@ __global__ __launch_bounds__(256) void kConvLds(ConvArgs<uint16_t, OutT> a)
         if (r < a.r0) storeOut<OutT>(a.out0 + v * a.r0 + r, val, a.accumulate);
         else storeOut<OutT>(a.out1 + v * a.r1 + (r - a.r0), val, a.accumulate);
       }
+  }
+}
+
+/* sum of the split-K chunks in chunk order, bias, ReLU, store */
+template <typename OutT>
+__global__ __launch_bounds__(256) void kConvSplitFinish(ConvArgs<uint16_t, OutT> a, const float *__restrict__ split,
+    int chunks)
+{
+  const int rows = a.r0 + a.r1;
+  const size_t total = (size_t)a.n * a.n * a.n * rows;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    float val = 0.f;
+    for (int z = 0; z < chunks; z++) val += split[(size_t)z * total + e];
+    size_t v = e / rows;
+    int r    = (int)(e - v * rows);
+    val += a.bias ? a.bias[r] : 0.f;
+    if (a.relu) val = val > 0.f ? val : 0.f;
+    if (r < a.r0) storeOut<OutT>(a.out0 + v * a.r0 + r, val, a.accumulate);
+    else storeOut<OutT>(a.out1 + v * a.r1 + (r - a.r0), val, a.accumulate);
   }
 }
 

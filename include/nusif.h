@@ -45,8 +45,10 @@ void nusifFreeSolver(Solver *s);
  *   mg_maxcycles  int     gmg: cycle cap per solve (100)
  *   mg_omega      double  gmg: smoother relaxation (1.0)
  *   mg_project    0 | 1   gmg: remove the rhs mean (all-Neumann compatibility)
- *   mg_tail       0 | 1   gmg: run the levels of <= 32768 cells in one workgroup
+ *   mg_tail       0 | 1   gmg: run the levels of <= 4096 cells in one workgroup
  *                         (1, default) or with one kernel per pass (0); same values
+ *   mg_graph      0 | 1   gmg: replay each cycle from a captured HIP graph (1,
+ *                         default) or launch its kernels one by one (0); same values
  *   lid           double  lid speed of "dcavity" (reference: 1.0)
  *   u_inflow      double  inflow velocity of "canal" (reference: 2.0)
  *   inflow_noise  double  relative uniform perturbation of the inflow (0)

@@ -118,3 +118,32 @@ def test_gpu_gmg_tail_cycle_equals_per_level_kernels(shape, levels, precision, g
         c.close()
     assert out[0][1] == out[1][1]
     np.testing.assert_array_equal(out[0][0], out[1][0])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gpu_gmg_graph_replay_equals_eager_launches(precision, gpu_ok):
+    """Cycles replayed from captured HIP graphs (mg_graph 1) against the same
+    cycles launched kernel by kernel (mg_graph 0): whole cavity steps, each
+    with several cycles (replays of the cached graphs) and the step's own
+    eager kernels and host synchronisations between them; every field equal
+    bit for bit after every step."""
+    print("gmg graph", precision, flush=True)
+    prm = dict(G.TRAJ["c1_rb"][1], imax=64, jmax=64, kmax=64, re=1000.0, levels=5,
+               presmooth=2, postsmooth=2)
+    cases = []
+    for graph in (0, 1):
+        opts = {"solver": "gmg", "precision": precision, "mg_tol": 1e-8, "mg_graph": graph}
+        cases.append(A.Case(lib("gpu_rb"), options=opts, **prm))
+    total_cycles = 0
+    for step in range(6):
+        for c in cases:
+            c.step()
+        it = [c.lib.nusifSolverIterations(C.byref(c.s)) for c in cases]
+        assert it[0] == it[1]
+        total_cycles += it[0]
+        fa, fb = cases[0].fields(), cases[1].fields()
+        for k in ("u", "v", "w", "p"):
+            np.testing.assert_array_equal(fa[k], fb[k], err_msg="%s step %d" % (k, step))
+    assert total_cycles >= 6
+    for c in cases:
+        c.close()
