@@ -104,7 +104,8 @@ void nusifDefaultOptions(NusifOptions *o)
   o->mgOmega     = 1.0;
   o->mgProject   = 0;
   o->mgTail      = 1;
-  o->mgGraph     = 1;
+  o->mgGraph     = 0;
+  o->mgTileRows  = 0;
   o->lid         = 1.0;
   o->uInflow     = 2.0;
   o->inflowNoise = 0.0;
@@ -218,6 +219,11 @@ extern "C" int nusifSetOption(Parameter *p, const char *key, const char *value)
   if (k == "mg_project") return parseInt(v.c_str(), &o->mgProject);
   if (k == "mg_tail") return parseInt(v.c_str(), &o->mgTail);
   if (k == "mg_graph") return parseInt(v.c_str(), &o->mgGraph);
+  if (k == "mg_tile_rows") {
+    int r = parseInt(v.c_str(), &o->mgTileRows);
+    if (r == 0 && o->mgTileRows != 0 && o->mgTileRows != 4 && o->mgTileRows != 8) return -2;
+    return r;
+  }
   if (k == "lid") return parseReal(v.c_str(), &o->lid);
   if (k == "u_inflow") return parseReal(v.c_str(), &o->uInflow);
   if (k == "inflow_noise") return parseReal(v.c_str(), &o->inflowNoise);

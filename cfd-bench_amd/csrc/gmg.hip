@@ -86,7 +86,8 @@ struct GmgState {
   float *tab32;
   int nl;
   int nu1, nu2, maxCycles, project, mixed;
-  int tail; /* first level of the one-workgroup tail cycle, -1: none */
+  int tail;     /* first level of the one-workgroup tail cycle, -1: none */
+  int tileRows; /* fused-sweep tile height for FP32: 0 automatic, 4 or 8 */
   double tol, omega;
   LevelT<double> L64[MAXLEV];
   LevelT<float> L32[MAXLEV];
@@ -110,18 +111,6 @@ __device__ inline T lap7(const T *x, long long c, long long sj, long long sk, co
   T xc = x[c];
   return (x[c + 1] - (T)2 * xc + x[c - 1]) * cf.idx2 + (x[c + sj] - (T)2 * xc + x[c - sj]) * cf.idy2 +
          (x[c + sk] - (T)2 * xc + x[c - sk]) * cf.idz2;
-}
-
-/* Face ghosts of cell (i, j, k) from value v (reference ranges: each ghost
- * plane over the interior of the other two directions). */
-template <typename T> __device__ inline void faceGhosts(T *x, const Geo &g, int i, int j, int k, long long c, T v)
-{
-  if (i == 1) x[c - 1] = v;
-  if (i == g.ni) x[c + 1] = v;
-  if (j == 1) x[c - g.sj] = v;
-  if (j == g.nj) x[c + g.sj] = v;
-  if (k == 1) x[c - g.sk] = v;
-  if (k == g.nk) x[c + g.sk] = v;
 }
 
 /* One colour pass. Thread x owns the cell pair (2x+1, 2x+2) of a row and
@@ -161,7 +150,8 @@ __global__ __launch_bounds__(256) void kSmooth(T *__restrict__ x, const T *__res
 }
 
 /* One full red-black sweep in one launch (ping-pong: reads xo, writes xn).
- * A workgroup owns a 64 x 4 (i, j) tile and marches a chunk of FT_KC planes
+ * A workgroup of 256 threads owns a 64 x TJ (i, j) tile (each thread TJ/4
+ * cells of a column, rows ty, ty + 4, ...) and marches a chunk of kc planes
  * in k. Plane k's odd ("red") cells are updated from the old values of
  * planes k-1..k+1 held in LDS (X), over the tile plus a one-cell ring, into
  * LDS (Y); then plane k-1's even ("black") cells are updated from Y. The ring
@@ -179,34 +169,33 @@ __device__ inline void ldsBarrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_ba
 
 #define NUM_XCD 8 /* MI355X: 8 XCDs, each with its own L2 */
 #define FT_I 64
-#define FT_J 4
 #define FT_KC 32
 #define FUSED_MIN_EXTENT 32 /* levels narrower than this use the two-pass sweep */
-#define XW (FT_I + 4)
-#define XH (FT_J + 4)
-#define YW (FT_I + 2)
-#define YH (FT_J + 2)
 
-template <typename T>
-__global__ __launch_bounds__(256, 8) void kSweepFused(const T *__restrict__ xo, T *__restrict__ xn,
+template <typename T, int TJ, int MINW>
+__global__ __launch_bounds__(256, MINW) void kSweepFused(const T *__restrict__ xo, T *__restrict__ xn,
     const T *__restrict__ b, Geo g, Coef<T> cf, int kc)
 {
-  constexpr int NX = (XH * XW + 255) / 256; /* X elements per thread */
-  constexpr int NY = (YH * YW + 255) / 256; /* Y elements per thread */
+  constexpr int NT  = 256;
+  constexpr int NBK = 64 * TJ / NT; /* black cells per thread (rows ty, ty + 4, ...) */
+  constexpr int RS  = NT / 64;      /* thread rows */
+  constexpr int XW_ = FT_I + 4, XH_ = TJ + 4, YW_ = FT_I + 2, YH_ = TJ + 2;
+  constexpr int NX = (XH_ * XW_ + (NT - 1)) / NT; /* X elements per thread */
+  constexpr int NY = (YH_ * YW_ + (NT - 1)) / NT; /* Y elements per thread */
   /* slots padded to whole thread multiples so every LDS access is unconditional */
-  __shared__ T X[4][NX * 256];
-  __shared__ T Y[4][NY * 256];
+  __shared__ T X[4][NX * NT];
+  __shared__ T Y[4][NY * NT];
   const int tid = threadIdx.x;
   /* XCD-aware tile order: workgroup ids are dealt round-robin over the 8
    * XCDs, so id w runs on XCD w % 8. Giving each XCD a contiguous run of
    * tiles (x fastest, then y, then the k chunk) keeps the halos that
    * neighbouring tiles share inside one L2. */
-  const int gxT = (g.ni + FT_I - 1) / FT_I, gyT = (g.nj + FT_J - 1) / FT_J;
+  const int gxT = (g.ni + FT_I - 1) / FT_I, gyT = (g.nj + TJ - 1) / TJ;
   const int nTiles = gxT * gyT * ((g.nk + kc - 1) / kc);
   const int perXcd = (nTiles + NUM_XCD - 1) / NUM_XCD;
   const int t = (blockIdx.x % NUM_XCD) * perXcd + blockIdx.x / NUM_XCD;
   if (t >= nTiles) return;
-  const int i0 = (t % gxT) * FT_I + 1, j0 = ((t / gxT) % gyT) * FT_J + 1;
+  const int i0 = (t % gxT) * FT_I + 1, j0 = ((t / gxT) % gyT) * TJ + 1;
   const int k0 = (t / (gxT * gyT)) * kc + 1;
   const int k1 = min(k0 + kc, g.nk + 1); /* black planes [k0, k1) */
   const int nk = g.nk, sj = g.sj;
@@ -222,30 +211,36 @@ __global__ __launch_bounds__(256, 8) void kSweepFused(const T *__restrict__ xo, 
   int yPar[NY];  /* (i + j) & 1, or 2 when the element is not an interior cell */
 #pragma unroll
   for (int q = 0; q < NX; q++) {
-    int e = min(tid + 256 * q, XH * XW - 1);
-    int i = min(max(i0 - 2 + e % XW, 0), g.ni + 1);
-    int j = min(max(j0 - 2 + e / XW, 0), g.nj + 1);
+    int e = min(tid + NT * q, XH_ * XW_ - 1);
+    int i = min(max(i0 - 2 + e % XW_, 0), g.ni + 1);
+    int j = min(max(j0 - 2 + e / XW_, 0), g.nj + 1);
     xOff[q] = j * sj + i;
   }
 #pragma unroll
   for (int q = 0; q < NY; q++) {
-    int e  = tid + 256 * q;
-    int ec = min(e, YH * YW - 1);
-    int yi = ec % YW, yj = ec / YW;
+    int e  = tid + NT * q;
+    int ec = min(e, YH_ * YW_ - 1);
+    int yi = ec % YW_, yj = ec / YW_;
     int i = i0 - 1 + yi, j = j0 - 1 + yj;
-    bool in = e < YH * YW && i >= 1 && i <= g.ni && j >= 1 && j <= g.nj;
-    yXe[q]  = (yj + 1) * XW + (yi + 1);
+    bool in = e < YH_ * YW_ && i >= 1 && i <= g.ni && j >= 1 && j <= g.nj;
+    yXe[q]  = (yj + 1) * XW_ + (yi + 1);
     yOff[q] = min(j, g.nj + 1) * sj + min(i, g.ni + 1);
     yPar[q] = in ? ((i + j) & 1) : 2;
   }
   const int tx = tid & (FT_I - 1), ty = tid / FT_I;
-  const int bi = i0 + tx, bj = j0 + ty;
-  const bool bIn = bi <= g.ni && bj <= g.nj;
-  const int bOff = min(bj, g.nj + 1) * sj + min(bi, g.ni + 1);
-  const int bPar = (bi + bj) & 1;
-  const int bYe  = (ty + 1) * YW + (tx + 1);
-  const int bXe  = (ty + 2) * XW + (tx + 2);
-  const bool fIlo = bi == 1, fIhi = bi == g.ni, fJlo = bj == 1, fJhi = bj == g.nj;
+  int bOff[NBK], bPar[NBK], bYe[NBK], bXe[NBK];
+  bool bIn[NBK], fIlo[NBK], fIhi[NBK], fJlo[NBK], fJhi[NBK];
+#pragma unroll
+  for (int q = 0; q < NBK; q++) {
+    const int tyq = ty + q * RS;
+    const int bi = i0 + tx, bj = j0 + tyq;
+    bIn[q]  = bi <= g.ni && bj <= g.nj;
+    bOff[q] = min(bj, g.nj + 1) * sj + min(bi, g.ni + 1);
+    bPar[q] = (bi + bj) & 1;
+    bYe[q]  = (tyq + 1) * YW_ + (tx + 1);
+    bXe[q]  = (tyq + 2) * XW_ + (tx + 2);
+    fIlo[q] = bi == 1; fIhi[q] = bi == g.ni; fJlo[q] = bj == 1; fJhi[q] = bj == g.nj;
+  }
 
   T xr[NX];
   auto fetchX = [&](int k) {
@@ -255,7 +250,7 @@ __global__ __launch_bounds__(256, 8) void kSweepFused(const T *__restrict__ xo, 
   };
   auto storeX = [&](int k) {
 #pragma unroll
-    for (int q = 0; q < NX; q++) X[k & 3][tid + 256 * q] = xr[q];
+    for (int q = 0; q < NX; q++) X[k & 3][tid + NT * q] = xr[q];
   };
   auto fetchB = [&](int k, T *out) { /* b under this thread's Y elements, plane k */
     long long base = (long long)min(max(k, 1), nk) * sk;
@@ -271,51 +266,57 @@ __global__ __launch_bounds__(256, 8) void kSweepFused(const T *__restrict__ xo, 
       int xe = yXe[q];
       T v    = Xc[xe];
       T lap  = (Xc[xe + 1] - (T)2 * v + Xc[xe - 1]) * cf.idx2 +
-              (Xc[xe + XW] - (T)2 * v + Xc[xe - XW]) * cf.idy2 +
+              (Xc[xe + XW_] - (T)2 * v + Xc[xe - XW_]) * cf.idy2 +
               (Xp[xe] - (T)2 * v + Xm[xe]) * cf.idz2;
       T r    = bred[q] - lap;
       T vNew = v - cf.factor * r;
       bool red = kin && ((yPar[q] + k) & 1) == 1 && yPar[q] < 2;
-      Yc[tid + 256 * q] = red ? vNew : v;
+      Yc[tid + NT * q] = red ? vNew : v;
     }
   };
-  auto blackPlane = [&](int k, T bc) {
+  auto blackPlane = [&](int k, const T *bc) {
     const T *Yc = Y[k & 3], *Ym = Y[(k + 3) & 3], *Yp = Y[(k + 1) & 3];
-    T xc  = X[k & 3][bXe];
-    T lap = (Yc[bYe + 1] - (T)2 * xc + Yc[bYe - 1]) * cf.idx2 +
-            (Yc[bYe + YW] - (T)2 * xc + Yc[bYe - YW]) * cf.idy2 +
-            (Yp[bYe] - (T)2 * xc + Ym[bYe]) * cf.idz2;
-    T r    = bc - lap;
-    T vNew = xc - cf.factor * r;
-    T v    = (((bPar + k) & 1) == 0) ? vNew : Yc[bYe];
-    if (bIn) {
-      long long c = k * sk + bOff;
-      xn[c]       = v;
-      if (fIlo) xn[c - 1] = v; /* face ghosts (reference ranges) */
-      if (fIhi) xn[c + 1] = v;
-      if (fJlo) xn[c - sj] = v;
-      if (fJhi) xn[c + sj] = v;
-      if (k == 1) xn[c - sk] = v;
-      if (k == nk) xn[c + sk] = v;
+#pragma unroll
+    for (int q = 0; q < NBK; q++) {
+      T xc  = X[k & 3][bXe[q]];
+      T lap = (Yc[bYe[q] + 1] - (T)2 * xc + Yc[bYe[q] - 1]) * cf.idx2 +
+              (Yc[bYe[q] + YW_] - (T)2 * xc + Yc[bYe[q] - YW_]) * cf.idy2 +
+              (Yp[bYe[q]] - (T)2 * xc + Ym[bYe[q]]) * cf.idz2;
+      T r    = bc[q] - lap;
+      T vNew = xc - cf.factor * r;
+      T v    = (((bPar[q] + k) & 1) == 0) ? vNew : Yc[bYe[q]];
+      if (bIn[q]) {
+        long long c = k * sk + bOff[q];
+        xn[c]       = v;
+        if (fIlo[q]) xn[c - 1] = v;
+        if (fIhi[q]) xn[c + 1] = v;
+        if (fJlo[q]) xn[c - sj] = v;
+        if (fJhi[q]) xn[c + sj] = v;
+        if (k == 1) xn[c - sk] = v;
+        if (k == nk) xn[c + sk] = v;
+      }
     }
   };
 
   /* One iteration of the plane loop with register set (xs, bs, bbs): store
    * plane kk+1 to LDS, issue the loads of plane kk+3 (x), kk+2 (red b) and
    * kk+1 (own b) two iterations ahead, then red(kk) and black(kk-1). */
-  auto iterate = [&](int kk, T(&xs)[NX], T(&bs)[NY], T &bbs) {
+  auto iterate = [&](int kk, T(&xs)[NX], T(&bs)[NY], T(&bbs)[NBK]) {
 #pragma unroll
     for (int q = 0; q < NX; q++) xr[q] = xs[q];
     storeX(kk + 1);
     T bUse[NY];
 #pragma unroll
     for (int q = 0; q < NY; q++) bUse[q] = bs[q];
-    T bbUse = bbs;
+    T bbUse[NBK];
+#pragma unroll
+    for (int q = 0; q < NBK; q++) bbUse[q] = bbs[q];
     fetchX(kk + 3);
 #pragma unroll
     for (int q = 0; q < NX; q++) xs[q] = xr[q];
     fetchB(kk + 2, bs);
-    bbs = b[(long long)min(max(kk + 1, 1), nk) * sk + bOff];
+#pragma unroll
+    for (int q = 0; q < NBK; q++) bbs[q] = b[(long long)min(max(kk + 1, 1), nk) * sk + bOff[q]];
     ldsBarrier();
     redPlane(kk, bUse); /* ghost planes (kk = 0, nk+1) copy old values */
     ldsBarrier();
@@ -327,7 +328,9 @@ __global__ __launch_bounds__(256, 8) void kSweepFused(const T *__restrict__ xo, 
   storeX(k0 - 2 + 4);
   fetchX(k0 - 1);
   storeX(k0 - 1);
-  T xA[NX], xB[NX], bA[NY], bB[NY], bbA = 0, bbB = 0;
+  T xA[NX], xB[NX], bA[NY], bB[NY], bbA[NBK], bbB[NBK];
+#pragma unroll
+  for (int q = 0; q < NBK; q++) bbA[q] = bbB[q] = (T)0;
   fetchX(k0);
 #pragma unroll
   for (int q = 0; q < NX; q++) xA[q] = xr[q];
@@ -345,22 +348,50 @@ __global__ __launch_bounds__(256, 8) void kSweepFused(const T *__restrict__ xo, 
   }
 }
 
-/* k-chunk of the fused sweep: the longest (<= FT_KC planes, a power of two
- * >= 4) that still gives >= 2048 workgroups (8 per CU), so small levels
- * trade some halo re-loading for enough waves to hide latency */
-static int fusedChunk(const Geo &g)
+/* Launch shape of the fused sweep on a level. Taller tiles (64 x 8, two
+ * cells per thread) measured 10 % faster for FP32 at 256^3 (58 vs 64 us)
+ * and slower for FP64 and on coarser levels, so they are used for FP32
+ * levels whose grid still has >= 1024 such workgroups at the full k-chunk.
+ * The k-chunk is the longest (<= FT_KC planes, a power of two >= 4) that
+ * still gives >= 2048 workgroups (8 per CU), so small levels trade some halo
+ * re-loading for enough waves to hide latency. */
+struct SweepShape {
+  int tj, kc;
+};
+static long long sweepTiles(const Geo &g, int tj, int kc)
 {
-  long long tiles = (long long)((g.ni + FT_I - 1) / FT_I) * ((g.nj + FT_J - 1) / FT_J);
-  int kc          = FT_KC;
-  while (kc > 4 && tiles * ((g.nk + kc - 1) / kc) < 2048) kc /= 2;
-  return kc;
+  return (long long)((g.ni + FT_I - 1) / FT_I) * ((g.nj + tj - 1) / tj) * ((g.nk + kc - 1) / kc);
+}
+template <typename T> static SweepShape sweepShape(const Geo &g, int rows)
+{
+  if (sizeof(T) == 4 && (rows == 8 || (rows == 0 && sweepTiles(g, 8, FT_KC) >= 1024))) return {8, FT_KC};
+  int kc = FT_KC;
+  while (kc > 4 && sweepTiles(g, 4, kc) < 2048) kc /= 2;
+  return {4, kc};
 }
 
 /* 1-D grid: NUM_XCD * ceil(tiles / NUM_XCD) workgroups (surplus ids exit) */
-static dim3 fusedGrid(const Geo &g, int kc)
+static dim3 fusedGrid(const Geo &g, SweepShape s)
 {
-  int n = ((g.ni + FT_I - 1) / FT_I) * ((g.nj + FT_J - 1) / FT_J) * ((g.nk + kc - 1) / kc);
-  return dim3(((n + NUM_XCD - 1) / NUM_XCD) * NUM_XCD);
+  long long n = sweepTiles(g, s.tj, s.kc);
+  return dim3((unsigned)(((n + NUM_XCD - 1) / NUM_XCD) * NUM_XCD));
+}
+
+template <typename T> static void launchSweep(const LevelT<T> &L, SweepShape s, bool timed)
+{
+  dim3 grd = fusedGrid(L.g, s);
+  if (s.tj == 8) {
+    if (timed)
+      NUSIF_TIMED("gmg_smooth32_l0", (kSweepFused<T, 8, 6><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, s.kc)));
+    else
+      kSweepFused<T, 8, 6><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, s.kc);
+  } else {
+    if (timed)
+      NUSIF_TIMED(sizeof(T) == 8 ? "gmg_smooth64_l0" : "gmg_smooth32_l0",
+          (kSweepFused<T, 4, 8><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, s.kc)));
+    else
+      kSweepFused<T, 4, 8><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, s.kc);
+  }
 }
 
 /* b_c(I,J,K) = (1/8) sum over the 8 children of (b - A x); the coarse x is
@@ -866,7 +897,7 @@ template <typename T> static void fill(T *x, long long n, T v)
 
 /* Timers cover the finest level only ("..._l0"), whose algorithmic bytes
  * nusifKernelBytes reports. Each sweep swaps x and xAlt. */
-template <typename T> static void smooth(LevelT<T> &L, int sweeps, bool fine)
+template <typename T> static void smooth(GmgState *G, LevelT<T> &L, int sweeps, bool fine)
 {
   if (L.g.ni < FUSED_MIN_EXTENT || L.g.nk < 4) { /* small levels: two colour passes */
     dim3 grd = sweepGrid(L.g);
@@ -876,14 +907,9 @@ template <typename T> static void smooth(LevelT<T> &L, int sweeps, bool fine)
     }
     return;
   }
-  const int kc = fusedChunk(L.g);
-  dim3 grd     = fusedGrid(L.g, kc);
+  const SweepShape shape = sweepShape<T>(L.g, G->tileRows);
   for (int s = 0; s < sweeps; s++) {
-    if (fine)
-      NUSIF_TIMED(sizeof(T) == 8 ? "gmg_smooth64_l0" : "gmg_smooth32_l0",
-          (kSweepFused<T><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc)));
-    else
-      kSweepFused<T><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc);
+    launchSweep(L, shape, fine);
     T *t   = L.x;
     L.x    = L.xAlt;
     L.xAlt = t;
@@ -962,7 +988,7 @@ template <typename T> static void vcycle(GmgState *G, LevelT<T> *L, int l)
     return;
   }
   bool fine = l == 0;
-  smooth(L[l], G->nu1, fine);
+  smooth(G, L[l], G->nu1, fine);
   LevelT<T> &C = L[l + 1];
   if (fine)
     NUSIF_TIMED(sizeof(T) == 8 ? "gmg_restrict64_l0" : "gmg_restrict32_l0",
@@ -980,7 +1006,7 @@ template <typename T> static void vcycle(GmgState *G, LevelT<T> *L, int l)
   else
     kProlongMarch<T><<<xcdGrid(pg), dim3(GTX, GTY), 0, nusifStream()>>>(
         L[l].x, L[l].g, C.x, C.g, pg.x, pg.y, pg.z);
-  smooth(L[l], G->nu2, fine);
+  smooth(G, L[l], G->nu2, fine);
 }
 
 template <typename T> static T *devAlloc(long long n)
@@ -1004,6 +1030,7 @@ GmgState *gmgCreate(NusifSolverCtx *sc, const Parameter *prm)
     G->L32[l] = LevelT<float>();
   }
   G->useGraphs = c->opt.mgGraph;
+  G->tileRows  = c->opt.mgTileRows;
   G->nu1       = prm->presmooth;
   G->nu2       = prm->postsmooth;
   G->maxCycles = c->opt.mgMaxCycles;
@@ -1250,14 +1277,18 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
     launchResidual(c, G, p, rhs, warmstartResidualBuffer(ws), nullptr, c->scal->norm2 + 2);
     warmstartCaptureBegin(ws, p, rhs, g);
     if (warmstartNet(ws)) {
-      warmstartGuess(ws, p, rhs, c->pAlt);
-      /* the guess's residual, in mixed mode straight into the first defect
-       * buffer (L0.b) with the first correction zeroed, so an accepted guess
-       * costs no second residual pass; then its restriction to the network
-       * grid, and one read-back for the decision */
+      const float *d = warmstartCorrection(ws, rhs);
+      /* the guess p + d into c->pAlt and its residual in one pass, in mixed
+       * mode straight into the first defect buffer (L0.b) with the first
+       * correction zeroed, so an accepted guess costs no further residual
+       * pass; then its restriction to the network grid, and one read-back
+       * for the decision */
       float *rg  = G->mixed ? G->L32[0].b : warmstartResidualBuffer(ws);
       float *e0  = G->mixed ? G->L32[0].x : nullptr;
-      launchResidual(c, G, c->pAlt, rhs, rg, e0, c->scal->norm2);
+      dim3 ng    = normGrid(g);
+      NUSIF_TIMED("gmg_residual64", (kResidualLds<true><<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(
+                                        p, d, rhs, g, G->cf0, c->pAlt, rg, e0, c->partials, ng.x, ng.y, ng.z)));
+      kFoldNorms<<<1, 256, 0, nusifStream()>>>(c->partials, (int)(ng.x * ng.y * ng.z), c->scal->norm2);
       warmstartCoarseNormLaunch(ws, rg);
       const double *wn = warmstartNormsAsync(ws);
       double nbg       = 0.0;

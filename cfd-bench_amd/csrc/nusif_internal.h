@@ -65,6 +65,19 @@ static inline dim3 xcdGrid(unsigned gx, unsigned gy, unsigned gz)
   return dim3(((n + NUM_XCDS - 1) / NUM_XCDS) * NUM_XCDS);
 }
 static inline dim3 xcdGrid(dim3 g) { return xcdGrid(g.x, g.y, g.z); }
+/* Face ghosts of cell (i, j, k) from value v: the Neumann copy of the
+ * pressure solvers (reference src/solver-rb.c:129-148), each ghost plane over
+ * the interior of the other two directions. */
+template <typename T> __device__ inline void faceGhosts(T *x, const Geo &g, int i, int j, int k, long long c, T v)
+{
+  if (i == 1) x[c - 1] = v;
+  if (i == g.ni) x[c + 1] = v;
+  if (j == 1) x[c - g.sj] = v;
+  if (j == g.nj) x[c + g.sj] = v;
+  if (k == 1) x[c - g.sk] = v;
+  if (k == g.nk) x[c + g.sk] = v;
+}
+
 /* Logical block (bx, by, bz) of this workgroup; false for surplus ids. */
 __device__ inline bool xcdBlock(int gx, int gy, int gz, int &bx, int &by, int &bz)
 {
@@ -90,6 +103,7 @@ struct NusifOptions {
   int mgProject;
   int mgTail;      /* gmg: coarse levels in one workgroup (1) or per-level kernels (0) */
   int mgGraph;     /* gmg: replay each cycle from a captured HIP graph (1) or launch eagerly (0) */
+  int mgTileRows;  /* gmg: FP32 fused-sweep tile height, 0 automatic, 4 or 8 */
   double lid;
   double uInflow;
   double inflowNoise;

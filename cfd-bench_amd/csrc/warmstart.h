@@ -20,7 +20,7 @@ bool warmstartCapturing(const WarmStart *w);
 float *warmstartResidualBuffer(WarmStart *w);
 void warmstartCoarseNormLaunch(WarmStart *w, const float *rFine);
 const double *warmstartNormsAsync(WarmStart *w);
-void warmstartGuess(WarmStart *w, const double *p, const double *rhs, double *guess);
+const float *warmstartCorrection(WarmStart *w, const double *rhs);
 void warmstartCaptureBegin(WarmStart *w, const double *p, const double *rhs, const Geo &g0);
 void warmstartCaptureEnd(WarmStart *w, const double *p);
 size_t warmstartSamples(const WarmStart *w);

@@ -78,14 +78,18 @@ __global__ __launch_bounds__(256) void kCast(const double *__restrict__ a, float
 }
 
 /* per-block partial sums of r^2 and b^2 over the interior of level Ln */
+/* sums of r^2 and b^2 over the interior; a workgroup covers 64 x 4 (i, j)
+ * and SQ_KC planes (enough workgroups on the small network grid) */
+#define SQ_KC 4
 __global__ __launch_bounds__(256) void kSumSq2(const float *r, const float *b, Geo g, double *partials)
 {
   __shared__ double lds[4];
   int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int k0 = blockIdx.z * SQ_KC + 1, k1 = min(k0 + SQ_KC, g.nk + 1);
   double a0 = 0.0, a1 = 0.0;
   if (i <= g.ni && j <= g.nj)
-    for (int k = 1; k <= g.nk; k++) {
+    for (int k = k0; k < k1; k++) {
       long long c = g.at(i, j, k);
       a0 += (double)r[c] * r[c];
       a1 += (double)b[c] * b[c];
@@ -93,10 +97,15 @@ __global__ __launch_bounds__(256) void kSumSq2(const float *r, const float *b, G
   a0 = blockSum(a0, lds);
   a1 = blockSum(a1, lds);
   if (threadIdx.x == 0 && threadIdx.y == 0) {
-    unsigned bid             = blockIdx.x + gridDim.x * blockIdx.y;
+    unsigned bid          = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     partials[2 * bid]     = a0;
     partials[2 * bid + 1] = a1;
   }
+}
+
+static dim3 sumGrid(const Geo &g)
+{
+  return dim3((g.ni + 63) / 64, (g.nj + 3) / 4, (g.nk + SQ_KC - 1) / SQ_KC);
 }
 
 __global__ __launch_bounds__(256) void kFoldSumSq2(const double *partials, int nb, double *out)
@@ -134,7 +143,10 @@ __global__ void kUnpack(const float *y, Geo g, const double *scal, double h2, fl
   float f   = (float)(sr * h2);
   for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (size_t)gridDim.x * blockDim.x) {
     int i = (int)(v % g.ni) + 1, j = (int)((v / g.ni) % g.nj) + 1, k = (int)(v / ((size_t)g.ni * g.nj)) + 1;
-    d[g.at(i, j, k)] = y[v] * f;
+    long long c = g.at(i, j, k);
+    float val   = y[v] * f;
+    d[c]        = val;
+    faceGhosts(d, g, i, j, k, c, val);
   }
 }
 
@@ -176,25 +188,10 @@ __global__ __launch_bounds__(256) void kProlongSet(float *xf, Geo gf, const floa
   float c21 = a * xc[gc.at(I, J2, K)] + q * xc[gc.at(I2, J2, K)];
   float c12 = a * xc[gc.at(I, J, K2)] + q * xc[gc.at(I2, J, K2)];
   float c22 = a * xc[gc.at(I, J2, K2)] + q * xc[gc.at(I2, J2, K2)];
-  xf[gf.at(i, j, k)] = a * (a * c11 + q * c21) + q * (a * c12 + q * c22);
-}
-
-/* guess = p + d (FP64), face ghosts of the guess written */
-__global__ __launch_bounds__(256) void kGuess(const double *p, const float *d, Geo g, double *out)
-{
-  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  int k = blockIdx.z + 1;
-  if (i > g.ni || j > g.nj) return;
-  long long c = g.at(i, j, k);
-  double v    = p[c] + (double)d[c];
-  out[c]      = v;
-  if (i == 1) out[c - 1] = v;
-  if (i == g.ni) out[c + 1] = v;
-  if (j == 1) out[c - g.sj] = v;
-  if (j == g.nj) out[c + g.sj] = v;
-  if (k == 1) out[c - g.sk] = v;
-  if (k == g.nk) out[c + g.sk] = v;
+  long long c = gf.at(i, j, k);
+  float v     = a * (a * c11 + q * c21) + q * (a * c12 + q * c22);
+  xf[c]       = v;
+  faceGhosts(xf, gf, i, j, k, c, v); /* Neumann copies: the finest level's are read as p + d */
 }
 
 /* ---- host ------------------------------------------------------------------ */
@@ -293,15 +290,18 @@ static void buildInput(WarmStart *w, const double *rhs)
     }
   }
   const WsLevel &L = w->lv[w->Ln];
-  dim3 g2((L.g.ni + WTX - 1) / WTX, (L.g.nj + WTY - 1) / WTY);
+  dim3 g2 = sumGrid(L.g);
   kSumSq2<<<g2, dim3(WTX, WTY), 0, st>>>(L.r, L.b, L.g, w->partials);
-  kFoldSumSq2<<<1, 256, 0, st>>>(w->partials, (int)(g2.x * g2.y), w->scal);
+  kFoldSumSq2<<<1, 256, 0, st>>>(w->partials, (int)(g2.x * g2.y * g2.z), w->scal);
   kPack<<<1024, 256, 0, st>>>(L.r, L.b, L.g, w->scal, w->x);
 }
 
-/* Network guess into `guess` (FP64, face ghosts set), from p and the r0
- * already in lv[0].r. */
-void warmstartGuess(WarmStart *w, const double *p, const double *rhs, double *guess)
+/* The network's correction on the finest grid (FP32, interior and face
+ * ghosts), from the r0 already in lv[0].r and the rhs: restrict, pack,
+ * forward, unpack on the network grid, prolong trilinearly to level 0. The
+ * guess is p + this correction (the caller forms it together with its
+ * residual). */
+const float *warmstartCorrection(WarmStart *w, const double *rhs)
 {
   hipStream_t st = nusifStream();
   buildInput(w, rhs);
@@ -310,7 +310,7 @@ void warmstartGuess(WarmStart *w, const double *p, const double *rhs, double *gu
   kUnpack<<<1024, 256, 0, st>>>(unetOutput(w->net), L.g, w->scal, w->h2, L.d);
   for (int l = w->Ln; l > 0; l--)
     kProlongSet<<<cells(w->lv[l - 1].g), dim3(WTX, WTY), 0, st>>>(w->lv[l - 1].d, w->lv[l - 1].g, w->lv[l].d, w->lv[l].g);
-  kGuess<<<cells(w->lv[0].g), dim3(WTX, WTY), 0, st>>>(p, w->lv[0].d, w->lv[0].g, guess);
+  return w->lv[0].d;
 }
 
 float *warmstartResidualBuffer(WarmStart *w) { return w->lv[0].r; }
@@ -328,9 +328,9 @@ void warmstartCoarseNormLaunch(WarmStart *w, const float *rFine)
     src = w->lv[l].r;
   }
   const WsLevel &L = w->lv[w->Ln];
-  dim3 g2((L.g.ni + WTX - 1) / WTX, (L.g.nj + WTY - 1) / WTY);
+  dim3 g2 = sumGrid(L.g);
   kSumSq2<<<g2, dim3(WTX, WTY), 0, st>>>(src, src, L.g, w->partials);
-  kFoldSumSq2<<<1, 256, 0, st>>>(w->partials, (int)(g2.x * g2.y), w->scal + 2);
+  kFoldSumSq2<<<1, 256, 0, st>>>(w->partials, (int)(g2.x * g2.y * g2.z), w->scal + 2);
 }
 
 /* enqueue the copy of scal[0..3] ([0] = sum (R r0)^2, [2] = the last
@@ -385,9 +385,22 @@ void warmstartTakeSamples(WarmStart *w, std::vector<uint16_t *> &xs, std::vector
 static const int kUnetLevels = 4;
 static const int kUnetChannels[4] = {16, 32, 64, 128};
 
+extern "C" int nusifUnetTrainCavityGrids(const char *weightsOut, const int *grids, int ngrids, int runs,
+    int stepsPerRun, int skipSteps, int epochs, int batch, double lr, int seed, double *lossPerEpoch);
+
 extern "C" int nusifUnetTrainCavity(const char *weightsOut, int grid, int runs, int stepsPerRun,
     int skipSteps, int epochs, int batch, double lr, int seed, double *lossPerEpoch)
 {
+  return nusifUnetTrainCavityGrids(weightsOut, &grid, 1, runs, stepsPerRun, skipSteps, epochs, batch, lr, seed,
+      lossPerEpoch);
+}
+
+/* as nusifUnetTrainCavity, run r on a grids[r % ngrids]^3 cavity (all grids
+ * must give the same network extent) */
+extern "C" int nusifUnetTrainCavityGrids(const char *weightsOut, const int *grids, int ngrids, int runs,
+    int stepsPerRun, int skipSteps, int epochs, int batch, double lr, int seed, double *lossPerEpoch)
+{
+  if (ngrids < 1) NUSIF_FATAL("unet training: no grid given");
   std::vector<uint16_t *> xs;
   std::vector<float *> ts;
   unsigned long long rng = 0x243F6A8885A308D3ull ^ ((unsigned long long)seed * 0x9E3779B97F4A7C15ull);
@@ -399,6 +412,7 @@ extern "C" int nusifUnetTrainCavity(const char *weightsOut, int grid, int runs, 
   };
   int netN = 0;
   for (int r = 0; r < runs; r++) {
+    const int grid = grids[r % ngrids];
     double re = 10.0 + 990.0 * uni(), lid = 0.5 + 1.5 * uni();
     Parameter prm;
     initParameter(&prm);
@@ -430,12 +444,14 @@ extern "C" int nusifUnetTrainCavity(const char *weightsOut, int grid, int runs, 
     NusifSolverCtx *sc = nusifSolverCtxOf(&s);
     if (skipSteps > 0) nusifRunSteps(&d, &s, skipSteps, nullptr);
     sc->ws = warmstartCreate(sc->dctx, nullptr, true);
-    netN   = sc->ws->n;
+    if (netN && sc->ws->n != netN)
+      NUSIF_FATAL("unet training: grid %d gives network extent %d, earlier grids %d", grid, sc->ws->n, netN);
+    netN = sc->ws->n;
     nusifRunSteps(&d, &s, stepsPerRun, nullptr);
     warmstartTakeSamples(sc->ws, xs, ts);
     nusifFreeSolver(&s);
     nusifFreeDiscretization(&d);
-    printf("[unet] run %d/%d: Re %.1f lid %.2f, %zu pairs so far\n", r + 1, runs, re, lid, ts.size());
+    printf("[unet] run %d/%d: %d^3, Re %.1f lid %.2f, %zu pairs so far\n", r + 1, runs, grid, re, lid, ts.size());
     fflush(stdout);
   }
   if (ts.empty()) return 0;

@@ -1,15 +1,16 @@
 """Trains the warm-start U-Net on the GPU and measures what it gains.
 
-  python cfd-bench_amd/train_unet.py [--grid 128] [--runs R] [--steps S] [--epochs E]
+  python cfd-bench_amd/train_unet.py [--grid 128 [256 ...]] [--runs R] [--steps S] [--epochs E]
                                       [--out cfd-bench_amd/weights/unet_cavity.bin]
 
 Training pairs are generated by the solver itself (lid-driven cavities of
-grid^3 cells, seeded Re ~ U[10, 1000], lid speed ~ U[0.5, 2], gmg with FP32
-V-cycles in FP64 defect correction to 1e-6), then the network is trained with
-Adam, all inside the library (nusifUnetTrainCavity). The evaluation runs a
-held-out cavity (Re and lid not drawn from the training seed) with and
-without the warm start and prints one JSON line: V-cycles per step, ms per
-step, and how often the network's guess beat the previous-step pressure.
+grid^3 cells, run r on the r-th grid in turn, seeded Re ~ U[10, 1000], lid
+speed ~ U[0.5, 2], gmg with FP32 V-cycles in FP64 defect correction to 1e-6),
+then the network is trained with Adam, all inside the library
+(nusifUnetTrainCavityGrids). The evaluation runs held-out cavities (Re and
+lid not drawn from the training seed) on every grid with and without the
+warm start and prints one JSON line: V-cycles per step, ms per step, and how
+often the network's guess beat the previous-step pressure.
 """
 import argparse
 import ctypes as C
@@ -60,7 +61,7 @@ warmup = 10
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--grid", type=int, nargs="+", default=[128])
     ap.add_argument("--runs", type=int, default=16)
     ap.add_argument("--skip", type=int, default=5)
     ap.add_argument("--steps", type=int, default=60)
@@ -80,16 +81,18 @@ def main():
     if not a.eval_only:
         losses = (C.c_double * a.epochs)()
         t0 = time.perf_counter()
-        n = lib.nusifUnetTrainCavity(a.out.encode(), a.grid, a.runs, a.steps, a.skip, a.epochs,
-                                     a.batch, a.lr, a.seed, losses)
+        grids = (C.c_int * len(a.grid))(*a.grid)
+        n = lib.nusifUnetTrainCavityGrids(a.out.encode(), grids, len(a.grid), a.runs, a.steps, a.skip,
+                                          a.epochs, a.batch, a.lr, a.seed, losses)
         report.update(pairs=n, train_seconds=time.perf_counter() - t0, loss_per_epoch=list(losses))
     # held-out case: Re and lid chosen outside the training draws
-    for re, lid in ((500.0, 1.0), (150.0, 1.6)):
-        base = evaluate(lib, a.out, a.grid, re, lid, False, a.eval_steps)
-        warm = evaluate(lib, a.out, a.grid, re, lid, True, a.eval_steps)
-        force = evaluate(lib, a.out, a.grid, re, lid, True, a.eval_steps, force=True)
-        report["eval_re%g_lid%g" % (re, lid)] = {"without": base, "with_unet": warm,
-                                                 "always_guess": force}
+    for grid in a.grid:
+        for re, lid in ((500.0, 1.0), (150.0, 1.6)):
+            base = evaluate(lib, a.out, grid, re, lid, False, a.eval_steps)
+            warm = evaluate(lib, a.out, grid, re, lid, True, a.eval_steps)
+            force = evaluate(lib, a.out, grid, re, lid, True, a.eval_steps, force=True)
+            report["eval_%d_re%g_lid%g" % (grid, re, lid)] = {"without": base, "with_unet": warm,
+                                                              "always_guess": force}
     print(json.dumps(report), flush=True)
 
 

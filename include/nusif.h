@@ -47,8 +47,10 @@ void nusifFreeSolver(Solver *s);
  *   mg_project    0 | 1   gmg: remove the rhs mean (all-Neumann compatibility)
  *   mg_tail       0 | 1   gmg: run the levels of <= 4096 cells in one workgroup
  *                         (1, default) or with one kernel per pass (0); same values
- *   mg_graph      0 | 1   gmg: replay each cycle from a captured HIP graph (1,
- *                         default) or launch its kernels one by one (0); same values
+ *   mg_graph      0 | 1   gmg: replay each cycle from a captured HIP graph (1) or
+ *                         launch its kernels one by one (0, default); same values
+ *   mg_tile_rows  0|4|8   gmg: FP32 fused-sweep tile height (0: automatic, 8 on
+ *                         the largest levels); same values
  *   lid           double  lid speed of "dcavity" (reference: 1.0)
  *   u_inflow      double  inflow velocity of "canal" (reference: 2.0)
  *   inflow_noise  double  relative uniform perturbation of the inflow (0)
@@ -107,6 +109,10 @@ double nusifCopyProbe(size_t bytes, int reps);
  * epoch. Returns the number of training pairs. */
 int nusifUnetTrainCavity(const char *weightsOut, int grid, int runs, int stepsPerRun,
     int skipSteps, int epochs, int batch, double lr, int seed, double *lossPerEpoch);
+/* The same with run r on a grids[r % ngrids]^3 cavity (every grid must give
+ * the same network extent, e.g. 128 and 256 both give 64). */
+int nusifUnetTrainCavityGrids(const char *weightsOut, const int *grids, int ngrids, int runs,
+    int stepsPerRun, int skipSteps, int epochs, int batch, double lr, int seed, double *lossPerEpoch);
 
 /* One forward and backward pass of the U-Net on the GPU, for gradient
  * checks: cfg = {levels, ch0, ch1, ch2, ch3, input channels}, x is n^3 x

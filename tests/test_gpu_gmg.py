@@ -147,3 +147,23 @@ def test_gpu_gmg_graph_replay_equals_eager_launches(precision, gpu_ok):
     assert total_cycles >= 6
     for c in cases:
         c.close()
+
+
+def test_gpu_gmg_tall_tile_sweep_equals_default(gpu_ok):
+    """The FP32 fused sweep with 64 x 8 tiles (two cells per thread, used on
+    the largest levels) against 64 x 4 tiles: same p bit for bit."""
+    print("gmg tile rows", flush=True)
+    prm = dict(G.KAT_PARAMS, imax=128, jmax=40, kmax=72, xlength=4.0, ylength=1.25,
+               zlength=2.25, levels=3, presmooth=2, postsmooth=2)
+    out = []
+    for rows in (4, 8):
+        opts = {"solver": "gmg", "precision": "mixed", "mg_tol": 1e-9, "mg_project": 1,
+                "mg_tile_rows": rows}
+        c = A.Case(lib("gpu_rb"), options=opts, **prm)
+        f = G.random_fields(c.shape, 9)
+        c.set_fields(p=np.zeros(c.shape), rhs=f["rhs"])
+        c.lib.solve(C.byref(c.s), c.d.p, c.d.rhs)
+        out.append((c.fields()["p"], c.lib.nusifSolverIterations(C.byref(c.s))))
+        c.close()
+    assert out[0][1] == out[1][1]
+    np.testing.assert_array_equal(out[0][0], out[1][0])
