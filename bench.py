@@ -43,13 +43,15 @@ EXTRA_OPTS = {}
 
 
 def mode_options(mode):
-    opts = {"solver": "gmg", "mg_tol": 1e-6}
-    opts.update(EXTRA_OPTS)
+    # smoother over-relaxed (omega 1.3) in every mode, the FP64 comparison included:
+    # 3.79 -> 3.08 V(2,2) cycles per step at 256^3 (measured over omega 1.0 .. 1.6)
+    opts = {"solver": "gmg", "mg_tol": 1e-6, "mg_omega": 1.3}
     if mode in ("mixed", "learned"):
         opts["precision"] = "mixed"
     if mode == "learned":
         opts["warmstart"] = "unet"
         opts["unet_weights"] = os.path.join(ROOT, "cfd-bench_amd", "weights", "unet_cavity.bin")
+    opts.update(EXTRA_OPTS)
     return opts
 
 
@@ -186,8 +188,8 @@ def main():
         "dtype": "f32+f64" if a.mode != "fp64" else "f64", "data": "synthetic",
         "mlups": cells * sps * 1e-6,
         "config": {"workload": "lid-driven cavity %d^3 cells, Re=1000, lid u=1, tau=0.5, "
-                               "pressure to 1e-6 relative residual per step, V(2,2) 6 levels; "
-                               "mode %s" % (n, a.mode),
+                               "pressure to 1e-6 relative residual per step, V(2,2) 6 levels, "
+                               "red-black smoother omega 1.3; mode %s" % (n, a.mode),
                    "grid": [n, n, n], "mode": a.mode, "device": name.value.decode(),
                    "options": dict(EXTRA_OPTS)},
         "cycles_per_step": sum(res["cycles"]) / len(res["cycles"]),

@@ -51,9 +51,9 @@ __global__ __launch_bounds__(256) void kAbsMax3(const double *__restrict__ u,
   }
 }
 
-__global__ __launch_bounds__(256) void kFoldMax3(const double *partials, int nb, DevScalars *sc)
+__global__ __launch_bounds__(1024) void kFoldMax3(const double *partials, int nb, DevScalars *sc)
 {
-  __shared__ double lds[4];
+  __shared__ double lds[16]; /* one per wavefront */
   for (int q = 0; q < 3; q++) {
     double m = foldMax(partials, nb, 3, q, DBL_MIN, lds);
     if (threadIdx.x == 0) sc->absmax[q] = (unsigned long long)__double_as_longlong(m);
@@ -68,12 +68,63 @@ static int reductionBlocks(long long n)
   return (int)b;
 }
 
+/* Maxima over the ghost shell of the padded box: the planes k = 0, nk+1,
+ * the rows j = 0, nj+1 and the cells i = 0, ni+1 of the planes between. */
+#define SHELL_BLOCKS 256
+__global__ __launch_bounds__(256) void kShellMax3(const double *__restrict__ u,
+    const double *__restrict__ v, const double *__restrict__ w, Geo g, double *__restrict__ partials)
+{
+  __shared__ double lds[4];
+  const long long plane = g.sk, rowsPerPlane = 2LL * g.sj + 2LL * g.nj;
+  const long long total = 2 * plane + (long long)g.nk * rowsPerPlane;
+  double mu = DBL_MIN, mv = DBL_MIN, mw = DBL_MIN;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long long)gridDim.x * blockDim.x) {
+    long long c;
+    if (q < 2 * plane) {
+      c = q < plane ? q : (long long)(g.nk + 1) * g.sk + (q - plane);
+    } else {
+      long long r = q - 2 * plane;
+      int k = (int)(r / rowsPerPlane) + 1;
+      long long e = r % rowsPerPlane;
+      if (e < 2LL * g.sj) {
+        int j = e < g.sj ? 0 : g.nj + 1;
+        c = g.at((int)(e % g.sj), j, k);
+      } else {
+        long long f = e - 2LL * g.sj; /* cells i = 0 and ni+1 of rows 1..nj */
+        c = g.at(f % 2 ? g.ni + 1 : 0, (int)(f / 2) + 1, k);
+      }
+    }
+    double a = fabs(u[c]), b = fabs(v[c]), d = fabs(w[c]);
+    mu = mu > a ? mu : a;
+    mv = mv > b ? mv : b;
+    mw = mw > d ? mw : d;
+  }
+  mu = blockMax(mu, lds);
+  mv = blockMax(mv, lds);
+  mw = blockMax(mw, lds);
+  if (threadIdx.x == 0) {
+    partials[3 * blockIdx.x + 0] = mu;
+    partials[3 * blockIdx.x + 1] = mv;
+    partials[3 * blockIdx.x + 2] = mw;
+  }
+}
+
+/* max |u|, |v|, |w| over the padded arrays into scal->absmax. After an
+ * adaptUV that nothing has overwritten since, only the ghost shell is read
+ * (the interior-range maxima are adaptUV's); otherwise the whole arrays. */
 void launchAbsMax3(NusifCtx *c)
 {
+  if (c->cflValid) {
+    double *shell = c->cflPartials + 3LL * c->cflInteriorBlocks;
+    kShellMax3<<<SHELL_BLOCKS, 256, 0, nusifStream()>>>(c->u, c->v, c->w, c->geo, shell);
+    kFoldMax3<<<1, 1024, 0, nusifStream()>>>(c->cflPartials, c->cflInteriorBlocks + SHELL_BLOCKS, c->scal);
+    return;
+  }
   int nb = reductionBlocks(c->geo.n);
   NUSIF_TIMED("absmax", (kAbsMax3<<<nb, 256, 0, nusifStream()>>>(c->u, c->v, c->w, c->geo.n,
                             c->partials, c->scal)));
-  kFoldMax3<<<1, 256, 0, nusifStream()>>>(c->partials, nb, c->scal);
+  kFoldMax3<<<1, 1024, 0, nusifStream()>>>(c->partials, nb, c->scal);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -160,6 +211,7 @@ static void launchFaces(NusifCtx *c, int fa, int ta, int fb, int tb)
   dim3 blk(64, 4);
   dim3 grd((ea + 63) / 64, (eb + 3) / 4);
   kVelocityFaces<<<grd, blk, 0, nusifStream()>>>(c->u, c->v, c->w, c->geo, fa, ta, fb, tb);
+  c->cflValid = false; /* wall faces of the interior range rewritten */
 }
 
 void launchVelocityBC(NusifCtx *c, const Discretization *d)
@@ -432,32 +484,65 @@ void launchNormalize(NusifCtx *c)
 /* ------------------------------------------------------------------------ */
 /* adaptUV (src/discretization.c:654-681)                                    */
 /* ------------------------------------------------------------------------ */
+/* A thread updates one (i, j) column over ADAPT_KC planes (p at k+1 is kept
+ * for the next plane) and keeps the maxima of |u|, |v|, |w| it wrote: the
+ * interior-range part of the next computeTimestep's CFL maxima, one partial
+ * per workgroup (logical block order). */
+#define ADAPT_KC 4
 __global__ __launch_bounds__(256) void kAdaptUV(const double *__restrict__ f,
     const double *__restrict__ g, const double *__restrict__ h, const double *__restrict__ p,
     double *__restrict__ u, double *__restrict__ v, double *__restrict__ w, Geo geo, double fx,
-    double fy, double fz, int gridGx, int gridGy)
+    double fy, double fz, double *__restrict__ partials, int gridGx, int gridGy, int gridGz)
 {
+  __shared__ double lds[4];
   int bx, by, bz;
-  if (!xcdBlock(gridGx, gridGy, geo.nk, bx, by, bz)) return;
+  if (!xcdBlock(gridGx, gridGy, gridGz, bx, by, bz)) return;
   int i = bx * blockDim.x + threadIdx.x + 1;
   int j = by * blockDim.y + threadIdx.y + 1;
-  int k = bz + 1;
-  if (i > geo.ni || j > geo.nj) return;
-  long long c = geo.at(i, j, k);
-  double pc   = p[c];
-  u[c]        = f[c] - (p[c + 1] - pc) * fx;
-  v[c]        = g[c] - (p[c + geo.sj] - pc) * fy;
-  w[c]        = h[c] - (p[c + geo.sk] - pc) * fz;
+  int k0 = bz * ADAPT_KC + 1, k1 = min(k0 + ADAPT_KC, geo.nk + 1);
+  double mu = DBL_MIN, mv = DBL_MIN, mw = DBL_MIN;
+  if (i <= geo.ni && j <= geo.nj) {
+    long long c = geo.at(i, j, k0);
+    double pc   = p[c];
+    for (int k = k0; k < k1; k++, c += geo.sk) {
+      double pz = p[c + geo.sk];
+      double a  = f[c] - (p[c + 1] - pc) * fx;
+      double b  = g[c] - (p[c + geo.sj] - pc) * fy;
+      double d  = h[c] - (pz - pc) * fz;
+      u[c]      = a;
+      v[c]      = b;
+      w[c]      = d;
+      a         = fabs(a);
+      b         = fabs(b);
+      d         = fabs(d);
+      mu        = mu > a ? mu : a;
+      mv        = mv > b ? mv : b;
+      mw        = mw > d ? mw : d;
+      pc        = pz;
+    }
+  }
+  mu = blockMax(mu, lds);
+  mv = blockMax(mv, lds);
+  mw = blockMax(mw, lds);
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    unsigned bid          = bx + gridGx * (by + gridGy * bz);
+    partials[3 * bid + 0] = mu;
+    partials[3 * bid + 1] = mv;
+    partials[3 * bid + 2] = mw;
+  }
 }
+
+static dim3 adaptGrid(const Geo &g) { return dim3((g.ni + TX - 1) / TX, (g.nj + TY - 1) / TY, (g.nk + ADAPT_KC - 1) / ADAPT_KC); }
 
 void launchAdaptUV(NusifCtx *c, const Discretization *d)
 {
   const Geo &g = c->geo;
   dim3 blk(TX, TY);
-  dim3 grd((g.ni + TX - 1) / TX, (g.nj + TY - 1) / TY, g.nk);
+  dim3 grd = adaptGrid(g);
   NUSIF_TIMED("adaptUV", (kAdaptUV<<<xcdGrid(grd), blk, 0, nusifStream()>>>(c->f, c->g, c->h,
                              c->p, c->u, c->v, c->w, g, d->dt / d->grid.dx, d->dt / d->grid.dy,
-                             d->dt / d->grid.dz, grd.x, grd.y)));
+                             d->dt / d->grid.dz, c->cflPartials, grd.x, grd.y, grd.z)));
+  c->cflValid = true;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -564,6 +649,12 @@ extern "C" void initDiscretization(Discretization *s, Parameter *params)
   c->pAlt = deviceField(n);
   c->nPartials = 65536;
   HIP_CHECK(hipMalloc((void **)&c->partials, c->nPartials * sizeof(double)));
+  {
+    dim3 ag              = adaptGrid(c->geo);
+    c->cflInteriorBlocks = (int)(ag.x * ag.y * ag.z);
+    HIP_CHECK(hipMalloc((void **)&c->cflPartials, 3 * ((size_t)c->cflInteriorBlocks + SHELL_BLOCKS) * sizeof(double)));
+    c->cflValid = false;
+  }
   HIP_CHECK(hipMalloc((void **)&c->scal, sizeof(DevScalars)));
   HIP_CHECK(hipHostMalloc((void **)&c->scalHost, sizeof(DevScalars), hipHostMallocDefault));
   memset(c->scalHost, 0, sizeof(DevScalars));
@@ -596,6 +687,7 @@ extern "C" void nusifSyncDevice(Discretization *d)
   for (int q = 0; q < 8; q++)
     HIP_CHECK(hipMemcpyAsync(dev[q], c->host[q], bytes, hipMemcpyHostToDevice, nusifStream()));
   HIP_CHECK(hipStreamSynchronize(nusifStream()));
+  c->cflValid = false;
 }
 
 extern "C" void nusifFreeDiscretization(Discretization *d)
@@ -605,6 +697,7 @@ extern "C" void nusifFreeDiscretization(Discretization *d)
   double *dev[9] = {c->p, c->rhs, c->f, c->g, c->h, c->u, c->v, c->w, c->pAlt};
   for (int q = 0; q < 9; q++) HIP_CHECK(hipFree(dev[q]));
   HIP_CHECK(hipFree(c->partials));
+  HIP_CHECK(hipFree(c->cflPartials));
   HIP_CHECK(hipFree(c->scal));
   if (c->inflow) HIP_CHECK(hipFree(c->inflow));
   HIP_CHECK(hipHostFree(c->scalHost));

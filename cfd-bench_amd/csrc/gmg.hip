@@ -172,7 +172,10 @@ __device__ inline void ldsBarrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_ba
 #define FT_KC 32
 #define FUSED_MIN_EXTENT 32 /* levels narrower than this use the two-pass sweep */
 
-template <typename T, int TJ, int MINW>
+/* ZX: the sweep starts from x = 0 (the first sweep of a cycle's error
+ * equation): old values are taken as zero instead of being read, which gives
+ * the same values as reading a zeroed array, without the zeroing pass. */
+template <typename T, int TJ, int MINW, bool ZX>
 __global__ __launch_bounds__(256, MINW) void kSweepFused(const T *__restrict__ xo, T *__restrict__ xn,
     const T *__restrict__ b, Geo g, Coef<T> cf, int kc)
 {
@@ -246,7 +249,7 @@ __global__ __launch_bounds__(256, MINW) void kSweepFused(const T *__restrict__ x
   auto fetchX = [&](int k) {
     long long base = (long long)min(max(k, 0), nk + 1) * sk;
 #pragma unroll
-    for (int q = 0; q < NX; q++) xr[q] = xo[base + xOff[q]];
+    for (int q = 0; q < NX; q++) xr[q] = ZX ? (T)0 : xo[base + xOff[q]];
   };
   auto storeX = [&](int k) {
 #pragma unroll
@@ -377,22 +380,34 @@ static dim3 fusedGrid(const Geo &g, SweepShape s)
   return dim3((unsigned)(((n + NUM_XCD - 1) / NUM_XCD) * NUM_XCD));
 }
 
-template <typename T> static void launchSweep(const LevelT<T> &L, SweepShape s, bool timed)
+template <typename T, int TJ, int MINW>
+static void launchSweepShape(const LevelT<T> &L, dim3 grd, int kc, bool timed, bool zx)
 {
-  dim3 grd = fusedGrid(L.g, s);
-  if (s.tj == 8) {
+  const char *name = sizeof(T) == 8 ? "gmg_smooth64_l0" : "gmg_smooth32_l0";
+  if (zx) {
     if (timed)
-      NUSIF_TIMED("gmg_smooth32_l0", (kSweepFused<T, 8, 6><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, s.kc)));
+      NUSIF_TIMED(name, (kSweepFused<T, TJ, MINW, true><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc)));
     else
-      kSweepFused<T, 8, 6><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, s.kc);
+      kSweepFused<T, TJ, MINW, true><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc);
   } else {
     if (timed)
-      NUSIF_TIMED(sizeof(T) == 8 ? "gmg_smooth64_l0" : "gmg_smooth32_l0",
-          (kSweepFused<T, 4, 8><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, s.kc)));
+      NUSIF_TIMED(name, (kSweepFused<T, TJ, MINW, false><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc)));
     else
-      kSweepFused<T, 4, 8><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, s.kc);
+      kSweepFused<T, TJ, MINW, false><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc);
   }
 }
+
+template <typename T> static void launchSweep(const LevelT<T> &L, SweepShape s, bool timed, bool zx)
+{
+  dim3 grd = fusedGrid(L.g, s);
+  if (s.tj == 8)
+    launchSweepShape<T, 8, 6>(L, grd, s.kc, timed, zx);
+  else
+    launchSweepShape<T, 4, 8>(L, grd, s.kc, timed, zx);
+}
+
+/* true when a level's smoother is the fused sweep */
+template <typename T> static bool fusedLevel(const LevelT<T> &L) { return L.g.ni >= FUSED_MIN_EXTENT && L.g.nk >= 4; }
 
 /* b_c(I,J,K) = (1/8) sum over the 8 children of (b - A x); the coarse x is
  * zeroed (interior and face ghosts: the coarse cycle's initial guess). */
@@ -897,9 +912,11 @@ template <typename T> static void fill(T *x, long long n, T v)
 
 /* Timers cover the finest level only ("..._l0"), whose algorithmic bytes
  * nusifKernelBytes reports. Each sweep swaps x and xAlt. */
-template <typename T> static void smooth(GmgState *G, LevelT<T> &L, int sweeps, bool fine)
+/* `zeroStart`: x is zero at the first sweep (for the fused sweep it is
+ * taken as zero, so the caller may skip zeroing it) */
+template <typename T> static void smooth(GmgState *G, LevelT<T> &L, int sweeps, bool fine, bool zeroStart)
 {
-  if (L.g.ni < FUSED_MIN_EXTENT || L.g.nk < 4) { /* small levels: two colour passes */
+  if (!fusedLevel(L)) { /* small levels: two colour passes */
     dim3 grd = sweepGrid(L.g);
     for (int s = 0; s < sweeps; s++) {
       kSmooth<T><<<grd, dim3(GTX, GTY), 0, nusifStream()>>>(L.x, L.b, L.g, L.cf, 1, 0);
@@ -909,7 +926,7 @@ template <typename T> static void smooth(GmgState *G, LevelT<T> &L, int sweeps, 
   }
   const SweepShape shape = sweepShape<T>(L.g, G->tileRows);
   for (int s = 0; s < sweeps; s++) {
-    launchSweep(L, shape, fine);
+    launchSweep(L, shape, fine, zeroStart && s == 0);
     T *t   = L.x;
     L.x    = L.xAlt;
     L.xAlt = t;
@@ -988,7 +1005,9 @@ template <typename T> static void vcycle(GmgState *G, LevelT<T> *L, int l)
     return;
   }
   bool fine = l == 0;
-  smooth(G, L[l], G->nu1, fine);
+  /* the coarse levels start from x = 0 (zeroed by the restriction); the
+   * finest one in mixed mode too (the error equation), FP64 mode smooths p */
+  smooth(G, L[l], G->nu1, fine, l > 0 || G->mixed);
   LevelT<T> &C = L[l + 1];
   if (fine)
     NUSIF_TIMED(sizeof(T) == 8 ? "gmg_restrict64_l0" : "gmg_restrict32_l0",
@@ -1006,7 +1025,7 @@ template <typename T> static void vcycle(GmgState *G, LevelT<T> *L, int l)
   else
     kProlongMarch<T><<<xcdGrid(pg), dim3(GTX, GTY), 0, nusifStream()>>>(
         L[l].x, L[l].g, C.x, C.g, pg.x, pg.y, pg.z);
-  smooth(G, L[l], G->nu2, fine);
+  smooth(G, L[l], G->nu2, fine, false);
 }
 
 template <typename T> static T *devAlloc(long long n)
@@ -1212,6 +1231,15 @@ template <typename Body> static void runCycle(NusifCtx *c, GmgState *G, Body bod
   G->graphs.push_back(e);
 }
 
+/* Mixed mode: whether the finest level's correction e must be zeroed in
+ * memory before a cycle. Not when the cycle's first pre-smoothing sweep is
+ * a fused sweep, which takes the old values as zero (ZX); a two-pass sweep,
+ * or a cycle without pre-smoothing, reads them. */
+static bool l0NeedsZero(const GmgState *G)
+{
+  return G->mixed && G->nl > 1 && (!fusedLevel(G->L32[0]) || G->nu1 == 0);
+}
+
 /* Residual of p on the finest level (FP64): partial sums of r^2 and rhs^2
  * folded into the device pair out[0..1]; optionally r as FP32 (r32) and an
  * FP32 array zeroed (eZero). No host synchronisation. */
@@ -1284,7 +1312,7 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
        * pass; then its restriction to the network grid, and one read-back
        * for the decision */
       float *rg  = G->mixed ? G->L32[0].b : warmstartResidualBuffer(ws);
-      float *e0  = G->mixed ? G->L32[0].x : nullptr;
+      float *e0  = l0NeedsZero(G) ? G->L32[0].x : nullptr;
       dim3 ng    = normGrid(g);
       NUSIF_TIMED("gmg_residual64", (kResidualLds<true><<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(
                                         p, d, rhs, g, G->cf0, c->pAlt, rg, e0, c->partials, ng.x, ng.y, ng.z)));
@@ -1341,7 +1369,7 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
      * pass that applies e to p (into the partner buffer), forms the next r
      * and zeroes the partner of e for the next cycle */
     LevelT<float> &L0 = G->L32[0];
-    double rn = haveInitial ? rnInit : residualNorm(c, G, p, rhs, L0.b, &nb, L0.x);
+    double rn = haveInitial ? rnInit : residualNorm(c, G, p, rhs, L0.b, &nb, l0NeedsZero(G) ? L0.x : nullptr);
     rel       = nb > 0.0 ? rn / nb : 0.0;
     dim3 ng   = normGrid(g);
     while (nb > 0.0 && rel > G->tol && cycles < G->maxCycles) {
@@ -1349,7 +1377,7 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
         vcycle<float>(G, G->L32, 0);
         NUSIF_TIMED("gmg_correct_residual",
             (kResidualLds<true><<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(c->p, L0.x, rhs, g,
-                G->cf0, c->pAlt, L0.b, L0.xAlt, c->partials, ng.x, ng.y, ng.z)));
+                G->cf0, c->pAlt, L0.b, l0NeedsZero(G) ? L0.xAlt : nullptr, c->partials, ng.x, ng.y, ng.z)));
         float *te  = L0.x; /* the next cycle's zeroed e */
         L0.x       = L0.xAlt;
         L0.xAlt    = te;

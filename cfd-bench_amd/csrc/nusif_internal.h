@@ -110,6 +110,7 @@ struct NusifOptions {
   int seed;
   int warmstart;
   char unetWeights[512];
+  int unetExtent;  /* warm start: the network runs on the first level of extent <= this (64) */
 };
 void nusifDefaultOptions(NusifOptions *o);
 /* Options registered for `p` (defaults when none were set). */
@@ -143,6 +144,12 @@ struct NusifCtx {
   DevScalars *scal;    /* device */
   DevScalars *scalHost;/* pinned mirror */
   double *inflow;      /* optional per-cell inflow profile (jmax+2)*(kmax+2) */
+  /* CFL maxima: adaptUV leaves per-workgroup maxima of the interior range it
+   * wrote in cflPartials; while no other kernel has written u, v, w since
+   * (cflValid), computeTimestep adds only the ghost shell to them */
+  double *cflPartials;
+  int cflInteriorBlocks;
+  bool cflValid;
   NusifOptions opt;
   double t;            /* nusifRunSteps clock */
   int nt;

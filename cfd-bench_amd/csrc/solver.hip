@@ -46,7 +46,10 @@ extern "C" void initSolver(Solver *s, Discretization *d, Parameter *p)
   if (sc->dctx->opt.warmstart) {
     if (sc->kind != SOLVER_GMG) NUSIF_FATAL("warmstart unet needs solver gmg");
     const char *path = sc->dctx->opt.unetWeights;
-    sc->ws           = warmstartCreate(sc->dctx, nullptr, false);
+    /* a weights file that records its training extent sets the network level */
+    int fileExtent = path[0] ? unetFileExtent(path) : 0;
+    if (fileExtent > 0) sc->dctx->opt.unetExtent = fileExtent;
+    sc->ws = warmstartCreate(sc->dctx, nullptr, false);
     UnetModel *net   = path[0] ? unetLoad(path, warmstartExtent(sc->ws)) : nullptr;
     if (!net) NUSIF_FATAL("warmstart unet: cannot read weights '%s' (option unet_weights)", path);
     warmstartSetNet(sc->ws, net);

@@ -25,6 +25,7 @@ double unetLastLoss(UnetModel *m);
 void unetAdam(UnetModel *m, float lr, int batch);
 int unetSave(UnetModel *m, const char *path);
 UnetModel *unetLoad(const char *path, int n);
+int unetFileExtent(const char *path);
 int unetLevels(const UnetModel *m);
 int unetExtent(const UnetModel *m);
 #endif

@@ -634,8 +634,31 @@ int unetSave(UnetModel *m, const char *path)
   fwrite(hdr, sizeof(int), 6, f);
   fwrite(&np, sizeof(np), 1, f);
   fwrite(h.data(), sizeof(float), h.size(), f);
+  /* trailer: the network extent it was trained on */
+  int ext = m->n;
+  fwrite("NUSIFEXT", 1, 8, f);
+  fwrite(&ext, sizeof(int), 1, f);
   fclose(f);
   return 0;
+}
+
+/* the network extent recorded in a weights file, 0 if it records none */
+int unetFileExtent(const char *path)
+{
+  FILE *f = fopen(path, "rb");
+  if (!f) return 0;
+  char magic[10];
+  int hdr[6];
+  long long np = 0;
+  int ext      = 0;
+  if (fread(magic, 1, 10, f) == 10 && memcmp(magic, "NUSIFUNET1", 10) == 0 &&
+      fread(hdr, sizeof(int), 6, f) == 6 && fread(&np, sizeof(np), 1, f) == 1 && np > 0 &&
+      fseek(f, (long)(np * (long long)sizeof(float)), SEEK_CUR) == 0) {
+    char tag[8];
+    if (fread(tag, 1, 8, f) != 8 || memcmp(tag, "NUSIFEXT", 8) != 0 || fread(&ext, sizeof(int), 1, f) != 1) ext = 0;
+  }
+  fclose(f);
+  return ext;
 }
 
 UnetModel *unetLoad(const char *path, int n)

@@ -206,10 +206,10 @@ static float *falloc(long long n)
 }
 
 /* first level whose largest extent is <= 64 (extents must stay even) */
-static int networkLevel(const Geo &g0)
+static int networkLevel(const Geo &g0, int maxExtent)
 {
   int ni = g0.ni, nj = g0.nj, nk = g0.nk, l = 0;
-  while ((ni > 64 || nj > 64 || nk > 64) && ni % 2 == 0 && nj % 2 == 0 && nk % 2 == 0) {
+  while ((ni > maxExtent || nj > maxExtent || nk > maxExtent) && ni % 2 == 0 && nj % 2 == 0 && nk % 2 == 0) {
     ni /= 2;
     nj /= 2;
     nk /= 2;
@@ -223,7 +223,7 @@ WarmStart *warmstartCreate(NusifCtx *c, UnetModel *net, bool capture)
   WarmStart *w = new WarmStart();
   w->net       = net;
   w->capture   = capture;
-  w->Ln        = networkLevel(c->geo);
+  w->Ln        = networkLevel(c->geo, c->opt.unetExtent > 0 ? c->opt.unetExtent : 64);
   Geo g        = c->geo;
   double h     = c->owner->grid.dx;
   for (int l = 0; l <= w->Ln; l++) {
@@ -385,20 +385,21 @@ void warmstartTakeSamples(WarmStart *w, std::vector<uint16_t *> &xs, std::vector
 static const int kUnetLevels = 4;
 static const int kUnetChannels[4] = {16, 32, 64, 128};
 
-extern "C" int nusifUnetTrainCavityGrids(const char *weightsOut, const int *grids, int ngrids, int runs,
-    int stepsPerRun, int skipSteps, int epochs, int batch, double lr, int seed, double *lossPerEpoch);
+extern "C" int nusifUnetTrainCavityGrids(const char *weightsOut, const int *grids, int ngrids, int netExtent,
+    int runs, int stepsPerRun, int skipSteps, int epochs, int batch, double lr, int seed, double *lossPerEpoch);
 
 extern "C" int nusifUnetTrainCavity(const char *weightsOut, int grid, int runs, int stepsPerRun,
     int skipSteps, int epochs, int batch, double lr, int seed, double *lossPerEpoch)
 {
-  return nusifUnetTrainCavityGrids(weightsOut, &grid, 1, runs, stepsPerRun, skipSteps, epochs, batch, lr, seed,
-      lossPerEpoch);
+  return nusifUnetTrainCavityGrids(weightsOut, &grid, 1, 64, runs, stepsPerRun, skipSteps, epochs, batch, lr,
+      seed, lossPerEpoch);
 }
 
-/* as nusifUnetTrainCavity, run r on a grids[r % ngrids]^3 cavity (all grids
+/* as nusifUnetTrainCavity, run r on a grids[r % ngrids]^3 cavity, with the
+ * network on the first multigrid level of extent <= netExtent (all grids
  * must give the same network extent) */
-extern "C" int nusifUnetTrainCavityGrids(const char *weightsOut, const int *grids, int ngrids, int runs,
-    int stepsPerRun, int skipSteps, int epochs, int batch, double lr, int seed, double *lossPerEpoch)
+extern "C" int nusifUnetTrainCavityGrids(const char *weightsOut, const int *grids, int ngrids, int netExtent,
+    int runs, int stepsPerRun, int skipSteps, int epochs, int batch, double lr, int seed, double *lossPerEpoch)
 {
   if (ngrids < 1) NUSIF_FATAL("unet training: no grid given");
   std::vector<uint16_t *> xs;
@@ -434,6 +435,8 @@ extern "C" int nusifUnetTrainCavityGrids(const char *weightsOut, const int *grid
     nusifSetOption(&prm, "mg_tol", "1e-6");
     snprintf(buf, sizeof buf, "%.17g", lid);
     nusifSetOption(&prm, "lid", buf);
+    snprintf(buf, sizeof buf, "%d", netExtent);
+    nusifSetOption(&prm, "unet_extent", buf);
     Discretization d;
     Solver s;
     memset(&d, 0, sizeof d);

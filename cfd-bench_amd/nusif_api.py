@@ -135,7 +135,7 @@ def _declare_optional(lib):
         "nusifWarmstartUsed": ([C.POINTER(Solver)], C.c_int),
         "nusifWarmstartResiduals": ([C.POINTER(Solver), _DP], None),
         "nusifUnetTrainCavity": ([C.c_char_p] + [C.c_int] * 6 + [C.c_double, C.c_int, _DP], C.c_int),
-        "nusifUnetTrainCavityGrids": ([C.c_char_p, C.POINTER(C.c_int)] + [C.c_int] * 6 +
+        "nusifUnetTrainCavityGrids": ([C.c_char_p, C.POINTER(C.c_int)] + [C.c_int] * 7 +
                                       [C.c_double, C.c_int, _DP], C.c_int),
         "nusifUnetLossGrad": ([C.POINTER(C.c_int), C.c_int, _DP, _DP, _DP, _DP, _DP], C.c_double),
         "nusifRunSteps": ([C.POINTER(Discretization), C.POINTER(Solver), C.c_int,

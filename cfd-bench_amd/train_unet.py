@@ -24,12 +24,13 @@ sys.path.insert(0, HERE)
 import nusif_api as A  # noqa: E402
 
 
-def evaluate(lib, weights, grid, re, lid, warm, steps, force=False):
+def evaluate(lib, weights, grid, re, lid, warm, steps, force=False, extent=64):
     prm = dict(name="dcavity", re=re, imax=grid, jmax=grid, kmax=grid, tau=0.5, te=1e9, levels=6,
                presmooth=2, postsmooth=2)
     opts = {"solver": "gmg", "precision": "mixed", "mg_tol": 1e-6, "lid": lid}
     if warm:
-        opts.update({"warmstart": "unet_force" if force else "unet", "unet_weights": weights})
+        opts.update({"warmstart": "unet_force" if force else "unet", "unet_weights": weights,
+                     "unet_extent": extent})
     case = A.Case(lib, options=opts, **prm)
     tr = (C.c_double * (4 * (warmup + steps)))()
     lib.nusifRunSteps(C.byref(case.d), C.byref(case.s), warmup, tr)
@@ -62,6 +63,7 @@ warmup = 10
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--grid", type=int, nargs="+", default=[128])
+    ap.add_argument("--extent", type=int, default=64, help="network grid extent (unet_extent)")
     ap.add_argument("--runs", type=int, default=16)
     ap.add_argument("--skip", type=int, default=5)
     ap.add_argument("--steps", type=int, default=60)
@@ -82,15 +84,15 @@ def main():
         losses = (C.c_double * a.epochs)()
         t0 = time.perf_counter()
         grids = (C.c_int * len(a.grid))(*a.grid)
-        n = lib.nusifUnetTrainCavityGrids(a.out.encode(), grids, len(a.grid), a.runs, a.steps, a.skip,
-                                          a.epochs, a.batch, a.lr, a.seed, losses)
+        n = lib.nusifUnetTrainCavityGrids(a.out.encode(), grids, len(a.grid), a.extent, a.runs, a.steps,
+                                          a.skip, a.epochs, a.batch, a.lr, a.seed, losses)
         report.update(pairs=n, train_seconds=time.perf_counter() - t0, loss_per_epoch=list(losses))
     # held-out case: Re and lid chosen outside the training draws
     for grid in a.grid:
         for re, lid in ((500.0, 1.0), (150.0, 1.6)):
             base = evaluate(lib, a.out, grid, re, lid, False, a.eval_steps)
-            warm = evaluate(lib, a.out, grid, re, lid, True, a.eval_steps)
-            force = evaluate(lib, a.out, grid, re, lid, True, a.eval_steps, force=True)
+            warm = evaluate(lib, a.out, grid, re, lid, True, a.eval_steps, extent=a.extent)
+            force = evaluate(lib, a.out, grid, re, lid, True, a.eval_steps, force=True, extent=a.extent)
             report["eval_%d_re%g_lid%g" % (grid, re, lid)] = {"without": base, "with_unet": warm,
                                                               "always_guess": force}
     print(json.dumps(report), flush=True)

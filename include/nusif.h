@@ -58,6 +58,9 @@ void nusifFreeSolver(Solver *s);
  *   warmstart     none | unet | unet_force   initial guess of the gmg solve
  *                 (unet_force: always take the network's guess, diagnostic)
  *   unet_weights  path    trained U-Net weights (nusifUnetSave format)
+ *   unet_extent   int     the network runs on the first multigrid level whose
+ *                         extents are all <= this (64); a weights file that
+ *                         records its training extent overrides it
  * Returns 0 on success, -1 for an unknown key, -2 for a bad value. */
 int nusifSetOption(Parameter *p, const char *key, const char *value);
 
@@ -109,10 +112,12 @@ double nusifCopyProbe(size_t bytes, int reps);
  * epoch. Returns the number of training pairs. */
 int nusifUnetTrainCavity(const char *weightsOut, int grid, int runs, int stepsPerRun,
     int skipSteps, int epochs, int batch, double lr, int seed, double *lossPerEpoch);
-/* The same with run r on a grids[r % ngrids]^3 cavity (every grid must give
- * the same network extent, e.g. 128 and 256 both give 64). */
-int nusifUnetTrainCavityGrids(const char *weightsOut, const int *grids, int ngrids, int runs,
-    int stepsPerRun, int skipSteps, int epochs, int batch, double lr, int seed, double *lossPerEpoch);
+/* The same with run r on a grids[r % ngrids]^3 cavity and the network on
+ * the first level of extent <= netExtent (every grid must give the same
+ * network extent, e.g. 128 and 256 both give 64). */
+int nusifUnetTrainCavityGrids(const char *weightsOut, const int *grids, int ngrids, int netExtent,
+    int runs, int stepsPerRun, int skipSteps, int epochs, int batch, double lr, int seed,
+    double *lossPerEpoch);
 
 /* One forward and backward pass of the U-Net on the GPU, for gradient
  * checks: cfg = {levels, ch0, ch1, ch2, ch3, input channels}, x is n^3 x

@@ -167,3 +167,49 @@ def test_gpu_gmg_tall_tile_sweep_equals_default(gpu_ok):
         c.close()
     assert out[0][1] == out[1][1]
     np.testing.assert_array_equal(out[0][0], out[1][0])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gpu_channel_steps_per_field_error(precision, gpu_ok):
+    """The channel form (canal.par: outflow left/right, no-slip walls,
+    uniform inflow u = 1 at x = 0, domain 4 x 1 x 1, Re 100) at 64 x 16 x 16
+    with the gmg solver to 1e-8 (rhs mean removed: the inflow/outflow rhs is
+    not compatible with the all-Neumann pressure problem): GPU steps against
+    the oracle's step functions + the oracle's FP64 gmg. The error per field
+    is printed and bounded: FP64 <= 1e-10, mixed (FP32 V-cycles inside FP64
+    defect correction) <= 1e-6."""
+    print("channel", precision, flush=True)
+    prm = dict(name="canal", re=100.0, imax=64, jmax=16, kmax=16, xlength=4.0, ylength=1.0,
+               zlength=1.0, bcLeft=3, bcRight=3, bcBottom=1, bcTop=1, bcFront=1, bcBack=1,
+               u_init=1.0, tau=0.5, te=1e9, gamma=0.9, levels=3, presmooth=2, postsmooth=2)
+    opts = {"solver": "gmg", "precision": precision, "mg_tol": 1e-8, "mg_project": 1,
+            "u_inflow": 1.0}
+    g = A.Case(lib("gpu_rb"), options=opts, **prm)
+    o_lib = lib("orc_rb")
+    o_lib.orcSetProblemExtras(1.0, 1.0)
+    try:
+        o = A.Case(o_lib, **prm)
+        for step in range(8):
+            g.step()
+            d = o.d
+            o.lib.computeTimestep(C.byref(d))
+            for fn in ("setBoundaryConditions", "setSpecialBoundaryCondition", "computeFG",
+                       "computeRHS"):
+                o.call(fn)
+            if step == 0:
+                o.call("normalizePressure")
+            n = int(np.prod(o.shape))
+            pv = np.ctypeslib.as_array(d.p, shape=(n,))
+            rv = np.ctypeslib.as_array(d.rhs, shape=(n,))
+            oracle_solve(pv, rv, dict(prm), dict(opts, precision="fp64"))
+            o.call("adaptUV")
+    finally:
+        o_lib.orcSetProblemExtras(1.0, 2.0)
+    fg, fo = g.fields(), o.fields()
+    assert np.all(fg["u"][1:-1, 1:-1, 0] == 1.0)
+    for k in ("u", "v", "w", "p"):
+        err = A.rel_maxnorm(fg[k], fo[k])
+        print(precision, "channel", k, "rel err vs FP64 oracle %.3e" % err)
+        assert err <= (1e-10 if precision == "fp64" else 1e-6), k
+    g.close()
+    o.close()
