@@ -394,6 +394,8 @@ extern "C" double nusifKernelBytes(Discretization *d, const char *kernel)
       {"rb_fused", 24.0, 0},         /* one full iteration: rd p, rd rhs, wr p */
       {"gmg_smooth64_l0", 24.0, 0},  /* one red-black sweep, FP64: rd x, rd b, wr x */
       {"gmg_smooth32_l0", 12.0, 0},  /* one red-black sweep, FP32 */
+      {"gmg_smooth64z_l0", 16.0, 0}, /* first sweep from x = 0, FP64: rd b, wr x */
+      {"gmg_smooth32z_l0", 8.0, 0},  /* first sweep from x = 0, FP32 */
       {"gmg_restrict64_l0", 17.0, 0},/* rd x, b; wr b_coarse (1/8) */
       {"gmg_restrict32_l0", 8.5, 0},
       {"gmg_prolong64_l0", 17.0, 0}, /* rd/wr x, rd x_coarse (1/8) */

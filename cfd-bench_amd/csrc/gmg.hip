@@ -27,6 +27,8 @@
 #include <math.h>
 #include <string.h>
 
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "nusif_internal.h"
@@ -383,7 +385,9 @@ static dim3 fusedGrid(const Geo &g, SweepShape s)
 template <typename T, int TJ, int MINW>
 static void launchSweepShape(const LevelT<T> &L, dim3 grd, int kc, bool timed, bool zx)
 {
-  const char *name = sizeof(T) == 8 ? "gmg_smooth64_l0" : "gmg_smooth32_l0";
+  /* the zero-start sweep moves 2/3 of a sweep's bytes (no x read): timed apart */
+  const char *name = sizeof(T) == 8 ? (zx ? "gmg_smooth64z_l0" : "gmg_smooth64_l0")
+                                    : (zx ? "gmg_smooth32z_l0" : "gmg_smooth32_l0");
   if (zx) {
     if (timed)
       NUSIF_TIMED(name, (kSweepFused<T, TJ, MINW, true><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc)));
@@ -1301,11 +1305,12 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
   bool haveInitial = false; /* mixed: residual of c->p already in L0.b, its norms in rnInit/nb */
   double rnInit = 0.0;
   if (ws) {
-    /* r0 = rhs - A p_prev as FP32 (network input); norms into norm2[2..3] */
-    launchResidual(c, G, p, rhs, warmstartResidualBuffer(ws), nullptr, c->scal->norm2 + 2);
-    warmstartCaptureBegin(ws, p, rhs, g);
+    /* the network input: r0 = rhs - A p_prev and rhs, restricted to the
+     * network grid, and their norms */
+    warmstartPrepare(ws, p, rhs, G->cf0.idx2, G->cf0.idy2, G->cf0.idz2);
+    warmstartCaptureBegin(ws, p, g);
     if (warmstartNet(ws)) {
-      const float *d = warmstartCorrection(ws, rhs);
+      const float *d = warmstartCorrection(ws);
       /* the guess p + d into c->pAlt and its residual in one pass, in mixed
        * mode straight into the first defect buffer (L0.b) with the first
        * correction zeroed, so an accepted guess costs no further residual
@@ -1326,7 +1331,7 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
        * fine residuals): a correction interpolated from that grid carries
        * interpolation error whose fine-grid residual the first smoothing
        * sweep removes, and which would otherwise hide any gain */
-      sc->wsRes[0] = sqrt(c->scalHost->norm2[2]);
+      sc->wsRes[0] = sqrt(wn[3]);
       sc->wsRes[1] = rng;
       sc->wsRes[2] = rc0;
       sc->wsRes[3] = rcg;
@@ -1399,3 +1404,55 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
 }
 
 
+
+/* Duration of the finest level's red-black sweep (the dominant kernel of a
+ * gmg solve): `reps` back-to-back launches on the solver's own level-0
+ * buffers, HIP events on the library stream around the batch only, so the
+ * per-launch event packets of the kernel timers do not inflate it. The
+ * sweeps change only the level-0 work arrays (in mixed mode the correction
+ * e, in FP64 mode p and its partner: use it after the steps it measures).
+ * reps is rounded up to an even number, which leaves x / xAlt as found.
+ * Returns microseconds per launch, or -1 when the solver is not gmg or the
+ * finest level does not use the fused sweep. */
+extern "C" double nusifTimeFinestSweep(Solver *s, int reps)
+{
+  NusifSolverCtx *sc = nusifSolverCtxOf(s);
+  if (sc->kind != SOLVER_GMG || !sc->gmg) return -1.0;
+  GmgState *G = sc->gmg;
+  NusifCtx *c = sc->dctx;
+  reps        = reps < 2 ? 2 : reps + (reps & 1);
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  float ms = 0.0f;
+  auto run = [&](auto &L) {
+    if (!fusedLevel(L)) return false;
+    auto shape = sweepShape<typename std::remove_pointer<decltype(L.x)>::type>(L.g, G->tileRows);
+    launchSweep(L, shape, false, false); /* warm */
+    std::swap(L.x, L.xAlt);
+    launchSweep(L, shape, false, false);
+    std::swap(L.x, L.xAlt);
+    HIP_CHECK(hipEventRecord(e0, nusifStream()));
+    for (int q = 0; q < reps; q++) {
+      launchSweep(L, shape, false, false);
+      std::swap(L.x, L.xAlt);
+    }
+    HIP_CHECK(hipEventRecord(e1, nusifStream()));
+    HIP_CHECK(hipEventSynchronize(e1));
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    return true;
+  };
+  bool ok;
+  if (G->mixed) {
+    ok = run(G->L32[0]);
+  } else {
+    LevelT<double> &L0 = G->L64[0];
+    L0.x    = c->p;
+    L0.xAlt = c->pAlt;
+    L0.b    = c->rhs;
+    ok      = run(L0);
+  }
+  HIP_CHECK(hipEventDestroy(e0));
+  HIP_CHECK(hipEventDestroy(e1));
+  return ok ? 1e3 * ms / reps : -1.0;
+}

@@ -205,8 +205,22 @@ __global__ __launch_bounds__(256) void kConvLds(ConvArgs<uint16_t, OutT> a, floa
       }
       *(uint4 *)&sIn[v * CL_CSTR + q * 8] = val;
     }
-    __syncthreads();
     const int G = 27 * g8;
+    /* weight fragments of K-step s: loaded one step ahead (double buffer), so
+     * the global latency overlaps the previous step's MFMAs */
+    auto loadA = [&](int s, bf16x8 *fa) {
+      int gi  = s + lg;
+      bool gv = gi < G;
+      int t = gv ? gi / g8 : 0, c8 = gv ? gi - t * g8 : 0;
+#pragma unroll
+      for (int nb = 0; nb < NB; nb++) {
+        int r  = row0 + nb * 16 + lr;
+        fa[nb] = (gv && r < rows) ? ld8(a.A + ((size_t)r * 27 + t) * cin + cc0 + c8 * 8) : zero8();
+      }
+    };
+    bf16x8 faNext[NB];
+    loadA(0, faNext);
+    __syncthreads();
     for (int s = 0; s < G; s += 4) {
       int gi  = s + lg;
       bool gv = gi < G;
@@ -214,10 +228,8 @@ __global__ __launch_bounds__(256) void kConvLds(ConvArgs<uint16_t, OutT> a, floa
       int dz = t / 9, dy = (t / 3) % 3, dx = t % 3; /* tap offset + 1 = halo offset */
       bf16x8 fa[NB], fb[4];
 #pragma unroll
-      for (int nb = 0; nb < NB; nb++) {
-        int r  = row0 + nb * 16 + lr;
-        fa[nb] = (gv && r < rows) ? ld8(a.A + ((size_t)r * 27 + t) * cin + cc0 + c8 * 8) : zero8();
-      }
+      for (int nb = 0; nb < NB; nb++) fa[nb] = faNext[nb];
+      if (s + 4 < G) loadA(s + 4, faNext);
 #pragma unroll
       for (int mb = 0; mb < 4; mb++) {
         int hv = ((wave + dz) * CL_HY + (mb + dy)) * CL_HX + (lr + dx);

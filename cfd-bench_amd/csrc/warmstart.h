@@ -20,8 +20,9 @@ bool warmstartCapturing(const WarmStart *w);
 float *warmstartResidualBuffer(WarmStart *w);
 void warmstartCoarseNormLaunch(WarmStart *w, const float *rFine);
 const double *warmstartNormsAsync(WarmStart *w);
-const float *warmstartCorrection(WarmStart *w, const double *rhs);
-void warmstartCaptureBegin(WarmStart *w, const double *p, const double *rhs, const Geo &g0);
+void warmstartPrepare(WarmStart *w, const double *p, const double *rhs, double idx2, double idy2, double idz2);
+const float *warmstartCorrection(WarmStart *w);
+void warmstartCaptureBegin(WarmStart *w, const double *p, const Geo &g0);
 void warmstartCaptureEnd(WarmStart *w, const double *p);
 size_t warmstartSamples(const WarmStart *w);
 void warmstartTakeSamples(WarmStart *w, std::vector<uint16_t *> &xs, std::vector<float *> &ts);

@@ -45,7 +45,8 @@ struct WarmStart {
   WsLevel lv[12];
   uint16_t *x;            /* network input n^3 x 8 */
   double *pPrev;          /* FP64 copy of p for capture */
-  double *scal;           /* [0] sum r0^2, [1] sum b^2 on level Ln; [2..3] scratch sums */
+  double *scal;           /* [0] sum r0^2, [1] sum b^2 on level Ln, [2] guess residual
+                           * on level Ln, [3] sum r0^2 on level 0 */
   double *scalHost;       /* pinned mirror of scal */
   double *partials;
   /* capture */
@@ -194,6 +195,76 @@ __global__ __launch_bounds__(256) void kProlongSet(float *xf, Geo gf, const floa
   faceGhosts(xf, gf, i, j, k, c, v); /* Neumann copies: the finest level's are read as p + d */
 }
 
+/* The network input's first stage in one pass over the finest grid: for
+ * each level-1 cell, the 8-child averages of r0 = rhs - A p (FP64, the
+ * 7-point Neumann Laplacian in the solvers' operand order) and of rhs, as
+ * FP32, plus per-workgroup partial sums of r0^2 over the fine cells. */
+struct Lap {
+  double idx2, idy2, idz2;
+};
+__global__ __launch_bounds__(256) void kInputRestrict(const double *__restrict__ p, const double *__restrict__ rhs,
+    Geo gf, Lap cf, float *__restrict__ rc, float *__restrict__ bc, Geo gc, double *partials)
+{
+  __shared__ double lds[4];
+  int I = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int J = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int K = blockIdx.z + 1;
+  double rr = 0.0;
+  if (I <= gc.ni && J <= gc.nj) {
+    double sr = 0.0, sb = 0.0;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      long long c = gf.at(2 * I - 1 + (e & 1), 2 * J - 1 + ((e >> 1) & 1), 2 * K - 1 + (e >> 2));
+      double xc  = p[c];
+      double lap = (p[c + 1] - 2.0 * xc + p[c - 1]) * cf.idx2 + (p[c + gf.sj] - 2.0 * xc + p[c - gf.sj]) * cf.idy2 +
+                   (p[c + gf.sk] - 2.0 * xc + p[c - gf.sk]) * cf.idz2;
+      double b = rhs[c];
+      double r = b - lap;
+      sr += r;
+      sb += b;
+      rr += r * r;
+    }
+    long long cc = gc.at(I, J, K);
+    rc[cc]       = (float)(0.125 * sr);
+    bc[cc]       = (float)(0.125 * sb);
+  }
+  rr = blockSum(rr, lds);
+  if (threadIdx.x == 0 && threadIdx.y == 0)
+    partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = rr;
+}
+
+/* network on the finest grid (small problems): r0 and rhs as FP32 there */
+__global__ __launch_bounds__(256) void kInputFine(const double *__restrict__ p, const double *__restrict__ rhs, Geo g,
+    Lap cf, float *__restrict__ r, float *__restrict__ b32, double *partials)
+{
+  __shared__ double lds[4];
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int k = blockIdx.z + 1;
+  double rr = 0.0;
+  if (i <= g.ni && j <= g.nj) {
+    long long c = g.at(i, j, k);
+    double xc   = p[c];
+    double lap  = (p[c + 1] - 2.0 * xc + p[c - 1]) * cf.idx2 + (p[c + g.sj] - 2.0 * xc + p[c - g.sj]) * cf.idy2 +
+                 (p[c + g.sk] - 2.0 * xc + p[c - g.sk]) * cf.idz2;
+    double bb = rhs[c];
+    double rv = bb - lap;
+    r[c]      = (float)rv;
+    b32[c]    = (float)bb;
+    rr        = rv * rv;
+  }
+  rr = blockSum(rr, lds);
+  if (threadIdx.x == 0 && threadIdx.y == 0)
+    partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = rr;
+}
+
+__global__ __launch_bounds__(256) void kFoldSum1(const double *partials, int nb, double *out)
+{
+  __shared__ double lds[4];
+  double s = foldSum(partials, nb, 1, 0, lds);
+  if (threadIdx.x == 0) *out = s;
+}
+
 /* ---- host ------------------------------------------------------------------ */
 static dim3 cells(const Geo &g) { return dim3((g.ni + WTX - 1) / WTX, (g.nj + WTY - 1) / WTY, g.nk); }
 
@@ -241,6 +312,13 @@ WarmStart *warmstartCreate(NusifCtx *c, UnetModel *net, bool capture)
     NUSIF_FATAL("warm start: the network needs a cubic grid on its level (got %dx%dx%d)", gn.ni, gn.nj, gn.nk);
   w->n  = gn.ni;
   w->h2 = (gn.ni * h / M_PI) * (gn.ni * h / M_PI);
+  {
+    /* one partial per workgroup of the first input pass (and of the norms) */
+    dim3 gr = cells(w->lv[w->Ln > 0 ? 1 : 0].g);
+    if ((size_t)gr.x * gr.y * gr.z > 65536)
+      NUSIF_FATAL("warm start: grid %dx%dx%d too large for the network input pass (coarsen to extents <= %d)",
+          c->geo.ni, c->geo.nj, c->geo.nk, c->opt.unetExtent > 0 ? c->opt.unetExtent : 64);
+  }
   HIP_CHECK(hipMalloc((void **)&w->x, (size_t)w->n * w->n * w->n * 8 * sizeof(uint16_t)));
   HIP_CHECK(hipMalloc((void **)&w->scal, 4 * sizeof(double)));
   HIP_CHECK(hipHostMalloc((void **)&w->scalHost, 4 * sizeof(double), hipHostMallocDefault));
@@ -274,16 +352,23 @@ void warmstartSetNet(WarmStart *w, UnetModel *net) { w->net = net; }
 UnetModel *warmstartNet(WarmStart *w) { return w->net; }
 bool warmstartCapturing(const WarmStart *w) { return w->capture; }
 
-/* level-0 FP32 residual r0 (already in lv[0].r) and the rhs -> level Ln,
- * norms on Ln, network input x */
-static void buildInput(WarmStart *w, const double *rhs)
+/* Network input from p and rhs: r0 = rhs - A p and rhs restricted to the
+ * network level (one fused pass to level 1, then 8-child averages), their
+ * sums of squares there (scal[0], scal[1]) and of r0 on the finest grid
+ * (scal[3]), packed into x. */
+void warmstartPrepare(WarmStart *w, const double *p, const double *rhs, double idx2, double idy2, double idz2)
 {
   hipStream_t st = nusifStream();
+  Lap cf         = {idx2, idy2, idz2};
+  const Geo &g0  = w->lv[0].g;
   if (w->Ln == 0) {
-    kCast<<<cells(w->lv[0].g), dim3(WTX, WTY), 0, st>>>(rhs, w->lv[0].b, w->lv[0].g);
+    dim3 gr = cells(g0);
+    kInputFine<<<gr, dim3(WTX, WTY), 0, st>>>(p, rhs, g0, cf, w->lv[0].r, w->lv[0].b, w->partials);
+    kFoldSum1<<<1, 256, 0, st>>>(w->partials, (int)(gr.x * gr.y * gr.z), w->scal + 3);
   } else {
-    kRestrictAvg<double><<<cells(w->lv[1].g), dim3(WTX, WTY), 0, st>>>(rhs, w->lv[0].g, w->lv[1].b, w->lv[1].g);
-    kRestrictAvg<float><<<cells(w->lv[1].g), dim3(WTX, WTY), 0, st>>>(w->lv[0].r, w->lv[0].g, w->lv[1].r, w->lv[1].g);
+    dim3 gr = cells(w->lv[1].g);
+    kInputRestrict<<<gr, dim3(WTX, WTY), 0, st>>>(p, rhs, g0, cf, w->lv[1].r, w->lv[1].b, w->lv[1].g, w->partials);
+    kFoldSum1<<<1, 256, 0, st>>>(w->partials, (int)(gr.x * gr.y * gr.z), w->scal + 3);
     for (int l = 2; l <= w->Ln; l++) {
       kRestrictAvg<float><<<cells(w->lv[l].g), dim3(WTX, WTY), 0, st>>>(w->lv[l - 1].b, w->lv[l - 1].g, w->lv[l].b, w->lv[l].g);
       kRestrictAvg<float><<<cells(w->lv[l].g), dim3(WTX, WTY), 0, st>>>(w->lv[l - 1].r, w->lv[l - 1].g, w->lv[l].r, w->lv[l].g);
@@ -297,14 +382,12 @@ static void buildInput(WarmStart *w, const double *rhs)
 }
 
 /* The network's correction on the finest grid (FP32, interior and face
- * ghosts), from the r0 already in lv[0].r and the rhs: restrict, pack,
- * forward, unpack on the network grid, prolong trilinearly to level 0. The
- * guess is p + this correction (the caller forms it together with its
- * residual). */
-const float *warmstartCorrection(WarmStart *w, const double *rhs)
+ * ghosts) from the input warmstartPrepare built: forward, unpack on the
+ * network grid, prolong trilinearly to level 0. The guess is p + this
+ * correction (the caller forms it together with its residual). */
+const float *warmstartCorrection(WarmStart *w)
 {
   hipStream_t st = nusifStream();
-  buildInput(w, rhs);
   NUSIF_TIMED("unet_forward", unetForward(w->net, w->x));
   const WsLevel &L = w->lv[w->Ln];
   kUnpack<<<1024, 256, 0, st>>>(unetOutput(w->net), L.g, w->scal, w->h2, L.d);
@@ -342,12 +425,12 @@ const double *warmstartNormsAsync(WarmStart *w)
   return w->scalHost;
 }
 
-/* capture, part 1 (before the solve): the input and a copy of p */
-void warmstartCaptureBegin(WarmStart *w, const double *p, const double *rhs, const Geo &g0)
+/* capture, part 1 (before the solve, after warmstartPrepare): the input and
+ * a copy of p */
+void warmstartCaptureBegin(WarmStart *w, const double *p, const Geo &g0)
 {
   if (!w->capture) return;
   HIP_CHECK(hipMemcpyAsync(w->pPrev, p, (size_t)g0.n * sizeof(double), hipMemcpyDeviceToDevice, nusifStream()));
-  buildInput(w, rhs);
   uint16_t *x = nullptr;
   size_t nv   = (size_t)w->n * w->n * w->n;
   HIP_CHECK(hipMalloc((void **)&x, nv * 8 * sizeof(uint16_t)));

@@ -127,6 +127,7 @@ def _declare_optional(lib):
         "nusifDeviceName": ([C.c_char_p, C.c_int], C.c_int),
         "nusifTimerEnable": ([C.c_char_p], C.c_int),
         "nusifTimerDisableAll": ([], None),
+        "nusifTimeFinestSweep": ([C.POINTER(Solver), C.c_int], C.c_double),
         "nusifTimerRead": ([C.c_char_p, C.POINTER(C.c_int)], C.c_double),
         "nusifTimerReset": ([], None),
         "nusifKernelBytes": ([C.POINTER(Discretization), C.c_char_p], C.c_double),

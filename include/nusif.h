@@ -92,6 +92,10 @@ int nusifTimerEnable(const char *kernel);
 double nusifTimerRead(const char *kernel, int *launches);
 /* Stops and removes every timer (read them first). */
 void nusifTimerDisableAll(void);
+/* gmg solver: microseconds per launch of the finest level's red-black sweep,
+ * from `reps` back-to-back launches on the solver's own buffers (changes its
+ * work arrays; -1 if not applicable). */
+double nusifTimeFinestSweep(Solver *s, int reps);
 void nusifTimerReset(void);
 /* Algorithmic HBM bytes of one launch of `kernel` on the grid of `d`
  * (each array element read once and/or written once). */
