@@ -1,19 +1,20 @@
 """Benchmark: time steps/s and MLUP/s of the 256^3 lid-driven cavity on one
 MI355X, pressure solved each step to a relative residual of 1e-6.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode M] [--size N]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode M] [--size N] [--opt KEY=VALUE]
 
 A step is the reference time step (computeTimestep, boundary conditions,
 computeFG, computeRHS, [normalizePressure], solve, adaptUV; reference
 src/main.c:60-88) on the GPU library; the K timed steps run inside one
 nusifRunSteps call with all fields resident in HBM. Modes:
-  learned  U-Net warm start + FP32 V-cycles in FP64 defect correction
+  learned  U-Net warm start + FP32 V-cycles in FP64 defect correction (default)
   mixed    FP32 V-cycles in FP64 defect correction (previous-step guess)
   fp64     plain FP64 V(2,2) cycles (the comparison path)
 The headline line also times the fp64 path on the same workload and reports
-the speedup, the roofline of the dominant kernel (HIP events on the
-library's stream) and a CPU baseline (the oracle's FP64 multigrid on one host
-core, a bounded sample of steps of the same workload).
+the speedup (and, for learned, the same FP32 solver without the network),
+the roofline of the dominant kernel (HIP events on the library's stream) and
+a CPU baseline (the oracle's FP64 multigrid on one host core, a bounded
+sample of steps of the same workload).
 """
 import argparse
 import ctypes as C
@@ -155,7 +156,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--mode", default="mixed", choices=["learned", "mixed", "fp64"])
+    ap.add_argument("--mode", default="learned", choices=["learned", "mixed", "fp64"])
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-compare", action="store_true", help="skip the fp64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -218,6 +219,13 @@ def main():
         out["fp64_mg"] = {"value": f64, "mlups": cells * f64 * 1e-6,
                           "cycles_per_step": sum(ref["cycles"]) / len(ref["cycles"])}
         out["speedup_vs_fp64_mg"] = sps / f64
+        if a.mode == "learned":  # the same solver without the network
+            _, mx = run_mode(lib, "mixed", n, a.warmup, a.steps)
+            vm = a.steps / mx["seconds"]
+            out["fp32_mg_without_warmstart"] = {
+                "value": vm, "mlups": cells * vm * 1e-6,
+                "cycles_per_step": sum(mx["cycles"]) / len(mx["cycles"]),
+                "speedup_vs_fp64_mg": vm / f64}
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, n)
     case.close()

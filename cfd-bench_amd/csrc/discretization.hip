@@ -51,13 +51,18 @@ __global__ __launch_bounds__(256) void kAbsMax3(const double *__restrict__ u,
   }
 }
 
-__global__ __launch_bounds__(1024) void kFoldMax3(const double *partials, int nb, DevScalars *sc)
+/* scHost: device address of the pinned, host-coherent mirror (also written) */
+__global__ __launch_bounds__(1024) void kFoldMax3(const double *partials, int nb, DevScalars *sc, DevScalars *scHost)
 {
   __shared__ double lds[16]; /* one per wavefront */
   for (int q = 0; q < 3; q++) {
     double m = foldMax(partials, nb, 3, q, DBL_MIN, lds);
-    if (threadIdx.x == 0) sc->absmax[q] = (unsigned long long)__double_as_longlong(m);
+    if (threadIdx.x == 0) {
+      sc->absmax[q]     = (unsigned long long)__double_as_longlong(m);
+      scHost->absmax[q] = (unsigned long long)__double_as_longlong(m);
+    }
   }
+  if (threadIdx.x == 0) __threadfence_system();
 }
 
 static int reductionBlocks(long long n)
@@ -118,13 +123,13 @@ void launchAbsMax3(NusifCtx *c)
   if (c->cflValid) {
     double *shell = c->cflPartials + 3LL * c->cflInteriorBlocks;
     kShellMax3<<<SHELL_BLOCKS, 256, 0, nusifStream()>>>(c->u, c->v, c->w, c->geo, shell);
-    kFoldMax3<<<1, 1024, 0, nusifStream()>>>(c->cflPartials, c->cflInteriorBlocks + SHELL_BLOCKS, c->scal);
+    kFoldMax3<<<1, 1024, 0, nusifStream()>>>(c->cflPartials, c->cflInteriorBlocks + SHELL_BLOCKS, c->scal, c->scalHostDev);
     return;
   }
   int nb = reductionBlocks(c->geo.n);
   NUSIF_TIMED("absmax", (kAbsMax3<<<nb, 256, 0, nusifStream()>>>(c->u, c->v, c->w, c->geo.n,
                             c->partials, c->scal)));
-  kFoldMax3<<<1, 1024, 0, nusifStream()>>>(c->partials, nb, c->scal);
+  kFoldMax3<<<1, 1024, 0, nusifStream()>>>(c->partials, nb, c->scal, c->scalHostDev);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -656,7 +661,8 @@ extern "C" void initDiscretization(Discretization *s, Parameter *params)
     c->cflValid = false;
   }
   HIP_CHECK(hipMalloc((void **)&c->scal, sizeof(DevScalars)));
-  HIP_CHECK(hipHostMalloc((void **)&c->scalHost, sizeof(DevScalars), hipHostMallocDefault));
+  HIP_CHECK(hipHostMalloc((void **)&c->scalHost, sizeof(DevScalars), hipHostMallocCoherent));
+  HIP_CHECK(hipHostGetDevicePointer((void **)&c->scalHostDev, c->scalHost, 0));
   memset(c->scalHost, 0, sizeof(DevScalars));
   HIP_CHECK(hipMemcpy(c->scal, c->scalHost, sizeof(DevScalars), hipMemcpyHostToDevice));
   if (c->opt.inflowNoise != 0.0) buildInflowProfile(c);
@@ -713,9 +719,7 @@ extern "C" void computeTimestep(Discretization *s)
 {
   NusifCtx *c = nusifCtxOf(s);
   launchAbsMax3(c);
-  HIP_CHECK(hipMemcpyAsync(c->scalHost->absmax, c->scal->absmax, sizeof(c->scal->absmax),
-      hipMemcpyDeviceToHost, nusifStream()));
-  HIP_CHECK(hipStreamSynchronize(nusifStream()));
+  HIP_CHECK(hipStreamSynchronize(nusifStream())); /* kFoldMax3 wrote the pinned mirror */
   double m[3];
   for (int q = 0; q < 3; q++) memcpy(&m[q], &c->scalHost->absmax[q], sizeof(double));
   double dt = s->dtBound;

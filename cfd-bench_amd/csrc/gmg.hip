@@ -41,6 +41,11 @@
 #define GKCH 8
 #define COARSE_MAX_CELLS 2048 /* interior cells of the coarsest level (LDS solve) */
 
+/* Set while cycles are enqueued speculatively (the device flag that makes
+ * their kernels return at once), null otherwise. The library is
+ * single-threaded, as the reference's API. */
+static const int *gCycleStop = nullptr;
+
 template <typename T> struct Coef {
   T idx2, idy2, idz2, factor;
 };
@@ -80,9 +85,19 @@ struct CycleGraph {
   hipGraphExec_t exec;
 };
 
+#define CTL_RING 8 /* cycles in flight at most (pinned read-back slots) */
+struct CycleCtl;
+
 struct GmgState {
   std::vector<CycleGraph> graphs; /* replayable cycles, at most GRAPH_CACHE */
   int useGraphs;
+  int speculate;  /* enqueue cycles ahead of the convergence test */
+  int predicted;  /* cycles of the previous solve: the first batch */
+  CycleCtl *dCtl; /* device cycle control */
+  CycleCtl *hRing;/* pinned copies of dCtl after each enqueued cycle */
+  CycleCtl *hRingDev; /* their device address (written by kFoldCycle) */
+  CycleCtl *hInit;
+  hipEvent_t ev[CTL_RING];
   DctTables dct;
   double *tab64;
   float *tab32;
@@ -121,8 +136,9 @@ __device__ inline T lap7(const T *x, long long c, long long sj, long long sk, co
  * values are final by then. */
 template <typename T>
 __global__ __launch_bounds__(256) void kSmooth(T *__restrict__ x, const T *__restrict__ b, Geo g,
-    Coef<T> cf, int parity, int ghosts)
+    Coef<T> cf, int parity, int ghosts, const int *__restrict__ stop)
 {
+  if (stop && *stop) return;
   int tx = blockIdx.x * blockDim.x + threadIdx.x;
   int j  = blockIdx.y * blockDim.y + threadIdx.y + 1;
   int k0 = blockIdx.z * GKCH + 1;
@@ -179,8 +195,9 @@ __device__ inline void ldsBarrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_ba
  * the same values as reading a zeroed array, without the zeroing pass. */
 template <typename T, int TJ, int MINW, bool ZX>
 __global__ __launch_bounds__(256, MINW) void kSweepFused(const T *__restrict__ xo, T *__restrict__ xn,
-    const T *__restrict__ b, Geo g, Coef<T> cf, int kc)
+    const T *__restrict__ b, Geo g, Coef<T> cf, int kc, const int *__restrict__ stop)
 {
+  if (stop && *stop) return;
   constexpr int NT  = 256;
   constexpr int NBK = 64 * TJ / NT; /* black cells per thread (rows ty, ty + 4, ...) */
   constexpr int RS  = NT / 64;      /* thread rows */
@@ -390,14 +407,14 @@ static void launchSweepShape(const LevelT<T> &L, dim3 grd, int kc, bool timed, b
                                     : (zx ? "gmg_smooth32z_l0" : "gmg_smooth32_l0");
   if (zx) {
     if (timed)
-      NUSIF_TIMED(name, (kSweepFused<T, TJ, MINW, true><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc)));
+      NUSIF_TIMED(name, (kSweepFused<T, TJ, MINW, true><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc, gCycleStop)));
     else
-      kSweepFused<T, TJ, MINW, true><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc);
+      kSweepFused<T, TJ, MINW, true><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc, gCycleStop);
   } else {
     if (timed)
-      NUSIF_TIMED(name, (kSweepFused<T, TJ, MINW, false><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc)));
+      NUSIF_TIMED(name, (kSweepFused<T, TJ, MINW, false><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc, gCycleStop)));
     else
-      kSweepFused<T, TJ, MINW, false><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc);
+      kSweepFused<T, TJ, MINW, false><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc, gCycleStop);
   }
 }
 
@@ -418,8 +435,9 @@ template <typename T> static bool fusedLevel(const LevelT<T> &L) { return L.g.ni
 template <typename T>
 __global__ __launch_bounds__(256) void kResidualRestrict(const T *__restrict__ x,
     const T *__restrict__ b, Geo gf, Coef<T> cf, T *__restrict__ bc, T *__restrict__ xc, Geo gc,
-    int gridGx, int gridGy)
+    int gridGx, int gridGy, const int *__restrict__ stop)
 {
+  if (stop && *stop) return;
   int bx, by, bz;
   if (!xcdBlock(gridGx, gridGy, gc.nk, bx, by, bz)) return;
   int I = bx * blockDim.x + threadIdx.x + 1;
@@ -464,8 +482,9 @@ template <typename T> __global__ __launch_bounds__(256) void kFill(T *__restrict
 #define PKC 4
 template <typename T>
 __global__ __launch_bounds__(256) void kProlongMarch(T *__restrict__ xf, Geo gf, const T *__restrict__ xc,
-    Geo gc, int gridGx, int gridGy, int gridGz)
+    Geo gc, int gridGx, int gridGy, int gridGz, const int *__restrict__ stop)
 {
+  if (stop && *stop) return;
   int bx, by, bz;
   if (!xcdBlock(gridGx, gridGy, gridGz, bx, by, bz)) return;
   int i = bx * blockDim.x + threadIdx.x + 1;
@@ -575,8 +594,9 @@ __device__ void coarseDct(T *__restrict__ x, const T *__restrict__ b, const Geo 
 
 template <typename T>
 __global__ __launch_bounds__(1024) void kCoarseDct(T *__restrict__ x, const T *__restrict__ b, Geo g,
-    const T *__restrict__ tab, DctTables L)
+    const T *__restrict__ tab, DctTables L, const int *__restrict__ stop)
 {
+  if (stop && *stop) return;
   extern __shared__ unsigned char smem[];
   coarseDct(x, b, g, tab, L, smem);
 }
@@ -607,6 +627,7 @@ template <typename T> struct TailArgs {
   Coef<T> cf[MAXTAIL];
   const T *tab;   /* DCT tables of the coarsest level */
   DctTables dct;
+  const int *stop; /* skip the launch when *stop (speculative cycles) */
 };
 
 __device__ inline void tailSync() { __syncthreads(); }
@@ -741,6 +762,7 @@ template <typename T> __device__ void tailZero(T *__restrict__ x, long long n)
 template <typename T>
 __global__ __launch_bounds__(TAIL_THREADS) void kTailCycle(TailArgs<T> a)
 {
+  if (a.stop && *a.stop) return;
   extern __shared__ unsigned char smem[];
   for (int l = 0; l < a.n; l++) tailZero(a.x[l], a.g[l].n);
   tailSync();
@@ -788,8 +810,10 @@ __global__ __launch_bounds__(TAIL_THREADS) void kTailCycle(TailArgs<T> a)
 template <bool CORR>
 __global__ __launch_bounds__(256) void kResidualLds(const double *__restrict__ p, const float *__restrict__ e,
     const double *__restrict__ rhs, Geo g, Coef<double> cf, double *__restrict__ pn, float *__restrict__ r32,
-    float *__restrict__ eZero, double *partials, int gridGx, int gridGy, int gridGz)
+    float *__restrict__ eZero, double *partials, int gridGx, int gridGy, int gridGz,
+    const int *__restrict__ stop = nullptr)
 {
+  if (stop && *stop) return;
   constexpr int HW = GTX + 2, HN = HW * (GTY + 2);
   __shared__ double lds[4];
   __shared__ double X[3][GTY + 2][HW];
@@ -853,14 +877,61 @@ __global__ __launch_bounds__(256) void kResidualLds(const double *__restrict__ p
   }
 }
 
-__global__ __launch_bounds__(256) void kFoldNorms(const double *partials, int nb, double *out)
+/* out: device norm pair; outHost (optional): device address of a pinned,
+ * host-coherent copy the host reads after the stream synchronises */
+__global__ __launch_bounds__(256) void kFoldNorms(const double *partials, int nb, double *out, double *outHost)
 {
   __shared__ double lds[4];
   double s0, s1;
   foldSum2(partials, nb, lds, s0, s1);
+  if (outHost && threadIdx.x == 0) {
+    outHost[0] = s0;
+    outHost[1] = s1;
+    __threadfence_system();
+  }
   if (threadIdx.x == 0) {
     out[0] = s0;
     out[1] = s1;
+  }
+}
+
+/* Cycle control of speculatively enqueued cycles: the host enqueues cycles
+ * ahead of knowing whether the previous one converged; every kernel of a
+ * cycle returns at once when `stop` is set, and the cycle's last kernel
+ * (kFoldCycle) sets it with the host loop's own test. */
+struct CycleCtl {
+  int stop;      /* no further cycle runs */
+  int executed;  /* cycles that ran */
+  int maxCycles;
+  int pad;
+  double tol;
+  double r2, b2; /* ||r||^2, ||rhs||^2 after the last cycle that ran */
+};
+
+/* hostRing: device address of CTL_RING pinned, host-coherent CycleCtl
+ * slots. A cycle that runs writes its result into slot (executed - 1) %
+ * CTL_RING, the slot of its own cycle index, which the host reads after the
+ * cycle's event. The slot comes from the device count, not from a launch
+ * argument, so a replayed captured cycle writes the right one. Skipped
+ * cycles write nothing: the host stops reading at the cycle that set stop. */
+__global__ __launch_bounds__(256) void kFoldCycle(const double *partials, int nb, double *norm2, CycleCtl *ctl,
+    CycleCtl *hostRing)
+{
+  __shared__ double lds[4];
+  if (ctl->stop) return;
+  double s0, s1;
+  foldSum2(partials, nb, lds, s0, s1);
+  if (threadIdx.x == 0) {
+    norm2[0]      = s0;
+    norm2[1]      = s1;
+    ctl->r2       = s0;
+    ctl->b2       = s1;
+    int e         = ctl->executed + 1;
+    ctl->executed = e;
+    double rn = sqrt(s0), nb2 = sqrt(s1);
+    ctl->stop = (nb2 > 0.0 && rn / nb2 > ctl->tol && e < ctl->maxCycles) ? 0 : 1;
+    hostRing[(e - 1) % CTL_RING] = *ctl;
+    __threadfence_system();
   }
 }
 
@@ -923,8 +994,8 @@ template <typename T> static void smooth(GmgState *G, LevelT<T> &L, int sweeps, 
   if (!fusedLevel(L)) { /* small levels: two colour passes */
     dim3 grd = sweepGrid(L.g);
     for (int s = 0; s < sweeps; s++) {
-      kSmooth<T><<<grd, dim3(GTX, GTY), 0, nusifStream()>>>(L.x, L.b, L.g, L.cf, 1, 0);
-      kSmooth<T><<<grd, dim3(GTX, GTY), 0, nusifStream()>>>(L.x, L.b, L.g, L.cf, 0, 1);
+      kSmooth<T><<<grd, dim3(GTX, GTY), 0, nusifStream()>>>(L.x, L.b, L.g, L.cf, 1, 0, gCycleStop);
+      kSmooth<T><<<grd, dim3(GTX, GTY), 0, nusifStream()>>>(L.x, L.b, L.g, L.cf, 0, 1, gCycleStop);
     }
     return;
   }
@@ -943,7 +1014,7 @@ static const float *tables(GmgState *G, float *) { return G->tab32; }
 template <typename T> static void coarseSolve(GmgState *G, LevelT<T> &L)
 {
   size_t lds = 2 * (size_t)L.g.ni * L.g.nj * L.g.nk * sizeof(T);
-  kCoarseDct<T><<<1, 1024, lds, nusifStream()>>>(L.x, L.b, L.g, tables(G, (T *)nullptr), G->dct);
+  kCoarseDct<T><<<1, 1024, lds, nusifStream()>>>(L.x, L.b, L.g, tables(G, (T *)nullptr), G->dct, gCycleStop);
 }
 
 /* DCT-II tables of the coarsest level (closed forms; host libm) */
@@ -991,8 +1062,9 @@ template <typename T> static void tailCycle(GmgState *G, LevelT<T> *L)
     a.g[q]  = L[G->tail + q].g;
     a.cf[q] = L[G->tail + q].cf;
   }
-  a.tab = tables(G, (T *)nullptr);
-  a.dct = G->dct;
+  a.tab  = tables(G, (T *)nullptr);
+  a.dct  = G->dct;
+  a.stop = gCycleStop;
   const Geo &gc = L[G->nl - 1].g;
   size_t lds    = 2 * (size_t)gc.ni * gc.nj * gc.nk * sizeof(T);
   kTailCycle<T><<<1, TAIL_THREADS, lds, nusifStream()>>>(a);
@@ -1016,19 +1088,19 @@ template <typename T> static void vcycle(GmgState *G, LevelT<T> *L, int l)
   if (fine)
     NUSIF_TIMED(sizeof(T) == 8 ? "gmg_restrict64_l0" : "gmg_restrict32_l0",
         (kResidualRestrict<T><<<xcdGrid(cellGrid(C.g)), dim3(GTX, GTY), 0, nusifStream()>>>(
-            L[l].x, L[l].b, L[l].g, L[l].cf, C.b, C.x, C.g, cellGrid(C.g).x, cellGrid(C.g).y)));
+            L[l].x, L[l].b, L[l].g, L[l].cf, C.b, C.x, C.g, cellGrid(C.g).x, cellGrid(C.g).y, gCycleStop)));
   else
     kResidualRestrict<T><<<xcdGrid(cellGrid(C.g)), dim3(GTX, GTY), 0, nusifStream()>>>(
-        L[l].x, L[l].b, L[l].g, L[l].cf, C.b, C.x, C.g, cellGrid(C.g).x, cellGrid(C.g).y);
+        L[l].x, L[l].b, L[l].g, L[l].cf, C.b, C.x, C.g, cellGrid(C.g).x, cellGrid(C.g).y, gCycleStop);
   vcycle(G, L, l + 1);
   dim3 pg((L[l].g.ni + GTX - 1) / GTX, (L[l].g.nj + GTY - 1) / GTY, (C.g.nk + PKC - 1) / PKC);
   if (fine)
     NUSIF_TIMED(sizeof(T) == 8 ? "gmg_prolong64_l0" : "gmg_prolong32_l0",
         (kProlongMarch<T><<<xcdGrid(pg), dim3(GTX, GTY), 0, nusifStream()>>>(
-            L[l].x, L[l].g, C.x, C.g, pg.x, pg.y, pg.z)));
+            L[l].x, L[l].g, C.x, C.g, pg.x, pg.y, pg.z, gCycleStop)));
   else
     kProlongMarch<T><<<xcdGrid(pg), dim3(GTX, GTY), 0, nusifStream()>>>(
-        L[l].x, L[l].g, C.x, C.g, pg.x, pg.y, pg.z);
+        L[l].x, L[l].g, C.x, C.g, pg.x, pg.y, pg.z, gCycleStop);
   smooth(G, L[l], G->nu2, fine, false);
 }
 
@@ -1053,6 +1125,13 @@ GmgState *gmgCreate(NusifSolverCtx *sc, const Parameter *prm)
     G->L32[l] = LevelT<float>();
   }
   G->useGraphs = c->opt.mgGraph;
+  G->speculate = c->opt.mgSpeculate;
+  G->predicted = 1;
+  HIP_CHECK(hipMalloc((void **)&G->dCtl, sizeof(CycleCtl)));
+  HIP_CHECK(hipHostMalloc((void **)&G->hRing, CTL_RING * sizeof(CycleCtl), hipHostMallocCoherent));
+  HIP_CHECK(hipHostGetDevicePointer((void **)&G->hRingDev, G->hRing, 0));
+  HIP_CHECK(hipHostMalloc((void **)&G->hInit, sizeof(CycleCtl), hipHostMallocDefault));
+  for (int q = 0; q < CTL_RING; q++) HIP_CHECK(hipEventCreateWithFlags(&G->ev[q], hipEventDisableTiming));
   G->tileRows  = c->opt.mgTileRows;
   G->nu1       = prm->presmooth;
   G->nu2       = prm->postsmooth;
@@ -1144,6 +1223,10 @@ GmgState *gmgCreate(NusifSolverCtx *sc, const Parameter *prm)
 
 void gmgFree(GmgState *G)
 {
+  HIP_CHECK(hipFree(G->dCtl));
+  HIP_CHECK(hipHostFree(G->hRing));
+  HIP_CHECK(hipHostFree(G->hInit));
+  for (int q = 0; q < CTL_RING; q++) HIP_CHECK(hipEventDestroy(G->ev[q]));
   for (CycleGraph &e : G->graphs) HIP_CHECK(hipGraphExecDestroy(e.exec));
   G->graphs.clear();
   if (G->tab64) HIP_CHECK(hipFree(G->tab64));
@@ -1254,24 +1337,95 @@ static void launchResidual(NusifCtx *c, GmgState *G, const double *p, const doub
   NUSIF_TIMED("gmg_residual64", (kResidualLds<false><<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(
                                     p, nullptr, rhs, c->geo, G->cf0, nullptr, r32, eZero, c->partials, ng.x,
                                     ng.y, ng.z)));
-  kFoldNorms<<<1, 256, 0, nusifStream()>>>(c->partials, (int)(ng.x * ng.y * ng.z), out);
+  kFoldNorms<<<1, 256, 0, nusifStream()>>>(c->partials, (int)(ng.x * ng.y * ng.z), out,
+      out == c->scal->norm2 ? c->scalHostDev->norm2 : (out == c->scal->norm2 + 2 ? c->scalHostDev->norm2 + 2 : nullptr));
 }
 
-/* read back norm2[0..1] (||r||^2, ||rhs||^2): returns ||r||, sets *nb */
+/* read norm2[0..1] (||r||^2, ||rhs||^2), which the fold kernels also write
+ * into the pinned mirror: returns ||r||, sets *nb */
 static double readNorms(NusifCtx *c, double *nb)
 {
-  HIP_CHECK(hipMemcpyAsync(c->scalHost->norm2, c->scal->norm2, 4 * sizeof(double),
-      hipMemcpyDeviceToHost, nusifStream()));
   HIP_CHECK(hipStreamSynchronize(nusifStream()));
   *nb = sqrt(c->scalHost->norm2[1]);
   return sqrt(c->scalHost->norm2[0]);
+}
+
+/* Runs V-cycles until ||rhs - A p|| / ||rhs|| <= tol or maxCycles, from the
+ * residual norms (rel, nb) of the starting p; updates them and returns the
+ * cycle count. `body` enqueues one cycle that ends with the residual
+ * partials of the new iterate (normGrid layout in c->partials).
+ *
+ * Speculative mode: the host enqueues cycles before it knows whether the
+ * previous one converged (as many as the previous solve needed, then one
+ * at a time), the device decides (kFoldCycle sets the stop flag every
+ * cycle kernel checks first), and the host reads the decision through
+ * pinned memory per cycle. Cycles enqueued past the stop do nothing. The
+ * kernels that run, and so the result, are those of the one-at-a-time loop;
+ * the buffer pointers are set back to their arrangement after the cycles
+ * that ran. */
+template <typename Body> static int runCycles(NusifCtx *c, GmgState *G, double &rel, double &nb, Body body)
+{
+  dim3 ng        = normGrid(c->geo);
+  const int nbk  = (int)(ng.x * ng.y * ng.z);
+  hipStream_t st = nusifStream();
+  int cycles     = 0;
+  if (!G->speculate || nusifTimersActive()) {
+    while (nb > 0.0 && rel > G->tol && cycles < G->maxCycles) {
+      runCycle(c, G, [&] {
+        body();
+        kFoldNorms<<<1, 256, 0, st>>>(c->partials, nbk, c->scal->norm2, c->scalHostDev->norm2);
+      });
+      cycles++;
+      double rn = readNorms(c, &nb);
+      rel       = rn / nb;
+    }
+    return cycles;
+  }
+  if (!(nb > 0.0 && rel > G->tol && G->maxCycles > 0)) return 0;
+  *G->hInit = CycleCtl{0, 0, G->maxCycles, 0, G->tol, 0.0, 0.0};
+  HIP_CHECK(hipMemcpyAsync(G->dCtl, G->hInit, sizeof(CycleCtl), hipMemcpyHostToDevice, st));
+  gCycleStop = &G->dCtl->stop;
+  std::vector<std::vector<void *>> states;
+  const int pred = G->predicted > 0 ? G->predicted : 1;
+  int enq = 0, chk = 0;
+  CycleCtl last = *G->hInit;
+  for (;;) {
+    int target = pred > chk + 1 ? pred : chk + 1;
+    while (enq < G->maxCycles && enq < target && enq < chk + CTL_RING) {
+      states.push_back(cycleState(c, G));
+      runCycle(c, G, [&] {
+        body();
+        kFoldCycle<<<1, 256, 0, st>>>(c->partials, nbk, c->scal->norm2, G->dCtl, G->hRingDev);
+      });
+      HIP_CHECK(hipEventRecord(G->ev[enq % CTL_RING], st));
+      enq++;
+    }
+    HIP_CHECK(hipEventSynchronize(G->ev[chk % CTL_RING]));
+    last = G->hRing[chk % CTL_RING];
+    if (last.executed != chk + 1) /* every cycle up to chk ran: the slot is cycle chk's */
+      NUSIF_FATAL("gmg: cycle control slot of cycle %d holds count %d", chk, last.executed);
+    chk++;
+    if (last.stop) break;
+    if (chk >= enq && enq >= G->maxCycles)
+      NUSIF_FATAL("gmg: cycle control: %d cycles ran without the stop flag", enq);
+  }
+  gCycleStop = nullptr;
+  states.push_back(cycleState(c, G));
+  if (last.executed < 1 || last.executed >= (int)states.size())
+    NUSIF_FATAL("gmg: %d cycles reported for %d enqueued", last.executed, enq);
+  setCycleState(c, G, states[last.executed]);
+  nb           = sqrt(last.b2);
+  rel          = sqrt(last.r2) / nb;
+  G->predicted = last.executed;
+  return last.executed;
 }
 
 /* fold the partials of a normGrid launch into norm2[0..1] and read them */
 static double foldNorms(NusifCtx *c, double *nb)
 {
   dim3 ng = normGrid(c->geo);
-  kFoldNorms<<<1, 256, 0, nusifStream()>>>(c->partials, (int)(ng.x * ng.y * ng.z), c->scal->norm2);
+  kFoldNorms<<<1, 256, 0, nusifStream()>>>(c->partials, (int)(ng.x * ng.y * ng.z), c->scal->norm2,
+      c->scalHostDev->norm2);
   return readNorms(c, nb);
 }
 
@@ -1321,7 +1475,8 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
       dim3 ng    = normGrid(g);
       NUSIF_TIMED("gmg_residual64", (kResidualLds<true><<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(
                                         p, d, rhs, g, G->cf0, c->pAlt, rg, e0, c->partials, ng.x, ng.y, ng.z)));
-      kFoldNorms<<<1, 256, 0, nusifStream()>>>(c->partials, (int)(ng.x * ng.y * ng.z), c->scal->norm2);
+      kFoldNorms<<<1, 256, 0, nusifStream()>>>(c->partials, (int)(ng.x * ng.y * ng.z), c->scal->norm2,
+          c->scalHostDev->norm2);
       warmstartCoarseNormLaunch(ws, rg);
       const double *wn = warmstartNormsAsync(ws);
       double nbg       = 0.0;
@@ -1356,46 +1511,38 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
     L0.b    = rhs;
     double rn = residualNorm(c, G, L0.x, rhs, nullptr, &nb);
     rel       = nb > 0.0 ? rn / nb : 0.0;
-    while (nb > 0.0 && rel > G->tol && cycles < G->maxCycles) {
-      runCycle(c, G, [&] {
-        vcycle<double>(G, G->L64, 0);
-        launchResidual(c, G, L0.x, rhs, nullptr, nullptr, c->scal->norm2);
-      });
-      cycles++;
-      rn  = readNorms(c, &nb);
-      rel = rn / nb;
-    }
+    dim3 ng   = normGrid(g);
+    cycles    = runCycles(c, G, rel, nb, [&] {
+      vcycle<double>(G, G->L64, 0);
+      NUSIF_TIMED("gmg_residual64", (kResidualLds<false><<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(
+                                        L0.x, nullptr, rhs, g, G->cf0, nullptr, nullptr, nullptr, c->partials,
+                                        ng.x, ng.y, ng.z, gCycleStop)));
+    });
     if (L0.x != c->p) { /* the result sits in the partner buffer */
       c->pAlt = c->p;
       c->p    = L0.x;
     }
   } else {
-    /* r (FP32) into L0.b, e = 0 into L0.x; each cycle then ends with one
-     * pass that applies e to p (into the partner buffer), forms the next r
-     * and zeroes the partner of e for the next cycle */
+    /* r (FP32) into L0.b (and e = 0 into L0.x where the first sweep reads
+     * it); each cycle then ends with one pass that applies e to p (into the
+     * partner buffer) and forms the next r */
     LevelT<float> &L0 = G->L32[0];
     double rn = haveInitial ? rnInit : residualNorm(c, G, p, rhs, L0.b, &nb, l0NeedsZero(G) ? L0.x : nullptr);
     rel       = nb > 0.0 ? rn / nb : 0.0;
     dim3 ng   = normGrid(g);
-    while (nb > 0.0 && rel > G->tol && cycles < G->maxCycles) {
-      runCycle(c, G, [&] {
-        vcycle<float>(G, G->L32, 0);
-        NUSIF_TIMED("gmg_correct_residual",
-            (kResidualLds<true><<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(c->p, L0.x, rhs, g,
-                G->cf0, c->pAlt, L0.b, l0NeedsZero(G) ? L0.xAlt : nullptr, c->partials, ng.x, ng.y, ng.z)));
-        float *te  = L0.x; /* the next cycle's zeroed e */
-        L0.x       = L0.xAlt;
-        L0.xAlt    = te;
-        double *tp = c->p; /* the corrected p is in c->pAlt */
-        c->p       = c->pAlt;
-        c->pAlt    = tp;
-        kFoldNorms<<<1, 256, 0, nusifStream()>>>(c->partials, (int)(ng.x * ng.y * ng.z), c->scal->norm2);
-      });
-      p = c->p;
-      cycles++;
-      rn  = readNorms(c, &nb);
-      rel = rn / nb;
-    }
+    cycles    = runCycles(c, G, rel, nb, [&] {
+      vcycle<float>(G, G->L32, 0);
+      NUSIF_TIMED("gmg_correct_residual",
+          (kResidualLds<true><<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(c->p, L0.x, rhs, g, G->cf0,
+              c->pAlt, L0.b, l0NeedsZero(G) ? L0.xAlt : nullptr, c->partials, ng.x, ng.y, ng.z, gCycleStop)));
+      float *te  = L0.x; /* the next cycle's (zeroed) e */
+      L0.x       = L0.xAlt;
+      L0.xAlt    = te;
+      double *tp = c->p; /* the corrected p is in c->pAlt */
+      c->p       = c->pAlt;
+      c->pAlt    = tp;
+    });
+    p = c->p;
   }
   if (ws) warmstartCaptureEnd(ws, c->p);
   sc->lastIterations = cycles;

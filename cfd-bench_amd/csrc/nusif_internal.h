@@ -104,6 +104,7 @@ struct NusifOptions {
   int mgTail;      /* gmg: coarse levels in one workgroup (1) or per-level kernels (0) */
   int mgGraph;     /* gmg: replay each cycle from a captured HIP graph (1) or launch eagerly (0) */
   int mgTileRows;  /* gmg: FP32 fused-sweep tile height, 0 automatic, 4 or 8 */
+  int mgSpeculate; /* gmg: enqueue cycles ahead of the convergence test (1) or one at a time (0) */
   double lid;
   double uInflow;
   double inflowNoise;
@@ -142,7 +143,8 @@ struct NusifCtx {
   double *partials;    /* reduction partials */
   size_t nPartials;
   DevScalars *scal;    /* device */
-  DevScalars *scalHost;/* pinned mirror */
+  DevScalars *scalHost;/* pinned, host-coherent mirror */
+  DevScalars *scalHostDev; /* its device address: reduction kernels write results there directly */
   double *inflow;      /* optional per-cell inflow profile (jmax+2)*(kmax+2) */
   /* CFL maxima: adaptUV leaves per-workgroup maxima of the interior range it
    * wrote in cflPartials; while no other kernel has written u, v, w since

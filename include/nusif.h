@@ -51,6 +51,10 @@ void nusifFreeSolver(Solver *s);
  *                         launch its kernels one by one (0, default); same values
  *   mg_tile_rows  0|4|8   gmg: FP32 fused-sweep tile height (0: automatic, 8 on
  *                         the largest levels); same values
+ *   mg_speculate  0 | 1   gmg: enqueue cycles before the previous one's
+ *                         convergence test is read back (1; the device skips
+ *                         the ones past convergence) or one at a time (0,
+ *                         default); same values
  *   lid           double  lid speed of "dcavity" (reference: 1.0)
  *   u_inflow      double  inflow velocity of "canal" (reference: 2.0)
  *   inflow_noise  double  relative uniform perturbation of the inflow (0)

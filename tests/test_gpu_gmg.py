@@ -213,3 +213,35 @@ def test_gpu_channel_steps_per_field_error(precision, gpu_ok):
         assert err <= (1e-10 if precision == "fp64" else 1e-6), k
     g.close()
     o.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gpu_gmg_speculative_cycles_equal_one_at_a_time(precision, gpu_ok):
+    """Cycles enqueued ahead of the convergence read-back (mg_speculate 1:
+    the device skips those past convergence and the host restores the
+    buffer arrangement) against one cycle per read-back (mg_speculate 0):
+    the same cycle count every step and every field equal bit for bit,
+    over steps whose cycle counts go up and down (a batch sized by the
+    previous step then overshoots or falls short)."""
+    print("gmg speculate", precision, flush=True)
+    prm = dict(G.TRAJ["c1_rb"][1], imax=64, jmax=64, kmax=64, re=1000.0, levels=5,
+               presmooth=2, postsmooth=2)
+    cases = []
+    for spec in (0, 1):
+        opts = {"solver": "gmg", "precision": precision, "mg_tol": 1e-7, "mg_omega": 1.3,
+                "mg_speculate": spec}
+        cases.append(A.Case(lib("gpu_rb"), options=opts, **prm))
+    counts = []
+    for step in range(10):
+        for c in cases:
+            c.step()
+        it = [c.lib.nusifSolverIterations(C.byref(c.s)) for c in cases]
+        assert it[0] == it[1], (step, it)
+        counts.append(it[0])
+        fa, fb = cases[0].fields(), cases[1].fields()
+        for k in ("u", "v", "w", "p"):
+            np.testing.assert_array_equal(fa[k], fb[k], err_msg="%s step %d" % (k, step))
+    print("cycles per step", counts)
+    assert min(counts) >= 1
+    for c in cases:
+        c.close()
