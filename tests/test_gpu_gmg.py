@@ -132,7 +132,10 @@ def test_gpu_gmg_graph_replay_equals_eager_launches(precision, gpu_ok):
                presmooth=2, postsmooth=2)
     cases = []
     for graph in (0, 1):
-        opts = {"solver": "gmg", "precision": precision, "mg_tol": 1e-8, "mg_graph": graph}
+        # speculative cycles too: the replayed fold kernel must write the
+        # cycle-control slot of the cycle that runs, not of the captured one
+        opts = {"solver": "gmg", "precision": precision, "mg_tol": 1e-8, "mg_graph": graph,
+                "mg_speculate": 1}
         cases.append(A.Case(lib("gpu_rb"), options=opts, **prm))
     total_cycles = 0
     for step in range(6):
