@@ -166,6 +166,76 @@ __global__ __launch_bounds__(256) void kConvMfma(ConvArgs<InT, OutT> a)
 #define CL_CC 32
 #define CL_CSTR 40
 
+/* One channel chunk of G8 8-channel groups (G8 = 4, 2 or 1): stage the
+ * brick's halo into LDS, then K-steps of four (tap, group) pairs. G8 is a
+ * compile-time constant so the (tap, group) split of the K index and the
+ * tap's 3-D offset are shifts and constant multiplies, not divisions. */
+template <int NB, int G8>
+__device__ inline void convChunk(const ConvArgs<uint16_t, uint16_t> &a, uint16_t *sIn, int cc0, int cin, int rows,
+    int row0, int x0, int y0, int z0, int wave, int lr, int lg, floatx4 (&acc)[NB][4])
+{
+  const int tid = threadIdx.x, n = a.n;
+  for (int u = tid; u < CL_HV * G8; u += 256) {
+    int v = u / G8, q = u - v * G8;
+    int hx = v % CL_HX, hy = (v / CL_HX) % CL_HY, hz = v / (CL_HX * CL_HY);
+    int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
+    uint4 val = make_uint4(0, 0, 0, 0);
+    if (x >= 0 && x < n && y >= 0 && y < n && z >= 0 && z < n) {
+      size_t vv = ((size_t)z * n + y) * n + x;
+      int c     = cc0 + q * 8;
+      val = c < a.c0 ? *(const uint4 *)(a.in0 + vv * a.c0 + c) : *(const uint4 *)(a.in1 + vv * a.c1 + (c - a.c0));
+    }
+    *(uint4 *)&sIn[v * CL_CSTR + q * 8] = val;
+  }
+  constexpr int G = 27 * G8;
+  /* per-lane bases: the A row offsets and the LDS voxel of each block at tap
+   * offset (-1, -1, -1); a tap adds (dz * HY + dy) * HX + dx */
+  size_t aRow[NB];
+  bool rowOk[NB];
+#pragma unroll
+  for (int nb = 0; nb < NB; nb++) {
+    int r     = row0 + nb * 16 + lr;
+    rowOk[nb] = r < rows;
+    aRow[nb]  = (size_t)(rowOk[nb] ? r : 0) * 27 * cin + cc0;
+  }
+  int hvBase[4];
+#pragma unroll
+  for (int mb = 0; mb < 4; mb++) hvBase[mb] = (wave * CL_HY + mb) * CL_HX + lr;
+  auto loadA = [&](int s, bf16x8 *fa) {
+    int gi  = s + lg;
+    bool gv = gi < G;
+    int t = gv ? gi / G8 : 0, c8 = gv ? gi % G8 : 0;
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++)
+      fa[nb] = (gv && rowOk[nb]) ? ld8(a.A + aRow[nb] + (size_t)t * cin + c8 * 8) : zero8();
+  };
+  bf16x8 faNext[NB];
+  loadA(0, faNext);
+  __syncthreads();
+#pragma unroll 2
+  for (int s = 0; s < G; s += 4) {
+    int gi  = s + lg;
+    bool gv = gi < G;
+    int t = gv ? gi / G8 : 0, c8 = gv ? gi % G8 : 0;
+    int off = ((t / 9) * CL_HY + (t / 3) % 3) * CL_HX + t % 3;
+    bf16x8 fa[NB], fb[4];
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) fa[nb] = faNext[nb];
+    if (s + 4 < G) loadA(s + 4, faNext);
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++) {
+      uint4 raw = *(const uint4 *)&sIn[(hvBase[mb] + off) * CL_CSTR + c8 * 8];
+      fb[mb]    = gv ? __builtin_bit_cast(bf16x8, raw) : zero8();
+    }
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+      for (int mb = 0; mb < 4; mb++)
+        acc[nb][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[nb], fb[mb], acc[nb][mb], 0, 0, 0);
+  }
+  __syncthreads();
+}
+
 template <typename OutT, int NB>
 __global__ __launch_bounds__(256) void kConvLds(ConvArgs<uint16_t, OutT> a, float *__restrict__ split)
 {
@@ -193,59 +263,21 @@ __global__ __launch_bounds__(256) void kConvLds(ConvArgs<uint16_t, OutT> a, floa
   const int cEnd = gridDim.z > 1 ? min(cBeg + CL_CC, cin) : cin;
   for (int cc0 = cBeg; cc0 < cEnd; cc0 += CL_CC) {
     const int g8 = min(CL_CC, cin - cc0) >> 3; /* 8-channel groups in this chunk */
-    for (int u = tid; u < CL_HV * g8; u += 256) {
-      int v = u / g8, q = u - v * g8;
-      int hx = v % CL_HX, hy = (v / CL_HX) % CL_HY, hz = v / (CL_HX * CL_HY);
-      int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
-      uint4 val = make_uint4(0, 0, 0, 0);
-      if (x >= 0 && x < n && y >= 0 && y < n && z >= 0 && z < n) {
-        size_t vv = ((size_t)z * n + y) * n + x;
-        int c     = cc0 + q * 8;
-        val = c < a.c0 ? *(const uint4 *)(a.in0 + vv * a.c0 + c) : *(const uint4 *)(a.in1 + vv * a.c1 + (c - a.c0));
-      }
-      *(uint4 *)&sIn[v * CL_CSTR + q * 8] = val;
+    if (g8 == 4)
+      convChunk<NB, 4>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
+    else if (g8 == 2)
+      convChunk<NB, 2>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
+    else if (g8 == 1)
+      convChunk<NB, 1>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
+    else /* three groups (24-channel chunk): as two chunks of 2 and 1 */ {
+      convChunk<NB, 2>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
+      convChunk<NB, 1>(a, sIn, cc0 + 16, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
     }
-    const int G = 27 * g8;
-    /* weight fragments of K-step s: loaded one step ahead (double buffer), so
-     * the global latency overlaps the previous step's MFMAs */
-    auto loadA = [&](int s, bf16x8 *fa) {
-      int gi  = s + lg;
-      bool gv = gi < G;
-      int t = gv ? gi / g8 : 0, c8 = gv ? gi - t * g8 : 0;
-#pragma unroll
-      for (int nb = 0; nb < NB; nb++) {
-        int r  = row0 + nb * 16 + lr;
-        fa[nb] = (gv && r < rows) ? ld8(a.A + ((size_t)r * 27 + t) * cin + cc0 + c8 * 8) : zero8();
-      }
-    };
-    bf16x8 faNext[NB];
-    loadA(0, faNext);
-    __syncthreads();
-    for (int s = 0; s < G; s += 4) {
-      int gi  = s + lg;
-      bool gv = gi < G;
-      int t = gv ? gi / g8 : 0, c8 = gv ? gi - t * g8 : 0;
-      int dz = t / 9, dy = (t / 3) % 3, dx = t % 3; /* tap offset + 1 = halo offset */
-      bf16x8 fa[NB], fb[4];
-#pragma unroll
-      for (int nb = 0; nb < NB; nb++) fa[nb] = faNext[nb];
-      if (s + 4 < G) loadA(s + 4, faNext);
-#pragma unroll
-      for (int mb = 0; mb < 4; mb++) {
-        int hv = ((wave + dz) * CL_HY + (mb + dy)) * CL_HX + (lr + dx);
-        uint4 raw = *(const uint4 *)&sIn[hv * CL_CSTR + c8 * 8];
-        fb[mb]    = gv ? __builtin_bit_cast(bf16x8, raw) : zero8();
-      }
-#pragma unroll
-      for (int nb = 0; nb < NB; nb++)
-#pragma unroll
-        for (int mb = 0; mb < 4; mb++)
-          acc[nb][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[nb], fb[mb], acc[nb][mb], 0, 0, 0);
-    }
-    __syncthreads();
   }
   const int x = x0 + lr, z = z0 + wave;
   if (x >= n || z >= n) return;
+  /* each lane holds four consecutive rows (channels) of one voxel: rows are
+   * multiples of 16, so they are one 16 B (FP32) or 8 B (bf16) store */
   if (gridDim.z > 1) {
     float *sp = split + (size_t)blockIdx.z * n * n * n * rows;
 #pragma unroll
@@ -254,12 +286,10 @@ __global__ __launch_bounds__(256) void kConvLds(ConvArgs<uint16_t, OutT> a, floa
       if (y >= n) continue;
       size_t v = ((size_t)z * n + y) * n + x;
 #pragma unroll
-      for (int nb = 0; nb < NB; nb++)
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          int r = row0 + nb * 16 + 4 * lg + q;
-          if (r < rows) sp[v * rows + r] = acc[nb][mb][q];
-        }
+      for (int nb = 0; nb < NB; nb++) {
+        int r = row0 + nb * 16 + 4 * lg;
+        if (r < rows) *(float4 *)(sp + v * rows + r) = make_float4(acc[nb][mb][0], acc[nb][mb][1], acc[nb][mb][2], acc[nb][mb][3]);
+      }
     }
     return;
   }
@@ -269,16 +299,29 @@ __global__ __launch_bounds__(256) void kConvLds(ConvArgs<uint16_t, OutT> a, floa
     if (y >= n) continue;
     size_t v = ((size_t)z * n + y) * n + x;
 #pragma unroll
-    for (int nb = 0; nb < NB; nb++)
+    for (int nb = 0; nb < NB; nb++) {
+      int r = row0 + nb * 16 + 4 * lg;
+      if (r >= rows) continue;
+      float val[4];
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        int r = row0 + nb * 16 + 4 * lg + q;
-        if (r >= rows) continue;
-        float val = acc[nb][mb][q] + (a.bias ? a.bias[r] : 0.f);
-        if (a.relu) val = val > 0.f ? val : 0.f;
-        if (r < a.r0) storeOut<OutT>(a.out0 + v * a.r0 + r, val, a.accumulate);
-        else storeOut<OutT>(a.out1 + v * a.r1 + (r - a.r0), val, a.accumulate);
+        val[q] = acc[nb][mb][q] + (a.bias ? a.bias[r + q] : 0.f);
+        if (a.relu) val[q] = val[q] > 0.f ? val[q] : 0.f;
       }
+      if (sizeof(OutT) == 2 && !a.accumulate && r + 3 < a.r0) {
+        uint2 pk;
+        pk.x = (uint32_t)f2bf(val[0]) | ((uint32_t)f2bf(val[1]) << 16);
+        pk.y = (uint32_t)f2bf(val[2]) | ((uint32_t)f2bf(val[3]) << 16);
+        *(uint2 *)(a.out0 + v * a.r0 + r) = pk;
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          int rq = r + q;
+          if (rq < a.r0) storeOut<OutT>(a.out0 + v * a.r0 + rq, val[q], a.accumulate);
+          else storeOut<OutT>(a.out1 + v * a.r1 + (rq - a.r0), val[q], a.accumulate);
+        }
+      }
+    }
   }
 }
 
