@@ -225,7 +225,7 @@ extern "C" int nusifSetOption(Parameter *p, const char *key, const char *value)
   if (k == "mg_speculate") return parseInt(v.c_str(), &o->mgSpeculate);
   if (k == "mg_tile_rows") {
     int r = parseInt(v.c_str(), &o->mgTileRows);
-    if (r == 0 && o->mgTileRows != 0 && o->mgTileRows != 4 && o->mgTileRows != 8) return -2;
+    if (r == 0 && o->mgTileRows != 0 && o->mgTileRows != 4 && o->mgTileRows != 8 && o->mgTileRows != 16) return -2;
     return r;
   }
   if (k == "lid") return parseReal(v.c_str(), &o->lid);

@@ -104,7 +104,7 @@ struct GmgState {
   int nl;
   int nu1, nu2, maxCycles, project, mixed;
   int tail;     /* first level of the one-workgroup tail cycle, -1: none */
-  int tileRows; /* fused-sweep tile height for FP32: 0 automatic, 4 or 8 */
+  int tileRows; /* fused-sweep tile height: 0 automatic, 4, 8 or 16 */
   double tol, omega;
   LevelT<double> L64[MAXLEV];
   LevelT<float> L32[MAXLEV];
@@ -167,17 +167,6 @@ __global__ __launch_bounds__(256) void kSmooth(T *__restrict__ x, const T *__res
   }
 }
 
-/* One full red-black sweep in one launch (ping-pong: reads xo, writes xn).
- * A workgroup of 256 threads owns a 64 x TJ (i, j) tile (each thread TJ/4
- * cells of a column, rows ty, ty + 4, ...) and marches a chunk of kc planes
- * in k. Plane k's odd ("red") cells are updated from the old values of
- * planes k-1..k+1 held in LDS (X), over the tile plus a one-cell ring, into
- * LDS (Y); then plane k-1's even ("black") cells are updated from Y. The ring
- * and the planes k0-1 and k1 are computed redundantly, so no workgroup reads
- * another's new values. Every cell gets exactly the values of the two-pass
- * sweep (same operands, same order); face ghosts are written with the new
- * values. X and Y rotate over 4 plane slots: loads of plane k+2 never touch
- * a slot still being read. */
 /* Workgroup barrier that orders LDS traffic only: waits for this wave's
  * LDS operations, then s_barrier. Unlike __syncthreads() it does not wait
  * for outstanding global loads, so the prefetches below stay in flight
@@ -190,28 +179,56 @@ __device__ inline void ldsBarrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_ba
 #define FT_KC 32
 #define FUSED_MIN_EXTENT 32 /* levels narrower than this use the two-pass sweep */
 
-/* ZX: the sweep starts from x = 0 (the first sweep of a cycle's error
- * equation): old values are taken as zero instead of being read, which gives
- * the same values as reading a zeroed array, without the zeroing pass. */
+/* One full red-black sweep in one launch (ping-pong: reads xo, writes xn).
+ * A workgroup of 32 * TJ threads owns a 64 x TJ (i, j) tile and marches a
+ * chunk of kc planes in k. Plane k's red cells ((i + j + k) odd) are updated
+ * from the old values of planes k-1..k+1 held in LDS (X), over the tile plus a
+ * one-cell ring, into LDS (Y); then plane k-1's black cells are updated from
+ * Y, one per thread. The ring and the planes k0-1 and k1 are computed
+ * redundantly, so no workgroup reads another's new values. Every cell gets
+ * exactly the values of the two-pass sweep (same operands, same order); face
+ * ghosts are written with the new values. X and Y rotate over 4 plane slots:
+ * loads of plane k+2 never touch a slot still being read.
+ *
+ * LDS holds every plane split by the parity of (i + j): half h, row lj,
+ * position p = li / 2 of the local column li (tile origin minus 2). On plane
+ * k the red cells are exactly half (k + 1) & 1 and the black cells half k & 1,
+ * and every neighbour a red or black update reads sits at position p - 1 + s,
+ * p + s (i -/+ 1, other half) or p (j, k neighbours) of its row, s = li & 1
+ * being uniform along a row. Consecutive lanes therefore read consecutive LDS
+ * words with no stride-2 bank conflicts, and no lane computes a value that is
+ * thrown away (apart from two corner positions of the red ring and lanes past
+ * the task count of the last red round).
+ *
+ * X: old values of plane k over local columns 0..67 and rows 0..TJ+3, both
+ * halves. Y: new red values over rows 1..TJ+2, positions 0..33 (the ghost and
+ * out-of-box entries keep their old values). */
+template <typename T, int TJ>
+struct SweepRBShape {
+  static constexpr int NT   = 32 * TJ;        /* threads: one black cell each */
+  static constexpr int XW   = FT_I + 4;       /* local columns 0..67 */
+  static constexpr int XH   = TJ + 4;         /* local rows 0..TJ+3 */
+  static constexpr int RW   = XW / 2;         /* positions per half row */
+  static constexpr int NXE  = XW * XH;        /* X elements per plane */
+  static constexpr int NX   = (NXE + NT - 1) / NT;
+  /* half stride = 16 (mod 32) words: a plane store's even and odd lanes land
+   * in disjoint banks */
+  static constexpr int HALF = XH * RW + ((16 - (XH * RW) % 32) + 32) % 32;
+  static constexpr int NRT  = RW * (TJ + 2);  /* red tasks per plane */
+  static constexpr int NR   = (NRT + NT - 1) / NT;
+  static constexpr int YS   = NRT;
+};
+
 template <typename T, int TJ, int MINW, bool ZX>
-__global__ __launch_bounds__(256, MINW) void kSweepFused(const T *__restrict__ xo, T *__restrict__ xn,
+__global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ xo, T *__restrict__ xn,
     const T *__restrict__ b, Geo g, Coef<T> cf, int kc, const int *__restrict__ stop)
 {
   if (stop && *stop) return;
-  constexpr int NT  = 256;
-  constexpr int NBK = 64 * TJ / NT; /* black cells per thread (rows ty, ty + 4, ...) */
-  constexpr int RS  = NT / 64;      /* thread rows */
-  constexpr int XW_ = FT_I + 4, XH_ = TJ + 4, YW_ = FT_I + 2, YH_ = TJ + 2;
-  constexpr int NX = (XH_ * XW_ + (NT - 1)) / NT; /* X elements per thread */
-  constexpr int NY = (YH_ * YW_ + (NT - 1)) / NT; /* Y elements per thread */
-  /* slots padded to whole thread multiples so every LDS access is unconditional */
-  __shared__ T X[4][NX * NT];
-  __shared__ T Y[4][NY * NT];
+  using S = SweepRBShape<T, TJ>;
+  constexpr int NT = S::NT, XW = S::XW, RW = S::RW, HALF = S::HALF, NX = S::NX, NR = S::NR;
+  __shared__ T X[4][2 * HALF];
+  __shared__ T Y[4][S::YS];
   const int tid = threadIdx.x;
-  /* XCD-aware tile order: workgroup ids are dealt round-robin over the 8
-   * XCDs, so id w runs on XCD w % 8. Giving each XCD a contiguous run of
-   * tiles (x fastest, then y, then the k chunk) keeps the halos that
-   * neighbouring tiles share inside one L2. */
   const int gxT = (g.ni + FT_I - 1) / FT_I, gyT = (g.nj + TJ - 1) / TJ;
   const int nTiles = gxT * gyT * ((g.nk + kc - 1) / kc);
   const int perXcd = (nTiles + NUM_XCD - 1) / NUM_XCD;
@@ -220,49 +237,40 @@ __global__ __launch_bounds__(256, MINW) void kSweepFused(const T *__restrict__ x
   const int i0 = (t % gxT) * FT_I + 1, j0 = ((t / gxT) % gyT) * TJ + 1;
   const int k0 = (t / (gxT * gyT)) * kc + 1;
   const int k1 = min(k0 + kc, g.nk + 1); /* black planes [k0, k1) */
-  const int nk = g.nk, sj = g.sj;
+  const int nk = g.nk, sj = g.sj, ni = g.ni, nj = g.nj;
   const long long sk = g.sk;
+  const int ib = i0 - 2, jb = j0 - 2; /* global (i, j) of local (0, 0) */
 
-  /* Plane-independent per-thread addressing, computed once. Positions
-   * outside the padded box load from a clamped (valid) address: those X
-   * entries are only neighbours of ghost or out-of-box Y entries, which are
-   * never updated, so their values are never used. */
-  int xOff[NX];  /* j*sj + i of this thread's X elements (clamped) */
-  int yXe[NY];   /* X index under this thread's Y elements */
-  int yOff[NY];  /* j*sj + i of the Y element (clamped) */
-  int yPar[NY];  /* (i + j) & 1, or 2 when the element is not an interior cell */
+  /* X loads: element e = (li, lj) row-major, clamped into the padded box
+   * (clamped entries are only read by updates whose results are not kept) */
+  int xOff[NX], xLds[NX];
 #pragma unroll
   for (int q = 0; q < NX; q++) {
-    int e = min(tid + NT * q, XH_ * XW_ - 1);
-    int i = min(max(i0 - 2 + e % XW_, 0), g.ni + 1);
-    int j = min(max(j0 - 2 + e / XW_, 0), g.nj + 1);
-    xOff[q] = j * sj + i;
+    int e  = min(tid + NT * q, S::NXE - 1);
+    int li = e % XW, lj = e / XW;
+    xOff[q] = min(max(jb + lj, 0), nj + 1) * sj + min(max(ib + li, 0), ni + 1);
+    xLds[q] = ((li + lj) & 1) * HALF + lj * RW + (li >> 1);
   }
+  /* red tasks: position p, row lj in 1..TJ+2; the column is 2p + s with
+   * s = (k + 1 + lj) & 1 on plane k. rIn bit s: that cell is interior in (i, j). */
+  int rP[NR], rLj[NR], rOff[NR], rIn[NR];
 #pragma unroll
-  for (int q = 0; q < NY; q++) {
-    int e  = tid + NT * q;
-    int ec = min(e, YH_ * YW_ - 1);
-    int yi = ec % YW_, yj = ec / YW_;
-    int i = i0 - 1 + yi, j = j0 - 1 + yj;
-    bool in = e < YH_ * YW_ && i >= 1 && i <= g.ni && j >= 1 && j <= g.nj;
-    yXe[q]  = (yj + 1) * XW_ + (yi + 1);
-    yOff[q] = min(j, g.nj + 1) * sj + min(i, g.ni + 1);
-    yPar[q] = in ? ((i + j) & 1) : 2;
+  for (int r = 0; r < NR; r++) {
+    int task = min(tid + NT * r, S::NRT - 1);
+    int p = task % RW, lj = 1 + task / RW;
+    int j = jb + lj, i = ib + 2 * p;
+    rP[r] = p; rLj[r] = lj;
+    rOff[r] = min(max(j, 0), nj + 1) * sj + min(max(i, 0), ni);
+    bool jin = j >= 1 && j <= nj;
+    rIn[r] = (jin && i >= 1 && i <= ni ? 1 : 0) | (jin && i + 1 >= 1 && i + 1 <= ni ? 2 : 0);
   }
-  const int tx = tid & (FT_I - 1), ty = tid / FT_I;
-  int bOff[NBK], bPar[NBK], bYe[NBK], bXe[NBK];
-  bool bIn[NBK], fIlo[NBK], fIhi[NBK], fJlo[NBK], fJhi[NBK];
-#pragma unroll
-  for (int q = 0; q < NBK; q++) {
-    const int tyq = ty + q * RS;
-    const int bi = i0 + tx, bj = j0 + tyq;
-    bIn[q]  = bi <= g.ni && bj <= g.nj;
-    bOff[q] = min(bj, g.nj + 1) * sj + min(bi, g.ni + 1);
-    bPar[q] = (bi + bj) & 1;
-    bYe[q]  = (tyq + 1) * YW_ + (tx + 1);
-    bXe[q]  = (tyq + 2) * XW_ + (tx + 2);
-    fIlo[q] = bi == 1; fIhi[q] = bi == g.ni; fJlo[q] = bj == 1; fJhi[q] = bj == g.nj;
-  }
+  /* black task: position p in 1..32, row lj in 2..TJ+1; black column 2p + s,
+   * s = (k + lj) & 1, its red partner 2p + 1 - s. */
+  const int bp = 1 + (tid & 31), blj = 2 + (tid >> 5);
+  const int bj = jb + blj, bi = ib + 2 * bp; /* global i of column 2p */
+  const int bRow = min(bj, nj + 1) * sj + min(bi, ni);
+  const bool bjIn = bj <= nj;
+  const bool bIn0 = bjIn && bi <= ni, bIn1 = bjIn && bi + 1 <= ni;
 
   T xr[NX];
   auto fetchX = [&](int k) {
@@ -271,98 +279,119 @@ __global__ __launch_bounds__(256, MINW) void kSweepFused(const T *__restrict__ x
     for (int q = 0; q < NX; q++) xr[q] = ZX ? (T)0 : xo[base + xOff[q]];
   };
   auto storeX = [&](int k) {
+    if (ZX) return;
 #pragma unroll
-    for (int q = 0; q < NX; q++) X[k & 3][tid + NT * q] = xr[q];
+    for (int q = 0; q < NX; q++)
+      if (q < NX - 1 || tid + NT * q < S::NXE) X[k & 3][xLds[q]] = xr[q];
   };
-  auto fetchB = [&](int k, T *out) { /* b under this thread's Y elements, plane k */
+  auto fetchBR = [&](int k, T *out) { /* b under this thread's red cells of plane k */
     long long base = (long long)min(max(k, 1), nk) * sk;
+    const int s0 = (k + 1) & 1;
 #pragma unroll
-    for (int q = 0; q < NY; q++) out[q] = b[base + yOff[q]];
+    for (int r = 0; r < NR; r++) out[r] = b[base + rOff[r] + ((s0 + rLj[r]) & 1)];
   };
   auto redPlane = [&](int k, const T *bred) {
-    const T *Xm = X[(k + 3) & 3], *Xc = X[k & 3], *Xp = X[(k + 1) & 3];
+    const int hr = (k + 1) & 1;
+    const T *Xm = X[(k + 3) & 3] + hr * HALF, *Xc = X[k & 3] + hr * HALF, *Xp = X[(k + 1) & 3] + hr * HALF;
+    const T *Xo = X[k & 3] + (1 - hr) * HALF; /* the other half of plane k */
     T *Yc = Y[k & 3];
-    bool kin = k >= 1 && k <= nk;
+    const bool kin = k >= 1 && k <= nk;
 #pragma unroll
-    for (int q = 0; q < NY; q++) {
-      int xe = yXe[q];
-      T v    = Xc[xe];
-      T lap  = (Xc[xe + 1] - (T)2 * v + Xc[xe - 1]) * cf.idx2 +
-              (Xc[xe + XW_] - (T)2 * v + Xc[xe - XW_]) * cf.idy2 +
-              (Xp[xe] - (T)2 * v + Xm[xe]) * cf.idz2;
-      T r    = bred[q] - lap;
-      T vNew = v - cf.factor * r;
-      bool red = kin && ((yPar[q] + k) & 1) == 1 && yPar[q] < 2;
-      Yc[tid + NT * q] = red ? vNew : v;
-    }
-  };
-  auto blackPlane = [&](int k, const T *bc) {
-    const T *Yc = Y[k & 3], *Ym = Y[(k + 3) & 3], *Yp = Y[(k + 1) & 3];
-#pragma unroll
-    for (int q = 0; q < NBK; q++) {
-      T xc  = X[k & 3][bXe[q]];
-      T lap = (Yc[bYe[q] + 1] - (T)2 * xc + Yc[bYe[q] - 1]) * cf.idx2 +
-              (Yc[bYe[q] + YW_] - (T)2 * xc + Yc[bYe[q] - YW_]) * cf.idy2 +
-              (Yp[bYe[q]] - (T)2 * xc + Ym[bYe[q]]) * cf.idz2;
-      T r    = bc[q] - lap;
-      T vNew = xc - cf.factor * r;
-      T v    = (((bPar[q] + k) & 1) == 0) ? vNew : Yc[bYe[q]];
-      if (bIn[q]) {
-        long long c = k * sk + bOff[q];
-        xn[c]       = v;
-        if (fIlo[q]) xn[c - 1] = v;
-        if (fIhi[q]) xn[c + 1] = v;
-        if (fJlo[q]) xn[c - sj] = v;
-        if (fJhi[q]) xn[c + sj] = v;
-        if (k == 1) xn[c - sk] = v;
-        if (k == nk) xn[c + sk] = v;
+    for (int r = 0; r < NR; r++) {
+      if (r < NR - 1 || tid + NT * r < S::NRT) {
+        const int s = (hr + rLj[r]) & 1;
+        const int a = rLj[r] * RW + rP[r];
+        T v, lap;
+        if (ZX) {
+          v   = (T)0;
+          lap = (((T)0 - (T)2 * v + (T)0) * cf.idx2 + ((T)0 - (T)2 * v + (T)0) * cf.idy2) +
+                ((T)0 - (T)2 * v + (T)0) * cf.idz2;
+        } else {
+          v   = Xc[a];
+          lap = (Xo[a + s] - (T)2 * v + Xo[a - 1 + s]) * cf.idx2 +
+                (Xo[a + RW] - (T)2 * v + Xo[a - RW]) * cf.idy2 +
+                (Xp[a] - (T)2 * v + Xm[a]) * cf.idz2;
+        }
+        T rr   = bred[r] - lap;
+        T vNew = v - cf.factor * rr;
+        bool in = kin && ((rIn[r] >> s) & 1);
+        Yc[a - RW] = in ? vNew : v;
       }
     }
   };
+  auto blackPlane = [&](int k, T bc) {
+    const int hb = k & 1, s = (hb + blj) & 1;
+    const T *Yc = Y[k & 3], *Ym = Y[(k + 3) & 3], *Yp = Y[(k + 1) & 3];
+    const int a = (blj - 1) * RW + bp; /* Y index of this row and position */
+    T xc  = ZX ? (T)0 : X[k & 3][hb * HALF + blj * RW + bp];
+    T lap = (Yc[a + s] - (T)2 * xc + Yc[a - 1 + s]) * cf.idx2 +
+            (Yc[a + RW] - (T)2 * xc + Yc[a - RW]) * cf.idy2 +
+            (Yp[a] - (T)2 * xc + Ym[a]) * cf.idz2;
+    T rr   = bc - lap;
+    T vB   = xc - cf.factor * rr;
+    T vR   = Yc[a];
+    /* column 2p holds the black value when s = 0, the red one when s = 1 */
+    T v0 = s ? vR : vB, v1 = s ? vB : vR;
+    long long c = k * sk + bRow;
+    if (bIn0) {
+      xn[c] = v0;
+      if (bi == 1) xn[c - 1] = v0;
+      if (bi == ni) xn[c + 1] = v0;
+      if (bj == 1) xn[c - sj] = v0;
+      if (bj == nj) xn[c + sj] = v0;
+      if (k == 1) xn[c - sk] = v0;
+      if (k == nk) xn[c + sk] = v0;
+    }
+    if (bIn1) {
+      long long c1 = c + 1;
+      xn[c1] = v1;
+      if (bi + 1 == ni) xn[c1 + 1] = v1;
+      if (bj == 1) xn[c1 - sj] = v1;
+      if (bj == nj) xn[c1 + sj] = v1;
+      if (k == 1) xn[c1 - sk] = v1;
+      if (k == nk) xn[c1 + sk] = v1;
+    }
+  };
+  auto fetchBB = [&](int k) {
+    int kk = min(max(k, 1), nk);
+    return b[(long long)kk * sk + bRow + ((kk + blj) & 1)];
+  };
 
-  /* One iteration of the plane loop with register set (xs, bs, bbs): store
-   * plane kk+1 to LDS, issue the loads of plane kk+3 (x), kk+2 (red b) and
-   * kk+1 (own b) two iterations ahead, then red(kk) and black(kk-1). */
-  auto iterate = [&](int kk, T(&xs)[NX], T(&bs)[NY], T(&bbs)[NBK]) {
+  /* one plane step with register set (xs, bs, bbs): store plane kk+1 to LDS,
+   * issue the loads of plane kk+3 (x), kk+2 (red b) and kk+1 (black b), then
+   * red(kk) and black(kk-1) */
+  auto iterate = [&](int kk, T(&xs)[NX], T(&bs)[NR], T &bbs) {
 #pragma unroll
     for (int q = 0; q < NX; q++) xr[q] = xs[q];
     storeX(kk + 1);
-    T bUse[NY];
+    T bUse[NR];
 #pragma unroll
-    for (int q = 0; q < NY; q++) bUse[q] = bs[q];
-    T bbUse[NBK];
-#pragma unroll
-    for (int q = 0; q < NBK; q++) bbUse[q] = bbs[q];
+    for (int r = 0; r < NR; r++) bUse[r] = bs[r];
+    T bbUse = bbs;
     fetchX(kk + 3);
 #pragma unroll
     for (int q = 0; q < NX; q++) xs[q] = xr[q];
-    fetchB(kk + 2, bs);
-#pragma unroll
-    for (int q = 0; q < NBK; q++) bbs[q] = b[(long long)min(max(kk + 1, 1), nk) * sk + bOff[q]];
+    fetchBR(kk + 2, bs);
+    bbs = fetchBB(kk + 1);
     ldsBarrier();
-    redPlane(kk, bUse); /* ghost planes (kk = 0, nk+1) copy old values */
+    redPlane(kk, bUse);
     ldsBarrier();
     if (kk - 1 >= k0) blackPlane(kk - 1, bbUse);
   };
 
-  /* prologue: planes k0-2, k0-1 straight to LDS; two register sets ahead */
-  fetchX(k0 - 2); /* clamped to plane 0 when k0 = 1 (then never read) */
+  fetchX(k0 - 2);
   storeX(k0 - 2 + 4);
   fetchX(k0 - 1);
   storeX(k0 - 1);
-  T xA[NX], xB[NX], bA[NY], bB[NY], bbA[NBK], bbB[NBK];
-#pragma unroll
-  for (int q = 0; q < NBK; q++) bbA[q] = bbB[q] = (T)0;
+  T xA[NX], xB[NX], bA[NR], bB[NR], bbA = (T)0, bbB = (T)0;
   fetchX(k0);
 #pragma unroll
   for (int q = 0; q < NX; q++) xA[q] = xr[q];
   fetchX(k0 + 1);
 #pragma unroll
   for (int q = 0; q < NX; q++) xB[q] = xr[q];
-  fetchB(k0 - 1, bA);
-  fetchB(k0, bB);
-  /* the own-b slots of both sets stand for planes k0-2 and k0-1 at their
-   * first use, where no black plane is updated */
+  fetchBR(k0 - 1, bA);
+  fetchBR(k0, bB);
   iterate(k0 - 1, xA, bA, bbA);
   for (int kk = k0; kk <= k1; kk += 2) {
     iterate(kk, xB, bB, bbB);
@@ -370,13 +399,15 @@ __global__ __launch_bounds__(256, MINW) void kSweepFused(const T *__restrict__ x
   }
 }
 
-/* Launch shape of the fused sweep on a level. Taller tiles (64 x 8, two
- * cells per thread) measured 10 % faster for FP32 at 256^3 (58 vs 64 us)
- * and slower for FP64 and on coarser levels, so they are used for FP32
- * levels whose grid still has >= 1024 such workgroups at the full k-chunk.
- * The k-chunk is the longest (<= FT_KC planes, a power of two >= 4) that
- * still gives >= 2048 workgroups (8 per CU), so small levels trade some halo
- * re-loading for enough waves to hide latency. */
+/* Launch shape of the fused sweep on a level, from a sweep of tile heights
+ * 4/8/16 and k-chunks 4..32 on 32^3..256^3 cubes (FP32 and FP64). Levels with
+ * nj >= 128 use 64 x 16 tiles (512 threads): fewer ring cells per interior
+ * cell, 44 us for FP32 at 256^3 against 49 us for 64 x 4. Their k-chunk is the
+ * longest (<= FT_KC planes, a power of two >= 4) that still gives >= 1024
+ * workgroups for FP32, >= 256 for FP64 (FP32 256^3: 16 planes; FP64 256^3:
+ * 32). Narrower levels use 64 x 4 tiles with the k-chunk that gives >= 2048
+ * workgroups (4 planes on 64^3 and 32^3). `rows` (4, 8 or 16) forces the tile
+ * height; 0 picks it. */
 struct SweepShape {
   int tj, kc;
 };
@@ -386,10 +417,11 @@ static long long sweepTiles(const Geo &g, int tj, int kc)
 }
 template <typename T> static SweepShape sweepShape(const Geo &g, int rows)
 {
-  if (sizeof(T) == 4 && (rows == 8 || (rows == 0 && sweepTiles(g, 8, FT_KC) >= 1024))) return {8, FT_KC};
+  const int tj = rows ? rows : (g.nj >= 128 ? 16 : 4);
+  const long long want = tj == 4 ? 2048 : (sizeof(T) == 4 ? 1024 : 256);
   int kc = FT_KC;
-  while (kc > 4 && sweepTiles(g, 4, kc) < 2048) kc /= 2;
-  return {4, kc};
+  while (kc > 4 && sweepTiles(g, tj, kc) < want) kc /= 2;
+  return {tj, kc};
 }
 
 /* 1-D grid: NUM_XCD * ceil(tiles / NUM_XCD) workgroups (surplus ids exit) */
@@ -405,23 +437,26 @@ static void launchSweepShape(const LevelT<T> &L, dim3 grd, int kc, bool timed, b
   /* the zero-start sweep moves 2/3 of a sweep's bytes (no x read): timed apart */
   const char *name = sizeof(T) == 8 ? (zx ? "gmg_smooth64z_l0" : "gmg_smooth64_l0")
                                     : (zx ? "gmg_smooth32z_l0" : "gmg_smooth32_l0");
+  constexpr int NT = SweepRBShape<T, TJ>::NT;
   if (zx) {
     if (timed)
-      NUSIF_TIMED(name, (kSweepFused<T, TJ, MINW, true><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc, gCycleStop)));
+      NUSIF_TIMED(name, (kSweepRB<T, TJ, MINW, true><<<grd, NT, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc, gCycleStop)));
     else
-      kSweepFused<T, TJ, MINW, true><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc, gCycleStop);
+      kSweepRB<T, TJ, MINW, true><<<grd, NT, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc, gCycleStop);
   } else {
     if (timed)
-      NUSIF_TIMED(name, (kSweepFused<T, TJ, MINW, false><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc, gCycleStop)));
+      NUSIF_TIMED(name, (kSweepRB<T, TJ, MINW, false><<<grd, NT, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc, gCycleStop)));
     else
-      kSweepFused<T, TJ, MINW, false><<<grd, 256, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc, gCycleStop);
+      kSweepRB<T, TJ, MINW, false><<<grd, NT, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc, gCycleStop);
   }
 }
 
 template <typename T> static void launchSweep(const LevelT<T> &L, SweepShape s, bool timed, bool zx)
 {
   dim3 grd = fusedGrid(L.g, s);
-  if (s.tj == 8)
+  if (s.tj == 16)
+    launchSweepShape<T, 16, 4>(L, grd, s.kc, timed, zx);
+  else if (s.tj == 8)
     launchSweepShape<T, 8, 6>(L, grd, s.kc, timed, zx);
   else
     launchSweepShape<T, 4, 8>(L, grd, s.kc, timed, zx);

@@ -103,7 +103,7 @@ struct NusifOptions {
   int mgProject;
   int mgTail;      /* gmg: coarse levels in one workgroup (1) or per-level kernels (0) */
   int mgGraph;     /* gmg: replay each cycle from a captured HIP graph (1) or launch eagerly (0) */
-  int mgTileRows;  /* gmg: FP32 fused-sweep tile height, 0 automatic, 4 or 8 */
+  int mgTileRows;  /* gmg: fused-sweep tile height, 0 automatic, 4, 8 or 16 */
   int mgSpeculate; /* gmg: enqueue cycles ahead of the convergence test (1) or one at a time (0) */
   double lid;
   double uInflow;

@@ -49,8 +49,8 @@ void nusifFreeSolver(Solver *s);
  *                         (1, default) or with one kernel per pass (0); same values
  *   mg_graph      0 | 1   gmg: replay each cycle from a captured HIP graph (1) or
  *                         launch its kernels one by one (0, default); same values
- *   mg_tile_rows  0|4|8   gmg: FP32 fused-sweep tile height (0: automatic, 8 on
- *                         the largest levels); same values
+ *   mg_tile_rows  0|4|8|16 gmg: fused-sweep tile height (0: automatic, 16 on
+ *                         levels with >= 128 rows); same values
  *   mg_speculate  0 | 1   gmg: enqueue cycles before the previous one's
  *                         convergence test is read back (1; the device skips
  *                         the ones past convergence) or one at a time (0,

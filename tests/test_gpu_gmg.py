@@ -116,8 +116,9 @@ def test_gpu_gmg_tail_cycle_equals_per_level_kernels(shape, levels, precision, g
         c.lib.solve(C.byref(c.s), c.d.p, c.d.rhs)
         out.append((c.fields()["p"], c.lib.nusifSolverIterations(C.byref(c.s))))
         c.close()
-    assert out[0][1] == out[1][1]
-    np.testing.assert_array_equal(out[0][0], out[1][0])
+    for o in out[1:]:
+        assert o[1] == out[0][1]
+        np.testing.assert_array_equal(o[0], out[0][0])
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
@@ -152,15 +153,19 @@ def test_gpu_gmg_graph_replay_equals_eager_launches(precision, gpu_ok):
         c.close()
 
 
-def test_gpu_gmg_tall_tile_sweep_equals_default(gpu_ok):
-    """The FP32 fused sweep with 64 x 8 tiles (two cells per thread, used on
-    the largest levels) against 64 x 4 tiles: same p bit for bit."""
-    print("gmg tile rows", flush=True)
-    prm = dict(G.KAT_PARAMS, imax=128, jmax=40, kmax=72, xlength=4.0, ylength=1.25,
-               zlength=2.25, levels=3, presmooth=2, postsmooth=2)
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gpu_gmg_tall_tile_sweep_equals_default(precision, gpu_ok):
+    """The fused sweep with 64 x 8 and 64 x 16 tiles (used on levels with
+    >= 128 rows) against 64 x 4 tiles: same p bit for bit. At 128 x 144 x 240
+    the three heights run with different k-chunks on the finest level (FP64:
+    8, 32 and 16 planes, the last of the 32-plane chunks partial; FP32: 8, 8
+    and 4), and the next levels' 72 and 36 rows are no multiple of 16."""
+    print("gmg tile rows", precision, flush=True)
+    prm = dict(G.KAT_PARAMS, imax=128, jmax=144, kmax=240, xlength=2.0, ylength=2.25,
+               zlength=3.75, levels=5, presmooth=2, postsmooth=2)
     out = []
-    for rows in (4, 8):
-        opts = {"solver": "gmg", "precision": "mixed", "mg_tol": 1e-9, "mg_project": 1,
+    for rows in (4, 8, 16):
+        opts = {"solver": "gmg", "precision": precision, "mg_tol": 1e-9, "mg_project": 1,
                 "mg_tile_rows": rows}
         c = A.Case(lib("gpu_rb"), options=opts, **prm)
         f = G.random_fields(c.shape, 9)
@@ -168,8 +173,9 @@ def test_gpu_gmg_tall_tile_sweep_equals_default(gpu_ok):
         c.lib.solve(C.byref(c.s), c.d.p, c.d.rhs)
         out.append((c.fields()["p"], c.lib.nusifSolverIterations(C.byref(c.s))))
         c.close()
-    assert out[0][1] == out[1][1]
-    np.testing.assert_array_equal(out[0][0], out[1][0])
+    for o in out[1:]:
+        assert o[1] == out[0][1]
+        np.testing.assert_array_equal(o[0], out[0][0])
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
