@@ -107,6 +107,7 @@ void nusifDefaultOptions(NusifOptions *o)
   o->mgGraph     = 0;
   o->mgTileRows  = 0;
   o->mgSpeculate = 0;
+  o->mgInner     = 1;
   o->unetExtent  = 64;
   o->lid         = 1.0;
   o->uInflow     = 2.0;
@@ -223,6 +224,11 @@ extern "C" int nusifSetOption(Parameter *p, const char *key, const char *value)
   if (k == "mg_graph") return parseInt(v.c_str(), &o->mgGraph);
   if (k == "unet_extent") return parseInt(v.c_str(), &o->unetExtent);
   if (k == "mg_speculate") return parseInt(v.c_str(), &o->mgSpeculate);
+  if (k == "mg_inner") {
+    int r = parseInt(v.c_str(), &o->mgInner);
+    if (r == 0 && o->mgInner < 0) return -2;
+    return r;
+  }
   if (k == "mg_tile_rows") {
     int r = parseInt(v.c_str(), &o->mgTileRows);
     if (r == 0 && o->mgTileRows != 0 && o->mgTileRows != 4 && o->mgTileRows != 8 && o->mgTileRows != 16) return -2;

@@ -27,6 +27,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -93,6 +94,9 @@ struct GmgState {
   int useGraphs;
   int speculate;  /* enqueue cycles ahead of the convergence test */
   int predicted;  /* cycles of the previous solve: the first batch */
+  int inner;      /* cycles before a solve's first stop test; 0: nextGroup */
+  int nextGroup;  /* mg_inner 0: the next solve's first group (starts at 1) */
+  int lastGroup;  /* the last solve's first group */
   CycleCtl *dCtl; /* device cycle control */
   CycleCtl *hRing;/* pinned copies of dCtl after each enqueued cycle */
   CycleCtl *hRingDev; /* their device address (written by kFoldCycle) */
@@ -1105,7 +1109,9 @@ template <typename T> static void tailCycle(GmgState *G, LevelT<T> *L)
   kTailCycle<T><<<1, TAIL_THREADS, lds, nusifStream()>>>(a);
 }
 
-template <typename T> static void vcycle(GmgState *G, LevelT<T> *L, int l)
+/* `cont` (mixed mode, finest level): the cycle continues the correction e
+ * of the previous cycle on the same defect instead of starting from 0 */
+template <typename T> static void vcycle(GmgState *G, LevelT<T> *L, int l, bool cont = false)
 {
   if (l == G->tail) {
     tailCycle(G, L);
@@ -1117,8 +1123,9 @@ template <typename T> static void vcycle(GmgState *G, LevelT<T> *L, int l)
   }
   bool fine = l == 0;
   /* the coarse levels start from x = 0 (zeroed by the restriction); the
-   * finest one in mixed mode too (the error equation), FP64 mode smooths p */
-  smooth(G, L[l], G->nu1, fine, l > 0 || G->mixed);
+   * finest one in mixed mode too (the error equation) unless the cycle
+   * continues e, FP64 mode smooths p */
+  smooth(G, L[l], G->nu1, fine, l > 0 || (G->mixed && !cont));
   LevelT<T> &C = L[l + 1];
   if (fine)
     NUSIF_TIMED(sizeof(T) == 8 ? "gmg_restrict64_l0" : "gmg_restrict32_l0",
@@ -1161,6 +1168,9 @@ GmgState *gmgCreate(NusifSolverCtx *sc, const Parameter *prm)
   }
   G->useGraphs = c->opt.mgGraph;
   G->speculate = c->opt.mgSpeculate;
+  G->inner     = c->opt.mgInner;
+  G->nextGroup = 1;
+  G->lastGroup = 0;
   G->predicted = 1;
   HIP_CHECK(hipMalloc((void **)&G->dCtl, sizeof(CycleCtl)));
   HIP_CHECK(hipHostMalloc((void **)&G->hRing, CTL_RING * sizeof(CycleCtl), hipHostMallocCoherent));
@@ -1323,13 +1333,17 @@ static void setCycleState(NusifCtx *c, GmgState *G, const std::vector<void *> &v
 
 extern bool nusifTimersActive();
 
-template <typename Body> static void runCycle(NusifCtx *c, GmgState *G, Body body)
+/* enqueues body() (one cycle group, `key` = its cycle count), from a
+ * captured graph when graphs are on: the graphs are keyed by the buffer
+ * arrangement before the group and its cycle count */
+template <typename Body> static void runCycle(NusifCtx *c, GmgState *G, int key, Body body)
 {
   if (!G->useGraphs || nusifTimersActive()) {
     body();
     return;
   }
   std::vector<void *> before = cycleState(c, G);
+  before.push_back((void *)(intptr_t)key);
   for (CycleGraph &e : G->graphs)
     if (e.before == before) {
       HIP_CHECK(hipGraphLaunch(e.exec, nusifStream()));
@@ -1346,7 +1360,7 @@ template <typename Body> static void runCycle(NusifCtx *c, GmgState *G, Body bod
   HIP_CHECK(hipStreamEndCapture(nusifStream(), &graph));
   CycleGraph e;
   e.before = before;
-  e.after  = cycleState(c, G);
+  e.after  = cycleState(c, G); /* (no key: setCycleState reads the buffers only) */
   HIP_CHECK(hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0));
   HIP_CHECK(hipGraphDestroy(graph));
   HIP_CHECK(hipGraphLaunch(e.exec, nusifStream()));
@@ -1387,8 +1401,17 @@ static double readNorms(NusifCtx *c, double *nb)
 
 /* Runs V-cycles until ||rhs - A p|| / ||rhs|| <= tol or maxCycles, from the
  * residual norms (rel, nb) of the starting p; updates them and returns the
- * cycle count. `body` enqueues one cycle that ends with the residual
+ * cycle count. `body(m)` enqueues m cycles that end with the residual
  * partials of the new iterate (normGrid layout in c->partials).
+ *
+ * The first stop test of a solve comes after a group of G->inner cycles,
+ * the later ones after every cycle. Skipped tests save the host round trip
+ * and, in mixed mode, the FP64 correction + residual pass between the
+ * cycles. G->inner = 0 sizes the group from the previous solve
+ * (nextGroupAfter): a group that did not converge gives the exact count the
+ * solve needed; one that converged shrinks by one when its own contraction
+ * says one cycle fewer would have reached the tolerance. A solve that needs
+ * fewer cycles than its group still runs the whole group.
  *
  * Speculative mode: the host enqueues cycles before it knows whether the
  * previous one converged (as many as the previous solve needed, then one
@@ -1398,6 +1421,23 @@ static double readNorms(NusifCtx *c, double *nb)
  * kernels that run, and so the result, are those of the one-at-a-time loop;
  * the buffer pointers are set back to their arrangement after the cycles
  * that ran. */
+/* the next solve's first group under mg_inner 0, from this solve's: group
+ * `first` took the relative residual from rel0 to relFirst; the solve ended
+ * after `cycles` */
+static int nextGroupAfter(const GmgState *G, int first, int cycles, double rel0, double relFirst)
+{
+  int next = first;
+  if (cycles > first) {
+    next = cycles;
+  } else if (first > 1 && rel0 > 0.0 && relFirst > 0.0) {
+    /* mean contraction per cycle over the group; the residual one cycle
+     * earlier was about relFirst / rho */
+    double rho = pow(relFirst / rel0, 1.0 / first);
+    if (rho < 1.0 && relFirst / rho <= G->tol) next = first - 1;
+  }
+  return std::max(1, std::min(next, G->maxCycles));
+}
+
 template <typename Body> static int runCycles(NusifCtx *c, GmgState *G, double &rel, double &nb, Body body)
 {
   dim3 ng        = normGrid(c->geo);
@@ -1405,15 +1445,24 @@ template <typename Body> static int runCycles(NusifCtx *c, GmgState *G, double &
   hipStream_t st = nusifStream();
   int cycles     = 0;
   if (!G->speculate || nusifTimersActive()) {
+    const int first   = G->inner > 0 ? G->inner : G->nextGroup;
+    const double rel0 = rel;
+    double relFirst   = 0.0;
+    G->lastGroup      = 0;
     while (nb > 0.0 && rel > G->tol && cycles < G->maxCycles) {
-      runCycle(c, G, [&] {
-        body();
+      int m = cycles == 0 ? first : 1;
+      if (m > G->maxCycles - cycles) m = G->maxCycles - cycles;
+      if (cycles == 0) G->lastGroup = m;
+      runCycle(c, G, m, [&] {
+        body(m);
         kFoldNorms<<<1, 256, 0, st>>>(c->partials, nbk, c->scal->norm2, c->scalHostDev->norm2);
       });
-      cycles++;
+      cycles += m;
       double rn = readNorms(c, &nb);
       rel       = rn / nb;
+      if (cycles == m) relFirst = rel;
     }
+    if (G->inner == 0 && cycles > 0) G->nextGroup = nextGroupAfter(G, G->lastGroup, cycles, rel0, relFirst);
     return cycles;
   }
   if (!(nb > 0.0 && rel > G->tol && G->maxCycles > 0)) return 0;
@@ -1428,8 +1477,8 @@ template <typename Body> static int runCycles(NusifCtx *c, GmgState *G, double &
     int target = pred > chk + 1 ? pred : chk + 1;
     while (enq < G->maxCycles && enq < target && enq < chk + CTL_RING) {
       states.push_back(cycleState(c, G));
-      runCycle(c, G, [&] {
-        body();
+      runCycle(c, G, 1, [&] {
+        body(1);
         kFoldCycle<<<1, 256, 0, st>>>(c->partials, nbk, c->scal->norm2, G->dCtl, G->hRingDev);
       });
       HIP_CHECK(hipEventRecord(G->ev[enq % CTL_RING], st));
@@ -1452,6 +1501,7 @@ template <typename Body> static int runCycles(NusifCtx *c, GmgState *G, double &
   nb           = sqrt(last.b2);
   rel          = sqrt(last.r2) / nb;
   G->predicted = last.executed;
+  G->lastGroup = 1;
   return last.executed;
 }
 
@@ -1547,8 +1597,8 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
     double rn = residualNorm(c, G, L0.x, rhs, nullptr, &nb);
     rel       = nb > 0.0 ? rn / nb : 0.0;
     dim3 ng   = normGrid(g);
-    cycles    = runCycles(c, G, rel, nb, [&] {
-      vcycle<double>(G, G->L64, 0);
+    cycles    = runCycles(c, G, rel, nb, [&](int m) {
+      for (int q = 0; q < m; q++) vcycle<double>(G, G->L64, 0);
       NUSIF_TIMED("gmg_residual64", (kResidualLds<false><<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(
                                         L0.x, nullptr, rhs, g, G->cf0, nullptr, nullptr, nullptr, c->partials,
                                         ng.x, ng.y, ng.z, gCycleStop)));
@@ -1565,8 +1615,8 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
     double rn = haveInitial ? rnInit : residualNorm(c, G, p, rhs, L0.b, &nb, l0NeedsZero(G) ? L0.x : nullptr);
     rel       = nb > 0.0 ? rn / nb : 0.0;
     dim3 ng   = normGrid(g);
-    cycles    = runCycles(c, G, rel, nb, [&] {
-      vcycle<float>(G, G->L32, 0);
+    cycles    = runCycles(c, G, rel, nb, [&](int m) {
+      for (int q = 0; q < m; q++) vcycle<float>(G, G->L32, 0, q > 0);
       NUSIF_TIMED("gmg_correct_residual",
           (kResidualLds<true><<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(c->p, L0.x, rhs, g, G->cf0,
               c->pAlt, L0.b, l0NeedsZero(G) ? L0.xAlt : nullptr, c->partials, ng.x, ng.y, ng.z, gCycleStop)));
@@ -1596,6 +1646,12 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
  * reps is rounded up to an even number, which leaves x / xAlt as found.
  * Returns microseconds per launch, or -1 when the solver is not gmg or the
  * finest level does not use the fused sweep. */
+extern "C" int nusifSolverFirstGroup(Solver *s)
+{
+  NusifSolverCtx *sc = nusifSolverCtxOf(s);
+  return sc->kind == SOLVER_GMG && sc->gmg ? sc->gmg->lastGroup : 0;
+}
+
 extern "C" double nusifTimeFinestSweep(Solver *s, int reps)
 {
   NusifSolverCtx *sc = nusifSolverCtxOf(s);

@@ -123,6 +123,7 @@ def _declare_optional(lib):
         "nusifFreeSolver": ([C.POINTER(Solver)], None),
         "nusifSetOption": ([C.POINTER(Parameter), C.c_char_p, C.c_char_p], C.c_int),
         "nusifSolverIterations": ([C.POINTER(Solver)], C.c_int),
+        "nusifSolverFirstGroup": ([C.POINTER(Solver)], C.c_int),
         "nusifSolverRelResidual": ([C.POINTER(Solver)], C.c_double),
         "nusifDeviceName": ([C.c_char_p, C.c_int], C.c_int),
         "nusifTimerEnable": ([C.c_char_p], C.c_int),
@@ -148,6 +149,8 @@ def _declare_optional(lib):
         "orcGmgLevels": ([C.c_int] * 4, C.c_int),
         "orcGmgSolve": ([C.c_int] * 3 + [C.c_double] * 3 + [_DP, _DP] + [C.c_int] * 3
                         + [C.c_double] * 2 + [C.c_int] * 3 + [_DP], C.c_int),
+        "orcGmgSolveGrouped": ([C.c_int] * 3 + [C.c_double] * 3 + [_DP, _DP] + [C.c_int] * 3
+                               + [C.c_double] * 2 + [C.c_int] * 4 + [_DP], C.c_int),
         "orcGmgCoarseSolve": ([C.c_int] * 3 + [C.c_double] * 3 + [_DP, _DP], None),
         "orcUnetParamCount": ([C.POINTER(C.c_int)], C.c_long),
         "orcUnetLoss": ([C.POINTER(C.c_int), C.c_int, _DP, _DP, _DP, _DP, _DP, C.c_int], C.c_double),

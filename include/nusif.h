@@ -51,6 +51,15 @@ void nusifFreeSolver(Solver *s);
  *                         launch its kernels one by one (0, default); same values
  *   mg_tile_rows  0|4|8|16 gmg: fused-sweep tile height (0: automatic, 16 on
  *                         levels with >= 128 rows); same values
+ *   mg_inner      int     gmg: cycles before a solve's first convergence test
+ *                         (1, default: test after every cycle; k > 1: the
+ *                         first k cycles run back to back, in mixed mode as
+ *                         FP32 cycles on one FP64 defect; 0: sized from
+ *                         the previous solve, its exact count when its
+ *                         group did not converge, one less when its own
+ *                         contraction says that would have sufficed); the
+ *                         tests after the first come after every cycle.
+ *                         Ignored with mg_speculate 1.
  *   mg_speculate  0 | 1   gmg: enqueue cycles before the previous one's
  *                         convergence test is read back (1; the device skips
  *                         the ones past convergence) or one at a time (0,
@@ -73,6 +82,9 @@ int nusifStepLimit(Parameter *p);
 
 /* Iterations (rb), sweeps (mg) or V-cycles (gmg) of the last solve(). */
 int nusifSolverIterations(Solver *s);
+/* gmg: V-cycles before the last solve's first convergence test (the
+ * mg_inner group; 0 when the solve ran no cycle or the solver is not gmg). */
+int nusifSolverFirstGroup(Solver *s);
 /* Relative residual ||rhs - A p||_2 / ||rhs||_2 after the last gmg solve
  * (-1 for rb/mg, whose stop tests use the reference's mean-square residual). */
 double nusifSolverRelResidual(Solver *s);

@@ -13,12 +13,18 @@
  *   - coarsest level: exact solve in the DCT-II eigenbasis of the Neumann
  *     Laplacian, constant mode dropped;
  *   - FP64 stop test ||rhs - A p|| / ||rhs|| <= tol after each cycle;
- *   - mixed: FP32 V-cycle on the FP64 residual, p += e (defect correction).
+ *   - mixed: FP32 V-cycle on the FP64 residual, p += e (defect correction);
+ *   - grouped (orcGmgSolveGrouped): the first test of a solve comes after
+ *     `firstGroup` cycles (mixed: that many FP32 cycles on one defect, the
+ *     later ones continuing from e), later tests after every cycle.
  * Pins (tests/test_oracle_gmg.py): convergence to the manufactured
  * eigenfunction solution, V-cycle contraction factor bounds, exactness of
  * the restriction/prolongation on constants and linear fields, the coarse
- * DCT solve against a dense least-squares solve, and the FP32 path agreeing
- * with the FP64 path to the tolerance.
+ * DCT solve against a dense least-squares solve, the FP32 path agreeing
+ * with the FP64 path to the tolerance, and the grouped loop: a group of one
+ * is the plain loop bit for bit, an FP64 group of m cycles gives the plain
+ * loop's m-th iterate bit for bit (the stop test does not touch p), and a
+ * mixed group converges at the plain loop's rate.
  */
 #include <math.h>
 #include <stdio.h>
@@ -271,6 +277,13 @@ void orcGmgCoarseSolve(int ni, int nj, int nk, double hx, double hy, double hz, 
   free(t);
 }
 
+/* cycles before the next stop test: the first group, then one */
+static int groupSize(int done, int firstGroup, int maxCycles)
+{
+  int m = done == 0 && firstGroup > 1 ? firstGroup : 1;
+  return m < maxCycles - done ? m : maxCycles - done;
+}
+
 static double resNorm(const G3 *g, const double *p, const double *rhs, const double *cf, float *r32,
     double *nb)
 {
@@ -289,11 +302,13 @@ static double resNorm(const G3 *g, const double *p, const double *rhs, const dou
 }
 
 /* Solves A p = rhs on the padded arrays p, rhs (rhs loses its mean when
- * `project`). Returns the number of cycles, -1 if the grid cannot be
- * coarsened; *relOut receives the final relative residual. */
-int orcGmgSolve(int ni, int nj, int nk, double dx, double dy, double dz, double *p, double *rhs,
+ * `project`). The first stop test comes after min(firstGroup, maxCycles)
+ * cycles, the later ones after each cycle. Returns the number of cycles, -1
+ * if the grid cannot be coarsened; *relOut receives the final relative
+ * residual. */
+int orcGmgSolveGrouped(int ni, int nj, int nk, double dx, double dy, double dz, double *p, double *rhs,
     int levels, int nu1, int nu2, double omega, double tol, int maxCycles, int project,
-    int mixed, double *relOut)
+    int mixed, int firstGroup, double *relOut)
 {
   int nl = orcGmgLevels(ni, nj, nk, levels);
   if (nl < 0) return -1;
@@ -355,8 +370,9 @@ int orcGmgSolve(int ni, int nj, int nk, double dx, double dy, double dz, double 
     double rn = resNorm(&g0, p, rhs, cf0, NULL, &nb);
     rel       = nb > 0.0 ? rn / nb : 0.0;
     while (nb > 0.0 && rel > tol && cycles < maxCycles) {
-      vcycleD(LD, 0, nl, nu1, nu2, tab64, o);
-      cycles++;
+      int m = groupSize(cycles, firstGroup, maxCycles);
+      for (int q = 0; q < m; q++) vcycleD(LD, 0, nl, nu1, nu2, tab64, o);
+      cycles += m;
       rn  = resNorm(&g0, p, rhs, cf0, NULL, &nb);
       rel = rn / nb;
     }
@@ -364,8 +380,9 @@ int orcGmgSolve(int ni, int nj, int nk, double dx, double dy, double dz, double 
     double rn = resNorm(&g0, p, rhs, cf0, LF[0].b, &nb);
     rel       = nb > 0.0 ? rn / nb : 0.0;
     while (nb > 0.0 && rel > tol && cycles < maxCycles) {
+      int m = groupSize(cycles, firstGroup, maxCycles);
       memset(LF[0].x, 0, g0.n * sizeof(float));
-      vcycleF(LF, 0, nl, nu1, nu2, tab32, o);
+      for (int q = 0; q < m; q++) vcycleF(LF, 0, nl, nu1, nu2, tab32, o);
       for (int k = 1; k <= nk; k++)
         for (int j = 1; j <= nj; j++)
           for (int i = 1; i <= ni; i++) {
@@ -373,7 +390,7 @@ int orcGmgSolve(int ni, int nj, int nk, double dx, double dy, double dz, double 
             p[q]     = p[q] + (double)LF[0].x[q];
           }
       ghostsD(p, &g0);
-      cycles++;
+      cycles += m;
       rn  = resNorm(&g0, p, rhs, cf0, LF[0].b, &nb);
       rel = rn / nb;
     }
@@ -390,6 +407,15 @@ int orcGmgSolve(int ni, int nj, int nk, double dx, double dy, double dz, double 
   free(tab32);
   if (relOut) *relOut = rel;
   return cycles;
+}
+
+/* the plain loop: a stop test after every cycle */
+int orcGmgSolve(int ni, int nj, int nk, double dx, double dy, double dz, double *p, double *rhs,
+    int levels, int nu1, int nu2, double omega, double tol, int maxCycles, int project,
+    int mixed, double *relOut)
+{
+  return orcGmgSolveGrouped(ni, nj, nk, dx, dy, dz, p, rhs, levels, nu1, nu2, omega, tol, maxCycles,
+      project, mixed, 1, relOut);
 }
 
 /* Test hooks: one prolongation (x_f += P x_c) and one residual restriction

@@ -13,17 +13,21 @@ pytestmark = pytest.mark.gpu
 DP = A._DP
 
 
-def oracle_solve(p, rhs, prm, opts):
+def oracle_solve(p, rhs, prm, opts, first=None):
+    """The oracle's gmg solve; `first` = cycles before the first stop test
+    (default: the mg_inner option, 1 when absent)."""
     prm = dict(A.DEFAULTS, **prm)
     rel = C.c_double()
-    cyc = lib("orc_gmg").orcGmgSolve(
+    if first is None:
+        first = max(1, int(opts.get("mg_inner", 1)))
+    cyc = lib("orc_gmg").orcGmgSolveGrouped(
         prm["imax"], prm["jmax"], prm["kmax"], prm["xlength"] / prm["imax"],
         prm["ylength"] / prm["jmax"], prm["zlength"] / prm["kmax"], p.ctypes.data_as(DP),
         rhs.ctypes.data_as(DP), prm["levels"], prm["presmooth"], prm["postsmooth"],
         float(opts.get("mg_omega", 1.0)),
         float(opts["mg_tol"]), int(opts.get("mg_maxcycles", 100)),
         int(opts.get("mg_project", 0)), 1 if opts.get("precision") == "mixed" else 0,
-        C.byref(rel))
+        first, C.byref(rel))
     return cyc, rel.value
 
 
@@ -63,15 +67,54 @@ def test_gpu_gmg_solve_matches_oracle(shape, levels, precision, gpu_ok):
     c.close()
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_gpu_gmg_cavity_steps(precision, gpu_ok):
+@pytest.mark.parametrize("shape,levels,precision,inner", [((64, 64, 64), 6, "fp64", 3),
+                                                         ((64, 64, 64), 6, "mixed", 3),
+                                                         ((128, 12, 72), 3, "mixed", 2)])
+def test_gpu_gmg_grouped_cycles_match_oracle(shape, levels, precision, inner, gpu_ok):
+    """mg_inner k: the first stop test after k cycles (mixed: k FP32 cycles on
+    one defect, continuing e): same cycle count and p as the oracle's grouped
+    loop."""
+    print("gmg grouped", shape, levels, precision, inner, flush=True)
+    ni, nj, nk = shape
+    prm = dict(G.KAT_PARAMS, imax=ni, jmax=nj, kmax=nk, xlength=ni / 32.0, ylength=nj / 32.0,
+               zlength=nk / 32.0, levels=levels, presmooth=2, postsmooth=2)
+    opts = {"solver": "gmg", "precision": precision, "mg_tol": 1e-10, "mg_project": 1,
+            "mg_inner": inner}
+    c = A.Case(lib("gpu_rb"), options=opts, **prm)
+    f = G.random_fields(c.shape, 33)
+    p0 = np.zeros(c.shape)
+    c.set_fields(p=p0, rhs=f["rhs"])
+    res = c.lib.solve(C.byref(c.s), c.d.p, c.d.rhs)
+    out = c.fields()
+    cyc_g = c.lib.nusifSolverIterations(C.byref(c.s))
+    p = p0.copy()
+    rhs = np.ascontiguousarray(f["rhs"]).copy()
+    cyc_o, rel_o = oracle_solve(p, rhs, prm, opts)
+    assert cyc_g == cyc_o
+    assert res == pytest.approx(rel_o, rel=1e-6)
+    a = out["p"][1:-1, 1:-1, 1:-1]
+    b = p[1:-1, 1:-1, 1:-1]
+    tol = 1e-12 if precision == "fp64" else 1e-9
+    assert np.max(np.abs(a - b)) <= tol * np.max(np.abs(b))
+    c.close()
+
+
+@pytest.mark.parametrize("precision,inner", [("fp64", 1), ("mixed", 1), ("mixed", 0)])
+def test_gpu_gmg_cavity_steps(precision, inner, gpu_ok):
     """Whole steps of the 32^3 cavity with the gmg solver: GPU vs the oracle's
-    step functions + oracle gmg; per-field error stated against FP64."""
+    step functions + oracle gmg; per-field error stated against FP64. With
+    mg_inner 0 the library sizes each solve's first group from the previous
+    solve: the oracle is given the group the library reports, the cycle
+    counts agree step by step, and each group follows from the previous
+    solve (its count when the group did not converge, else the same group or
+    one less)."""
+    print("gmg cavity", precision, inner, flush=True)
     prm = dict(G.TRAJ["c1_rb"][1], re=1000.0, levels=4, presmooth=2, postsmooth=2)
-    opts = {"solver": "gmg", "precision": precision, "mg_tol": 1e-8}
+    opts = {"solver": "gmg", "precision": precision, "mg_tol": 1e-8, "mg_inner": inner}
     g = A.Case(lib("gpu_rb"), options=opts, **prm)
     o = A.Case(lib("orc_rb"), **prm)
-    for step in range(5):
+    prev = None  # (group, cycles) of the previous solve
+    for step in range(6):
         g.step()
         d = o.d
         if d.tau > 0:
@@ -84,7 +127,17 @@ def test_gpu_gmg_cavity_steps(precision, gpu_ok):
         n = int(np.prod(o.shape))
         pv = np.ctypeslib.as_array(d.p, shape=(n,))
         rv = np.ctypeslib.as_array(d.rhs, shape=(n,))
-        oracle_solve(pv, rv, prm, opts)
+        first = g.lib.nusifSolverFirstGroup(C.byref(g.s)) if inner == 0 else inner
+        cyc, _ = oracle_solve(pv, rv, prm, opts, first)
+        assert g.lib.nusifSolverIterations(C.byref(g.s)) == cyc, step
+        if inner == 0:
+            if prev is None:
+                assert first == 1
+            elif prev[1] > prev[0]:
+                assert first == prev[1], (step, prev, first)
+            else:
+                assert first in (prev[0], max(1, prev[0] - 1)), (step, prev, first)
+        prev = (first, cyc)
         o.call("adaptUV")
     fg, fo = g.fields(), o.fields()
     for k in ("u", "v", "w", "p"):

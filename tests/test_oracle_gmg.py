@@ -110,6 +110,62 @@ def test_mixed_matches_fp64():
     assert np.max(np.abs((a - a.mean()) - (b - b.mean()))) <= 1e-9 * np.max(np.abs(a))
 
 
+def solve_grouped(p, rhs, n, first, levels=5, tol=1e-10, maxc=60, mixed=0, project=1):
+    ni, nj, nk = n
+    rel = C.c_double()
+    cyc = lib("orc_gmg").orcGmgSolveGrouped(ni, nj, nk, 1.0 / ni, 1.0 / nj, 1.0 / nk, ptr(p), ptr(rhs),
+                                            levels, 2, 2, 1.0, tol, maxc, project, mixed, first,
+                                            C.byref(rel))
+    return cyc, rel.value
+
+
+@pytest.mark.parametrize("mixed", [0, 1])
+def test_group_of_one_is_the_plain_loop(mixed):
+    n = (32, 16, 24)
+    rhs = np.random.default_rng(7).uniform(-1, 1, (26, 18, 34))
+    pa, pb = np.zeros_like(rhs), np.zeros_like(rhs)
+    ca = solve(pa, rhs.copy(), n, levels=4, tol=1e-9, mixed=mixed)
+    cb = solve_grouped(pb, rhs.copy(), n, 1, levels=4, tol=1e-9, mixed=mixed)
+    assert ca == cb
+    np.testing.assert_array_equal(pa, pb)
+
+
+@pytest.mark.parametrize("m", [2, 3])
+def test_fp64_group_gives_the_plain_iterate(m):
+    """The stop test does not touch p: m FP64 cycles in one group are the
+    plain loop's m-th iterate, bit for bit."""
+    n = (32, 32, 16)
+    rhs = np.random.default_rng(8).uniform(-1, 1, (18, 34, 34))
+    pa, pb = np.zeros_like(rhs), np.zeros_like(rhs)
+    ca, ra = solve(pa, rhs.copy(), n, levels=4, tol=0.0, maxc=m)
+    cb, rb = solve_grouped(pb, rhs.copy(), n, m, levels=4, tol=0.0, maxc=m)
+    assert ca == cb == m and ra == rb
+    np.testing.assert_array_equal(pa, pb)
+
+
+def test_mixed_group_continues_the_correction():
+    """Three FP32 cycles on one defect reduce the residual like three plain
+    mixed cycles (within 10x), and well below one cycle (10x; one cycle
+    contracts by ~0.03 here): a group that restarted e from zero, or ran one
+    cycle, fails this."""
+    n = (32, 32, 32)
+    rhs = np.random.default_rng(9).uniform(-1, 1, (34, 34, 34))
+    rels = {}
+    for name, first, maxc in (("one", 1, 1), ("plain3", 1, 3), ("group3", 3, 3)):
+        p = np.zeros_like(rhs)
+        cyc, rel = solve_grouped(p, rhs.copy(), n, first, levels=5, tol=0.0, maxc=maxc, mixed=1)
+        assert cyc == maxc
+        rels[name] = rel
+    assert rels["group3"] <= 10.0 * rels["plain3"]
+    assert rels["group3"] <= 0.1 * rels["one"]
+    # and a grouped solve still stops at the tolerance, overshooting by at
+    # most the group's extra cycles
+    p0, p1 = np.zeros_like(rhs), np.zeros_like(rhs)
+    c0, r0 = solve_grouped(p0, rhs.copy(), n, 1, tol=1e-9, mixed=1)
+    c1, r1 = solve_grouped(p1, rhs.copy(), n, 4, tol=1e-9, mixed=1)
+    assert r0 <= 1e-9 and r1 <= 1e-9 and c0 <= c1 <= max(c0, 4)
+
+
 def test_coarse_dct_solve_is_the_min_norm_least_squares_solution():
     """The coarsest-level solve equals the pseudo-inverse of the dense
     Neumann Laplacian (numpy lstsq) on a small anisotropic box."""
