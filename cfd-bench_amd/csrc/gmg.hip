@@ -404,14 +404,16 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
 }
 
 /* Launch shape of the fused sweep on a level, from a sweep of tile heights
- * 4/8/16 and k-chunks 4..32 on 32^3..256^3 cubes (FP32 and FP64). Levels with
+ * 4/8/16 and k-chunks 1..32 on 32^3..256^3 cubes (FP32 and FP64). Levels with
  * nj >= 128 use 64 x 16 tiles (512 threads): fewer ring cells per interior
  * cell, 44 us for FP32 at 256^3 against 49 us for 64 x 4. Their k-chunk is the
  * longest (<= FT_KC planes, a power of two >= 4) that still gives >= 1024
- * workgroups for FP32, >= 256 for FP64 (FP32 256^3: 16 planes; FP64 256^3:
- * 32). Narrower levels use 64 x 4 tiles with the k-chunk that gives >= 2048
- * workgroups (4 planes on 64^3 and 32^3). `rows` (4, 8 or 16) forces the tile
- * height; 0 picks it. */
+ * workgroups for FP32, >= 256 for FP64 (FP32 256^3: 16 planes, 128^3: 4;
+ * FP64 256^3: 32, 128^3: 8). The small levels are latency-bound: 64 x 8
+ * tiles for nj >= 64, 64 x 4 below, and the k-chunk that gives >= 2048
+ * workgroups, down to one plane (64^3 and 32^3: one plane, 6.1 and 5.1 us
+ * for FP32 against 8.0 and 7.7 us with four). `rows` (4, 8 or 16) forces the
+ * tile height; 0 picks it. */
 struct SweepShape {
   int tj, kc;
 };
@@ -421,10 +423,11 @@ static long long sweepTiles(const Geo &g, int tj, int kc)
 }
 template <typename T> static SweepShape sweepShape(const Geo &g, int rows)
 {
-  const int tj = rows ? rows : (g.nj >= 128 ? 16 : 4);
-  const long long want = tj == 4 ? 2048 : (sizeof(T) == 4 ? 1024 : 256);
+  const int tj = rows ? rows : (g.nj >= 128 ? 16 : (g.nj >= 64 ? 8 : 4));
+  const long long want = tj == 16 ? (sizeof(T) == 4 ? 1024 : 256) : 2048;
+  const int kmin       = tj == 16 ? 4 : 1;
   int kc = FT_KC;
-  while (kc > 4 && sweepTiles(g, tj, kc) < want) kc /= 2;
+  while (kc > kmin && sweepTiles(g, tj, kc) < want) kc /= 2;
   return {tj, kc};
 }
 

@@ -50,7 +50,7 @@ void nusifFreeSolver(Solver *s);
  *   mg_graph      0 | 1   gmg: replay each cycle from a captured HIP graph (1) or
  *                         launch its kernels one by one (0, default); same values
  *   mg_tile_rows  0|4|8|16 gmg: fused-sweep tile height (0: automatic, 16 on
- *                         levels with >= 128 rows); same values
+ *                         levels with >= 128 rows, 8 with >= 64); same values
  *   mg_inner      int     gmg: cycles before a solve's first convergence test
  *                         (1, default: test after every cycle; k > 1: the
  *                         first k cycles run back to back, in mixed mode as

@@ -206,16 +206,19 @@ def test_gpu_gmg_graph_replay_equals_eager_launches(precision, gpu_ok):
         c.close()
 
 
+@pytest.mark.parametrize("shape,levels", [((128, 144, 240), 5), ((64, 12, 14), 2)])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_gpu_gmg_tall_tile_sweep_equals_default(precision, gpu_ok):
-    """The fused sweep with 64 x 8 and 64 x 16 tiles (used on levels with
-    >= 128 rows) against 64 x 4 tiles: same p bit for bit. At 128 x 144 x 240
-    the three heights run with different k-chunks on the finest level (FP64:
-    8, 32 and 16 planes, the last of the 32-plane chunks partial; FP32: 8, 8
-    and 4), and the next levels' 72 and 36 rows are no multiple of 16."""
-    print("gmg tile rows", precision, flush=True)
-    prm = dict(G.KAT_PARAMS, imax=128, jmax=144, kmax=240, xlength=2.0, ylength=2.25,
-               zlength=3.75, levels=5, presmooth=2, postsmooth=2)
+def test_gpu_gmg_tall_tile_sweep_equals_default(shape, levels, precision, gpu_ok):
+    """The fused sweep with 64 x 8 and 64 x 16 tiles against 64 x 4 tiles:
+    same p bit for bit. At 128 x 144 x 240 the three heights run with
+    different k-chunks on the finest level (FP64: 8, 4 and 16 planes; FP32:
+    8, 4 and 4) and the next levels' 72 and 36 rows are no multiple of 16; at
+    64 x 12 x 14 the 64 x 16 tile is partial in j and its last 4-plane chunk
+    holds 2 planes."""
+    print("gmg tile rows", shape, precision, flush=True)
+    ni, nj, nk = shape
+    prm = dict(G.KAT_PARAMS, imax=ni, jmax=nj, kmax=nk, xlength=ni / 64.0, ylength=nj / 64.0,
+               zlength=nk / 64.0, levels=levels, presmooth=2, postsmooth=2)
     out = []
     for rows in (4, 8, 16):
         opts = {"solver": "gmg", "precision": precision, "mg_tol": 1e-9, "mg_project": 1,
