@@ -223,6 +223,37 @@ struct SweepRBShape {
   static constexpr int YS   = NRT;
 };
 
+/* Plane-relative addressing through buffer instructions: a plane's buffer
+ * descriptor (base = the plane's first cell) is uniform and sits in scalar
+ * registers, and each lane passes a 32-bit byte offset that stays fixed for
+ * the whole march. Global loads with 64-bit addresses cost two 64-bit VALU
+ * adds per access here (about a third of the sweep's VALU instructions);
+ * buffer accesses cost none. Every offset is inside the padded box (the
+ * callers clamp), so the descriptor's range (2^31 - 1 bytes) is no bound
+ * check. 0x00020000: the descriptor's fourth word for gfx9-family GPUs
+ * (32-bit data format, no swizzle). */
+template <typename T> __device__ inline __amdgpu_buffer_rsrc_t planeRsrc(const T *a, const Geo &g, int k)
+{
+  return __builtin_amdgcn_make_buffer_rsrc((void *)(a + (long long)k * g.sk), (short)0, 0x7fffffff, 0x00020000);
+}
+__device__ inline float ldP(float *, __amdgpu_buffer_rsrc_t r, unsigned off)
+{
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ inline double ldP(double *, __amdgpu_buffer_rsrc_t r, unsigned off)
+{
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ inline void stP(__amdgpu_buffer_rsrc_t r, unsigned off, float v)
+{
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+}
+__device__ inline void stP(__amdgpu_buffer_rsrc_t r, unsigned off, double v)
+{
+  using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, 0, 0);
+}
+
 template <typename T, int TJ, int MINW, bool ZX>
 __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ xo, T *__restrict__ xn,
     const T *__restrict__ b, Geo g, Coef<T> cf, int kc, const int *__restrict__ stop)
@@ -247,24 +278,27 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
 
   /* X loads: element e = (li, lj) row-major, clamped into the padded box
    * (clamped entries are only read by updates whose results are not kept) */
-  int xOff[NX], xLds[NX];
+  constexpr unsigned ES = sizeof(T);
+  unsigned xOff[NX]; /* byte offsets within a plane */
+  int xLds[NX];
 #pragma unroll
   for (int q = 0; q < NX; q++) {
     int e  = min(tid + NT * q, S::NXE - 1);
     int li = e % XW, lj = e / XW;
-    xOff[q] = min(max(jb + lj, 0), nj + 1) * sj + min(max(ib + li, 0), ni + 1);
+    xOff[q] = (unsigned)(min(max(jb + lj, 0), nj + 1) * sj + min(max(ib + li, 0), ni + 1)) * ES;
     xLds[q] = ((li + lj) & 1) * HALF + lj * RW + (li >> 1);
   }
   /* red tasks: position p, row lj in 1..TJ+2; the column is 2p + s with
    * s = (k + 1 + lj) & 1 on plane k. rIn bit s: that cell is interior in (i, j). */
-  int rP[NR], rLj[NR], rOff[NR], rIn[NR];
+  int rP[NR], rLj[NR], rIn[NR];
+  unsigned rOff[NR]; /* byte offset of column 2p within a plane */
 #pragma unroll
   for (int r = 0; r < NR; r++) {
     int task = min(tid + NT * r, S::NRT - 1);
     int p = task % RW, lj = 1 + task / RW;
     int j = jb + lj, i = ib + 2 * p;
     rP[r] = p; rLj[r] = lj;
-    rOff[r] = min(max(j, 0), nj + 1) * sj + min(max(i, 0), ni);
+    rOff[r] = (unsigned)(min(max(j, 0), nj + 1) * sj + min(max(i, 0), ni)) * ES;
     bool jin = j >= 1 && j <= nj;
     rIn[r] = (jin && i >= 1 && i <= ni ? 1 : 0) | (jin && i + 1 >= 1 && i + 1 <= ni ? 2 : 0);
   }
@@ -272,15 +306,21 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
    * s = (k + lj) & 1, its red partner 2p + 1 - s. */
   const int bp = 1 + (tid & 31), blj = 2 + (tid >> 5);
   const int bj = jb + blj, bi = ib + 2 * bp; /* global i of column 2p */
-  const int bRow = min(bj, nj + 1) * sj + min(bi, ni);
+  const unsigned bRow = (unsigned)(min(bj, nj + 1) * sj + min(bi, ni)) * ES; /* bytes */
+  const unsigned sjB  = (unsigned)sj * ES;
   const bool bjIn = bj <= nj;
   const bool bIn0 = bjIn && bi <= ni, bIn1 = bjIn && bi + 1 <= ni;
 
   T xr[NX];
   auto fetchX = [&](int k) {
-    long long base = (long long)min(max(k, 0), nk + 1) * sk;
+    if (ZX) {
 #pragma unroll
-    for (int q = 0; q < NX; q++) xr[q] = ZX ? (T)0 : xo[base + xOff[q]];
+      for (int q = 0; q < NX; q++) xr[q] = (T)0;
+      return;
+    }
+    const __amdgpu_buffer_rsrc_t pl = planeRsrc(xo, g, min(max(k, 0), nk + 1));
+#pragma unroll
+    for (int q = 0; q < NX; q++) xr[q] = ldP((T *)nullptr, pl, xOff[q]);
   };
   auto storeX = [&](int k) {
     if (ZX) return;
@@ -289,10 +329,10 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
       if (q < NX - 1 || tid + NT * q < S::NXE) X[k & 3][xLds[q]] = xr[q];
   };
   auto fetchBR = [&](int k, T *out) { /* b under this thread's red cells of plane k */
-    long long base = (long long)min(max(k, 1), nk) * sk;
+    const __amdgpu_buffer_rsrc_t pl = planeRsrc(b, g, min(max(k, 1), nk));
     const int s0 = (k + 1) & 1;
 #pragma unroll
-    for (int r = 0; r < NR; r++) out[r] = b[base + rOff[r] + ((s0 + rLj[r]) & 1)];
+    for (int r = 0; r < NR; r++) out[r] = ldP((T *)nullptr, pl, rOff[r] + ((s0 + rLj[r]) & 1) * ES);
   };
   auto redPlane = [&](int k, const T *bred) {
     const int hr = (k + 1) & 1;
@@ -336,34 +376,43 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
     T vR   = Yc[a];
     /* column 2p holds the black value when s = 0, the red one when s = 1 */
     T v0 = s ? vR : vB, v1 = s ? vB : vR;
-    long long c = k * sk + bRow;
+    const __amdgpu_buffer_rsrc_t pl = planeRsrc(xn, g, k);
+    const unsigned c0 = bRow, c1 = bRow + ES;
     if (bIn0) {
-      xn[c] = v0;
-      if (bi == 1) xn[c - 1] = v0;
-      if (bi == ni) xn[c + 1] = v0;
-      if (bj == 1) xn[c - sj] = v0;
-      if (bj == nj) xn[c + sj] = v0;
-      if (k == 1) xn[c - sk] = v0;
-      if (k == nk) xn[c + sk] = v0;
+      stP(pl, c0, v0);
+      if (bi == 1) stP(pl, c0 - ES, v0);
+      if (bi == ni) stP(pl, c1, v0);
+      if (bj == 1) stP(pl, c0 - sjB, v0);
+      if (bj == nj) stP(pl, c0 + sjB, v0);
     }
     if (bIn1) {
-      long long c1 = c + 1;
-      xn[c1] = v1;
-      if (bi + 1 == ni) xn[c1 + 1] = v1;
-      if (bj == 1) xn[c1 - sj] = v1;
-      if (bj == nj) xn[c1 + sj] = v1;
-      if (k == 1) xn[c1 - sk] = v1;
-      if (k == nk) xn[c1 + sk] = v1;
+      stP(pl, c1, v1);
+      if (bi + 1 == ni) stP(pl, c1 + ES, v1);
+      if (bj == 1) stP(pl, c1 - sjB, v1);
+      if (bj == nj) stP(pl, c1 + sjB, v1);
+    }
+    if (k == 1 || k == nk) { /* the k-face ghost planes */
+      if (k == 1) {
+        const __amdgpu_buffer_rsrc_t pg = planeRsrc(xn, g, 0);
+        if (bIn0) stP(pg, c0, v0);
+        if (bIn1) stP(pg, c1, v1);
+      }
+      if (k == nk) {
+        const __amdgpu_buffer_rsrc_t pg = planeRsrc(xn, g, nk + 1);
+        if (bIn0) stP(pg, c0, v0);
+        if (bIn1) stP(pg, c1, v1);
+      }
     }
   };
   auto fetchBB = [&](int k) {
     int kk = min(max(k, 1), nk);
-    return b[(long long)kk * sk + bRow + ((kk + blj) & 1)];
+    return ldP((T *)nullptr, planeRsrc(b, g, kk), bRow + ((kk + blj) & 1) * ES);
   };
 
   /* one plane step with register set (xs, bs, bbs): store plane kk+1 to LDS,
    * issue the loads of plane kk+3 (x), kk+2 (red b) and kk+1 (black b), then
-   * red(kk) and black(kk-1) */
+   * red(kk) and black(kk-1). Loads for planes past the chunk's last use (x:
+   * k1+1, red b: k1, black b: k1-1) are skipped. */
   auto iterate = [&](int kk, T(&xs)[NX], T(&bs)[NR], T &bbs) {
 #pragma unroll
     for (int q = 0; q < NX; q++) xr[q] = xs[q];
@@ -372,11 +421,15 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
 #pragma unroll
     for (int r = 0; r < NR; r++) bUse[r] = bs[r];
     T bbUse = bbs;
-    fetchX(kk + 3);
+    if (kk + 3 <= k1 + 1) {
+      fetchX(kk + 3);
 #pragma unroll
-    for (int q = 0; q < NX; q++) xs[q] = xr[q];
-    fetchBR(kk + 2, bs);
-    bbs = fetchBB(kk + 1);
+      for (int q = 0; q < NX; q++) xs[q] = xr[q];
+    }
+    if (kk + 2 <= k1) {
+      fetchBR(kk + 2, bs);
+      bbs = fetchBB(kk + 1);
+    }
     ldsBarrier();
     redPlane(kk, bUse);
     ldsBarrier();
