@@ -62,7 +62,7 @@ DOMINANT = {"fp64": "gmg_smooth64_l0", "mixed": "gmg_smooth32_l0", "learned": "g
 TIMERS = ["gmg_smooth64_l0", "gmg_smooth32_l0", "gmg_smooth32z_l0", "gmg_restrict32_l0", "gmg_prolong32_l0",
           "gmg_restrict64_l0", "gmg_prolong64_l0", "gmg_residual64", "gmg_correct_residual",
           "computeFG",
-          "computeRHS", "adaptUV", "absmax"]
+          "computeRHS", "adaptUV", "absmax", "unet_forward", "unet_conv"]
 
 
 def measured_traffic(kernel, n):
@@ -228,6 +228,17 @@ def main():
                 "value": vm, "mlups": cells * vm * 1e-6,
                 "cycles_per_step": sum(mx["cycles"]) / len(mx["cycles"]),
                 "speedup_vs_fp64_mg": vm / f64}
+    # what plain streaming kernels reach on this device (256 MiB arrays): the
+    # practical ceiling under the 8 TB/s vendor peak
+    ms = (C.c_double * 3)()
+    lib.nusifStreamProbe(256 << 20, 20, ms)
+    nb = float(256 << 20)
+    out["hbm_probe"] = {"copy_GBs": 2 * nb / (ms[0] * 1e-3) / 1e9,
+                        "two_reads_one_write_GBs": 3 * nb / (ms[1] * 1e-3) / 1e9,
+                        "read_GBs": nb / (ms[2] * 1e-3) / 1e9, "array_bytes": int(nb)}
+    if "roofline" in out:
+        out["roofline"]["frac_of_probe_2r1w"] = (out["roofline"]["achieved"]
+                                                 / out["hbm_probe"]["two_reads_one_write_GBs"])
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, n)
     case.close()

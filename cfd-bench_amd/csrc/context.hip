@@ -7,6 +7,7 @@
  * reference src/solver.h:35). */
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -442,4 +443,74 @@ extern "C" double nusifCopyProbe(size_t bytes, int reps)
   HIP_CHECK(hipFree(a));
   HIP_CHECK(hipFree(b));
   return reps > 0 ? ms / reps : 0.0;
+}
+
+/* ---- streaming probes: the HBM rate a plain kernel reaches ------------- */
+/* Grid-stride loops over float4 (16 B per lane, one 1 KiB row per wave). */
+__global__ __launch_bounds__(256) void kProbeCopy(const float4 *__restrict__ a, float4 *__restrict__ b, long long n)
+{
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) b[i] = a[i];
+}
+
+/* two reads, one write: the access pattern of a smoother sweep (x, b -> x) */
+__global__ __launch_bounds__(256) void kProbeAdd(const float4 *__restrict__ a, const float4 *__restrict__ c,
+    float4 *__restrict__ b, long long n)
+{
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    float4 x = a[i], y = c[i];
+    b[i]     = make_float4(x.x + y.x, x.y + y.y, x.z + y.z, x.w + y.w);
+  }
+}
+
+/* read only: the sum decides a store that never happens for finite input,
+ * so every load is needed */
+__global__ __launch_bounds__(256) void kProbeRead(const float4 *__restrict__ a, float *__restrict__ out, long long n)
+{
+  float s = 0.0f;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    float4 x = a[i];
+    s += (x.x + x.y) + (x.z + x.w);
+  }
+  if (s == -1.0f) out[blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+/* Milliseconds per launch of the three probes over arrays of `bytes` bytes
+ * each (rounded down to 16 B), `reps` launches each: ms[0] copy (rd + wr),
+ * ms[1] two reads and one write, ms[2] read only. The arrays start zeroed. */
+extern "C" void nusifStreamProbe(size_t bytes, int reps, double *ms)
+{
+  const long long n = (long long)(bytes / 16);
+  float4 *a = nullptr, *b = nullptr, *c = nullptr;
+  float *out = nullptr;
+  const int grid = (int)std::min<long long>((n + 255) / 256, 16384);
+  HIP_CHECK(hipMalloc(&a, (size_t)n * 16));
+  HIP_CHECK(hipMalloc(&b, (size_t)n * 16));
+  HIP_CHECK(hipMalloc(&c, (size_t)n * 16));
+  HIP_CHECK(hipMalloc(&out, (size_t)grid * 256 * sizeof(float)));
+  HIP_CHECK(hipMemsetAsync(a, 0, (size_t)n * 16, nusifStream()));
+  HIP_CHECK(hipMemsetAsync(b, 0, (size_t)n * 16, nusifStream()));
+  HIP_CHECK(hipMemsetAsync(c, 0, (size_t)n * 16, nusifStream()));
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  for (int v = 0; v < 3; v++) {
+    for (int r = -1; r < reps; r++) { /* r = -1: untimed warm-up launch */
+      if (r == 0) HIP_CHECK(hipEventRecord(e0, nusifStream()));
+      if (v == 0) kProbeCopy<<<grid, 256, 0, nusifStream()>>>(a, b, n);
+      if (v == 1) kProbeAdd<<<grid, 256, 0, nusifStream()>>>(a, c, b, n);
+      if (v == 2) kProbeRead<<<grid, 256, 0, nusifStream()>>>(a, out, n);
+    }
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(e1, nusifStream()));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float t = 0.0f;
+    HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+    ms[v] = reps > 0 ? t / reps : 0.0;
+  }
+  HIP_CHECK(hipEventDestroy(e0));
+  HIP_CHECK(hipEventDestroy(e1));
+  HIP_CHECK(hipFree(a));
+  HIP_CHECK(hipFree(b));
+  HIP_CHECK(hipFree(c));
+  HIP_CHECK(hipFree(out));
 }

@@ -55,11 +55,33 @@ __global__ __launch_bounds__(256) void kAbsMax3(const double *__restrict__ u,
 __global__ __launch_bounds__(1024) void kFoldMax3(const double *partials, int nb, DevScalars *sc, DevScalars *scHost)
 {
   __shared__ double lds[16]; /* one per wavefront */
-  for (int q = 0; q < 3; q++) {
-    double m = foldMax(partials, nb, 3, q, DBL_MIN, lds);
+  /* the three maxima in one pass, eight partials in flight per thread; a
+   * maximum does not depend on the order it is taken in */
+  double m[3] = {DBL_MIN, DBL_MIN, DBL_MIN};
+  const int st = blockDim.x;
+  int q        = threadIdx.x;
+  for (; q + 7 * st < nb; q += 8 * st) {
+    double x[8][3];
+#pragma unroll
+    for (int e = 0; e < 8; e++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) x[e][c] = partials[3LL * (q + e * st) + c];
+#pragma unroll
+    for (int e = 0; e < 8; e++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) m[c] = m[c] > x[e][c] ? m[c] : x[e][c];
+  }
+  for (; q < nb; q += st)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      double x = partials[3LL * q + c];
+      m[c]     = m[c] > x ? m[c] : x;
+    }
+  for (int c = 0; c < 3; c++) {
+    double r = blockMax(m[c], lds);
     if (threadIdx.x == 0) {
-      sc->absmax[q]     = (unsigned long long)__double_as_longlong(m);
-      scHost->absmax[q] = (unsigned long long)__double_as_longlong(m);
+      sc->absmax[c]     = (unsigned long long)__double_as_longlong(r);
+      scHost->absmax[c] = (unsigned long long)__double_as_longlong(r);
     }
   }
   if (threadIdx.x == 0) __threadfence_system();

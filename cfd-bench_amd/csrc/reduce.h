@@ -65,8 +65,20 @@ __device__ inline double blockMax(double x, double *lds)
  * before these reads; no device-scope atomics or fences are needed. */
 __device__ inline double foldSum(const double *part, int n, int stride, int comp, double *lds)
 {
-  double s = 0.0;
-  for (int q = threadIdx.x; q < n; q += blockDim.x) s += part[(long long)q * stride + comp];
+  /* each thread adds its elements in ascending index order; four loads are
+   * issued before their adds, so the fold is not one latency per element */
+  double s     = 0.0;
+  const int st = blockDim.x;
+  int q        = threadIdx.x;
+  for (; q + 3 * st < n; q += 4 * st) {
+    double a0 = part[(long long)q * stride + comp], a1 = part[(long long)(q + st) * stride + comp];
+    double a2 = part[(long long)(q + 2 * st) * stride + comp], a3 = part[(long long)(q + 3 * st) * stride + comp];
+    s += a0;
+    s += a1;
+    s += a2;
+    s += a3;
+  }
+  for (; q < n; q += st) s += part[(long long)q * stride + comp];
   return blockSum(s, lds);
 }
 

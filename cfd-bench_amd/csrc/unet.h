@@ -17,7 +17,8 @@ void unetGetGrads(UnetModel *m, float *hostGrads);
 void unetZeroGrads(UnetModel *m);
 void unetSyncWeights(UnetModel *m);
 /* x: n^3 x cin bf16 (channels-last); the prediction stays on the device */
-void unetForward(UnetModel *m, const uint16_t *x);
+/* forTraining: also keep the upsampled decoder inputs unetBackward reads */
+void unetForward(UnetModel *m, const uint16_t *x, bool forTraining = false);
 const float *unetOutput(UnetModel *m);
 /* after unetForward: loss against t (n^3 FP32), gradients added into G */
 double unetBackward(UnetModel *m, const uint16_t *x, const float *t, bool wantLoss);

@@ -14,6 +14,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <type_traits>
 #include <vector>
 
 #include "nusif_internal.h"
@@ -387,18 +388,24 @@ void unetInit(UnetModel *m, unsigned seed)
   m->step = 0;
 }
 
+/* The library stream is non-blocking: a plain hipMemcpy (null stream) does
+ * not wait for the kernels queued on it, so each host copy below first
+ * drains that stream (the kernels that write, or still read, the buffer). */
 void unetSetParams(UnetModel *m, const float *hostParams)
 {
+  HIP_CHECK(hipStreamSynchronize(nusifStream()));
   HIP_CHECK(hipMemcpy(m->P, hostParams, m->nParams * sizeof(float), hipMemcpyHostToDevice));
 }
 
 void unetGetParams(UnetModel *m, float *hostParams)
 {
+  HIP_CHECK(hipStreamSynchronize(nusifStream()));
   HIP_CHECK(hipMemcpy(hostParams, m->P, m->nParams * sizeof(float), hipMemcpyDeviceToHost));
 }
 
 void unetGetGrads(UnetModel *m, float *hostGrads)
 {
+  HIP_CHECK(hipStreamSynchronize(nusifStream()));
   HIP_CHECK(hipMemcpy(hostGrads, m->G, m->nParams * sizeof(float), hipMemcpyDeviceToHost));
 }
 
@@ -447,12 +454,23 @@ static void launchConvLds(UnetModel *m, const ConvArgs<uint16_t, uint16_t> &a)
       (size_t)chunks * n * n * n * rows <= m->splitCap) split = chunks;
   auto grid = [&](int NB) { return dim3(bricks, (rows + 16 * NB - 1) / (16 * NB), split); };
   hipStream_t st = nusifStream();
-  if (rows >= 64 && bricks >= 512)
-    NUSIF_TIMED("unet_conv", (kConvLds<uint16_t, 4><<<grid(4), 256, 0, st>>>(a, m->split)));
-  else if (rows >= 32 && bricks >= 512)
-    NUSIF_TIMED("unet_conv", (kConvLds<uint16_t, 2><<<grid(2), 256, 0, st>>>(a, m->split)));
-  else
-    NUSIF_TIMED("unet_conv", (kConvLds<uint16_t, 1><<<grid(1), 256, 0, st>>>(a, m->split)));
+  const int nb = rows >= 64 && bricks >= 512 ? 4 : (rows >= 32 && bricks >= 512 ? 2 : 1);
+  const int gb = (cin % CL_CC) / 8; /* groups of the partial chunk */
+  auto launch = [&](auto nbC, auto gbC) {
+    constexpr int NB = decltype(nbC)::value, GB = decltype(gbC)::value;
+    NUSIF_TIMED("unet_conv", (kConvLds<uint16_t, NB, GB><<<grid(NB), 256, 0, st>>>(a, m->split)));
+  };
+  auto withGb = [&](auto nbC) {
+    switch (gb) {
+    case 0: launch(nbC, std::integral_constant<int, 0>()); break;
+    case 1: launch(nbC, std::integral_constant<int, 1>()); break;
+    case 2: launch(nbC, std::integral_constant<int, 2>()); break;
+    default: launch(nbC, std::integral_constant<int, 3>()); break;
+    }
+  };
+  if (nb == 4) withGb(std::integral_constant<int, 4>());
+  else if (nb == 2) withGb(std::integral_constant<int, 2>());
+  else withGb(std::integral_constant<int, 1>());
   if (split > 1) {
     size_t total = (size_t)n * n * n * rows;
     unsigned nb  = (unsigned)std::min<size_t>((total + 255) / 256, 4096);
@@ -462,7 +480,7 @@ static void launchConvLds(UnetModel *m, const ConvArgs<uint16_t, uint16_t> &a)
 
 /* forward conv k: out = relu(conv(in0 ++ in1) + b) */
 static void convForward(UnetModel *m, int k, const uint16_t *in0, int c0, const uint16_t *in1, int c1,
-    uint16_t *out, int n)
+    uint16_t *out, int n, bool half0 = false)
 {
   const ConvL &c = m->conv[k];
   ConvArgs<uint16_t, uint16_t> a;
@@ -480,8 +498,11 @@ static void convForward(UnetModel *m, int k, const uint16_t *in0, int c0, const 
   a.tapSign = 1;
   a.relu = 1;
   a.accumulate = 0;
+  a.half0 = half0 ? 1 : 0;
   if ((c0 + c1) % 8 == 0)
     launchConvLds(m, a);
+  else if (half0)
+    NUSIF_FATAL("unet: an upsampled input needs channels in multiples of 8");
   else
     launchConv(a);
 }
@@ -521,7 +542,7 @@ static void convBackward(UnetModel *m, int k, const float *dOut, const uint16_t 
 }
 
 /* ---- forward / backward --------------------------------------------------- */
-void unetForward(UnetModel *m, const uint16_t *x)
+void unetForward(UnetModel *m, const uint16_t *x, bool forTraining)
 {
   int L = m->L;
   const uint16_t *cur = x;
@@ -542,8 +563,12 @@ void unetForward(UnetModel *m, const uint16_t *x)
   for (int l = L - 2; l >= 0; l--) {
     LevelBuf &b = m->lv[l];
     size_t nup  = cube(b.n) * m->ch[l + 1];
-    kUpFwd<<<blocksFor(nup), 256, 0, st>>>(prev, b.up, b.n, m->ch[l + 1]);
-    convForward(m, q++, b.up, m->ch[l + 1], m->lv[l].s, m->ch[l], b.da, b.n);
+    if (forTraining) { /* the backward pass reads the upsampled tensor */
+      kUpFwd<<<blocksFor(nup), 256, 0, st>>>(prev, b.up, b.n, m->ch[l + 1]);
+      convForward(m, q++, b.up, m->ch[l + 1], m->lv[l].s, m->ch[l], b.da, b.n);
+    } else { /* the conv reads the coarse tensor through its parent index */
+      convForward(m, q++, prev, m->ch[l + 1], m->lv[l].s, m->ch[l], b.da, b.n, true);
+    }
     convForward(m, q++, b.da, m->ch[l], nullptr, 0, b.d, b.n);
     prev = b.d;
   }

@@ -61,6 +61,8 @@ template <typename InT, typename OutT> struct ConvArgs {
   int tapSign;            /* +1 forward, -1 data gradient */
   int relu;               /* ReLU on the result */
   int accumulate;         /* out += result (FP32 outputs) */
+  int half0 = 0;          /* in0 is stored at extent n / 2 and read upsampled
+                             (nearest parent voxel); LDS forward conv only */
 };
 
 template <typename T> __device__ inline void storeOut(T *p, float v, int accumulate);
@@ -175,19 +177,11 @@ __device__ inline void convChunk(const ConvArgs<uint16_t, uint16_t> &a, uint16_t
     int row0, int x0, int y0, int z0, int wave, int lr, int lg, floatx4 (&acc)[NB][4])
 {
   const int tid = threadIdx.x, n = a.n;
-  for (int u = tid; u < CL_HV * G8; u += 256) {
-    int v = u / G8, q = u - v * G8;
-    int hx = v % CL_HX, hy = (v / CL_HX) % CL_HY, hz = v / (CL_HX * CL_HY);
-    int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
-    uint4 val = make_uint4(0, 0, 0, 0);
-    if (x >= 0 && x < n && y >= 0 && y < n && z >= 0 && z < n) {
-      size_t vv = ((size_t)z * n + y) * n + x;
-      int c     = cc0 + q * 8;
-      val = c < a.c0 ? *(const uint4 *)(a.in0 + vv * a.c0 + c) : *(const uint4 *)(a.in1 + vv * a.c1 + (c - a.c0));
-    }
-    *(uint4 *)&sIn[v * CL_CSTR + q * 8] = val;
-  }
-  constexpr int G = 27 * G8;
+  constexpr int G   = 27 * G8;
+  constexpr int NIT = (G + 3) / 4; /* K-steps of four (tap, group) pairs */
+  /* weight fragments run D K-steps ahead of the MFMAs in a register ring, so
+   * their global (L2) latency is paid once per chunk, not once per K-step */
+  constexpr int D = NB == 1 ? 8 : (NB == 2 ? 4 : 2);
   /* per-lane bases: the A row offsets and the LDS voxel of each block at tap
    * offset (-1, -1, -1); a tap adds (dz * HY + dy) * HX + dx */
   size_t aRow[NB];
@@ -209,19 +203,60 @@ __device__ inline void convChunk(const ConvArgs<uint16_t, uint16_t> &a, uint16_t
     for (int nb = 0; nb < NB; nb++)
       fa[nb] = (gv && rowOk[nb]) ? ld8(a.A + aRow[nb] + (size_t)t * cin + c8 * 8) : zero8();
   };
-  bf16x8 faNext[NB];
-  loadA(0, faNext);
+  bf16x8 ring[D][NB];
+#pragma unroll
+  for (int p = 0; p < D; p++)
+    if (p < NIT) loadA(4 * p, ring[p]);
+  /* stage the brick and its halo (this chunk's channels) into LDS: every
+   * thread issues all its global loads before its first LDS store, so the
+   * staging costs one memory latency, not one per element */
+  constexpr int NST = (CL_HV * G8 + 255) / 256;
+  constexpr int SB  = 6; /* loads in flight per thread (register budget) */
+#pragma unroll
+  for (int r0 = 0; r0 < NST; r0 += SB) {
+    uint4 val[SB];
+#pragma unroll
+    for (int r = 0; r < SB; r++) {
+      const int u = tid + 256 * (r0 + r);
+      val[r]      = make_uint4(0, 0, 0, 0);
+      if (r0 + r < NST && u < CL_HV * G8) {
+        int v = u / G8, q = u - v * G8;
+        int hx = v % CL_HX, hy = (v / CL_HX) % CL_HY, hz = v / (CL_HX * CL_HY);
+        int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
+        if (x >= 0 && x < n && y >= 0 && y < n && z >= 0 && z < n) {
+          size_t vv = ((size_t)z * n + y) * n + x;
+          int c     = cc0 + q * 8;
+          if (c < a.c0) {
+            const int h = n >> 1;
+            size_t v0   = a.half0 ? ((size_t)(z >> 1) * h + (y >> 1)) * h + (x >> 1) : vv;
+            val[r]      = *(const uint4 *)(a.in0 + v0 * a.c0 + c);
+          } else {
+            val[r] = *(const uint4 *)(a.in1 + vv * a.c1 + (c - a.c0));
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < SB; r++) {
+      const int u = tid + 256 * (r0 + r);
+      if (r0 + r < NST && u < CL_HV * G8) {
+        int v = u / G8, q = u - v * G8;
+        *(uint4 *)&sIn[v * CL_CSTR + q * 8] = val[r];
+      }
+    }
+  }
   __syncthreads();
-#pragma unroll 2
-  for (int s = 0; s < G; s += 4) {
-    int gi  = s + lg;
-    bool gv = gi < G;
+#pragma unroll
+  for (int it = 0; it < NIT; it++) {
+    const int s = 4 * it;
+    int gi      = s + lg;
+    bool gv     = gi < G;
     int t = gv ? gi / G8 : 0, c8 = gv ? gi % G8 : 0;
     int off = ((t / 9) * CL_HY + (t / 3) % 3) * CL_HX + t % 3;
     bf16x8 fa[NB], fb[4];
 #pragma unroll
-    for (int nb = 0; nb < NB; nb++) fa[nb] = faNext[nb];
-    if (s + 4 < G) loadA(s + 4, faNext);
+    for (int nb = 0; nb < NB; nb++) fa[nb] = ring[it % D][nb];
+    if (it + D < NIT) loadA(4 * (it + D), ring[it % D]);
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) {
       uint4 raw = *(const uint4 *)&sIn[(hvBase[mb] + off) * CL_CSTR + c8 * 8];
@@ -236,7 +271,23 @@ __device__ inline void convChunk(const ConvArgs<uint16_t, uint16_t> &a, uint16_t
   __syncthreads();
 }
 
-template <typename OutT, int NB>
+/* GB: 8-channel groups of the last, partial chunk (cin % 32 / 8; 0 when
+ * every chunk is full). Full chunks take four groups. Only the chunk shapes a
+ * layer has are instantiated, which keeps the register count (and so the
+ * number of workgroups per CU) at what one chunk loop needs. */
+template <int NB, int GB>
+__device__ inline void convTail(const ConvArgs<uint16_t, uint16_t> &a, uint16_t *sIn, int cc0, int cin, int rows,
+    int row0, int x0, int y0, int z0, int wave, int lr, int lg, floatx4 (&acc)[NB][4])
+{
+  if constexpr (GB == 3) { /* 24 channels: as groups of 2 and 1 */
+    convChunk<NB, 2>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
+    convChunk<NB, 1>(a, sIn, cc0 + 16, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
+  } else if constexpr (GB > 0) {
+    convChunk<NB, GB>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
+  }
+}
+
+template <typename OutT, int NB, int GB>
 __global__ __launch_bounds__(256) void kConvLds(ConvArgs<uint16_t, OutT> a, float *__restrict__ split)
 {
   __shared__ __attribute__((aligned(16))) uint16_t sIn[CL_HV * CL_CSTR];
@@ -259,20 +310,17 @@ __global__ __launch_bounds__(256) void kConvLds(ConvArgs<uint16_t, OutT> a, floa
   /* split-K: with gridDim.z > 1, workgroup z takes channel chunk z only and
    * writes its FP32 partial sums to split[z][vox][rows] (kConvSplitFinish
    * adds the chunks in order) */
-  const int cBeg = gridDim.z > 1 ? blockIdx.z * CL_CC : 0;
-  const int cEnd = gridDim.z > 1 ? min(cBeg + CL_CC, cin) : cin;
-  for (int cc0 = cBeg; cc0 < cEnd; cc0 += CL_CC) {
-    const int g8 = min(CL_CC, cin - cc0) >> 3; /* 8-channel groups in this chunk */
-    if (g8 == 4)
+  const int nFull = cin / CL_CC;
+  if (gridDim.z > 1) {
+    const int cc0 = blockIdx.z * CL_CC;
+    if ((int)blockIdx.z < nFull)
       convChunk<NB, 4>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
-    else if (g8 == 2)
-      convChunk<NB, 2>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
-    else if (g8 == 1)
-      convChunk<NB, 1>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
-    else /* three groups (24-channel chunk): as two chunks of 2 and 1 */ {
-      convChunk<NB, 2>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
-      convChunk<NB, 1>(a, sIn, cc0 + 16, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
-    }
+    else
+      convTail<NB, GB>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
+  } else {
+    for (int q = 0; q < nFull; q++)
+      convChunk<NB, 4>(a, sIn, q * CL_CC, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
+    convTail<NB, GB>(a, sIn, nFull * CL_CC, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
   }
   const int x = x0 + lr, z = z0 + wave;
   if (x >= n || z >= n) return;

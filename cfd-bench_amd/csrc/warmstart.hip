@@ -552,7 +552,7 @@ extern "C" int nusifUnetTrainCavityGrids(const char *weightsOut, const int *grid
     double sum = 0.0;
     float lrE  = (float)(lr * 0.5 * (1.0 + cos(M_PI * e / (double)epochs))); /* cosine decay */
     for (size_t q = 0; q < order.size(); q++) {
-      unetForward(m, xs[order[q]]);
+      unetForward(m, xs[order[q]], true);
       bool last = (q + 1) % batch == 0 || q + 1 == order.size();
       sum += unetBackward(m, xs[order[q]], ts[order[q]], true);
       if (last) unetAdam(m, lrE, batch);
@@ -574,11 +574,9 @@ extern "C" int nusifUnetTrainCavityGrids(const char *weightsOut, const int *grid
 }
 
 /* ---- C ABI: one forward/backward for gradient checks ------------------------ */
-/* cfg = {L, ch0, ch1, ch2, ch3, cin}; x: n^3 x cin (channels-last), target n^3;
- * params in the oracle's flat layout. Returns the loss; grad and pred (may be
- * NULL) receive the gradient and the prediction. */
-extern "C" double nusifUnetLossGrad(const int *cfg, int n, const double *params, const double *x,
-    const double *target, double *grad, double *pred)
+/* a model with the oracle's flat parameters and x (n^3 x cin, channels-last)
+ * uploaded as bf16 */
+static UnetModel *checkModel(const int *cfg, int n, const double *params, const double *x, uint16_t **dx)
 {
   UnetModel *m = unetNew(cfg[0], cfg + 1, cfg[5], n);
   size_t np    = unetParamCount(m);
@@ -586,25 +584,42 @@ extern "C" double nusifUnetLossGrad(const int *cfg, int n, const double *params,
   for (size_t q = 0; q < np; q++) hp[q] = (float)params[q];
   unetSetParams(m, hp.data());
   unetSyncWeights(m);
-  unetZeroGrads(m);
   size_t nv = (size_t)n * n * n;
   std::vector<uint16_t> hx(nv * cfg[5]);
   for (size_t q = 0; q < hx.size(); q++) hx[q] = f2bf((float)x[q]);
+  HIP_CHECK(hipMalloc((void **)dx, hx.size() * sizeof(uint16_t)));
+  HIP_CHECK(hipMemcpy(*dx, hx.data(), hx.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+  return m;
+}
+
+static void readPrediction(UnetModel *m, int n, double *pred)
+{
+  size_t nv = (size_t)n * n * n;
+  std::vector<float> hy(nv);
+  /* the forward was queued on the (non-blocking) library stream */
+  HIP_CHECK(hipStreamSynchronize(nusifStream()));
+  HIP_CHECK(hipMemcpy(hy.data(), unetOutput(m), nv * sizeof(float), hipMemcpyDeviceToHost));
+  for (size_t q = 0; q < nv; q++) pred[q] = hy[q];
+}
+
+/* cfg = {L, ch0, ch1, ch2, ch3, cin}; x: n^3 x cin (channels-last), target n^3;
+ * params in the oracle's flat layout. Returns the loss; grad and pred (may be
+ * NULL) receive the gradient and the prediction. */
+extern "C" double nusifUnetLossGrad(const int *cfg, int n, const double *params, const double *x,
+    const double *target, double *grad, double *pred)
+{
+  uint16_t *dx = nullptr;
+  UnetModel *m = checkModel(cfg, n, params, x, &dx);
+  unetZeroGrads(m);
+  size_t nv = (size_t)n * n * n, np = unetParamCount(m);
   std::vector<float> ht(nv);
   for (size_t q = 0; q < nv; q++) ht[q] = (float)target[q];
-  uint16_t *dx = nullptr;
-  float *dt    = nullptr;
-  HIP_CHECK(hipMalloc((void **)&dx, hx.size() * sizeof(uint16_t)));
+  float *dt = nullptr;
   HIP_CHECK(hipMalloc((void **)&dt, nv * sizeof(float)));
-  HIP_CHECK(hipMemcpy(dx, hx.data(), hx.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(dt, ht.data(), nv * sizeof(float), hipMemcpyHostToDevice));
-  unetForward(m, dx);
+  unetForward(m, dx, true);
   double loss = unetBackward(m, dx, dt, true);
-  if (pred) {
-    std::vector<float> hy(nv);
-    HIP_CHECK(hipMemcpy(hy.data(), unetOutput(m), nv * sizeof(float), hipMemcpyDeviceToHost));
-    for (size_t q = 0; q < nv; q++) pred[q] = hy[q];
-  }
+  if (pred) readPrediction(m, n, pred);
   if (grad) {
     std::vector<float> hg(np);
     unetGetGrads(m, hg.data());
@@ -614,4 +629,16 @@ extern "C" double nusifUnetLossGrad(const int *cfg, int n, const double *params,
   HIP_CHECK(hipFree(dt));
   unetFree(m);
   return loss;
+}
+
+/* The inference forward (the one the warm start runs: decoder inputs read
+ * upsampled in place) on the same arguments; pred receives n^3 values. */
+extern "C" void nusifUnetPredict(const int *cfg, int n, const double *params, const double *x, double *pred)
+{
+  uint16_t *dx = nullptr;
+  UnetModel *m = checkModel(cfg, n, params, x, &dx);
+  unetForward(m, dx, false);
+  readPrediction(m, n, pred);
+  HIP_CHECK(hipFree(dx));
+  unetFree(m);
 }

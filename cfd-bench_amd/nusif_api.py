@@ -134,12 +134,14 @@ def _declare_optional(lib):
         "nusifKernelBytes": ([C.POINTER(Discretization), C.c_char_p], C.c_double),
         "nusifDeviceSynchronize": ([], None),
         "nusifCopyProbe": ([C.c_size_t, C.c_int], C.c_double),
+        "nusifStreamProbe": ([C.c_size_t, C.c_int, _DP], None),
         "nusifWarmstartUsed": ([C.POINTER(Solver)], C.c_int),
         "nusifWarmstartResiduals": ([C.POINTER(Solver), _DP], None),
         "nusifUnetTrainCavity": ([C.c_char_p] + [C.c_int] * 6 + [C.c_double, C.c_int, _DP], C.c_int),
         "nusifUnetTrainCavityGrids": ([C.c_char_p, C.POINTER(C.c_int)] + [C.c_int] * 7 +
                                       [C.c_double, C.c_int, _DP], C.c_int),
         "nusifUnetLossGrad": ([C.POINTER(C.c_int), C.c_int, _DP, _DP, _DP, _DP, _DP], C.c_double),
+        "nusifUnetPredict": ([C.POINTER(C.c_int), C.c_int, _DP, _DP, _DP], None),
         "nusifRunSteps": ([C.POINTER(Discretization), C.POINTER(Solver), C.c_int,
                            C.POINTER(C.c_double)], C.c_int),
         # oracle extras

@@ -122,6 +122,12 @@ double nusifKernelBytes(Discretization *d, const char *kernel);
  * achievable-bandwidth reference). */
 double nusifCopyProbe(size_t bytes, int reps);
 
+/* Milliseconds per launch of three hand-written streaming kernels over
+ * arrays of `bytes` bytes each, `reps` launches each: ms[0] a copy (one read,
+ * one write), ms[1] two reads and one write (a smoother sweep's pattern),
+ * ms[2] one read. The HBM rate a plain kernel reaches on this device. */
+void nusifStreamProbe(size_t bytes, int reps, double *ms);
+
 /* Trains the warm-start U-Net on the GPU from pressure-solve pairs the
  * solver generates itself: `runs` lid-driven cavities of grid^3 cells with
  * seeded Re ~ U[10, 1000] and lid speed ~ U[0.5, 2], `skipSteps` steps of
@@ -145,6 +151,10 @@ int nusifUnetTrainCavityGrids(const char *weightsOut, const int *grids, int ngri
  * the loss; grad and pred (may be NULL) receive the results. */
 double nusifUnetLossGrad(const int *cfg, int n, const double *params, const double *x,
     const double *target, double *grad, double *pred);
+/* The inference forward on the same arguments (the warm start's: decoder
+ * inputs read upsampled in place, nothing kept for a backward pass); pred
+ * receives n^3 values, bit-identical to nusifUnetLossGrad's. */
+void nusifUnetPredict(const int *cfg, int n, const double *params, const double *x, double *pred);
 
 /* Runs the reference time loop (src/main.c:60-88) for `nsteps` steps entirely
  * inside the library: the same calls in the same order, without the Python

@@ -50,6 +50,12 @@ def test_gpu_unet_forward_and_gradient_match_oracle(L, ch, n, gpu_ok):
     loss_orc = lib("orc_unet").orcUnetLoss(cfg, n, params.ctypes.data_as(DP), x.ctypes.data_as(DP),
                                           t.ctypes.data_as(DP), g_orc.ctypes.data_as(DP),
                                           p_orc.ctypes.data_as(DP), 1)
+    # the inference forward (decoder inputs upsampled inside the conv) gives
+    # the training forward's prediction bit for bit
+    p_inf = np.zeros((n, n, n))
+    lib("gpu_rb").nusifUnetPredict(cfg, n, params.ctypes.data_as(DP), x.ctypes.data_as(DP),
+                                   p_inf.ctypes.data_as(DP))
+    assert np.array_equal(p_inf, p_gpu)
     perr = np.linalg.norm(p_gpu - p_orc) / np.linalg.norm(p_orc)
     gerr = np.linalg.norm(g_gpu - g_orc) / np.linalg.norm(g_orc)
     cos = float(g_gpu @ g_orc / (np.linalg.norm(g_gpu) * np.linalg.norm(g_orc)))
