@@ -343,24 +343,41 @@ __global__ __launch_bounds__(256) void kComputeFG(const double *__restrict__ u,
   const int hi0 = tid % HW, hj0 = tid / HW, hi1 = e1 % HW, hj1 = has1 ? e1 / HW : 0;
   const long long o0 = (long long)min(by * TY + hj0, geo.nj + 1) * geo.sj + min(bx * TX + hi0, geo.ni + 1);
   const long long o1 = (long long)min(by * TY + hj1, geo.nj + 1) * geo.sj + min(bx * TX + hi1, geo.ni + 1);
-  auto stage = [&](int k, int s) {
-    long long base = (long long)k * geo.sk;
-    U[s][hj0][hi0] = u[base + o0];
-    V[s][hj0][hi0] = v[base + o0];
-    W[s][hj0][hi0] = w[base + o0];
+  /* plane k + 2 is loaded into registers while plane k is computed and
+   * stored to LDS at the next step, so no step waits on global memory */
+  double ru0 = 0.0, rv0 = 0.0, rw0 = 0.0, ru1 = 0.0, rv1 = 0.0, rw1 = 0.0;
+  auto fetch = [&](int k) {
+    long long base = (long long)min(k, geo.nk + 1) * geo.sk;
+    ru0 = u[base + o0];
+    rv0 = v[base + o0];
+    rw0 = w[base + o0];
     if (has1) {
-      U[s][hj1][hi1] = u[base + o1];
-      V[s][hj1][hi1] = v[base + o1];
-      W[s][hj1][hi1] = w[base + o1];
+      ru1 = u[base + o1];
+      rv1 = v[base + o1];
+      rw1 = w[base + o1];
     }
   };
-  stage(k0 - 1, (k0 - 1) % 3);
-  stage(k0, k0 % 3);
+  auto store = [&](int s) {
+    U[s][hj0][hi0] = ru0;
+    V[s][hj0][hi0] = rv0;
+    W[s][hj0][hi0] = rw0;
+    if (has1) {
+      U[s][hj1][hi1] = ru1;
+      V[s][hj1][hi1] = rv1;
+      W[s][hj1][hi1] = rw1;
+    }
+  };
+  fetch(k0 - 1);
+  store((k0 - 1) % 3);
+  fetch(k0);
+  store(k0 % 3);
+  fetch(k0 + 1);
   const bool in = i <= geo.ni && j <= geo.nj;
   const int X = tx + 1, Y = ty + 1;
   const double gam = cf.gamma;
   for (int k = k0; k < k1; k++) {
-    stage(k + 1, (k + 1) % 3);
+    store((k + 1) % 3);
+    if (k + 1 < k1) fetch(k + 2);
     __syncthreads();
     if (in) {
       const int m = (k + 2) % 3, c0 = k % 3, pz = (k + 1) % 3;

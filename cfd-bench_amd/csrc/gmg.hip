@@ -723,6 +723,9 @@ template <typename T> struct TailArgs {
   const T *tab;   /* DCT tables of the coarsest level */
   DctTables dct;
   const int *stop; /* skip the launch when *stop (speculative cycles) */
+  /* LDS residency (the kernel's RES): byte offsets of each level's x (RES
+   * >= 1) and b (RES == 2) in the dynamic LDS, and of the DCT workspace */
+  unsigned xo[MAXTAIL], bo[MAXTAIL], wo;
 };
 
 __device__ inline void tailSync() { __syncthreads(); }
@@ -854,35 +857,49 @@ template <typename T> __device__ void tailZero(T *__restrict__ x, long long n)
   for (long long q = threadIdx.x; q < n; q += TAIL_THREADS) x[q] = (T)0;
 }
 
-template <typename T>
+/* RES: which level arrays live in LDS for the whole cycle (0: none, the
+ * arrays stay in global memory; 1: every x; 2: every x and b). Every phase
+ * then reads and writes LDS instead of L2, and a phase costs an LDS round
+ * trip and a barrier, not an L2 one. The tail's first level takes its b
+ * from global memory (the restriction above wrote it) and gives its x back
+ * (the prolongation above reads it); the coarser levels exist only in LDS.
+ * The passes are the same code on other pointers: same values. */
+template <typename T, int RES>
 __global__ __launch_bounds__(TAIL_THREADS) void kTailCycle(TailArgs<T> a)
 {
   if (a.stop && *a.stop) return;
-  extern __shared__ unsigned char smem[];
-  for (int l = 0; l < a.n; l++) tailZero(a.x[l], a.g[l].n);
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  auto X = [&](int l) -> T * { return RES >= 1 ? (T *)(smem + a.xo[l]) : a.x[l]; };
+  auto B = [&](int l) -> T * { return RES == 2 ? (T *)(smem + a.bo[l]) : a.b[l]; };
+  unsigned char *work = RES >= 1 ? smem + a.wo : smem;
+  for (int l = 0; l < a.n; l++) tailZero(X(l), a.g[l].n);
+  if (RES == 2)
+    for (long long q = threadIdx.x; q < a.g[0].n; q += TAIL_THREADS) B(0)[q] = a.b[0][q];
   tailSync();
   for (int l = 0; l < a.n - 1; l++) {
     for (int s = 0; s < a.nu1; s++) {
-      tailPass(a.x[l], a.b[l], a.g[l], a.cf[l], 1, 0);
+      tailPass(X(l), B(l), a.g[l], a.cf[l], 1, 0);
       tailSync();
-      tailPass(a.x[l], a.b[l], a.g[l], a.cf[l], 0, 1);
+      tailPass(X(l), B(l), a.g[l], a.cf[l], 0, 1);
       tailSync();
     }
-    tailRestrict(a.x[l], a.b[l], a.g[l], a.cf[l], a.b[l + 1], a.g[l + 1]);
+    tailRestrict(X(l), B(l), a.g[l], a.cf[l], B(l + 1), a.g[l + 1]);
     tailSync();
   }
-  coarseDct(a.x[a.n - 1], a.b[a.n - 1], a.g[a.n - 1], a.tab, a.dct, smem);
+  coarseDct(X(a.n - 1), B(a.n - 1), a.g[a.n - 1], a.tab, a.dct, work);
   tailSync();
   for (int l = a.n - 2; l >= 0; l--) {
-    tailProlong(a.x[l], a.g[l], a.x[l + 1], a.g[l + 1]);
+    tailProlong(X(l), a.g[l], X(l + 1), a.g[l + 1]);
     tailSync();
     for (int s = 0; s < a.nu2; s++) {
-      tailPass(a.x[l], a.b[l], a.g[l], a.cf[l], 1, 0);
+      tailPass(X(l), B(l), a.g[l], a.cf[l], 1, 0);
       tailSync();
-      tailPass(a.x[l], a.b[l], a.g[l], a.cf[l], 0, 1);
+      tailPass(X(l), B(l), a.g[l], a.cf[l], 0, 1);
       tailSync();
     }
   }
+  if (RES >= 1) /* the first level's x (every padded cell) back to global */
+    for (long long q = threadIdx.x; q < a.g[0].n; q += TAIL_THREADS) a.x[0][q] = X(0)[q];
 }
 
 /* ---- FP64 finest-level kernels: norms, defect correction ---------------- */
@@ -928,17 +945,28 @@ __global__ __launch_bounds__(256) void kResidualLds(const double *__restrict__ p
     if constexpr (CORR) return p[q] + (double)e[q];
     else return p[q];
   };
-  auto stage = [&](int k, int slot) {
+  /* plane k + 2 is loaded into registers while plane k is computed and
+   * stored to LDS at the next step, so no step waits on global memory */
+  double r0 = 0.0, r1 = 0.0;
+  auto fetch = [&](int k) {
     long long base = (long long)k * g.sk;
-    X[slot][hj0][hi0] = value(base + o0);
-    if (has1) X[slot][hj1][hi1] = value(base + o1);
+    r0 = value(base + o0);
+    if (has1) r1 = value(base + o1);
   };
-  stage(k0 - 1, (k0 - 1) % 3);
-  stage(k0, k0 % 3);
+  auto store = [&](int slot) {
+    X[slot][hj0][hi0] = r0;
+    if (has1) X[slot][hj1][hi1] = r1;
+  };
+  fetch(k0 - 1);
+  store((k0 - 1) % 3);
+  fetch(k0);
+  store(k0 % 3);
+  fetch(k0 + 1);
   const bool in = i <= g.ni && j <= g.nj;
   double a0 = 0.0, a1 = 0.0;
   for (int k = k0; k < k1; k++) {
-    stage(k + 1, (k + 1) % 3);
+    store((k + 1) % 3);
+    if (k + 1 < k1) fetch(k + 2);
     __syncthreads();
     if (in) {
       const int sm = (k + 2) % 3, sc = k % 3, sp = (k + 1) % 3;
@@ -1161,8 +1189,30 @@ template <typename T> static void tailCycle(GmgState *G, LevelT<T> *L)
   a.dct  = G->dct;
   a.stop = gCycleStop;
   const Geo &gc = L[G->nl - 1].g;
-  size_t lds    = 2 * (size_t)gc.ni * gc.nj * gc.nk * sizeof(T);
-  kTailCycle<T><<<1, TAIL_THREADS, lds, nusifStream()>>>(a);
+  const size_t work = 2 * (size_t)gc.ni * gc.nj * gc.nk * sizeof(T);
+  /* LDS residency: x and b of every tail level, or x only, when it fits the
+   * 64 KiB a workgroup may take without an attribute; else none */
+  auto align16 = [](size_t v) { return (v + 15) & ~(size_t)15; };
+  size_t xs = 0, bs = 0;
+  for (int q = 0; q < a.n; q++) {
+    a.xo[q] = (unsigned)xs;
+    xs += align16((size_t)a.g[q].n * sizeof(T));
+  }
+  for (int q = 0; q < a.n; q++) {
+    a.bo[q] = (unsigned)(xs + bs);
+    bs += align16((size_t)a.g[q].n * sizeof(T));
+  }
+  const size_t cap = 64 * 1024;
+  if (xs + bs + work <= cap) {
+    a.wo = (unsigned)(xs + bs);
+    kTailCycle<T, 2><<<1, TAIL_THREADS, xs + bs + work, nusifStream()>>>(a);
+  } else if (xs + work <= cap) {
+    a.wo = (unsigned)xs;
+    kTailCycle<T, 1><<<1, TAIL_THREADS, xs + work, nusifStream()>>>(a);
+  } else {
+    a.wo = 0;
+    kTailCycle<T, 0><<<1, TAIL_THREADS, work, nusifStream()>>>(a);
+  }
 }
 
 /* `cont` (mixed mode, finest level): the cycle continues the correction e

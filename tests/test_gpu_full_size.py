@@ -1,0 +1,112 @@
+"""Parity at the benchmark sizes: the 256^3 lid-driven cavity (Re 1000, the
+workload bench.py times, with its solver options) stepped on the GPU against
+the oracle's step functions + the oracle's multigrid, and the 512 x 128 x 128
+channel (Re 100, inflow u = 1, outflow) in mixed precision against the
+GPU's FP64 path, whose parity with the oracle the smaller tests pin.
+
+Tolerances: FP64 GPU vs oracle <= 1e-10 relative max-norm per field (only
+the order of the norm sums differs); mixed precision (FP32 V-cycles inside
+FP64 defect correction, stopped at the same FP64 relative residual) <= 1e-6
+per field, printed per field."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import nusif_api as A
+from conftest import lib
+from test_gpu_gmg import oracle_solve
+
+pytestmark = pytest.mark.gpu
+
+
+def cavity256():
+    return dict(name="dcavity", re=1000.0, imax=256, jmax=256, kmax=256, tau=0.5, omg=1.9,
+                eps=1e-3, itermax=1000, gamma=0.9, te=1e9, levels=6, presmooth=2, postsmooth=2)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gpu_cavity256_steps_match_oracle(precision, gpu_ok):
+    """Three steps of the benchmark workload (gmg V(2,2), 6 levels, omega 1.3,
+    relative residual 1e-6, first stop test sized from the previous solve):
+    cycle counts equal step by step, fields within the stated bound."""
+    print("cavity 256^3", precision, flush=True)
+    prm = cavity256()
+    opts = {"solver": "gmg", "precision": precision, "mg_tol": 1e-6, "mg_omega": 1.3,
+            "mg_inner": 0}
+    g = A.Case(lib("gpu_rb"), options=opts, **prm)
+    o = A.Case(lib("orc_rb"), **prm)
+    n = int(np.prod(o.shape))
+    for step in range(3):
+        g.step()
+        d = o.d
+        o.lib.computeTimestep(C.byref(d))
+        for fn in ("setBoundaryConditions", "setSpecialBoundaryCondition", "computeFG",
+                   "computeRHS"):
+            o.call(fn)
+        if step == 0:
+            o.call("normalizePressure")
+        pv = np.ctypeslib.as_array(d.p, shape=(n,))
+        rv = np.ctypeslib.as_array(d.rhs, shape=(n,))
+        first = g.lib.nusifSolverFirstGroup(C.byref(g.s))
+        cyc, rel = oracle_solve(pv, rv, prm, opts, first)
+        gcyc = g.lib.nusifSolverIterations(C.byref(g.s))
+        grel = g.lib.nusifSolverRelResidual(C.byref(g.s))
+        print("step", step, "cycles", gcyc, cyc, "rel residual %.3e %.3e" % (grel, rel), flush=True)
+        assert gcyc == cyc, step
+        assert grel <= 1e-6
+        o.call("adaptUV")
+    fg, fo = g.fields(), o.fields()
+    for k in ("u", "v", "w", "p"):
+        err = A.rel_maxnorm(fg[k], fo[k])
+        print(precision, "cavity256", k, "rel err vs oracle %.3e" % err)
+        assert err <= (1e-10 if precision == "fp64" else 1e-6), k
+    g.close()
+    o.close()
+
+
+CHANNELS = {
+    # the channel configuration: 512 x 128 x 128, Re 100, uniform inflow
+    "channel512": dict(imax=512, jmax=128, kmax=128, re=100.0),
+    # the combined configuration at scale: 768 x 192 x 192, Re 500, inflow
+    # perturbed by a seeded 1% uniform noise
+    "channel768": dict(imax=768, jmax=192, kmax=192, re=500.0, inflow_noise=0.01, seed=7),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CHANNELS))
+def test_gpu_channel_full_size_mixed_vs_fp64(case, gpu_ok):
+    """The channel (domain 4 x 1 x 1, inflow u = 1 at x = 0, zero-gradient
+    outflow, no-slip walls) at full size, five steps solved to 1e-8 relative
+    residual (rhs mean removed) in FP64 and in mixed precision: every solve
+    reaches the tolerance, and the mixed fields stay within 1e-6 of the FP64
+    ones."""
+    c = dict(CHANNELS[case])
+    noise = {k: c.pop(k) for k in ("inflow_noise", "seed") if k in c}
+    prm = dict(name="canal", xlength=4.0, ylength=1.0, zlength=1.0, bcLeft=3, bcRight=3,
+               bcBottom=1, bcTop=1, bcFront=1, bcBack=1, u_init=1.0, tau=0.5, te=1e9,
+               gamma=0.9, levels=6, presmooth=2, postsmooth=2, **c)
+    fields = {}
+    for precision in ("fp64", "mixed"):
+        print(case, precision, flush=True)
+        opts = dict({"solver": "gmg", "precision": precision, "mg_tol": 1e-8, "mg_project": 1,
+                     "u_inflow": 1.0, "mg_omega": 1.3}, **noise)
+        g = A.Case(lib("gpu_rb"), options=opts, **prm)
+        for step in range(5):
+            g.step()
+            rel = g.lib.nusifSolverRelResidual(C.byref(g.s))
+            cyc = g.lib.nusifSolverIterations(C.byref(g.s))
+            print(precision, "step", step, "cycles", cyc, "rel residual %.3e" % rel, flush=True)
+            assert rel <= 1e-8
+        fields[precision] = g.fields()
+        g.close()
+    inflow = fields["mixed"]["u"][1:-1, 1:-1, 0]
+    if noise:
+        assert np.array_equal(inflow, fields["fp64"]["u"][1:-1, 1:-1, 0])
+        assert 0.0 < np.max(np.abs(inflow - 1.0)) <= 0.01
+    else:
+        assert np.all(inflow == 1.0)
+    for k in ("u", "v", "w", "p"):
+        err = A.rel_maxnorm(fields["mixed"][k], fields["fp64"][k])
+        print(case, "mixed", k, "rel err vs FP64 %.3e" % err)
+        assert err <= 1e-6, k
