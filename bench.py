@@ -89,10 +89,17 @@ def run_mode(lib, mode, n, warmup, steps, profile_steps=20):
         lib.nusifRunSteps(C.byref(case.d), C.byref(case.s), warmup, trace)
     log("[bench] %s: %d warmup steps done" % (mode, warmup))
     lib.nusifDeviceSynchronize()
+    tried0, taken0 = C.c_longlong(), C.c_longlong()
+    lib.nusifWarmstartCounts(C.byref(case.s), C.byref(tried0), C.byref(taken0))
     t0 = time.perf_counter()
     lib.nusifRunSteps(C.byref(case.d), C.byref(case.s), steps, trace)
     lib.nusifDeviceSynchronize()
     t1 = time.perf_counter()
+    tried, taken = C.c_longlong(), C.c_longlong()
+    lib.nusifWarmstartCounts(C.byref(case.s), C.byref(tried), C.byref(taken))
+    ws = None
+    if tried.value > tried0.value:
+        ws = (taken.value - taken0.value) / float(tried.value - tried0.value)
     cycles = [trace[4 * q + 3] for q in range(steps)]
     rels = [trace[4 * q + 2] for q in range(steps)]
     for t in TIMERS:
@@ -114,7 +121,8 @@ def run_mode(lib, mode, n, warmup, steps, profile_steps=20):
     # packets): the duration the roofline uses
     sweep_us = lib.nusifTimeFinestSweep(C.byref(case.s), 20)
     log("[bench] %s: %d steps in %.3f s" % (mode, steps, t1 - t0))
-    return case, dict(seconds=t1 - t0, cycles=cycles, rel=rels, kernels=kern, sweep_us=sweep_us)
+    return case, dict(seconds=t1 - t0, cycles=cycles, rel=rels, kernels=kern, sweep_us=sweep_us,
+                      warmstart_taken=ws)
 
 
 def cpu_baseline(case_gpu, n, budget_s=20.0):
@@ -202,6 +210,8 @@ def main():
         "cycles_per_step": sum(res["cycles"]) / len(res["cycles"]),
         "max_rel_residual": max(res["rel"]),
     }
+    if res["warmstart_taken"] is not None:
+        out["warmstart_guess_taken_fraction"] = res["warmstart_taken"]
     dom = DOMINANT[a.mode]
     k = res["kernels"].get(dom)
     if k and res["sweep_us"] > 0:

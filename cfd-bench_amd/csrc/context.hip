@@ -411,7 +411,8 @@ extern "C" double nusifKernelBytes(Discretization *d, const char *kernel)
       {"gmg_prolong32_l0", 8.5, 0},
       {"gmg_residual64", 16.0, 0},   /* rd p, rhs (+4 when the FP32 copy is written) */
       {"gmg_correct", 20.0, 0},      /* rd p, e32; wr p */
-      {"gmg_correct_residual", 36.0, 0}, /* rd p, e32, rhs; wr p', r32, e32 = 0 */
+      {"gmg_correct_residual", 32.0, 0}, /* rd p, e32, rhs; wr p', r32 (the zero-start
+                                            sweep needs no zeroed e) */
   };
   for (auto &e : table)
     if (strcmp(kernel, e.name) == 0) return e.perCell * (e.padded ? Npad : N);

@@ -1682,6 +1682,8 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
       sc->wsRes[2] = rc0;
       sc->wsRes[3] = rcg;
       sc->warmUsed = (c->opt.warmstart == 2 || rcg < rc0) ? 1 : 0;
+      sc->wsTried++;
+      sc->wsTaken += sc->warmUsed;
       if (sc->warmUsed) { /* start from the network's guess */
         double *t = c->p;
         c->p      = c->pAlt;

@@ -170,6 +170,7 @@ struct NusifSolverCtx {
   GmgState *gmg;
   WarmStart *ws; /* learned warm start / training-pair capture (gmg only) */
   double wsRes[4]; /* last warm start: fine ||r0||, fine ||r_guess||, restricted both */
+  long long wsTried, wsTaken; /* network guesses formed / accepted since init */
 };
 
 hipStream_t nusifStream(void);

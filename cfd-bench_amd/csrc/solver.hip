@@ -40,6 +40,8 @@ extern "C" void initSolver(Solver *s, Discretization *d, Parameter *p)
   sc->kind     = sc->dctx->opt.solver != SOLVER_DEFAULT ? sc->dctx->opt.solver : NUSIF_DEFAULT_SOLVER;
   sc->lastRel  = -1.0;
   sc->warmUsed = -1;
+  sc->wsTried  = 0;
+  sc->wsTaken  = 0;
   if (sc->kind == SOLVER_MG && commIsMaster(s->comm))
     printf("Using Multigrid solver with %d levels\n", s->levels);
   if (sc->kind == SOLVER_GMG) sc->gmg = gmgCreate(sc, p);
@@ -111,6 +113,13 @@ extern "C" double solve(Solver *s, double *p, const double *rhs)
 extern "C" int nusifSolverIterations(Solver *s) { return nusifSolverCtxOf(s)->lastIterations; }
 extern "C" double nusifSolverRelResidual(Solver *s) { return nusifSolverCtxOf(s)->lastRel; }
 extern "C" int nusifWarmstartUsed(Solver *s) { return nusifSolverCtxOf(s)->warmUsed; }
+
+extern "C" void nusifWarmstartCounts(Solver *s, long long *tried, long long *taken)
+{
+  NusifSolverCtx *sc = nusifSolverCtxOf(s);
+  *tried             = sc->wsTried;
+  *taken             = sc->wsTaken;
+}
 
 extern "C" void nusifWarmstartResiduals(Solver *s, double *out4)
 {

@@ -137,6 +137,8 @@ def _declare_optional(lib):
         "nusifStreamProbe": ([C.c_size_t, C.c_int, _DP], None),
         "nusifWarmstartUsed": ([C.POINTER(Solver)], C.c_int),
         "nusifWarmstartResiduals": ([C.POINTER(Solver), _DP], None),
+        "nusifWarmstartCounts": ([C.POINTER(Solver), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
+                                 None),
         "nusifUnetTrainCavity": ([C.c_char_p] + [C.c_int] * 6 + [C.c_double, C.c_int, _DP], C.c_int),
         "nusifUnetTrainCavityGrids": ([C.c_char_p, C.POINTER(C.c_int)] + [C.c_int] * 7 +
                                       [C.c_double, C.c_int, _DP], C.c_int),

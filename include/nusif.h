@@ -95,6 +95,9 @@ int nusifWarmstartUsed(Solver *s);
  * fine ||r_guess||, and both restricted to the network grid (the acceptance
  * test compares the restricted pair). */
 void nusifWarmstartResiduals(Solver *s, double *out4);
+/* Network guesses formed and accepted by the solves of `s` since
+ * initSolver (the acceptance rate of the warm start over a run). */
+void nusifWarmstartCounts(Solver *s, long long *tried, long long *taken);
 
 /* Name of the GPU the library runs on; returns the device count. */
 int nusifDeviceName(char *buf, int len);
