@@ -61,7 +61,7 @@ def mode_options(mode):
 DOMINANT = {"fp64": "gmg_smooth64_l0", "mixed": "gmg_smooth32_l0", "learned": "gmg_smooth32_l0"}
 TIMERS = ["gmg_smooth64_l0", "gmg_smooth32_l0", "gmg_smooth32z_l0", "gmg_restrict32_l0", "gmg_prolong32_l0",
           "gmg_restrict64_l0", "gmg_prolong64_l0", "gmg_residual64", "gmg_correct_residual",
-          "computeFG",
+          "computeFG", "computeFG_rhs",
           "computeRHS", "adaptUV", "absmax", "unet_forward", "unet_conv"]
 
 

@@ -110,6 +110,7 @@ void nusifDefaultOptions(NusifOptions *o)
   o->mgSpeculate = 0;
   o->mgInner     = 1;
   o->unetExtent  = 64;
+  o->fgRhs       = 1;
   o->lid         = 1.0;
   o->uInflow     = 2.0;
   o->inflowNoise = 0.0;
@@ -224,6 +225,7 @@ extern "C" int nusifSetOption(Parameter *p, const char *key, const char *value)
   if (k == "mg_tail") return parseInt(v.c_str(), &o->mgTail);
   if (k == "mg_graph") return parseInt(v.c_str(), &o->mgGraph);
   if (k == "unet_extent") return parseInt(v.c_str(), &o->unetExtent);
+  if (k == "fg_rhs") return parseInt(v.c_str(), &o->fgRhs);
   if (k == "mg_speculate") return parseInt(v.c_str(), &o->mgSpeculate);
   if (k == "mg_inner") {
     int r = parseInt(v.c_str(), &o->mgInner);
@@ -397,6 +399,7 @@ extern "C" double nusifKernelBytes(Discretization *d, const char *kernel)
   } table[] = {
       {"computeFG", 48.0, 0},        /* rd u v w, wr f g h */
       {"computeRHS", 32.0, 0},       /* rd f g h, wr rhs */
+      {"computeFG_rhs", 56.0, 0},    /* rd u v w, wr f g h rhs */
       {"adaptUV", 56.0, 0},          /* rd f g h p, wr u v w */
       {"absmax", 24.0, 1},           /* rd u v w, padded */
       {"rb_pass", 16.0, 0},          /* one colour: rd p (all) + rd rhs / wr p (half) */

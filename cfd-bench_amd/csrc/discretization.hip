@@ -312,6 +312,45 @@ struct FGCoef {
   double gamma, dt, iRe, iDx, iDy, iDz, gx, gy, gz;
 };
 
+/* F, G and H of one cell from its operands, in the reference's operand
+ * order (src/discretization.c:491-594). Names: X? = value at an offset of
+ * the cell: E/W = i+-1, N/S = j+-1, T/B = k+-1, two letters for a diagonal
+ * (uNW = U(i-1,j+1,k)). */
+__device__ inline double fCell(const FGCoef &cf, double Uc, double uE, double uW, double uN, double uS,
+    double uT, double uB, double Vc, double vE, double vS, double vSE, double Wc, double wE, double wB,
+    double wBE)
+{
+  double du2dx = donor(cf.iDx, cf.gamma, Uc + uE, Uc + uE, Uc - uE, Uc + uW, Uc + uW, Uc - uW);
+  double duvdy = donor(cf.iDy, cf.gamma, Vc + vE, Uc + uN, Uc - uN, vS + vSE, Uc + uS, Uc - uS);
+  double duwdz = donor(cf.iDz, cf.gamma, Wc + wE, Uc + uT, Uc - uT, wB + wBE, Uc + uB, Uc - uB);
+  double lapU  = d2(cf.iDx, uE, Uc, uW) + d2(cf.iDy, uN, Uc, uS) + d2(cf.iDz, uT, Uc, uB);
+  return Uc + cf.dt * (cf.iRe * lapU - du2dx - duvdy - duwdz + cf.gx);
+}
+
+/* dvwdz keeps the reference's V(i,j,k+1) in both halves of its k-1/2 flux
+ * (src/discretization.c:543-550) */
+__device__ inline double gCell(const FGCoef &cf, double Uc, double uN, double uW, double uNW, double Vc,
+    double vE, double vW, double vN, double vS, double vT, double vB, double Wc, double wN, double wB,
+    double wBN)
+{
+  double duvdx = donor(cf.iDx, cf.gamma, Uc + uN, Vc + vE, Vc - vE, uW + uNW, Vc + vW, Vc - vW);
+  double dv2dy = donor(cf.iDy, cf.gamma, Vc + vN, Vc + vN, Vc - vN, Vc + vS, Vc + vS, Vc - vS);
+  double dvwdz = donor(cf.iDz, cf.gamma, Wc + wN, Vc + vT, Vc - vT, wB + wBN, Vc + vT, Vc - vT);
+  double lapV  = d2(cf.iDx, vE, Vc, vW) + d2(cf.iDy, vN, Vc, vS) + d2(cf.iDz, vT, Vc, vB);
+  return Vc + cf.dt * (cf.iRe * lapV - duvdx - dv2dy - dvwdz + cf.gy);
+}
+
+__device__ inline double hCell(const FGCoef &cf, double Uc, double uT, double uW, double uTW, double Vc,
+    double vT, double vS, double vTS, double Wc, double wE, double wW, double wN, double wS, double wT,
+    double wB)
+{
+  double duwdx = donor(cf.iDx, cf.gamma, Uc + uT, Wc + wE, Wc - wE, uW + uTW, Wc + wW, Wc - wW);
+  double dvwdy = donor(cf.iDy, cf.gamma, Vc + vT, Wc + wN, Wc - wN, vTS + vS, Wc + wS, Wc - wS);
+  double dw2dz = donor(cf.iDz, cf.gamma, Wc + wT, Wc + wT, Wc - wT, Wc + wB, Wc + wB, Wc - wB);
+  double lapW  = d2(cf.iDx, wE, Wc, wW) + d2(cf.iDy, wN, Wc, wS) + d2(cf.iDz, wT, Wc, wB);
+  return Wc + cf.dt * (cf.iRe * lapW - duwdx - dvwdy - dw2dz + cf.gz);
+}
+
 /* A workgroup owns a 64 x 4 (i, j) tile and marches FG_KC planes in k. Per
  * plane, the tile's u, v, w plus a one-cell ring in i and j are staged in
  * LDS (three planes rotate), and every stencil operand of F, G and H is read
@@ -374,7 +413,6 @@ __global__ __launch_bounds__(256) void kComputeFG(const double *__restrict__ u,
   fetch(k0 + 1);
   const bool in = i <= geo.ni && j <= geo.nj;
   const int X = tx + 1, Y = ty + 1;
-  const double gam = cf.gamma;
   for (int k = k0; k < k1; k++) {
     store((k + 1) % 3);
     if (k + 1 < k1) fetch(k + 2);
@@ -389,34 +427,15 @@ __global__ __launch_bounds__(256) void kComputeFG(const double *__restrict__ u,
       double vT = V[pz][Y][X], vB = V[m][Y][X];
       double wE = W[c0][Y][X + 1], wW = W[c0][Y][X - 1], wN = W[c0][Y + 1][X], wS = W[c0][Y - 1][X];
       double wT = W[pz][Y][X], wB = W[m][Y][X];
-      double fval, gval, hval;
-      if (i == geo.ni) {
-        fval = Uc;
-      } else {
-        double du2dx = donor(cf.iDx, gam, Uc + uE, Uc + uE, Uc - uE, Uc + uW, Uc + uW, Uc - uW);
-        double duvdy = donor(cf.iDy, gam, Vc + vE, Uc + uN, Uc - uN, vS + V[c0][Y - 1][X + 1], Uc + uS, Uc - uS);
-        double duwdz = donor(cf.iDz, gam, Wc + wE, Uc + uT, Uc - uT, wB + W[m][Y][X + 1], Uc + uB, Uc - uB);
-        double lapU  = d2(cf.iDx, uE, Uc, uW) + d2(cf.iDy, uN, Uc, uS) + d2(cf.iDz, uT, Uc, uB);
-        fval         = Uc + cf.dt * (cf.iRe * lapU - du2dx - duvdy - duwdz + cf.gx);
-      }
-      if (j == geo.nj) {
-        gval = Vc;
-      } else {
-        double duvdx = donor(cf.iDx, gam, Uc + uN, Vc + vE, Vc - vE, uW + U[c0][Y + 1][X - 1], Vc + vW, Vc - vW);
-        double dv2dy = donor(cf.iDy, gam, Vc + vN, Vc + vN, Vc - vN, Vc + vS, Vc + vS, Vc - vS);
-        double dvwdz = donor(cf.iDz, gam, Wc + wN, Vc + vT, Vc - vT, wB + W[m][Y + 1][X], Vc + vT, Vc - vT);
-        double lapV  = d2(cf.iDx, vE, Vc, vW) + d2(cf.iDy, vN, Vc, vS) + d2(cf.iDz, vT, Vc, vB);
-        gval         = Vc + cf.dt * (cf.iRe * lapV - duvdx - dv2dy - dvwdz + cf.gy);
-      }
-      if (k == geo.nk) {
-        hval = Wc;
-      } else {
-        double duwdx = donor(cf.iDx, gam, Uc + uT, Wc + wE, Wc - wE, uW + U[pz][Y][X - 1], Wc + wW, Wc - wW);
-        double dvwdy = donor(cf.iDy, gam, Vc + vT, Wc + wN, Wc - wN, V[pz][Y - 1][X] + vS, Wc + wS, Wc - wS);
-        double dw2dz = donor(cf.iDz, gam, Wc + wT, Wc + wT, Wc - wT, Wc + wB, Wc + wB, Wc - wB);
-        double lapW  = d2(cf.iDx, wE, Wc, wW) + d2(cf.iDy, wN, Wc, wS) + d2(cf.iDz, wT, Wc, wB);
-        hval         = Wc + cf.dt * (cf.iRe * lapW - duwdx - dvwdy - dw2dz + cf.gz);
-      }
+      double fval = i == geo.ni ? Uc
+                                : fCell(cf, Uc, uE, uW, uN, uS, uT, uB, Vc, vE, vS, V[c0][Y - 1][X + 1], Wc, wE,
+                                      wB, W[m][Y][X + 1]);
+      double gval = j == geo.nj ? Vc
+                                : gCell(cf, Uc, uN, uW, U[c0][Y + 1][X - 1], Vc, vE, vW, vN, vS, vT, vB, Wc, wN,
+                                      wB, W[m][Y + 1][X]);
+      double hval = k == geo.nk ? Wc
+                                : hCell(cf, Uc, uT, uW, U[pz][Y][X - 1], Vc, vT, vS, V[pz][Y - 1][X], Wc, wE, wW,
+                                      wN, wS, wT, wB);
       f[c] = fval;
       g[c] = gval;
       h[c] = hval;
@@ -425,6 +444,159 @@ __global__ __launch_bounds__(256) void kComputeFG(const double *__restrict__ u,
       if (k == 1) h[c - geo.sk] = wB;
     }
     __syncthreads();
+  }
+}
+
+/* computeFG and computeRHS in one pass (src/discretization.c:334-362 on the
+ * F, G, H this kernel has just formed): rhs(i,j,k) needs F(i-1), G(j-1) and
+ * H(k-1). Inside the tile they come from the thread to the left (a wave
+ * shuffle), the row below (LDS) and the previous plane (a register). At the
+ * tile's low edges they are computed again here, with the same cell
+ * functions, hence the same bits as the neighbouring tile's: wave 0 forms
+ * G of the row below the tile, FR_TY lanes of wave 1 F of the column left
+ * of it, and every thread H of the plane below its k-chunk once. Each such halo
+ * cell needs one operand outside the staged ring (V(i,j0-2), U(i0-2,j),
+ * W(i,j,k0-2)), loaded from global memory. At the walls the halo value is
+ * the reference's face value (F(0) = U(0), G(i,0) = V(i,0), H(i,j,0) =
+ * W(i,j,0)). 64 x 4 tiles (256 threads, 148 VGPRs: three workgroups per
+ * CU): 251 us at 256^3 against 261 us for 64 x 8 tiles (one workgroup per
+ * CU at this register count) and 201 + 125 us for the two kernels. */
+#define FR_TY 4
+__global__ __launch_bounds__(TX * FR_TY) void kComputeFGRHS(const double *__restrict__ u,
+    const double *__restrict__ v, const double *__restrict__ w, double *__restrict__ f,
+    double *__restrict__ g, double *__restrict__ h, double *__restrict__ rhs, Geo geo, FGCoef cf, double idt,
+    int gridGx, int gridGy, int gridGz)
+{
+  constexpr int HW = TX + 2, HH = FR_TY + 2, HN = HW * HH, NT = TX * FR_TY;
+  __shared__ double U[3][HH][HW], V[3][HH][HW], W[3][HH][HW];
+  __shared__ double Gs[FR_TY + 1][TX]; /* G of rows j0-1 .. j0+FR_TY-1 of this plane */
+  __shared__ double Fh[FR_TY];         /* F(i0-1, j) of this plane */
+  int bx, by, bz;
+  if (!xcdBlock(gridGx, gridGy, gridGz, bx, by, bz)) return;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
+  const int i0 = bx * TX + 1, j0 = by * FR_TY + 1;
+  const int i = i0 + tx, j = j0 + ty;
+  const int k0 = bz * FG_KC + 1, k1 = min(k0 + FG_KC, geo.nk + 1);
+  /* ring elements: two at most (HN <= 2 NT) */
+  const int e1 = tid + NT;
+  const bool has1 = e1 < HN;
+  const int hi0 = tid % HW, hj0 = tid / HW, hi1 = e1 % HW, hj1 = has1 ? e1 / HW : 0;
+  const long long o0 = (long long)min(by * FR_TY + hj0, geo.nj + 1) * geo.sj + min(bx * TX + hi0, geo.ni + 1);
+  const long long o1 = (long long)min(by * FR_TY + hj1, geo.nj + 1) * geo.sj + min(bx * TX + hi1, geo.ni + 1);
+  /* the halo operands: V(i, j0-2) for wave 0, U(i0-2, j0+lane) for lanes
+   * 0..FR_TY-1 of wave 1 (clamped into the box; at a wall they are not used) */
+  const bool gHalo = ty == 0, fHalo = ty == 1 && tx < FR_TY;
+  const long long oHalo = gHalo ? (long long)max(j0 - 2, 0) * geo.sj + min(i, geo.ni + 1)
+                                : (long long)min(j0 + tx, geo.nj + 1) * geo.sj + max(i0 - 2, 0);
+  double ru0 = 0.0, rv0 = 0.0, rw0 = 0.0, ru1 = 0.0, rv1 = 0.0, rw1 = 0.0;
+  auto fetch = [&](int k) {
+    long long base = (long long)k * geo.sk;
+    ru0 = u[base + o0];
+    rv0 = v[base + o0];
+    rw0 = w[base + o0];
+    if (has1) {
+      ru1 = u[base + o1];
+      rv1 = v[base + o1];
+      rw1 = w[base + o1];
+    }
+  };
+  auto store = [&](int s) {
+    U[s][hj0][hi0] = ru0;
+    V[s][hj0][hi0] = rv0;
+    W[s][hj0][hi0] = rw0;
+    if (has1) {
+      U[s][hj1][hi1] = ru1;
+      V[s][hj1][hi1] = rv1;
+      W[s][hj1][hi1] = rw1;
+    }
+  };
+  auto fetchHalo = [&](int k) -> double {
+    if (gHalo) return v[(long long)k * geo.sk + oHalo];
+    if (fHalo) return u[(long long)k * geo.sk + oHalo];
+    return 0.0;
+  };
+  const bool in = i <= geo.ni && j <= geo.nj;
+  const int X = tx + 1, Y = ty + 1;
+  /* W(i, j, k0-2): the operand of the halo H outside the ring */
+  const double wBB = in ? w[geo.at(i, j, max(k0 - 2, 0))] : 0.0;
+  fetch(k0 - 1);
+  store((k0 - 1) % 3);
+  fetch(k0);
+  store(k0 % 3);
+  double haloCur = fetchHalo(k0);
+  fetch(k0 + 1);
+  __syncthreads();
+  /* H of the plane below the chunk (k0 - 1), as the thread that owns it
+   * computes it (or W there at the bottom wall) */
+  double hPrev = 0.0;
+  if (in) {
+    const int c0 = (k0 - 1) % 3, pz = k0 % 3;
+    const double Wc = W[c0][Y][X];
+    hPrev = k0 == 1 ? Wc
+                    : hCell(cf, U[c0][Y][X], U[pz][Y][X], U[c0][Y][X - 1], U[pz][Y][X - 1], V[c0][Y][X],
+                          V[pz][Y][X], V[c0][Y - 1][X], V[pz][Y - 1][X], Wc, W[c0][Y][X + 1], W[c0][Y][X - 1],
+                          W[c0][Y + 1][X], W[c0][Y - 1][X], W[pz][Y][X], wBB);
+  }
+  for (int k = k0; k < k1; k++) {
+    store((k + 1) % 3);
+    double haloNext = 0.0;
+    if (k + 1 < k1) {
+      fetch(k + 2);
+      haloNext = fetchHalo(k + 1);
+    }
+    __syncthreads();
+    const int m = (k + 2) % 3, c0 = k % 3, pz = (k + 1) % 3;
+    double fval = 0.0, gval = 0.0, hval = 0.0;
+    if (in) {
+      double Uc = U[c0][Y][X], Vc = V[c0][Y][X], Wc = W[c0][Y][X];
+      double uE = U[c0][Y][X + 1], uW = U[c0][Y][X - 1], uN = U[c0][Y + 1][X], uS = U[c0][Y - 1][X];
+      double uT = U[pz][Y][X], uB = U[m][Y][X];
+      double vE = V[c0][Y][X + 1], vW = V[c0][Y][X - 1], vN = V[c0][Y + 1][X], vS = V[c0][Y - 1][X];
+      double vT = V[pz][Y][X], vB = V[m][Y][X];
+      double wE = W[c0][Y][X + 1], wW = W[c0][Y][X - 1], wN = W[c0][Y + 1][X], wS = W[c0][Y - 1][X];
+      double wT = W[pz][Y][X], wB = W[m][Y][X];
+      fval = i == geo.ni ? Uc
+                         : fCell(cf, Uc, uE, uW, uN, uS, uT, uB, Vc, vE, vS, V[c0][Y - 1][X + 1], Wc, wE, wB,
+                               W[m][Y][X + 1]);
+      gval = j == geo.nj ? Vc
+                         : gCell(cf, Uc, uN, uW, U[c0][Y + 1][X - 1], Vc, vE, vW, vN, vS, vT, vB, Wc, wN, wB,
+                               W[m][Y + 1][X]);
+      hval = k == geo.nk ? Wc
+                         : hCell(cf, Uc, uT, uW, U[pz][Y][X - 1], Vc, vT, vS, V[pz][Y - 1][X], Wc, wE, wW, wN,
+                               wS, wT, wB);
+      const long long c = geo.at(i, j, k);
+      f[c] = fval;
+      g[c] = gval;
+      h[c] = hval;
+      if (i == 1) f[c - 1] = uW;
+      if (j == 1) g[c - geo.sj] = vS;
+      if (k == 1) h[c - geo.sk] = wB;
+    }
+    Gs[ty + 1][tx] = gval;
+    if (gHalo && i <= geo.ni) { /* G(i, j0-1): row 0 of the ring */
+      const int Yh = 0;
+      Gs[0][tx] = j0 == 1 ? V[c0][Yh][X]
+                          : gCell(cf, U[c0][Yh][X], U[c0][Yh + 1][X], U[c0][Yh][X - 1], U[c0][Yh + 1][X - 1],
+                                V[c0][Yh][X], V[c0][Yh][X + 1], V[c0][Yh][X - 1], V[c0][Yh + 1][X], haloCur,
+                                V[pz][Yh][X], V[m][Yh][X], W[c0][Yh][X], W[c0][Yh + 1][X], W[m][Yh][X],
+                                W[m][Yh + 1][X]);
+    }
+    if (fHalo && j0 + tx <= geo.nj) { /* F(i0-1, j0+lane): column 0 of the ring */
+      const int Yh = tx + 1;
+      Fh[tx] = i0 == 1 ? U[c0][Yh][0]
+                       : fCell(cf, U[c0][Yh][0], U[c0][Yh][1], haloCur, U[c0][Yh + 1][0], U[c0][Yh - 1][0],
+                             U[pz][Yh][0], U[m][Yh][0], V[c0][Yh][0], V[c0][Yh][1], V[c0][Yh - 1][0],
+                             V[c0][Yh - 1][1], W[c0][Yh][0], W[c0][Yh][1], W[m][Yh][0], W[m][Yh][1]);
+    }
+    const double fLeft = __shfl_up(fval, 1, 64);
+    __syncthreads();
+    if (in) {
+      const double fPrev = tx > 0 ? fLeft : Fh[ty];
+      const double gPrev = Gs[ty][tx];
+      rhs[geo.at(i, j, k)] = ((fval - fPrev) * cf.iDx + (gval - gPrev) * cf.iDy + (hval - hPrev) * cf.iDz) * idt;
+    }
+    hPrev   = hval;
+    haloCur = haloNext;
   }
 }
 
@@ -441,10 +613,24 @@ void launchComputeFG(NusifCtx *c, const Discretization *d)
   cf.gy    = d->gy;
   cf.gz    = d->gz;
   const Geo &g = c->geo;
+  if (c->opt.fgRhs) {
+    /* rhs from the new F, G, H in the same pass, into the spare rhs
+     * buffer: computeRHS makes it the rhs (a pointer swap) while f, g, h
+     * and dt stay as they are; until then rhs keeps its old values, as the
+     * reference's computeFG leaves them */
+    dim3 blk(TX, FR_TY);
+    dim3 grd((g.ni + TX - 1) / TX, (g.nj + FR_TY - 1) / FR_TY, (g.nk + FG_KC - 1) / FG_KC);
+    NUSIF_TIMED("computeFG_rhs", (kComputeFGRHS<<<xcdGrid(grd), blk, 0, nusifStream()>>>(c->u, c->v, c->w,
+                                     c->f, c->g, c->h, c->rhsNext, g, cf, 1.0 / d->dt, grd.x, grd.y, grd.z)));
+    c->rhsReady = true;
+    c->rhsDt    = d->dt;
+    return;
+  }
   dim3 blk(TX, TY);
   dim3 grd((g.ni + TX - 1) / TX, (g.nj + TY - 1) / TY, (g.nk + FG_KC - 1) / FG_KC);
   NUSIF_TIMED("computeFG", (kComputeFG<<<xcdGrid(grd), blk, 0, nusifStream()>>>(
                                 c->u, c->v, c->w, c->f, c->g, c->h, g, cf, grd.x, grd.y, grd.z)));
+  c->rhsReady = false;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -467,6 +653,14 @@ __global__ __launch_bounds__(256) void kComputeRHS(const double *__restrict__ f,
 
 void launchComputeRHS(NusifCtx *c, const Discretization *d)
 {
+  if (c->rhsReady && c->rhsDt == d->dt) { /* computeFG formed it from these f, g, h */
+    double *t   = c->rhs;
+    c->rhs      = c->rhsNext;
+    c->rhsNext  = t;
+    c->rhsReady = false;
+    return;
+  }
+  c->rhsReady  = false;
   const Geo &g = c->geo;
   dim3 blk(TX, TY);
   dim3 grd((g.ni + TX - 1) / TX, (g.nj + TY - 1) / TY, g.nk);
@@ -691,6 +885,7 @@ extern "C" void initDiscretization(Discretization *s, Parameter *params)
   c->v    = deviceField(n);
   c->w    = deviceField(n);
   c->pAlt = deviceField(n);
+  c->rhsNext = deviceField(n);
   c->nPartials = 65536;
   HIP_CHECK(hipMalloc((void **)&c->partials, c->nPartials * sizeof(double)));
   {
@@ -699,6 +894,8 @@ extern "C" void initDiscretization(Discretization *s, Parameter *params)
     HIP_CHECK(hipMalloc((void **)&c->cflPartials, 3 * ((size_t)c->cflInteriorBlocks + SHELL_BLOCKS) * sizeof(double)));
     c->cflValid = false;
   }
+  c->rhsReady = false;
+  c->rhsDt    = 0.0;
   HIP_CHECK(hipMalloc((void **)&c->scal, sizeof(DevScalars)));
   HIP_CHECK(hipHostMalloc((void **)&c->scalHost, sizeof(DevScalars), hipHostMallocCoherent));
   HIP_CHECK(hipHostGetDevicePointer((void **)&c->scalHostDev, c->scalHost, 0));
@@ -733,14 +930,15 @@ extern "C" void nusifSyncDevice(Discretization *d)
     HIP_CHECK(hipMemcpyAsync(dev[q], c->host[q], bytes, hipMemcpyHostToDevice, nusifStream()));
   HIP_CHECK(hipStreamSynchronize(nusifStream()));
   c->cflValid = false;
+  c->rhsReady = false;
 }
 
 extern "C" void nusifFreeDiscretization(Discretization *d)
 {
   NusifCtx *c = nusifCtxOf(d);
   HIP_CHECK(hipStreamSynchronize(nusifStream()));
-  double *dev[9] = {c->p, c->rhs, c->f, c->g, c->h, c->u, c->v, c->w, c->pAlt};
-  for (int q = 0; q < 9; q++) HIP_CHECK(hipFree(dev[q]));
+  double *dev[10] = {c->p, c->rhs, c->f, c->g, c->h, c->u, c->v, c->w, c->pAlt, c->rhsNext};
+  for (int q = 0; q < 10; q++) HIP_CHECK(hipFree(dev[q]));
   HIP_CHECK(hipFree(c->partials));
   HIP_CHECK(hipFree(c->cflPartials));
   HIP_CHECK(hipFree(c->scal));

@@ -1418,6 +1418,9 @@ static std::vector<void *> cycleState(NusifCtx *c, GmgState *G)
       v.push_back(G->L64[l].xAlt);
     }
   }
+  /* the rhs buffer (computeRHS may swap it): part of a captured graph's
+   * arguments, so part of its key; setCycleState does not read it */
+  v.push_back(c->rhs);
   return v;
 }
 

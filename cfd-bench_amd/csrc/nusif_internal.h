@@ -113,6 +113,7 @@ struct NusifOptions {
   int warmstart;
   char unetWeights[512];
   int unetExtent;  /* warm start: the network runs on the first level of extent <= this (64) */
+  int fgRhs;       /* computeFG also forms rhs (1, default), computeRHS then skips its pass */
 };
 void nusifDefaultOptions(NusifOptions *o);
 /* Options registered for `p` (defaults when none were set). */
@@ -141,6 +142,7 @@ struct NusifCtx {
   Geo geo;
   double *p, *rhs, *f, *g, *h, *u, *v, *w; /* device fields */
   double *pAlt;        /* ping-pong partner of p for the fused RB kernel */
+  double *rhsNext;     /* the rhs computeFG forms (fg_rhs); computeRHS swaps it in */
   double *partials;    /* reduction partials */
   size_t nPartials;
   DevScalars *scal;    /* device */
@@ -153,6 +155,10 @@ struct NusifCtx {
   double *cflPartials;
   int cflInteriorBlocks;
   bool cflValid;
+  /* computeFG with fg_rhs also wrote rhsNext (from the f, g, h it wrote,
+   * with dt rhsDt); cleared by anything else that writes f, g, h or rhs */
+  bool rhsReady;
+  double rhsDt;
   NusifOptions opt;
   double t;            /* nusifRunSteps clock */
   int nt;

@@ -79,6 +79,7 @@ extern "C" double solve(Solver *s, double *p, const double *rhs)
 {
   NusifSolverCtx *sc = nusifSolverCtxOf(s);
   NusifCtx *c        = sc->dctx;
+  c->rhsReady        = false; /* a solve may rewrite rhs (staging, mean removal) */
   int ip = -1, ir = -1;
   NusifCtx *cp = nusifCtxOfHost(p, &ip);
   NusifCtx *cr = nusifCtxOfHost(rhs, &ir);

@@ -74,6 +74,11 @@ void nusifFreeSolver(Solver *s);
  *   unet_extent   int     the network runs on the first multigrid level whose
  *                         extents are all <= this (64); a weights file that
  *                         records its training extent overrides it
+ *   fg_rhs        0 | 1   computeFG also forms the right-hand side in the same
+ *                         pass, into a spare buffer that computeRHS then swaps
+ *                         in (1, default; computeRHS recomputes when f, g, h,
+ *                         rhs or dt changed in between); 0: computeRHS runs
+ *                         its own pass. Same values either way
  * Returns 0 on success, -1 for an unknown key, -2 for a bad value. */
 int nusifSetOption(Parameter *p, const char *key, const char *value);
 

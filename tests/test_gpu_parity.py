@@ -152,3 +152,36 @@ def test_gpu_canal_inflow_option(gpu_ok):
         assert A.rel_maxnorm(fg[k], fo[k]) <= FIELD_RTOL, k
     g.close()
     o.close()
+
+
+@pytest.mark.parametrize("shape", [(33, 17, 9), (64, 40, 36)])
+def test_gpu_fused_rhs_equals_separate_pass(shape, gpu_ok):
+    """computeFG forms rhs in its own pass (option fg_rhs, default on) and
+    computeRHS publishes it: every field after boundary conditions, computeFG
+    and computeRHS has the bits of the separate computeRHS pass (fg_rhs 0),
+    on shapes with partial tiles in i, j and k and random fields. computeFG
+    alone leaves rhs as it was, and a dt changed between the two calls makes
+    computeRHS recompute with it."""
+    print("fused rhs", shape, flush=True)
+    ni, nj, nk = shape
+    prm = dict(G.KAT_PARAMS, imax=ni, jmax=nj, kmax=nk, bcLeft=3, bcRight=2, bcBottom=1,
+               bcTop=1, bcFront=2, bcBack=3)
+    runs = {}
+    for fused in (1, 0):
+        c = A.Case(lib("gpu_rb"), options={"fg_rhs": fused}, **prm)
+        c.set_fields(**G.random_fields(c.shape, 5))
+        before = c.fields()["rhs"]
+        for fn in ("setBoundaryConditions", "setSpecialBoundaryCondition", "computeFG"):
+            c.call(fn)
+        assert np.array_equal(c.fields()["rhs"], before)
+        c.call("computeRHS")
+        first = c.fields()
+        # computeFG, then a new dt before computeRHS: the rhs must use it
+        c.call("computeFG")
+        c.d.dt = c.d.dt * 0.5
+        c.call("computeRHS")
+        runs[fused] = (first, c.fields())
+        c.close()
+    for q in range(2):
+        for k in A.FIELDS:
+            np.testing.assert_array_equal(runs[1][q][k], runs[0][q][k], err_msg="%d %s" % (q, k))
