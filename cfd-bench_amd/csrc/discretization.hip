@@ -97,29 +97,32 @@ static int reductionBlocks(long long n)
 
 /* Maxima over the ghost shell of the padded box: the planes k = 0, nk+1,
  * the rows j = 0, nj+1 and the cells i = 0, ni+1 of the planes between. */
-#define SHELL_BLOCKS 256
+#define SHELL_BLOCKS 1024
+/* 32-bit index arithmetic (the shell of any grid this library allocates
+ * has far fewer than 2^31 cells; checked at launch): 64-bit divisions were
+ * most of this kernel's time */
 __global__ __launch_bounds__(256) void kShellMax3(const double *__restrict__ u,
     const double *__restrict__ v, const double *__restrict__ w, Geo g, double *__restrict__ partials)
 {
   __shared__ double lds[4];
-  const long long plane = g.sk, rowsPerPlane = 2LL * g.sj + 2LL * g.nj;
-  const long long total = 2 * plane + (long long)g.nk * rowsPerPlane;
+  const int plane = (int)g.sk, sj = g.sj, rowsPerPlane = 2 * g.sj + 2 * g.nj;
+  const int total = 2 * plane + g.nk * rowsPerPlane;
   double mu = DBL_MIN, mv = DBL_MIN, mw = DBL_MIN;
-  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
-       q += (long long)gridDim.x * blockDim.x) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
     long long c;
     if (q < 2 * plane) {
       c = q < plane ? q : (long long)(g.nk + 1) * g.sk + (q - plane);
     } else {
-      long long r = q - 2 * plane;
-      int k = (int)(r / rowsPerPlane) + 1;
-      long long e = r % rowsPerPlane;
-      if (e < 2LL * g.sj) {
-        int j = e < g.sj ? 0 : g.nj + 1;
-        c = g.at((int)(e % g.sj), j, k);
+      int r = q - 2 * plane;
+      int k = r / rowsPerPlane;
+      int e = r - k * rowsPerPlane;
+      k += 1;
+      if (e < 2 * sj) {
+        int j = e < sj ? 0 : g.nj + 1;
+        c = g.at(e < sj ? e : e - sj, j, k);
       } else {
-        long long f = e - 2LL * g.sj; /* cells i = 0 and ni+1 of rows 1..nj */
-        c = g.at(f % 2 ? g.ni + 1 : 0, (int)(f / 2) + 1, k);
+        int f = e - 2 * sj; /* cells i = 0 and ni+1 of rows 1..nj */
+        c = g.at((f & 1) ? g.ni + 1 : 0, (f >> 1) + 1, k);
       }
     }
     double a = fabs(u[c]), b = fabs(v[c]), d = fabs(w[c]);
@@ -893,6 +896,8 @@ extern "C" void initDiscretization(Discretization *s, Parameter *params)
     c->cflInteriorBlocks = (int)(ag.x * ag.y * ag.z);
     HIP_CHECK(hipMalloc((void **)&c->cflPartials, 3 * ((size_t)c->cflInteriorBlocks + SHELL_BLOCKS) * sizeof(double)));
     c->cflValid = false;
+    const long long shell = 2LL * c->geo.sk + (long long)c->geo.nk * (2LL * c->geo.sj + 2LL * c->geo.nj);
+    if (shell >= (1LL << 31)) NUSIF_FATAL("grid too large: %lld ghost-shell cells", shell);
   }
   c->rhsReady = false;
   c->rhsDt    = 0.0;
@@ -921,6 +926,16 @@ extern "C" void nusifSyncHost(Discretization *d)
   HIP_CHECK(hipStreamSynchronize(nusifStream()));
 }
 
+/* The spare rhs buffer mirrors rhs after every host upload: computeFG
+ * writes only its interior, so the ghost cells a computeRHS swap brings in
+ * are the ones rhs had (the reference's computeRHS leaves them alone). */
+void nusifRhsSpareSync(NusifCtx *c)
+{
+  HIP_CHECK(hipMemcpyAsync(c->rhsNext, c->rhs, (size_t)c->geo.n * sizeof(double), hipMemcpyDeviceToDevice,
+      nusifStream()));
+  c->rhsReady = false;
+}
+
 extern "C" void nusifSyncDevice(Discretization *d)
 {
   NusifCtx *c   = nusifCtxOf(d);
@@ -928,9 +943,9 @@ extern "C" void nusifSyncDevice(Discretization *d)
   double *dev[8] = {c->p, c->rhs, c->f, c->g, c->h, c->u, c->v, c->w};
   for (int q = 0; q < 8; q++)
     HIP_CHECK(hipMemcpyAsync(dev[q], c->host[q], bytes, hipMemcpyHostToDevice, nusifStream()));
+  nusifRhsSpareSync(c);
   HIP_CHECK(hipStreamSynchronize(nusifStream()));
   c->cflValid = false;
-  c->rhsReady = false;
 }
 
 extern "C" void nusifFreeDiscretization(Discretization *d)

@@ -726,6 +726,8 @@ template <typename T> struct TailArgs {
   /* LDS residency (the kernel's RES): byte offsets of each level's x (RES
    * >= 1) and b (RES == 2) in the dynamic LDS, and of the DCT workspace */
   unsigned xo[MAXTAIL], bo[MAXTAIL], wo;
+  int tabLds;   /* the DCT tables copied to LDS at offset `to` (RES >= 1) */
+  unsigned to;
 };
 
 __device__ inline void tailSync() { __syncthreads(); }
@@ -785,6 +787,85 @@ __device__ void tailPass(T *__restrict__ x, const T *__restrict__ b, const Geo &
         if (in) faceGhosts(x, g, ii[u], jj[u], kk[u], cc[u], v);
         if (io[u] <= g.ni) faceGhosts(x, g, io[u], jj[u], kk[u], co[u], xo[u]);
       }
+    }
+  }
+}
+
+/* A thread's cell pairs on one tail level, decomposed once for all the
+ * colour passes of a smoothing block (the decomposition's integer divisions
+ * by runtime extents were the larger part of a pass): pair q = tid + u *
+ * TAIL_THREADS holds cells i1 = 2 p + 1 and i1 + 1 of row (j, k). Valid when
+ * the level has at most TailB * TAIL_THREADS pairs. */
+template <typename T> struct TailPlan {
+  static constexpr int B = TailB<T>::value;
+  int c1[B]; /* cell index of (i1, j, k) */
+  int i1[B], j[B], k[B];
+  bool live[B];
+};
+
+template <typename T> __device__ bool tailPlan(const Geo &g, TailPlan<T> &P)
+{
+  constexpr int B = TailB<T>::value;
+  const int np = (g.ni + 1) >> 1, total = np * g.nj * g.nk;
+  if (total > B * TAIL_THREADS) return false;
+#pragma unroll
+  for (int u = 0; u < B; u++) {
+    int q     = threadIdx.x + u * TAIL_THREADS;
+    P.live[u] = q < total;
+    int qq    = P.live[u] ? q : 0;
+    int tx = qq % np, r = qq / np;
+    P.j[u]  = r % g.nj + 1;
+    P.k[u]  = r / g.nj + 1;
+    P.i1[u] = 2 * tx + 1;
+    P.c1[u] = (int)g.at(P.i1[u], P.j[u], P.k[u]);
+  }
+  return true;
+}
+
+/* tailPass on a plan: the same cells, operands and order */
+template <typename T>
+__device__ void tailPassPlan(T *__restrict__ x, const T *__restrict__ b, const Geo &g, const Coef<T> &cf,
+    int parity, int ghosts, const TailPlan<T> &P)
+{
+  constexpr int B = TailB<T>::value;
+  T xc[B], xm[B], xp[B], ym[B], yp[B], zm[B], zp[B], bb[B], xo[B];
+  long long cc[B], co[B];
+  int ii[B], io[B];
+#pragma unroll
+  for (int u = 0; u < B; u++) {
+    const int i1 = P.i1[u];
+    const int i  = (((1 + P.j[u] + P.k[u]) & 1) == parity) ? i1 : i1 + 1;
+    ii[u] = i;
+    io[u] = (i == i1) ? i1 + 1 : i1;
+    /* a pair's missing second cell (odd ni) reads a valid dummy cell */
+    long long c = P.c1[u] + (i <= g.ni ? i - i1 : 0);
+    cc[u] = c;
+    co[u] = P.c1[u] + (io[u] <= g.ni ? io[u] - i1 : 0);
+    xc[u] = x[c];
+    xm[u] = x[c - 1];
+    xp[u] = x[c + 1];
+    ym[u] = x[c - g.sj];
+    yp[u] = x[c + g.sj];
+    zm[u] = x[c - g.sk];
+    zp[u] = x[c + g.sk];
+    bb[u] = b[c];
+    xo[u] = x[co[u]];
+  }
+#pragma unroll
+  for (int u = 0; u < B; u++) {
+    if (!P.live[u]) continue;
+    bool in = ii[u] <= g.ni;
+    T v     = 0;
+    if (in) {
+      T lap = (xp[u] - (T)2 * xc[u] + xm[u]) * cf.idx2 + (yp[u] - (T)2 * xc[u] + ym[u]) * cf.idy2 +
+              (zp[u] - (T)2 * xc[u] + zm[u]) * cf.idz2;
+      T r   = bb[u] - lap;
+      v     = xc[u] - cf.factor * r;
+      x[cc[u]] = v;
+    }
+    if (ghosts) {
+      if (in) faceGhosts(x, g, ii[u], P.j[u], P.k[u], cc[u], v);
+      if (io[u] <= g.ni) faceGhosts(x, g, io[u], P.j[u], P.k[u], co[u], xo[u]);
     }
   }
 }
@@ -872,31 +953,40 @@ __global__ __launch_bounds__(TAIL_THREADS) void kTailCycle(TailArgs<T> a)
   auto X = [&](int l) -> T * { return RES >= 1 ? (T *)(smem + a.xo[l]) : a.x[l]; };
   auto B = [&](int l) -> T * { return RES == 2 ? (T *)(smem + a.bo[l]) : a.b[l]; };
   unsigned char *work = RES >= 1 ? smem + a.wo : smem;
+  const T *tab        = a.tab;
+  if (RES >= 1 && a.tabLds) {
+    T *t = (T *)(smem + a.to);
+    for (int q = threadIdx.x; q < a.dct.total; q += TAIL_THREADS) t[q] = a.tab[q];
+    tab = t;
+  }
   for (int l = 0; l < a.n; l++) tailZero(X(l), a.g[l].n);
   if (RES == 2)
     for (long long q = threadIdx.x; q < a.g[0].n; q += TAIL_THREADS) B(0)[q] = a.b[0][q];
   tailSync();
-  for (int l = 0; l < a.n - 1; l++) {
-    for (int s = 0; s < a.nu1; s++) {
-      tailPass(X(l), B(l), a.g[l], a.cf[l], 1, 0);
+  /* nu colour-pass pairs on level l, on a plan when the level allows one */
+  auto smoothLevel = [&](int l, int nu) {
+    TailPlan<T> P;
+    const bool planned = tailPlan(a.g[l], P);
+    for (int s = 0; s < nu; s++) {
+      if (planned) tailPassPlan(X(l), B(l), a.g[l], a.cf[l], 1, 0, P);
+      else tailPass(X(l), B(l), a.g[l], a.cf[l], 1, 0);
       tailSync();
-      tailPass(X(l), B(l), a.g[l], a.cf[l], 0, 1);
+      if (planned) tailPassPlan(X(l), B(l), a.g[l], a.cf[l], 0, 1, P);
+      else tailPass(X(l), B(l), a.g[l], a.cf[l], 0, 1);
       tailSync();
     }
+  };
+  for (int l = 0; l < a.n - 1; l++) {
+    smoothLevel(l, a.nu1);
     tailRestrict(X(l), B(l), a.g[l], a.cf[l], B(l + 1), a.g[l + 1]);
     tailSync();
   }
-  coarseDct(X(a.n - 1), B(a.n - 1), a.g[a.n - 1], a.tab, a.dct, work);
+  coarseDct(X(a.n - 1), B(a.n - 1), a.g[a.n - 1], tab, a.dct, work);
   tailSync();
   for (int l = a.n - 2; l >= 0; l--) {
     tailProlong(X(l), a.g[l], X(l + 1), a.g[l + 1]);
     tailSync();
-    for (int s = 0; s < a.nu2; s++) {
-      tailPass(X(l), B(l), a.g[l], a.cf[l], 1, 0);
-      tailSync();
-      tailPass(X(l), B(l), a.g[l], a.cf[l], 0, 1);
-      tailSync();
-    }
+    smoothLevel(l, a.nu2);
   }
   if (RES >= 1) /* the first level's x (every padded cell) back to global */
     for (long long q = threadIdx.x; q < a.g[0].n; q += TAIL_THREADS) a.x[0][q] = X(0)[q];
@@ -1171,6 +1261,21 @@ static void buildDctTables(GmgState *G, const Geo &g, double hx, double hy, doub
   HIP_CHECK(hipMemcpy(G->tab32, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
 }
 
+/* dynamic LDS a workgroup may take on this device (64 KiB if the query
+ * fails) */
+static size_t tailLdsCap()
+{
+  static size_t cap = 0;
+  if (!cap) {
+    int dev = 0, v = 0;
+    cap = 64 * 1024;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+      cap = (size_t)v;
+  }
+  return cap;
+}
+
 /* levels G->tail .. nl-1 in one workgroup */
 template <typename T> static void tailCycle(GmgState *G, LevelT<T> *L)
 {
@@ -1190,8 +1295,9 @@ template <typename T> static void tailCycle(GmgState *G, LevelT<T> *L)
   a.stop = gCycleStop;
   const Geo &gc = L[G->nl - 1].g;
   const size_t work = 2 * (size_t)gc.ni * gc.nj * gc.nk * sizeof(T);
-  /* LDS residency: x and b of every tail level, or x only, when it fits the
-   * 64 KiB a workgroup may take without an attribute; else none */
+  /* LDS residency: x and b of every tail level, or x only, when it fits
+   * what a workgroup may take on this device; else none. The DCT tables go
+   * to LDS too when there is room */
   auto align16 = [](size_t v) { return (v + 15) & ~(size_t)15; };
   size_t xs = 0, bs = 0;
   for (int q = 0; q < a.n; q++) {
@@ -1202,17 +1308,23 @@ template <typename T> static void tailCycle(GmgState *G, LevelT<T> *L)
     a.bo[q] = (unsigned)(xs + bs);
     bs += align16((size_t)a.g[q].n * sizeof(T));
   }
-  const size_t cap = 64 * 1024;
+  const size_t cap  = tailLdsCap();
+  const size_t tabs = align16((size_t)G->dct.total * sizeof(T));
   if (xs + bs + work <= cap) {
-    a.wo = (unsigned)(xs + bs);
-    kTailCycle<T, 2><<<1, TAIL_THREADS, xs + bs + work, nusifStream()>>>(a);
+    a.wo     = (unsigned)(xs + bs);
+    a.to     = (unsigned)(xs + bs + work);
+    a.tabLds = xs + bs + work + tabs <= cap;
+    kTailCycle<T, 2><<<1, TAIL_THREADS, xs + bs + work + (a.tabLds ? tabs : 0), nusifStream()>>>(a);
   } else if (xs + work <= cap) {
-    a.wo = (unsigned)xs;
-    kTailCycle<T, 1><<<1, TAIL_THREADS, xs + work, nusifStream()>>>(a);
+    a.wo     = (unsigned)xs;
+    a.to     = (unsigned)(xs + work);
+    a.tabLds = xs + work + tabs <= cap;
+    kTailCycle<T, 1><<<1, TAIL_THREADS, xs + work + (a.tabLds ? tabs : 0), nusifStream()>>>(a);
   } else {
     a.wo = 0;
     kTailCycle<T, 0><<<1, TAIL_THREADS, work, nusifStream()>>>(a);
   }
+  HIP_CHECK(hipGetLastError());
 }
 
 /* `cont` (mixed mode, finest level): the cycle continues the correction e

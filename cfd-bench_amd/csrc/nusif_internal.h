@@ -209,6 +209,8 @@ void launchAbsMax3(NusifCtx *c);
 void launchVelocityBC(NusifCtx *c, const Discretization *d);
 void launchSpecialBC(NusifCtx *c, const Discretization *d);
 void launchComputeFG(NusifCtx *c, const Discretization *d);
+/* copy rhs into the spare rhs buffer (after a host upload of rhs) */
+void nusifRhsSpareSync(NusifCtx *c);
 void launchComputeRHS(NusifCtx *c, const Discretization *d);
 void launchNormalize(NusifCtx *c);
 void launchAdaptUV(NusifCtx *c, const Discretization *d);

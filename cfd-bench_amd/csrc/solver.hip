@@ -89,6 +89,7 @@ extern "C" double solve(Solver *s, double *p, const double *rhs)
     /* foreign arrays: copy in, solve on the context's own p / rhs, copy out */
     HIP_CHECK(hipMemcpyAsync(c->p, p, bytes, hipMemcpyHostToDevice, nusifStream()));
     HIP_CHECK(hipMemcpyAsync(c->rhs, rhs, bytes, hipMemcpyHostToDevice, nusifStream()));
+    nusifRhsSpareSync(c);
   }
   double res = 0.0;
   switch (sc->kind) {
