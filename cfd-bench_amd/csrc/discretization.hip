@@ -51,8 +51,16 @@ __global__ __launch_bounds__(256) void kAbsMax3(const double *__restrict__ u,
   }
 }
 
+/* dt from the three maxima, with the operations of computeTimestep on the
+ * host (src/discretization.c:431-453): on = 0 leaves dt alone */
+struct DtArgs {
+  int on;
+  double dtBound, dx, dy, dz, tau;
+};
+
 /* scHost: device address of the pinned, host-coherent mirror (also written) */
-__global__ __launch_bounds__(1024) void kFoldMax3(const double *partials, int nb, DevScalars *sc, DevScalars *scHost)
+__global__ __launch_bounds__(1024) void kFoldMax3(const double *partials, int nb, DevScalars *sc, DevScalars *scHost,
+    DtArgs da)
 {
   __shared__ double lds[16]; /* one per wavefront */
   /* the three maxima in one pass, eight partials in flight per thread; a
@@ -77,12 +85,22 @@ __global__ __launch_bounds__(1024) void kFoldMax3(const double *partials, int nb
       double x = partials[3LL * q + c];
       m[c]     = m[c] > x ? m[c] : x;
     }
+  double r[3];
   for (int c = 0; c < 3; c++) {
-    double r = blockMax(m[c], lds);
+    r[c] = blockMax(m[c], lds);
     if (threadIdx.x == 0) {
-      sc->absmax[c]     = (unsigned long long)__double_as_longlong(r);
-      scHost->absmax[c] = (unsigned long long)__double_as_longlong(r);
+      sc->absmax[c]     = (unsigned long long)__double_as_longlong(r[c]);
+      scHost->absmax[c] = (unsigned long long)__double_as_longlong(r[c]);
     }
+  }
+  if (threadIdx.x == 0 && da.on) {
+    double dt = da.dtBound;
+    if (r[0] > 0) dt = (dt > da.dx / r[0]) ? da.dx / r[0] : dt;
+    if (r[1] > 0) dt = (dt > da.dy / r[1]) ? da.dy / r[1] : dt;
+    if (r[2] > 0) dt = (dt > da.dz / r[2]) ? da.dz / r[2] : dt;
+    dt         = dt * da.tau;
+    sc->dt     = dt;
+    scHost->dt = dt;
   }
   if (threadIdx.x == 0) __threadfence_system();
 }
@@ -143,19 +161,26 @@ __global__ __launch_bounds__(256) void kShellMax3(const double *__restrict__ u,
 /* max |u|, |v|, |w| over the padded arrays into scal->absmax. After an
  * adaptUV that nothing has overwritten since, only the ghost shell is read
  * (the interior-range maxima are adaptUV's); otherwise the whole arrays. */
-void launchAbsMax3(NusifCtx *c)
+void launchAbsMax3(NusifCtx *c, const Discretization *d)
 {
+  DtArgs da = {0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (d) da = {1, d->dtBound, d->grid.dx, d->grid.dy, d->grid.dz, d->tau};
   if (c->cflValid) {
     double *shell = c->cflPartials + 3LL * c->cflInteriorBlocks;
     kShellMax3<<<SHELL_BLOCKS, 256, 0, nusifStream()>>>(c->u, c->v, c->w, c->geo, shell);
-    kFoldMax3<<<1, 1024, 0, nusifStream()>>>(c->cflPartials, c->cflInteriorBlocks + SHELL_BLOCKS, c->scal, c->scalHostDev);
+    kFoldMax3<<<1, 1024, 0, nusifStream()>>>(c->cflPartials, c->cflInteriorBlocks + SHELL_BLOCKS, c->scal,
+        c->scalHostDev, da);
     return;
   }
   int nb = reductionBlocks(c->geo.n);
   NUSIF_TIMED("absmax", (kAbsMax3<<<nb, 256, 0, nusifStream()>>>(c->u, c->v, c->w, c->geo.n,
                             c->partials, c->scal)));
-  kFoldMax3<<<1, 1024, 0, nusifStream()>>>(c->partials, nb, c->scal, c->scalHostDev);
+  kFoldMax3<<<1, 1024, 0, nusifStream()>>>(c->partials, nb, c->scal, c->scalHostDev, da);
 }
+
+/* device address of dt while the loop keeps it on the device, else null
+ * (the kernels then take d->dt from their arguments) */
+static const double *dtDevice(NusifCtx *c) { return c->dtOnDevice ? &c->scal->dt : nullptr; }
 
 /* ------------------------------------------------------------------------ */
 /* Velocity boundary conditions (src/discretization.c:119-332)              */
@@ -369,8 +394,10 @@ __device__ inline double hCell(const FGCoef &cf, double Uc, double uT, double uW
 #define FG_KC 16
 __global__ __launch_bounds__(256) void kComputeFG(const double *__restrict__ u,
     const double *__restrict__ v, const double *__restrict__ w, double *__restrict__ f,
-    double *__restrict__ g, double *__restrict__ h, Geo geo, FGCoef cf, int gridGx, int gridGy, int gridGz)
+    double *__restrict__ g, double *__restrict__ h, Geo geo, FGCoef cf, int gridGx, int gridGy, int gridGz,
+    const double *__restrict__ dtDev)
 {
+  if (dtDev) cf.dt = *dtDev;
   constexpr int HW = TX + 2, HH = TY + 2, HN = HW * HH;
   __shared__ double U[3][HH][HW], V[3][HH][HW], W[3][HH][HW];
   int bx, by, bz;
@@ -468,8 +495,12 @@ __global__ __launch_bounds__(256) void kComputeFG(const double *__restrict__ u,
 __global__ __launch_bounds__(TX * FR_TY) void kComputeFGRHS(const double *__restrict__ u,
     const double *__restrict__ v, const double *__restrict__ w, double *__restrict__ f,
     double *__restrict__ g, double *__restrict__ h, double *__restrict__ rhs, Geo geo, FGCoef cf, double idt,
-    int gridGx, int gridGy, int gridGz)
+    int gridGx, int gridGy, int gridGz, const double *__restrict__ dtDev)
 {
+  if (dtDev) {
+    cf.dt = *dtDev;
+    idt   = 1.0 / cf.dt;
+  }
   constexpr int HW = TX + 2, HH = FR_TY + 2, HN = HW * HH, NT = TX * FR_TY;
   __shared__ double U[3][HH][HW], V[3][HH][HW], W[3][HH][HW];
   __shared__ double Gs[FR_TY + 1][TX]; /* G of rows j0-1 .. j0+FR_TY-1 of this plane */
@@ -624,7 +655,8 @@ void launchComputeFG(NusifCtx *c, const Discretization *d)
     dim3 blk(TX, FR_TY);
     dim3 grd((g.ni + TX - 1) / TX, (g.nj + FR_TY - 1) / FR_TY, (g.nk + FG_KC - 1) / FG_KC);
     NUSIF_TIMED("computeFG_rhs", (kComputeFGRHS<<<xcdGrid(grd), blk, 0, nusifStream()>>>(c->u, c->v, c->w,
-                                     c->f, c->g, c->h, c->rhsNext, g, cf, 1.0 / d->dt, grd.x, grd.y, grd.z)));
+                                     c->f, c->g, c->h, c->rhsNext, g, cf, 1.0 / d->dt, grd.x, grd.y, grd.z,
+                                     dtDevice(c))));
     c->rhsReady = true;
     c->rhsDt    = d->dt;
     return;
@@ -632,7 +664,7 @@ void launchComputeFG(NusifCtx *c, const Discretization *d)
   dim3 blk(TX, TY);
   dim3 grd((g.ni + TX - 1) / TX, (g.nj + TY - 1) / TY, (g.nk + FG_KC - 1) / FG_KC);
   NUSIF_TIMED("computeFG", (kComputeFG<<<xcdGrid(grd), blk, 0, nusifStream()>>>(
-                                c->u, c->v, c->w, c->f, c->g, c->h, g, cf, grd.x, grd.y, grd.z)));
+                                c->u, c->v, c->w, c->f, c->g, c->h, g, cf, grd.x, grd.y, grd.z, dtDevice(c))));
   c->rhsReady = false;
 }
 
@@ -641,8 +673,10 @@ void launchComputeFG(NusifCtx *c, const Discretization *d)
 /* ------------------------------------------------------------------------ */
 __global__ __launch_bounds__(256) void kComputeRHS(const double *__restrict__ f,
     const double *__restrict__ g, const double *__restrict__ h, double *__restrict__ rhs,
-    Geo geo, double idx, double idy, double idz, double idt, int gridGx, int gridGy)
+    Geo geo, double idx, double idy, double idz, double idt, int gridGx, int gridGy,
+    const double *__restrict__ dtDev)
 {
+  if (dtDev) idt = 1.0 / *dtDev;
   int bx, by, bz;
   if (!xcdBlock(gridGx, gridGy, geo.nk, bx, by, bz)) return;
   int i = bx * blockDim.x + threadIdx.x + 1;
@@ -656,7 +690,9 @@ __global__ __launch_bounds__(256) void kComputeRHS(const double *__restrict__ f,
 
 void launchComputeRHS(NusifCtx *c, const Discretization *d)
 {
-  if (c->rhsReady && c->rhsDt == d->dt) { /* computeFG formed it from these f, g, h */
+  /* computeFG formed it from these f, g, h (with the same dt: the device's,
+   * or the host's if it did not change) */
+  if (c->rhsReady && (c->dtOnDevice || c->rhsDt == d->dt)) {
     double *t   = c->rhs;
     c->rhs      = c->rhsNext;
     c->rhsNext  = t;
@@ -669,7 +705,7 @@ void launchComputeRHS(NusifCtx *c, const Discretization *d)
   dim3 grd((g.ni + TX - 1) / TX, (g.nj + TY - 1) / TY, g.nk);
   NUSIF_TIMED("computeRHS",
       (kComputeRHS<<<xcdGrid(grd), blk, 0, nusifStream()>>>(c->f, c->g, c->h, c->rhs, g,
-          1.0 / d->grid.dx, 1.0 / d->grid.dy, 1.0 / d->grid.dz, 1.0 / d->dt, grd.x, grd.y)));
+          1.0 / d->grid.dx, 1.0 / d->grid.dy, 1.0 / d->grid.dz, 1.0 / d->dt, grd.x, grd.y, dtDevice(c))));
 }
 
 /* ------------------------------------------------------------------------ */
@@ -733,8 +769,15 @@ void launchNormalize(NusifCtx *c)
 __global__ __launch_bounds__(256) void kAdaptUV(const double *__restrict__ f,
     const double *__restrict__ g, const double *__restrict__ h, const double *__restrict__ p,
     double *__restrict__ u, double *__restrict__ v, double *__restrict__ w, Geo geo, double fx,
-    double fy, double fz, double *__restrict__ partials, int gridGx, int gridGy, int gridGz)
+    double fy, double fz, double *__restrict__ partials, int gridGx, int gridGy, int gridGz,
+    const double *__restrict__ dtDev, double dx, double dy, double dz)
 {
+  if (dtDev) { /* dt / dx etc., as the host forms them */
+    const double dt = *dtDev;
+    fx              = dt / dx;
+    fy              = dt / dy;
+    fz              = dt / dz;
+  }
   __shared__ double lds[4];
   int bx, by, bz;
   if (!xcdBlock(gridGx, gridGy, gridGz, bx, by, bz)) return;
@@ -782,7 +825,8 @@ void launchAdaptUV(NusifCtx *c, const Discretization *d)
   dim3 grd = adaptGrid(g);
   NUSIF_TIMED("adaptUV", (kAdaptUV<<<xcdGrid(grd), blk, 0, nusifStream()>>>(c->f, c->g, c->h,
                              c->p, c->u, c->v, c->w, g, d->dt / d->grid.dx, d->dt / d->grid.dy,
-                             d->dt / d->grid.dz, c->cflPartials, grd.x, grd.y, grd.z)));
+                             d->dt / d->grid.dz, c->cflPartials, grd.x, grd.y, grd.z, dtDevice(c), d->grid.dx,
+                             d->grid.dy, d->grid.dz)));
   c->cflValid = true;
 }
 
@@ -901,6 +945,8 @@ extern "C" void initDiscretization(Discretization *s, Parameter *params)
   }
   c->rhsReady = false;
   c->rhsDt    = 0.0;
+  c->dtOnDevice = false;
+  HIP_CHECK(hipEventCreateWithFlags(&c->dtEvent, hipEventDisableTiming));
   HIP_CHECK(hipMalloc((void **)&c->scal, sizeof(DevScalars)));
   HIP_CHECK(hipHostMalloc((void **)&c->scalHost, sizeof(DevScalars), hipHostMallocCoherent));
   HIP_CHECK(hipHostGetDevicePointer((void **)&c->scalHostDev, c->scalHost, 0));
@@ -954,6 +1000,7 @@ extern "C" void nusifFreeDiscretization(Discretization *d)
   HIP_CHECK(hipStreamSynchronize(nusifStream()));
   double *dev[10] = {c->p, c->rhs, c->f, c->g, c->h, c->u, c->v, c->w, c->pAlt, c->rhsNext};
   for (int q = 0; q < 10; q++) HIP_CHECK(hipFree(dev[q]));
+  HIP_CHECK(hipEventDestroy(c->dtEvent));
   HIP_CHECK(hipFree(c->partials));
   HIP_CHECK(hipFree(c->cflPartials));
   HIP_CHECK(hipFree(c->scal));

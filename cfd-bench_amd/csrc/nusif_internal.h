@@ -129,6 +129,7 @@ struct DevScalars {
   int stopAt;                   /* iterations to run (stop test)  } layout  */
   unsigned int counter[4];      /* last-block counters (self-resetting) */
   double norm2[4];              /* gmg norms */
+  double dt;                    /* the step's dt, formed by kFoldMax3 (device dt) */
 };
 
 struct GmgState;  /* gmg.hip */
@@ -159,6 +160,11 @@ struct NusifCtx {
    * with dt rhsDt); cleared by anything else that writes f, g, h or rhs */
   bool rhsReady;
   double rhsDt;
+  /* while nusifRunSteps drives the loop: computeFG, computeRHS and adaptUV
+   * read dt from scal->dt (formed on the device by the CFL fold), so no
+   * step waits for a host round trip; dtEvent follows that fold */
+  bool dtOnDevice;
+  hipEvent_t dtEvent;
   NusifOptions opt;
   double t;            /* nusifRunSteps clock */
   int nt;
@@ -205,7 +211,13 @@ void nusifTimerEnd(KernelTimer *t);
   } while (0)
 
 /* Launchers (step.hip) */
-void launchAbsMax3(NusifCtx *c);
+/* CFL maxima into scal (and its pinned mirror); with d, also dt (the
+ * reference's arithmetic, src/discretization.c:431-453) into scal->dt */
+void launchAbsMax3(NusifCtx *c, const Discretization *d = nullptr);
+void launchVelocityBC(NusifCtx *c, const Discretization *d);
+void launchSpecialBC(NusifCtx *c, const Discretization *d);
+void launchNormalize(NusifCtx *c);
+void launchAdaptUV(NusifCtx *c, const Discretization *d);
 void launchVelocityBC(NusifCtx *c, const Discretization *d);
 void launchSpecialBC(NusifCtx *c, const Discretization *d);
 void launchComputeFG(NusifCtx *c, const Discretization *d);

@@ -133,8 +133,18 @@ extern "C" void nusifWarmstartResiduals(Solver *s, double *out4)
 extern "C" int nusifRunSteps(Discretization *d, Solver *s, int nsteps, double *trace)
 {
   NusifCtx *c = nusifCtxOf(d);
+  /* with an adaptive dt, the CFL fold forms dt on the device and computeFG,
+   * computeRHS and adaptUV read it there: the step is enqueued without
+   * waiting for the maxima. The host learns dt (same bits as
+   * computeTimestep's own arithmetic) once the fold has run, which the
+   * solve's first read-back has already waited for. */
+  const bool devDt = d->tau > 0.0;
   for (int q = 0; q < nsteps; q++) {
-    if (d->tau > 0.0) computeTimestep(d);
+    if (devDt) {
+      launchAbsMax3(c, d);
+      HIP_CHECK(hipEventRecord(c->dtEvent, nusifStream()));
+      c->dtOnDevice = true;
+    }
     setBoundaryConditions(d);
     setSpecialBoundaryCondition(d);
     computeFG(d);
@@ -142,6 +152,11 @@ extern "C" int nusifRunSteps(Discretization *d, Solver *s, int nsteps, double *t
     if (c->nt % 100 == 0) normalizePressure(d);
     double res = solve(s, d->p, d->rhs);
     adaptUV(d);
+    if (devDt) {
+      c->dtOnDevice = false;
+      HIP_CHECK(hipEventSynchronize(c->dtEvent));
+      d->dt = c->scalHost->dt;
+    }
     if (trace) {
       trace[4 * q + 0] = c->t;
       trace[4 * q + 1] = d->dt;
