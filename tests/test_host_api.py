@@ -237,3 +237,29 @@ def test_vtk_writer(fmt, tmp_path, monkeypatch):
     if os.path.exists(A.ref_lib_path("rb")):
         theirs = _write_vtk(_ref_lib(), "theirs", fmt, grid, s, u, v, w)
         assert ours == theirs
+
+
+def test_set_option_accepts_documented_keys():
+    """nusifSetOption (host code only) accepts every key include/nusif.h
+    documents, with a valid value, rejects bad values (-2) and unknown keys
+    (-1); nusifStepLimit reads nsteps back."""
+    lib = _gpu_lib()
+    doc = open(os.path.join(ROOT, "include", "nusif.h")).read()
+    block = doc[doc.index("Options beyond the reference"):doc.index("int nusifSetOption")]
+    keys = set(re.findall(r"^ \*   ([a-z_0-9]+)\s", block, flags=re.M))
+    valid = {"nsteps": "7", "solver": "gmg", "precision": "mixed", "mg_tol": "1e-8",
+             "mg_maxcycles": "50", "mg_omega": "1.3", "mg_project": "1", "mg_tail": "0",
+             "mg_graph": "0", "mg_tile_rows": "8", "mg_inner": "0", "mg_speculate": "0",
+             "lid": "1.5", "u_inflow": "1.0", "inflow_noise": "0.01", "seed": "3",
+             "warmstart": "none", "unet_weights": "/tmp/w.bin", "unet_extent": "32", "fg_rhs": "0"}
+    assert keys == set(valid), sorted(keys ^ set(valid))
+    prm = A.Parameter()
+    lib.initParameter(C.byref(prm))
+    for k, v in valid.items():
+        assert lib.nusifSetOption(C.byref(prm), k.encode(), v.encode()) == 0, k
+    assert lib.nusifStepLimit(C.byref(prm)) == 7
+    bad = {"solver": "cg", "precision": "fp16", "mg_tile_rows": "5", "mg_inner": "-1",
+           "warmstart": "maybe", "mg_tol": "tight"}
+    for k, v in bad.items():
+        assert lib.nusifSetOption(C.byref(prm), k.encode(), v.encode()) == -2, k
+    assert lib.nusifSetOption(C.byref(prm), b"no_such_key", b"1") == -1
