@@ -945,6 +945,8 @@ extern "C" void initDiscretization(Discretization *s, Parameter *params)
   }
   c->rhsReady = false;
   c->rhsDt    = 0.0;
+  c->saveP    = nullptr;
+  c->saveRhs  = nullptr;
   c->dtOnDevice = false;
   HIP_CHECK(hipEventCreateWithFlags(&c->dtEvent, hipEventDisableTiming));
   HIP_CHECK(hipMalloc((void **)&c->scal, sizeof(DevScalars)));
@@ -1001,6 +1003,8 @@ extern "C" void nusifFreeDiscretization(Discretization *d)
   double *dev[10] = {c->p, c->rhs, c->f, c->g, c->h, c->u, c->v, c->w, c->pAlt, c->rhsNext};
   for (int q = 0; q < 10; q++) HIP_CHECK(hipFree(dev[q]));
   HIP_CHECK(hipEventDestroy(c->dtEvent));
+  if (c->saveP) HIP_CHECK(hipFree(c->saveP));
+  if (c->saveRhs) HIP_CHECK(hipFree(c->saveRhs));
   HIP_CHECK(hipFree(c->partials));
   HIP_CHECK(hipFree(c->cflPartials));
   HIP_CHECK(hipFree(c->scal));

@@ -144,6 +144,8 @@ struct NusifCtx {
   double *p, *rhs, *f, *g, *h, *u, *v, *w; /* device fields */
   double *pAlt;        /* ping-pong partner of p for the fused RB kernel */
   double *rhsNext;     /* the rhs computeFG forms (fg_rhs); computeRHS swaps it in */
+  double *saveP, *saveRhs; /* p and rhs kept aside while solve() runs on foreign arrays
+                            * (allocated on first use) */
   double *partials;    /* reduction partials */
   size_t nPartials;
   DevScalars *scal;    /* device */

@@ -86,7 +86,15 @@ extern "C" double solve(Solver *s, double *p, const double *rhs)
   bool staged  = !(cp == c && ip == 0 && cr == c && ir == 1);
   size_t bytes = (size_t)c->geo.n * sizeof(double);
   if (staged) {
-    /* foreign arrays: copy in, solve on the context's own p / rhs, copy out */
+    /* foreign arrays: the context's own p and rhs are set aside, the foreign
+     * ones solved in their place and copied out, then p and rhs restored
+     * (the reference's solve touches only the arrays it is given) */
+    if (!c->saveP) {
+      HIP_CHECK(hipMalloc((void **)&c->saveP, bytes));
+      HIP_CHECK(hipMalloc((void **)&c->saveRhs, bytes));
+    }
+    HIP_CHECK(hipMemcpyAsync(c->saveP, c->p, bytes, hipMemcpyDeviceToDevice, nusifStream()));
+    HIP_CHECK(hipMemcpyAsync(c->saveRhs, c->rhs, bytes, hipMemcpyDeviceToDevice, nusifStream()));
     HIP_CHECK(hipMemcpyAsync(c->p, p, bytes, hipMemcpyHostToDevice, nusifStream()));
     HIP_CHECK(hipMemcpyAsync(c->rhs, rhs, bytes, hipMemcpyHostToDevice, nusifStream()));
     nusifRhsSpareSync(c);
@@ -107,7 +115,11 @@ extern "C" double solve(Solver *s, double *p, const double *rhs)
   }
   if (staged) {
     HIP_CHECK(hipMemcpyAsync(p, c->p, bytes, hipMemcpyDeviceToHost, nusifStream()));
+    HIP_CHECK(hipMemcpyAsync(c->p, c->saveP, bytes, hipMemcpyDeviceToDevice, nusifStream()));
+    HIP_CHECK(hipMemcpyAsync(c->rhs, c->saveRhs, bytes, hipMemcpyDeviceToDevice, nusifStream()));
+    nusifRhsSpareSync(c);
     HIP_CHECK(hipStreamSynchronize(nusifStream()));
+    c->cflValid = false;
   }
   return res;
 }

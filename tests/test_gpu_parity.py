@@ -123,9 +123,13 @@ def test_gpu_run_steps_matches_step_calls(gpu_ok):
 
 def test_gpu_solve_on_foreign_host_arrays(gpu_ok):
     """solve() on host arrays that are not the Discretization's own is staged
-    through the device and gives the same result."""
+    through the device and gives the same result, and leaves the
+    Discretization's own p and rhs as they were (the reference's solve
+    touches only the arrays it is given)."""
     prm = dict(G.KAT_PARAMS)
     c = A.Case(lib("gpu_rb"), **prm)
+    own = G.random_fields(c.shape, 3)
+    c.set_fields(p=own["p"], rhs=own["rhs"])
     inputs = G.random_fields(c.shape, 10)
     p = np.ascontiguousarray(inputs["p"]).copy()
     rhs = np.ascontiguousarray(inputs["rhs"]).copy()
@@ -134,6 +138,9 @@ def test_gpu_solve_on_foreign_host_arrays(gpu_ok):
     _, ref = run_kat("orc", ("solve_rb", "rb", prm, 10, ["solve"]))
     np.testing.assert_array_equal(p, ref["p"])
     assert res == pytest.approx(ref["res"], rel=SUM_RTOL)
+    after = c.fields()
+    np.testing.assert_array_equal(after["p"], own["p"])
+    np.testing.assert_array_equal(after["rhs"], own["rhs"])
     c.close()
 
 
