@@ -575,7 +575,7 @@ template <typename T> __global__ __launch_bounds__(256) void kFill(T *__restrict
  * are read once per column rather than eight times per fine cell. The tail
  * cycle's tailProlong evaluates the same expression per cell. */
 #define PKC 4
-template <typename T>
+template <typename T, bool SET = false>
 __global__ __launch_bounds__(256) void kProlongMarch(T *__restrict__ xf, Geo gf, const T *__restrict__ xc,
     Geo gc, int gridGx, int gridGy, int gridGz, const int *__restrict__ stop)
 {
@@ -603,11 +603,12 @@ __global__ __launch_bounds__(256) void kProlongMarch(T *__restrict__ xf, Geo gf,
     T sp = S(min(K + 1, gc.nk));
     int k = 2 * K - 1;
     long long c = gf.at(i, j, k);
-    T v = xf[c] + (a * s0 + q * sm);
+    /* SET: x = P x_c (no old value added) */
+    T v = SET ? (a * s0 + q * sm) : xf[c] + (a * s0 + q * sm);
     xf[c] = v;
     faceGhosts(xf, gf, i, j, k, c, v);
     c += gf.sk;
-    v     = xf[c] + (a * s0 + q * sp);
+    v     = SET ? (a * s0 + q * sp) : xf[c] + (a * s0 + q * sp);
     xf[c] = v;
     faceGhosts(xf, gf, i, j, k + 1, c, v);
     sm = s0;
@@ -1009,16 +1010,30 @@ __global__ __launch_bounds__(TAIL_THREADS) void kTailCycle(TailArgs<T> a)
  * and rhs^2 per workgroup (slots 2 b, 2 b + 1). pn, eZero must not alias
  * p or e. */
 #define NKC 16
-template <bool CORR>
+/* Optional second output of kResidualLds: the residual (rc) and rhs (bc,
+ * may be null) averaged over the 8 children of each cell of the next
+ * coarser level (extents halved), summed in the children's order (i, then
+ * j, then k) as double, r rounded to FP32 first when roundR: the values of
+ * a separate 8-child restriction of r (or of its FP32 copy) and rhs. The
+ * tile (64 x 4, k-chunks of 16 from an odd plane) holds whole children
+ * groups when the extents are even. */
+struct ResidualRestrict {
+  float *rc = nullptr, *bc = nullptr;
+  Geo gc;
+  int roundR = 0;
+};
+
+template <bool CORR, bool RESTR = false>
 __global__ __launch_bounds__(256) void kResidualLds(const double *__restrict__ p, const float *__restrict__ e,
     const double *__restrict__ rhs, Geo g, Coef<double> cf, double *__restrict__ pn, float *__restrict__ r32,
     float *__restrict__ eZero, double *partials, int gridGx, int gridGy, int gridGz,
-    const int *__restrict__ stop = nullptr)
+    const int *__restrict__ stop = nullptr, ResidualRestrict ro = ResidualRestrict())
 {
   if (stop && *stop) return;
   constexpr int HW = GTX + 2, HN = HW * (GTY + 2);
   __shared__ double lds[4];
   __shared__ double X[3][GTY + 2][HW];
+  __shared__ double sR[GTY][GTX], sB[GTY][GTX]; /* this plane's r and rhs, for the restriction */
   int bx, by, bz;
   if (!xcdBlock(gridGx, gridGy, gridGz, bx, by, bz)) return;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + GTX * ty;
@@ -1054,10 +1069,15 @@ __global__ __launch_bounds__(256) void kResidualLds(const double *__restrict__ p
   fetch(k0 + 1);
   const bool in = i <= g.ni && j <= g.nj;
   double a0 = 0.0, a1 = 0.0;
+  /* restriction: the thread of each 2 x 2 group's first cell (even tx, ty)
+   * sums the group's 8 children over a plane pair */
+  const bool rOwner = RESTR && ro.rc && (tx & 1) == 0 && (ty & 1) == 0;
+  double accR = 0.0, accB = 0.0;
   for (int k = k0; k < k1; k++) {
     store((k + 1) % 3);
     if (k + 1 < k1) fetch(k + 2);
     __syncthreads();
+    double rv = 0.0, bv = 0.0;
     if (in) {
       const int sm = (k + 2) % 3, sc = k % 3, sp = (k + 1) % 3;
       const long long c = g.at(i, j, k);
@@ -1078,8 +1098,36 @@ __global__ __launch_bounds__(256) void kResidualLds(const double *__restrict__ p
       }
       a0 += r * r;
       a1 += bb * bb;
+      if (RESTR) {
+        rv = ro.roundR ? (double)(float)r : r;
+        bv = bb;
+      }
+    }
+    if (RESTR && ro.rc) {
+      sR[ty][tx] = rv;
+      sB[ty][tx] = bv;
     }
     __syncthreads();
+    if (RESTR && rOwner) {
+      if (k & 1) { /* first plane of the pair */
+        accR = 0.0;
+        accB = 0.0;
+      }
+      accR += sR[ty][tx];
+      accR += sR[ty][tx + 1];
+      accR += sR[ty + 1][tx];
+      accR += sR[ty + 1][tx + 1];
+      accB += sB[ty][tx];
+      accB += sB[ty][tx + 1];
+      accB += sB[ty + 1][tx];
+      accB += sB[ty + 1][tx + 1];
+      const int I = (i + 1) >> 1, J = (j + 1) >> 1, K = k >> 1;
+      if (!(k & 1) && I <= ro.gc.ni && J <= ro.gc.nj && K <= ro.gc.nk) {
+        const long long cc = ro.gc.at(I, J, K);
+        ro.rc[cc] = (float)(0.125 * accR);
+        if (ro.bc) ro.bc[cc] = (float)(0.125 * accB);
+      }
+    }
   }
   a0 = blockSum(a0, lds);
   a1 = blockSum(a1, lds);
@@ -1189,6 +1237,38 @@ static dim3 sweepGrid(const Geo &g)
 static dim3 normGrid(const Geo &g)
 {
   return dim3((g.ni + GTX - 1) / GTX, (g.nj + GTY - 1) / GTY, (g.nk + NKC - 1) / NKC);
+}
+
+/* The warm start's input pass: r0 = rhs - A p (FP64) and rhs averaged over
+ * the 8 children into the next level's rc, bc (FP32), and per-workgroup
+ * partials of r0^2, rhs^2 (slots 2 b, 2 b + 1). Returns the partial count.
+ * Extents must be even. */
+int launchInputRestrict(const double *p, const double *rhs, const Geo &g, double idx2, double idy2, double idz2,
+    float *rc, float *bc, const Geo &gc, double *partials)
+{
+  const Coef<double> cf = {idx2, idy2, idz2, 0.0};
+  ResidualRestrict ro;
+  ro.rc   = rc;
+  ro.bc   = bc;
+  ro.gc   = gc;
+  dim3 ng = normGrid(g);
+  kResidualLds<false, true><<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(p, nullptr, rhs, g, cf, nullptr,
+      nullptr, nullptr, partials, ng.x, ng.y, ng.z, nullptr, ro);
+  return (int)(ng.x * ng.y * ng.z);
+}
+
+int inputRestrictPartials(const Geo &g)
+{
+  dim3 ng = normGrid(g);
+  return (int)(ng.x * ng.y * ng.z);
+}
+
+/* x_f = P x_c (trilinear, cell-centred, face ghosts written), FP32 */
+void launchProlongSet32(float *xf, const Geo &gf, const float *xc, const Geo &gc)
+{
+  dim3 pg((gf.ni + GTX - 1) / GTX, (gf.nj + GTY - 1) / GTY, (gc.nk + PKC - 1) / PKC);
+  kProlongMarch<float, true><<<xcdGrid(pg), dim3(GTX, GTY), 0, nusifStream()>>>(xf, gf, xc, gc, pg.x, pg.y, pg.z,
+      nullptr);
 }
 
 template <typename T> static void fill(T *x, long long n, T v)
@@ -1779,11 +1859,17 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
       float *rg  = G->mixed ? G->L32[0].b : warmstartResidualBuffer(ws);
       float *e0  = l0NeedsZero(G) ? G->L32[0].x : nullptr;
       dim3 ng    = normGrid(g);
-      NUSIF_TIMED("gmg_residual64", (kResidualLds<true><<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(
-                                        p, d, rhs, g, G->cf0, c->pAlt, rg, e0, c->partials, ng.x, ng.y, ng.z)));
+      /* the same pass also averages the FP32 residual onto level 1 (the
+       * first step of its restriction to the network grid) */
+      ResidualRestrict ro;
+      ro.rc     = warmstartLevelResidual(ws, 1, &ro.gc);
+      ro.roundR = 1;
+      NUSIF_TIMED("gmg_residual64", (kResidualLds<true, true><<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(
+                                        p, d, rhs, g, G->cf0, c->pAlt, rg, e0, c->partials, ng.x, ng.y, ng.z,
+                                        nullptr, ro)));
       kFoldNorms<<<1, 256, 0, nusifStream()>>>(c->partials, (int)(ng.x * ng.y * ng.z), c->scal->norm2,
           c->scalHostDev->norm2);
-      warmstartCoarseNormLaunch(ws, rg);
+      warmstartCoarseNormLaunch(ws, rg, ro.rc != nullptr);
       const double *wn = warmstartNormsAsync(ws);
       double nbg       = 0.0;
       double rng       = readNorms(c, &nbg);
@@ -1869,6 +1955,52 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
  * reps is rounded up to an even number, which leaves x / xAlt as found.
  * Returns microseconds per launch, or -1 when the solver is not gmg or the
  * finest level does not use the fused sweep. */
+/* ---- C ABI: checks of the warm start's grid transfers ---------------------
+ * Host arrays in, host arrays out (padded layouts), for tests. */
+extern "C" void nusifRestrictResidualCheck(int ni, int nj, int nk, double dx, double dy, double dz,
+    const double *p, const double *rhs, float *rc, float *bc, double *sumsq)
+{
+  if (ni % 2 || nj % 2 || nk % 2) NUSIF_FATAL("restriction check: extents must be even");
+  const Geo g = makeGeo(ni, nj, nk), gc = makeGeo(ni / 2, nj / 2, nk / 2);
+  const Coef<double> cf = coefFor<double>(dx, dy, dz, 1.0);
+  const int nb = inputRestrictPartials(g);
+  double *dp = devAlloc<double>(g.n), *dr = devAlloc<double>(g.n), *dpart = devAlloc<double>(2LL * nb);
+  float *drc = devAlloc<float>(gc.n), *dbc = devAlloc<float>(gc.n);
+  hipStream_t st = nusifStream();
+  HIP_CHECK(hipMemcpyAsync(dp, p, g.n * sizeof(double), hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipMemcpyAsync(dr, rhs, g.n * sizeof(double), hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipMemsetAsync(drc, 0, gc.n * sizeof(float), st));
+  HIP_CHECK(hipMemsetAsync(dbc, 0, gc.n * sizeof(float), st));
+  launchInputRestrict(dp, dr, g, cf.idx2, cf.idy2, cf.idz2, drc, dbc, gc, dpart);
+  std::vector<double> part(2 * (size_t)nb);
+  HIP_CHECK(hipMemcpyAsync(rc, drc, gc.n * sizeof(float), hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipMemcpyAsync(bc, dbc, gc.n * sizeof(float), hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipMemcpyAsync(part.data(), dpart, part.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipStreamSynchronize(st));
+  double sum = 0.0;
+  for (int q = 0; q < nb; q++) sum += part[2 * (size_t)q];
+  *sumsq = sum;
+  HIP_CHECK(hipFree(dp));
+  HIP_CHECK(hipFree(dr));
+  HIP_CHECK(hipFree(dpart));
+  HIP_CHECK(hipFree(drc));
+  HIP_CHECK(hipFree(dbc));
+}
+
+extern "C" void nusifProlongSetCheck(int nic, int njc, int nkc, const float *xc, float *xf)
+{
+  const Geo gc = makeGeo(nic, njc, nkc), gf = makeGeo(2 * nic, 2 * njc, 2 * nkc);
+  float *dc = devAlloc<float>(gc.n), *df = devAlloc<float>(gf.n);
+  hipStream_t st = nusifStream();
+  HIP_CHECK(hipMemcpyAsync(dc, xc, gc.n * sizeof(float), hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipMemsetAsync(df, 0, gf.n * sizeof(float), st));
+  launchProlongSet32(df, gf, dc, gc);
+  HIP_CHECK(hipMemcpyAsync(xf, df, gf.n * sizeof(float), hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipStreamSynchronize(st));
+  HIP_CHECK(hipFree(dc));
+  HIP_CHECK(hipFree(df));
+}
+
 extern "C" int nusifSolverFirstGroup(Solver *s)
 {
   NusifSolverCtx *sc = nusifSolverCtxOf(s);

@@ -223,6 +223,14 @@ void launchAdaptUV(NusifCtx *c, const Discretization *d);
 void launchVelocityBC(NusifCtx *c, const Discretization *d);
 void launchSpecialBC(NusifCtx *c, const Discretization *d);
 void launchComputeFG(NusifCtx *c, const Discretization *d);
+/* gmg.hip kernels the warm start also runs: r0 = rhs - A p restricted to
+ * the next level with rhs (returns the partial count; partials hold r0^2
+ * and rhs^2 per workgroup), its partial count for a grid, and the FP32
+ * trilinear prolongation x_f = P x_c */
+int launchInputRestrict(const double *p, const double *rhs, const Geo &g, double idx2, double idy2, double idz2,
+    float *rc, float *bc, const Geo &gc, double *partials);
+int inputRestrictPartials(const Geo &g);
+void launchProlongSet32(float *xf, const Geo &gf, const float *xc, const Geo &gc);
 /* copy rhs into the spare rhs buffer (after a host upload of rhs) */
 void nusifRhsSpareSync(NusifCtx *c);
 void launchComputeRHS(NusifCtx *c, const Discretization *d);

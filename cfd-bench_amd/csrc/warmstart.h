@@ -18,7 +18,12 @@ UnetModel *warmstartNet(WarmStart *w);
 bool warmstartCapturing(const WarmStart *w);
 /* FP32 level-0 buffer the caller fills with r0 = rhs - A p before the guess */
 float *warmstartResidualBuffer(WarmStart *w);
-void warmstartCoarseNormLaunch(WarmStart *w, const float *rFine);
+/* level l's residual buffer and geometry when 1 <= l <= the network level,
+ * else null */
+float *warmstartLevelResidual(WarmStart *w, int l, Geo *g);
+/* sum of squares of the residual restricted to the network grid, from the
+ * fine FP32 residual, or from level 1 when it is already there (level1) */
+void warmstartCoarseNormLaunch(WarmStart *w, const float *rFine, bool level1);
 const double *warmstartNormsAsync(WarmStart *w);
 void warmstartPrepare(WarmStart *w, const double *p, const double *rhs, double idx2, double idy2, double idz2);
 const float *warmstartCorrection(WarmStart *w);

@@ -173,65 +173,10 @@ __global__ void kTarget(const float *d, Geo g, const double *scal, double h2, fl
   }
 }
 
-/* cell-centred trilinear x_f = P x_c (overwrite), face ghosts written */
-__global__ __launch_bounds__(256) void kProlongSet(float *xf, Geo gf, const float *xc, Geo gc)
-{
-  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  int k = blockIdx.z + 1;
-  if (i > gf.ni || j > gf.nj) return;
-  int I = (i + 1) >> 1, J = (j + 1) >> 1, K = (k + 1) >> 1;
-  int I2 = min(max(I + ((i & 1) ? -1 : 1), 1), gc.ni);
-  int J2 = min(max(J + ((j & 1) ? -1 : 1), 1), gc.nj);
-  int K2 = min(max(K + ((k & 1) ? -1 : 1), 1), gc.nk);
-  const float a = 0.75f, q = 0.25f;
-  float c11 = a * xc[gc.at(I, J, K)] + q * xc[gc.at(I2, J, K)];
-  float c21 = a * xc[gc.at(I, J2, K)] + q * xc[gc.at(I2, J2, K)];
-  float c12 = a * xc[gc.at(I, J, K2)] + q * xc[gc.at(I2, J, K2)];
-  float c22 = a * xc[gc.at(I, J2, K2)] + q * xc[gc.at(I2, J2, K2)];
-  long long c = gf.at(i, j, k);
-  float v     = a * (a * c11 + q * c21) + q * (a * c12 + q * c22);
-  xf[c]       = v;
-  faceGhosts(xf, gf, i, j, k, c, v); /* Neumann copies: the finest level's are read as p + d */
-}
-
-/* The network input's first stage in one pass over the finest grid: for
- * each level-1 cell, the 8-child averages of r0 = rhs - A p (FP64, the
- * 7-point Neumann Laplacian in the solvers' operand order) and of rhs, as
- * FP32, plus per-workgroup partial sums of r0^2 over the fine cells. */
+/* 7-point Neumann Laplacian coefficients */
 struct Lap {
   double idx2, idy2, idz2;
 };
-__global__ __launch_bounds__(256) void kInputRestrict(const double *__restrict__ p, const double *__restrict__ rhs,
-    Geo gf, Lap cf, float *__restrict__ rc, float *__restrict__ bc, Geo gc, double *partials)
-{
-  __shared__ double lds[4];
-  int I = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int J = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  int K = blockIdx.z + 1;
-  double rr = 0.0;
-  if (I <= gc.ni && J <= gc.nj) {
-    double sr = 0.0, sb = 0.0;
-#pragma unroll
-    for (int e = 0; e < 8; e++) {
-      long long c = gf.at(2 * I - 1 + (e & 1), 2 * J - 1 + ((e >> 1) & 1), 2 * K - 1 + (e >> 2));
-      double xc  = p[c];
-      double lap = (p[c + 1] - 2.0 * xc + p[c - 1]) * cf.idx2 + (p[c + gf.sj] - 2.0 * xc + p[c - gf.sj]) * cf.idy2 +
-                   (p[c + gf.sk] - 2.0 * xc + p[c - gf.sk]) * cf.idz2;
-      double b = rhs[c];
-      double r = b - lap;
-      sr += r;
-      sb += b;
-      rr += r * r;
-    }
-    long long cc = gc.at(I, J, K);
-    rc[cc]       = (float)(0.125 * sr);
-    bc[cc]       = (float)(0.125 * sb);
-  }
-  rr = blockSum(rr, lds);
-  if (threadIdx.x == 0 && threadIdx.y == 0)
-    partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = rr;
-}
 
 /* network on the finest grid (small problems): r0 and rhs as FP32 there */
 __global__ __launch_bounds__(256) void kInputFine(const double *__restrict__ p, const double *__restrict__ rhs, Geo g,
@@ -256,6 +201,14 @@ __global__ __launch_bounds__(256) void kInputFine(const double *__restrict__ p, 
   rr = blockSum(rr, lds);
   if (threadIdx.x == 0 && threadIdx.y == 0)
     partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = rr;
+}
+
+/* component 0 of stride-2 partials (the input pass's r0^2) */
+__global__ __launch_bounds__(256) void kFoldComp0(const double *partials, int nb, double *out)
+{
+  __shared__ double lds[4];
+  double s = foldSum(partials, nb, 2, 0, lds);
+  if (threadIdx.x == 0) *out = s;
 }
 
 __global__ __launch_bounds__(256) void kFoldSum1(const double *partials, int nb, double *out)
@@ -313,9 +266,11 @@ WarmStart *warmstartCreate(NusifCtx *c, UnetModel *net, bool capture)
   w->n  = gn.ni;
   w->h2 = (gn.ni * h / M_PI) * (gn.ni * h / M_PI);
   {
-    /* one partial per workgroup of the first input pass (and of the norms) */
-    dim3 gr = cells(w->lv[w->Ln > 0 ? 1 : 0].g);
-    if ((size_t)gr.x * gr.y * gr.z > 65536)
+    /* partials of the first input pass (two per workgroup when the network
+     * is on a coarser level, one on the finest) and of the norms */
+    dim3 gr = cells(w->lv[0].g);
+    size_t need = w->Ln > 0 ? 2 * (size_t)inputRestrictPartials(w->lv[0].g) : (size_t)gr.x * gr.y * gr.z;
+    if (need > 65536)
       NUSIF_FATAL("warm start: grid %dx%dx%d too large for the network input pass (coarsen to extents <= %d)",
           c->geo.ni, c->geo.nj, c->geo.nk, c->opt.unetExtent > 0 ? c->opt.unetExtent : 64);
   }
@@ -366,9 +321,9 @@ void warmstartPrepare(WarmStart *w, const double *p, const double *rhs, double i
     kInputFine<<<gr, dim3(WTX, WTY), 0, st>>>(p, rhs, g0, cf, w->lv[0].r, w->lv[0].b, w->partials);
     kFoldSum1<<<1, 256, 0, st>>>(w->partials, (int)(gr.x * gr.y * gr.z), w->scal + 3);
   } else {
-    dim3 gr = cells(w->lv[1].g);
-    kInputRestrict<<<gr, dim3(WTX, WTY), 0, st>>>(p, rhs, g0, cf, w->lv[1].r, w->lv[1].b, w->lv[1].g, w->partials);
-    kFoldSum1<<<1, 256, 0, st>>>(w->partials, (int)(gr.x * gr.y * gr.z), w->scal + 3);
+    /* r0 and rhs averaged onto level 1 in the residual pass itself */
+    int nb = launchInputRestrict(p, rhs, g0, idx2, idy2, idz2, w->lv[1].r, w->lv[1].b, w->lv[1].g, w->partials);
+    kFoldComp0<<<1, 256, 0, st>>>(w->partials, nb, w->scal + 3);
     for (int l = 2; l <= w->Ln; l++) {
       kRestrictAvg<float><<<cells(w->lv[l].g), dim3(WTX, WTY), 0, st>>>(w->lv[l - 1].b, w->lv[l - 1].g, w->lv[l].b, w->lv[l].g);
       kRestrictAvg<float><<<cells(w->lv[l].g), dim3(WTX, WTY), 0, st>>>(w->lv[l - 1].r, w->lv[l - 1].g, w->lv[l].r, w->lv[l].g);
@@ -391,8 +346,7 @@ const float *warmstartCorrection(WarmStart *w)
   NUSIF_TIMED("unet_forward", unetForward(w->net, w->x));
   const WsLevel &L = w->lv[w->Ln];
   kUnpack<<<1024, 256, 0, st>>>(unetOutput(w->net), L.g, w->scal, w->h2, L.d);
-  for (int l = w->Ln; l > 0; l--)
-    kProlongSet<<<cells(w->lv[l - 1].g), dim3(WTX, WTY), 0, st>>>(w->lv[l - 1].d, w->lv[l - 1].g, w->lv[l].d, w->lv[l].g);
+  for (int l = w->Ln; l > 0; l--) launchProlongSet32(w->lv[l - 1].d, w->lv[l - 1].g, w->lv[l].d, w->lv[l].g);
   return w->lv[0].d;
 }
 
@@ -402,11 +356,18 @@ float *warmstartResidualBuffer(WarmStart *w) { return w->lv[0].r; }
  * averaging, of a level-0 FP32 residual (any padded buffer on the finest
  * grid), into the device slot scal[2]; no host synchronisation. lv[l].r
  * (l >= 1) are reused as scratch. */
-void warmstartCoarseNormLaunch(WarmStart *w, const float *rFine)
+float *warmstartLevelResidual(WarmStart *w, int l, Geo *g)
+{
+  if (l < 1 || l > w->Ln) return nullptr;
+  *g = w->lv[l].g;
+  return w->lv[l].r;
+}
+
+void warmstartCoarseNormLaunch(WarmStart *w, const float *rFine, bool level1)
 {
   hipStream_t st = nusifStream();
-  const float *src = rFine;
-  for (int l = 1; l <= w->Ln; l++) {
+  const float *src = level1 ? w->lv[1].r : rFine;
+  for (int l = level1 ? 2 : 1; l <= w->Ln; l++) {
     kRestrictAvg<float><<<cells(w->lv[l].g), dim3(WTX, WTY), 0, st>>>(src, w->lv[l - 1].g, w->lv[l].r, w->lv[l].g);
     src = w->lv[l].r;
   }

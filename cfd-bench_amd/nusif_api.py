@@ -144,6 +144,9 @@ def _declare_optional(lib):
                                       [C.c_double, C.c_int, _DP], C.c_int),
         "nusifUnetLossGrad": ([C.POINTER(C.c_int), C.c_int, _DP, _DP, _DP, _DP, _DP], C.c_double),
         "nusifUnetPredict": ([C.POINTER(C.c_int), C.c_int, _DP, _DP, _DP], None),
+        "nusifRestrictResidualCheck": ([C.c_int] * 3 + [C.c_double] * 3 + [_DP, _DP, C.POINTER(C.c_float),
+                                       C.POINTER(C.c_float), _DP], None),
+        "nusifProlongSetCheck": ([C.c_int] * 3 + [C.POINTER(C.c_float), C.POINTER(C.c_float)], None),
         "nusifRunSteps": ([C.POINTER(Discretization), C.POINTER(Solver), C.c_int,
                            C.POINTER(C.c_double)], C.c_int),
         # oracle extras

@@ -164,6 +164,16 @@ double nusifUnetLossGrad(const int *cfg, int n, const double *params, const doub
  * receives n^3 values, bit-identical to nusifUnetLossGrad's. */
 void nusifUnetPredict(const int *cfg, int n, const double *params, const double *x, double *pred);
 
+/* Checks of the warm start's grid transfers on host arrays (padded layouts,
+ * for tests): r0 = rhs - A p (FP64, the solvers' 7-point Neumann Laplacian
+ * with spacings dx, dy, dz) and rhs averaged over the 8 children of each cell
+ * of the grid of halved extents (FP32 rc, bc; extents even), and sumsq =
+ * sum of r0^2; and the FP32 trilinear prolongation xf = P xc from a grid of
+ * nic x njc x nkc cells to the doubled one (face ghosts written). */
+void nusifRestrictResidualCheck(int ni, int nj, int nk, double dx, double dy, double dz, const double *p,
+    const double *rhs, float *rc, float *bc, double *sumsq);
+void nusifProlongSetCheck(int nic, int njc, int nkc, const float *xc, float *xf);
+
 /* Runs the reference time loop (src/main.c:60-88) for `nsteps` steps entirely
  * inside the library: the same calls in the same order, without the Python
  * or host-driver overhead between them. trace (may be NULL) receives

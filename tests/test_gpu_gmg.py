@@ -310,3 +310,85 @@ def test_gpu_gmg_speculative_cycles_equal_one_at_a_time(precision, gpu_ok):
     assert min(counts) >= 1
     for c in cases:
         c.close()
+
+
+def _np_lap7(x, ni, nj, nk, idx2, idy2, idz2):
+    """The solvers' 7-point Neumann Laplacian on the interior, in their
+    operand order ((E - 2C + W) * idx2 + (N - 2C + S) * idy2 + (T - 2C + B) * idz2)."""
+    c = x[1:-1, 1:-1, 1:-1]
+    e, w = x[1:-1, 1:-1, 2:], x[1:-1, 1:-1, :-2]
+    n, s = x[1:-1, 2:, 1:-1], x[1:-1, :-2, 1:-1]
+    t, b = x[2:, 1:-1, 1:-1], x[:-2, 1:-1, 1:-1]
+    return (e - 2.0 * c + w) * idx2 + (n - 2.0 * c + s) * idy2 + (t - 2.0 * c + b) * idz2
+
+
+@pytest.mark.parametrize("shape", [(64, 8, 32), (96, 22, 18)])
+def test_gpu_input_restriction_matches_numpy(shape, gpu_ok):
+    """The warm start's input pass (r0 = rhs - A p in FP64 with the 8-child
+    averages of r0 and rhs written by the residual kernel itself) against a
+    numpy restatement: the children summed in the order i, j, k as double,
+    times 1/8, as FP32: bit for bit. The r0^2 sum within 1e-12 relative
+    (workgroup partials, another summation order)."""
+    ni, nj, nk = shape
+    dx, dy, dz = 1.0 / ni, 0.7 / nj, 1.3 / nk
+    rng = np.random.default_rng(4)
+    p = rng.uniform(-1, 1, (nk + 2, nj + 2, ni + 2))
+    rhs = rng.uniform(-1, 1, (nk + 2, nj + 2, ni + 2))
+    gc = (nk // 2 + 2, nj // 2 + 2, ni // 2 + 2)
+    rc = np.zeros(gc, dtype=np.float32)
+    bc = np.zeros(gc, dtype=np.float32)
+    ss = C.c_double()
+    fp = C.POINTER(C.c_float)
+    lib("gpu_rb").nusifRestrictResidualCheck(ni, nj, nk, dx, dy, dz, p.ctypes.data_as(DP), rhs.ctypes.data_as(DP),
+                                             rc.ctypes.data_as(fp), bc.ctypes.data_as(fp), C.byref(ss))
+    r = rhs[1:-1, 1:-1, 1:-1] - _np_lap7(p, ni, nj, nk, 1.0 / (dx * dx), 1.0 / (dy * dy), 1.0 / (dz * dz))
+    b = rhs[1:-1, 1:-1, 1:-1]
+    for src, out in ((r, rc), (b, bc)):
+        s = np.zeros((nk // 2, nj // 2, ni // 2))
+        for dk in (0, 1):
+            for dj in (0, 1):
+                for di in (0, 1):
+                    s = s + src[dk::2, dj::2, di::2]
+        np.testing.assert_array_equal(out[1:-1, 1:-1, 1:-1], (0.125 * s).astype(np.float32))
+    assert ss.value == pytest.approx(float(np.sum(r * r)), rel=1e-12)
+
+
+@pytest.mark.parametrize("shape", [(32, 4, 16), (48, 11, 7)])
+def test_gpu_prolong_set_matches_numpy(shape, gpu_ok):
+    """The warm start's FP32 trilinear prolongation (the multigrid's
+    column-marching kernel with the old value not added) against numpy:
+    weights 3/4, 1/4 per direction, coarse neighbour clamped at the walls,
+    in-plane terms first; interior and face ghosts bit for bit."""
+    nic, njc, nkc = shape
+    rng = np.random.default_rng(6)
+    xc = rng.uniform(-1, 1, (nkc + 2, njc + 2, nic + 2)).astype(np.float32)
+    ni, nj, nk = 2 * nic, 2 * njc, 2 * nkc
+    xf = np.zeros((nk + 2, nj + 2, ni + 2), dtype=np.float32)
+    fp = C.POINTER(C.c_float)
+    lib("gpu_rb").nusifProlongSetCheck(nic, njc, nkc, xc.ctypes.data_as(fp), xf.ctypes.data_as(fp))
+    a, q = np.float32(0.75), np.float32(0.25)
+
+    def idx(n, nc):
+        f = np.arange(1, n + 1)
+        I = (f + 1) // 2
+        I2 = np.clip(I + np.where(f & 1, -1, 1), 1, nc)
+        return I, I2
+
+    I, I2 = idx(ni, nic)
+    J, J2 = idx(nj, njc)
+    K, K2 = idx(nk, nkc)
+
+    def S(Kp):
+        pl = xc[Kp]  # (n fine k, nj+2, ni+2)
+        c1 = a * pl[:, J][:, :, I] + q * pl[:, J][:, :, I2]
+        c2 = a * pl[:, J2][:, :, I] + q * pl[:, J2][:, :, I2]
+        return a * c1 + q * c2
+
+    ref = a * S(K) + q * S(K2)
+    np.testing.assert_array_equal(xf[1:-1, 1:-1, 1:-1], ref)
+    np.testing.assert_array_equal(xf[1:-1, 1:-1, 0], ref[:, :, 0])
+    np.testing.assert_array_equal(xf[1:-1, 1:-1, -1], ref[:, :, -1])
+    np.testing.assert_array_equal(xf[1:-1, 0, 1:-1], ref[:, 0, :])
+    np.testing.assert_array_equal(xf[1:-1, -1, 1:-1], ref[:, -1, :])
+    np.testing.assert_array_equal(xf[0, 1:-1, 1:-1], ref[0])
+    np.testing.assert_array_equal(xf[-1, 1:-1, 1:-1], ref[-1])
