@@ -994,10 +994,12 @@ __global__ __launch_bounds__(TAIL_THREADS) void kTailCycle(TailArgs<T> a)
 }
 
 /* ---- FP64 finest-level kernels: norms, defect correction ---------------- */
-/* Residual of the finest level with its norms, one workgroup per 64 x 4
+/* Residual of the finest level with its norms, one workgroup per 64 x RTY
  * (i, j) tile marching NKC planes: the tile's values plus a one-cell ring
  * are staged in LDS per plane (three planes rotate), so the 7-point stencil
- * reads LDS instead of seven global loads per cell.
+ * reads LDS instead of seven global loads per cell. (64 x 8 tiles, with
+ * fewer ring loads per cell, measured no faster at 256^3: L2 catches the
+ * ring's re-reads.)
  *   CORR = false: x = p;      r = rhs - A x
  *   CORR = true:  x = p + e;  pn = x (face ghosts refreshed), r = rhs - A x
  *                 (the mixed-precision defect correction and the next
@@ -1010,12 +1012,13 @@ __global__ __launch_bounds__(TAIL_THREADS) void kTailCycle(TailArgs<T> a)
  * and rhs^2 per workgroup (slots 2 b, 2 b + 1). pn, eZero must not alias
  * p or e. */
 #define NKC 16
+#define RTY 4 /* rows of a residual tile */
 /* Optional second output of kResidualLds: the residual (rc) and rhs (bc,
  * may be null) averaged over the 8 children of each cell of the next
  * coarser level (extents halved), summed in the children's order (i, then
  * j, then k) as double, r rounded to FP32 first when roundR: the values of
  * a separate 8-child restriction of r (or of its FP32 copy) and rhs. The
- * tile (64 x 4, k-chunks of 16 from an odd plane) holds whole children
+ * tile (64 x RTY, k-chunks of 16 from an odd plane) holds whole children
  * groups when the extents are even. */
 struct ResidualRestrict {
   float *rc = nullptr, *bc = nullptr;
@@ -1024,28 +1027,29 @@ struct ResidualRestrict {
 };
 
 template <bool CORR, bool RESTR = false>
-__global__ __launch_bounds__(256) void kResidualLds(const double *__restrict__ p, const float *__restrict__ e,
+__global__ __launch_bounds__(GTX * RTY, RESTR ? 6 : 8) void kResidualLds(const double *__restrict__ p, const float *__restrict__ e,
     const double *__restrict__ rhs, Geo g, Coef<double> cf, double *__restrict__ pn, float *__restrict__ r32,
     float *__restrict__ eZero, double *partials, int gridGx, int gridGy, int gridGz,
     const int *__restrict__ stop = nullptr, ResidualRestrict ro = ResidualRestrict())
 {
   if (stop && *stop) return;
-  constexpr int HW = GTX + 2, HN = HW * (GTY + 2);
-  __shared__ double lds[4];
-  __shared__ double X[3][GTY + 2][HW];
-  __shared__ double sR[GTY][GTX], sB[GTY][GTX]; /* this plane's r and rhs, for the restriction */
+  constexpr int NT = GTX * RTY, HW = GTX + 2, HN = HW * (RTY + 2);
+  static_assert(HN <= 2 * NT, "two ring elements per thread at most");
+  __shared__ double lds[NT / 64];
+  __shared__ double X[3][RTY + 2][HW];
+  __shared__ double sR[RTY][GTX], sB[RTY][GTX]; /* this plane's r and rhs, for the restriction */
   int bx, by, bz;
   if (!xcdBlock(gridGx, gridGy, gridGz, bx, by, bz)) return;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + GTX * ty;
-  const int i = bx * GTX + tx + 1, j = by * GTY + ty + 1;
+  const int i = bx * GTX + tx + 1, j = by * RTY + ty + 1;
   const int k0 = bz * NKC + 1, k1 = min(k0 + NKC, g.nk + 1);
-  /* this thread's ring elements (two at most: HN <= 512); positions past the
-   * padded box load a clamped cell, used only by cells outside the interior */
-  const int e1 = tid + 256;
+  /* this thread's ring elements (two at most); positions past the padded box
+   * load a clamped cell, used only by cells outside the interior */
+  const int e1 = tid + NT;
   const bool has1 = e1 < HN;
   const int hi0 = tid % HW, hj0 = tid / HW, hi1 = e1 % HW, hj1 = has1 ? e1 / HW : 0;
-  const long long o0 = (long long)min(by * GTY + hj0, g.nj + 1) * g.sj + min(bx * GTX + hi0, g.ni + 1);
-  const long long o1 = (long long)min(by * GTY + hj1, g.nj + 1) * g.sj + min(bx * GTX + hi1, g.ni + 1);
+  const long long o0 = (long long)min(by * RTY + hj0, g.nj + 1) * g.sj + min(bx * GTX + hi0, g.ni + 1);
+  const long long o1 = (long long)min(by * RTY + hj1, g.nj + 1) * g.sj + min(bx * GTX + hi1, g.ni + 1);
   auto value = [&](long long q) {
     if constexpr (CORR) return p[q] + (double)e[q];
     else return p[q];
@@ -1068,6 +1072,8 @@ __global__ __launch_bounds__(256) void kResidualLds(const double *__restrict__ p
   store(k0 % 3);
   fetch(k0 + 1);
   const bool in = i <= g.ni && j <= g.nj;
+  /* rhs one plane ahead, issued with the p loads */
+  double bNext = in ? rhs[g.at(i, j, k0)] : 0.0;
   double a0 = 0.0, a1 = 0.0;
   /* restriction: the thread of each 2 x 2 group's first cell (even tx, ty)
    * sums the group's 8 children over a plane pair */
@@ -1076,6 +1082,8 @@ __global__ __launch_bounds__(256) void kResidualLds(const double *__restrict__ p
   for (int k = k0; k < k1; k++) {
     store((k + 1) % 3);
     if (k + 1 < k1) fetch(k + 2);
+    const double bb = bNext;
+    if (in && k + 1 < k1) bNext = rhs[g.at(i, j, k + 1)];
     __syncthreads();
     double rv = 0.0, bv = 0.0;
     if (in) {
@@ -1085,7 +1093,6 @@ __global__ __launch_bounds__(256) void kResidualLds(const double *__restrict__ p
       double lap = (X[sc][ty + 1][tx + 2] - 2.0 * xc + X[sc][ty + 1][tx]) * cf.idx2 +
                    (X[sc][ty + 2][tx + 1] - 2.0 * xc + X[sc][ty][tx + 1]) * cf.idy2 +
                    (X[sp][ty + 1][tx + 1] - 2.0 * xc + X[sm][ty + 1][tx + 1]) * cf.idz2;
-      double bb = rhs[c];
       double r  = bb - lap;
       if constexpr (CORR) {
         pn[c] = xc;
@@ -1236,7 +1243,7 @@ static dim3 sweepGrid(const Geo &g)
 }
 static dim3 normGrid(const Geo &g)
 {
-  return dim3((g.ni + GTX - 1) / GTX, (g.nj + GTY - 1) / GTY, (g.nk + NKC - 1) / NKC);
+  return dim3((g.ni + GTX - 1) / GTX, (g.nj + RTY - 1) / RTY, (g.nk + NKC - 1) / NKC);
 }
 
 /* The warm start's input pass: r0 = rhs - A p (FP64) and rhs averaged over
@@ -1252,7 +1259,7 @@ int launchInputRestrict(const double *p, const double *rhs, const Geo &g, double
   ro.bc   = bc;
   ro.gc   = gc;
   dim3 ng = normGrid(g);
-  kResidualLds<false, true><<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(p, nullptr, rhs, g, cf, nullptr,
+  kResidualLds<false, true><<<xcdGrid(ng), dim3(GTX, RTY), 0, nusifStream()>>>(p, nullptr, rhs, g, cf, nullptr,
       nullptr, nullptr, partials, ng.x, ng.y, ng.z, nullptr, ro);
   return (int)(ng.x * ng.y * ng.z);
 }
@@ -1684,7 +1691,7 @@ static void launchResidual(NusifCtx *c, GmgState *G, const double *p, const doub
     float *eZero, double *out)
 {
   dim3 ng = normGrid(c->geo);
-  NUSIF_TIMED("gmg_residual64", (kResidualLds<false><<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(
+  NUSIF_TIMED("gmg_residual64", (kResidualLds<false><<<xcdGrid(ng), dim3(GTX, RTY), 0, nusifStream()>>>(
                                     p, nullptr, rhs, c->geo, G->cf0, nullptr, r32, eZero, c->partials, ng.x,
                                     ng.y, ng.z)));
   kFoldNorms<<<1, 256, 0, nusifStream()>>>(c->partials, (int)(ng.x * ng.y * ng.z), out,
@@ -1864,7 +1871,7 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
       ResidualRestrict ro;
       ro.rc     = warmstartLevelResidual(ws, 1, &ro.gc);
       ro.roundR = 1;
-      NUSIF_TIMED("gmg_residual64", (kResidualLds<true, true><<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(
+      NUSIF_TIMED("gmg_residual64", (kResidualLds<true, true><<<xcdGrid(ng), dim3(GTX, RTY), 0, nusifStream()>>>(
                                         p, d, rhs, g, G->cf0, c->pAlt, rg, e0, c->partials, ng.x, ng.y, ng.z,
                                         nullptr, ro)));
       kFoldNorms<<<1, 256, 0, nusifStream()>>>(c->partials, (int)(ng.x * ng.y * ng.z), c->scal->norm2,
@@ -1908,7 +1915,7 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
     dim3 ng   = normGrid(g);
     cycles    = runCycles(c, G, rel, nb, [&](int m) {
       for (int q = 0; q < m; q++) vcycle<double>(G, G->L64, 0);
-      NUSIF_TIMED("gmg_residual64", (kResidualLds<false><<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(
+      NUSIF_TIMED("gmg_residual64", (kResidualLds<false><<<xcdGrid(ng), dim3(GTX, RTY), 0, nusifStream()>>>(
                                         L0.x, nullptr, rhs, g, G->cf0, nullptr, nullptr, nullptr, c->partials,
                                         ng.x, ng.y, ng.z, gCycleStop)));
     });
@@ -1927,7 +1934,7 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
     cycles    = runCycles(c, G, rel, nb, [&](int m) {
       for (int q = 0; q < m; q++) vcycle<float>(G, G->L32, 0, q > 0);
       NUSIF_TIMED("gmg_correct_residual",
-          (kResidualLds<true><<<xcdGrid(ng), dim3(GTX, GTY), 0, nusifStream()>>>(c->p, L0.x, rhs, g, G->cf0,
+          (kResidualLds<true><<<xcdGrid(ng), dim3(GTX, RTY), 0, nusifStream()>>>(c->p, L0.x, rhs, g, G->cf0,
               c->pAlt, L0.b, l0NeedsZero(G) ? L0.xAlt : nullptr, c->partials, ng.x, ng.y, ng.z, gCycleStop)));
       float *te  = L0.x; /* the next cycle's (zeroed) e */
       L0.x       = L0.xAlt;
@@ -1946,15 +1953,6 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
 
 
 
-/* Duration of the finest level's red-black sweep (the dominant kernel of a
- * gmg solve): `reps` back-to-back launches on the solver's own level-0
- * buffers, HIP events on the library stream around the batch only, so the
- * per-launch event packets of the kernel timers do not inflate it. The
- * sweeps change only the level-0 work arrays (in mixed mode the correction
- * e, in FP64 mode p and its partner: use it after the steps it measures).
- * reps is rounded up to an even number, which leaves x / xAlt as found.
- * Returns microseconds per launch, or -1 when the solver is not gmg or the
- * finest level does not use the fused sweep. */
 /* ---- C ABI: checks of the warm start's grid transfers ---------------------
  * Host arrays in, host arrays out (padded layouts), for tests. */
 extern "C" void nusifRestrictResidualCheck(int ni, int nj, int nk, double dx, double dy, double dz,
@@ -2007,6 +2005,15 @@ extern "C" int nusifSolverFirstGroup(Solver *s)
   return sc->kind == SOLVER_GMG && sc->gmg ? sc->gmg->lastGroup : 0;
 }
 
+/* Duration of the finest level's red-black sweep (the dominant kernel of a
+ * gmg solve): `reps` back-to-back launches on the solver's own level-0
+ * buffers, HIP events on the library stream around the batch only, so the
+ * per-launch event packets of the kernel timers do not inflate it. The
+ * sweeps change only the level-0 work arrays (in mixed mode the correction
+ * e, in FP64 mode p and its partner: use it after the steps it measures).
+ * reps is rounded up to an even number, which leaves x / xAlt as found.
+ * Returns microseconds per launch, or -1 when the solver is not gmg or the
+ * finest level does not use the fused sweep. */
 extern "C" double nusifTimeFinestSweep(Solver *s, int reps)
 {
   NusifSolverCtx *sc = nusifSolverCtxOf(s);
