@@ -61,6 +61,7 @@ def mode_options(mode):
 DOMINANT = {"fp64": "gmg_smooth64_l0", "mixed": "gmg_smooth32_l0", "learned": "gmg_smooth32_l0"}
 TIMERS = ["gmg_smooth64_l0", "gmg_smooth32_l0", "gmg_smooth32z_l0", "gmg_restrict32_l0", "gmg_prolong32_l0",
           "gmg_restrict64_l0", "gmg_prolong64_l0", "gmg_residual64", "gmg_correct_residual",
+          "ws_guess_residual",
           "computeFG", "computeFG_rhs",
           "computeRHS", "adaptUV", "absmax", "unet_forward", "unet_conv"]
 
@@ -82,7 +83,8 @@ def run_mode(lib, mode, n, warmup, steps, profile_steps=20):
     """Warmup, then `steps` timed steps with no per-kernel timing, then
     `profile_steps` more steps with HIP-event timers around the kernels the
     roofline and the kernel table report (the events cost launch time, so
-    they stay out of the headline timing)."""
+    they stay out of the headline timing), then `profile_steps` more with
+    one event pair around each solve (the solver rate)."""
     case = A.Case(lib, options=mode_options(mode), **cavity_params(n))
     trace = (C.c_double * (4 * max(warmup, steps, profile_steps, 1)))()
     if warmup:
@@ -116,13 +118,33 @@ def run_mode(lib, mode, n, warmup, steps, profile_steps=20):
                            us_per_launch=1e3 * ms / launches.value,
                            bytes_per_launch=lib.nusifKernelBytes(C.byref(case.d), t.encode()))
     lib.nusifTimerDisableAll()
+    # the solver region alone, as the reference rates it (src/solver-rb.c:161,
+    # src/profiler.c:61): iterations x interior cells / solve time, an
+    # iteration of the gmg solve being one V-cycle
+    lib.nusifSolveTiming(C.byref(case.d), 1)
+    lib.nusifRunSteps(C.byref(case.d), C.byref(case.s), profile_steps, trace)
+    lib.nusifSolveTiming(C.byref(case.d), 0)
+    solve_ms, solves = C.c_double(), C.c_longlong()
+    lib.nusifSolveTimes(C.byref(case.d), C.byref(solve_ms), C.byref(solves))
+    solve = dict(ms=solve_ms.value, solves=solves.value,
+                 cycles=sum(trace[4 * q + 3] for q in range(profile_steps)))
     # the dominant kernel once more, 20 launches back to back on the live
     # level-0 buffers with events around the batch only (no per-launch event
     # packets): the duration the roofline uses
     sweep_us = lib.nusifTimeFinestSweep(C.byref(case.s), 20)
     log("[bench] %s: %d steps in %.3f s" % (mode, steps, t1 - t0))
     return case, dict(seconds=t1 - t0, cycles=cycles, rel=rels, kernels=kern, sweep_us=sweep_us,
-                      warmstart_taken=ws)
+                      warmstart_taken=ws, solve=solve)
+
+
+def solver_rate(res, cells):
+    """solver MLUP/s (V-cycles x interior cells / solve time) and the solve's
+    milliseconds per step, from the solve-timed steps"""
+    s = res["solve"]
+    if not s["solves"] or s["ms"] <= 0:
+        return {}
+    return {"solver_mlups": s["cycles"] * cells / (s["ms"] * 1e-3) * 1e-6,
+            "solve_ms_per_step": s["ms"] / s["solves"]}
 
 
 def cpu_baseline(case_gpu, n, budget_s=20.0):
@@ -210,6 +232,7 @@ def main():
         "cycles_per_step": sum(res["cycles"]) / len(res["cycles"]),
         "max_rel_residual": max(res["rel"]),
     }
+    out.update(solver_rate(res, cells))
     if res["warmstart_taken"] is not None:
         out["warmstart_guess_taken_fraction"] = res["warmstart_taken"]
     dom = DOMINANT[a.mode]
@@ -230,6 +253,7 @@ def main():
         f64 = a.steps / ref["seconds"]
         out["fp64_mg"] = {"value": f64, "mlups": cells * f64 * 1e-6,
                           "cycles_per_step": sum(ref["cycles"]) / len(ref["cycles"])}
+        out["fp64_mg"].update(solver_rate(ref, cells))
         out["speedup_vs_fp64_mg"] = sps / f64
         if a.mode == "learned":  # the same solver without the network
             _, mx = run_mode(lib, "mixed", n, a.warmup, a.steps)
@@ -238,6 +262,7 @@ def main():
                 "value": vm, "mlups": cells * vm * 1e-6,
                 "cycles_per_step": sum(mx["cycles"]) / len(mx["cycles"]),
                 "speedup_vs_fp64_mg": vm / f64}
+            out["fp32_mg_without_warmstart"].update(solver_rate(mx, cells))
     # what plain streaming kernels reach on this device (256 MiB arrays): the
     # practical ceiling under the 8 TB/s vendor peak
     ms = (C.c_double * 3)()

@@ -416,6 +416,8 @@ extern "C" double nusifKernelBytes(Discretization *d, const char *kernel)
       {"gmg_correct", 20.0, 0},      /* rd p, e32; wr p */
       {"gmg_correct_residual", 32.0, 0}, /* rd p, e32, rhs; wr p', r32 (the zero-start
                                             sweep needs no zeroed e) */
+      {"ws_guess_residual", 32.5, 0}, /* warm-start guess: rd p, d32, rhs; wr p + d,
+                                         r32, and r averaged onto level 1 (1/8 x 4 B) */
   };
   for (auto &e : table)
     if (strcmp(kernel, e.name) == 0) return e.perCell * (e.padded ? Npad : N);

@@ -949,6 +949,11 @@ extern "C" void initDiscretization(Discretization *s, Parameter *params)
   c->saveRhs  = nullptr;
   c->dtOnDevice = false;
   HIP_CHECK(hipEventCreateWithFlags(&c->dtEvent, hipEventDisableTiming));
+  c->solveTiming  = 0;
+  c->solveEvUsed  = 0;
+  c->solveEvReady = false;
+  c->solveMs      = 0.0;
+  c->solveCount   = 0;
   HIP_CHECK(hipMalloc((void **)&c->scal, sizeof(DevScalars)));
   HIP_CHECK(hipHostMalloc((void **)&c->scalHost, sizeof(DevScalars), hipHostMallocCoherent));
   HIP_CHECK(hipHostGetDevicePointer((void **)&c->scalHostDev, c->scalHost, 0));
@@ -1003,6 +1008,8 @@ extern "C" void nusifFreeDiscretization(Discretization *d)
   double *dev[10] = {c->p, c->rhs, c->f, c->g, c->h, c->u, c->v, c->w, c->pAlt, c->rhsNext};
   for (int q = 0; q < 10; q++) HIP_CHECK(hipFree(dev[q]));
   HIP_CHECK(hipEventDestroy(c->dtEvent));
+  if (c->solveEvReady)
+    for (int q = 0; q < 2 * SOLVE_EV_POOL; q++) HIP_CHECK(hipEventDestroy(c->solveEv[q]));
   if (c->saveP) HIP_CHECK(hipFree(c->saveP));
   if (c->saveRhs) HIP_CHECK(hipFree(c->saveRhs));
   HIP_CHECK(hipFree(c->partials));

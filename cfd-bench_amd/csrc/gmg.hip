@@ -1871,9 +1871,9 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
       ResidualRestrict ro;
       ro.rc     = warmstartLevelResidual(ws, 1, &ro.gc);
       ro.roundR = 1;
-      NUSIF_TIMED("gmg_residual64", (kResidualLds<true, true><<<xcdGrid(ng), dim3(GTX, RTY), 0, nusifStream()>>>(
-                                        p, d, rhs, g, G->cf0, c->pAlt, rg, e0, c->partials, ng.x, ng.y, ng.z,
-                                        nullptr, ro)));
+      NUSIF_TIMED("ws_guess_residual",
+          (kResidualLds<true, true><<<xcdGrid(ng), dim3(GTX, RTY), 0, nusifStream()>>>(
+              p, d, rhs, g, G->cf0, c->pAlt, rg, e0, c->partials, ng.x, ng.y, ng.z, nullptr, ro)));
       kFoldNorms<<<1, 256, 0, nusifStream()>>>(c->partials, (int)(ng.x * ng.y * ng.z), c->scal->norm2,
           c->scalHostDev->norm2);
       warmstartCoarseNormLaunch(ws, rg, ro.rc != nullptr);

@@ -137,6 +137,7 @@ struct UnetModel; /* unet.hip */
 struct WarmStart; /* warmstart.hip */
 
 /* One Discretization's device state. */
+#define SOLVE_EV_POOL 64
 struct NusifCtx {
   Discretization *owner;
   double *host[8];     /* p rhs f g h u v w host arrays, identity for lookups */
@@ -167,6 +168,15 @@ struct NusifCtx {
    * step waits for a host round trip; dtEvent follows that fold */
   bool dtOnDevice;
   hipEvent_t dtEvent;
+  /* nusifSolveTiming: an event pair around each solve of nusifRunSteps,
+   * from a pool folded into solveMs / solveCount when it fills and at the
+   * end of the call (no per-step wait) */
+  int solveTiming;
+  int solveEvUsed;
+  bool solveEvReady;
+  hipEvent_t solveEv[2 * SOLVE_EV_POOL];
+  double solveMs;
+  long long solveCount;
   NusifOptions opt;
   double t;            /* nusifRunSteps clock */
   int nt;

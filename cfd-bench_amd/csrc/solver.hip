@@ -142,6 +142,39 @@ extern "C" void nusifWarmstartResiduals(Solver *s, double *out4)
 }
 
 /* The loop body of src/main.c:60-88, driven from inside the library. */
+/* elapsed times of the recorded solve event pairs into solveMs / solveCount */
+static void foldSolveTimes(NusifCtx *c)
+{
+  for (int q = 0; q < c->solveEvUsed; q++) {
+    float ms = 0.0f;
+    HIP_CHECK(hipEventSynchronize(c->solveEv[2 * q + 1]));
+    HIP_CHECK(hipEventElapsedTime(&ms, c->solveEv[2 * q], c->solveEv[2 * q + 1]));
+    c->solveMs += ms;
+    c->solveCount++;
+  }
+  c->solveEvUsed = 0;
+}
+
+extern "C" void nusifSolveTiming(Discretization *d, int on)
+{
+  NusifCtx *c = nusifCtxOf(d);
+  if (on && !c->solveEvReady) {
+    for (int q = 0; q < 2 * SOLVE_EV_POOL; q++) HIP_CHECK(hipEventCreate(&c->solveEv[q]));
+    c->solveEvReady = true;
+  }
+  c->solveTiming = on ? 1 : 0;
+}
+
+extern "C" void nusifSolveTimes(Discretization *d, double *ms, long long *solves)
+{
+  NusifCtx *c = nusifCtxOf(d);
+  foldSolveTimes(c);
+  if (ms) *ms = c->solveMs;
+  if (solves) *solves = c->solveCount;
+  c->solveMs    = 0.0;
+  c->solveCount = 0;
+}
+
 extern "C" int nusifRunSteps(Discretization *d, Solver *s, int nsteps, double *trace)
 {
   NusifCtx *c = nusifCtxOf(d);
@@ -162,7 +195,13 @@ extern "C" int nusifRunSteps(Discretization *d, Solver *s, int nsteps, double *t
     computeFG(d);
     computeRHS(d);
     if (c->nt % 100 == 0) normalizePressure(d);
+    const bool timeSolve = c->solveTiming && c->solveEvReady;
+    if (timeSolve) {
+      if (c->solveEvUsed == SOLVE_EV_POOL) foldSolveTimes(c);
+      HIP_CHECK(hipEventRecord(c->solveEv[2 * c->solveEvUsed], nusifStream()));
+    }
     double res = solve(s, d->p, d->rhs);
+    if (timeSolve) HIP_CHECK(hipEventRecord(c->solveEv[2 * c->solveEvUsed++ + 1], nusifStream()));
     adaptUV(d);
     if (devDt) {
       c->dtOnDevice = false;
@@ -179,5 +218,6 @@ extern "C" int nusifRunSteps(Discretization *d, Solver *s, int nsteps, double *t
     c->nt++;
   }
   HIP_CHECK(hipStreamSynchronize(nusifStream()));
+  foldSolveTimes(c);
   return nsteps;
 }

@@ -149,6 +149,8 @@ def _declare_optional(lib):
         "nusifProlongSetCheck": ([C.c_int] * 3 + [C.POINTER(C.c_float), C.POINTER(C.c_float)], None),
         "nusifRunSteps": ([C.POINTER(Discretization), C.POINTER(Solver), C.c_int,
                            C.POINTER(C.c_double)], C.c_int),
+        "nusifSolveTiming": ([C.POINTER(Discretization), C.c_int], None),
+        "nusifSolveTimes": ([C.POINTER(Discretization), _DP, C.POINTER(C.c_longlong)], None),
         # oracle extras
         "orcSetProblemExtras": ([C.c_double, C.c_double], None),
         "orcLastIterations": ([], C.c_int),

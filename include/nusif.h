@@ -180,6 +180,15 @@ void nusifProlongSetCheck(int nic, int njc, int nkc, const float *xc, float *xf)
  * (t, dt, res, iterations) per step. Returns the steps run. */
 int nusifRunSteps(Discretization *d, Solver *s, int nsteps, double *trace);
 
+/* Solve time inside nusifRunSteps: with timing on, each step's solve is
+ * bracketed by HIP events on the library stream (no host wait per step;
+ * the reference times the solve as its SOLVER region, src/solver-rb.c:161,
+ * and rates it in printProfile, src/profiler.c:49-74). nusifSolveTimes
+ * returns the summed milliseconds and the number of timed solves since its
+ * last call, and resets them. */
+void nusifSolveTiming(Discretization *d, int on);
+void nusifSolveTimes(Discretization *d, double *ms, long long *solves);
+
 #ifdef __cplusplus
 }
 #endif

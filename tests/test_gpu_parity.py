@@ -121,6 +121,35 @@ def test_gpu_run_steps_matches_step_calls(gpu_ok):
     b.close()
 
 
+def test_gpu_solve_timing(gpu_ok):
+    """Events around each solve of nusifRunSteps (70 steps: more than one
+    pool of event pairs) count every solve, sum to less than the run's wall
+    time, and change no result."""
+    import time
+    prm = dict(G.TRAJ["c1_rb"][1], imax=24, jmax=20, kmax=16)
+    a = A.Case(lib("gpu_rb"), **prm)
+    b = A.Case(lib("gpu_rb"), **prm)
+    n = 70
+    ta, tb = (C.c_double * (4 * n))(), (C.c_double * (4 * n))()
+    assert a.lib.nusifRunSteps(C.byref(a.d), C.byref(a.s), n, ta) == n
+    b.lib.nusifSolveTiming(C.byref(b.d), 1)
+    t0 = time.perf_counter()
+    assert b.lib.nusifRunSteps(C.byref(b.d), C.byref(b.s), n, tb) == n
+    wall_ms = 1e3 * (time.perf_counter() - t0)
+    ms, solves = C.c_double(), C.c_longlong()
+    b.lib.nusifSolveTimes(C.byref(b.d), C.byref(ms), C.byref(solves))
+    assert solves.value == n
+    assert 0.0 < ms.value < wall_ms
+    b.lib.nusifSolveTimes(C.byref(b.d), C.byref(ms), C.byref(solves))
+    assert solves.value == 0 and ms.value == 0.0  # reset by the read
+    np.testing.assert_array_equal(np.array(ta), np.array(tb))
+    fa, fb = a.fields(), b.fields()
+    for k in A.FIELDS:
+        np.testing.assert_array_equal(fa[k], fb[k], err_msg=k)
+    a.close()
+    b.close()
+
+
 def test_gpu_solve_on_foreign_host_arrays(gpu_ok):
     """solve() on host arrays that are not the Discretization's own is staged
     through the device and gives the same result, and leaves the
