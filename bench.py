@@ -14,7 +14,9 @@ The headline line also times the fp64 path on the same workload and reports
 the speedup (and, for learned, the same FP32 solver without the network),
 the roofline of the dominant kernel (HIP events on the library's stream) and
 a CPU baseline (the oracle's FP64 multigrid on one host core, a bounded
-sample of steps of the same workload).
+sample of steps of the same workload). `rb_sor` times the red-black SOR solver
+on the same state with the reference's own stop test, next to the oracle's
+SOR on one host core (the reference's solver rate, iterations x cells / time).
 """
 import argparse
 import ctypes as C
@@ -157,7 +159,6 @@ def cpu_baseline(case_gpu, n, budget_s=20.0):
     o.set_fields(**case_gpu.fields())
     d = o.d
     cells = n ** 3
-    npad = int(o.shape[0] * o.shape[1] * o.shape[2])
     steps, t_total = 0, 0.0
     while t_total < budget_s and steps < 50:
         t0 = time.perf_counter()
@@ -175,12 +176,80 @@ def cpu_baseline(case_gpu, n, budget_s=20.0):
         steps += 1
         log("[bench] cpu baseline step %d: %.2f s" % (steps, t_total))
     o.close()
-    del npad
     return dict(value=steps / t_total, unit="time steps/s", cores=1, kind="port",
                 sample="%d step(s) of the %d^3 cavity from the GPU state after warmup; oracle "
                        "C restatement (-O3, one host core), FP64 V(2,2) 6-level multigrid to "
                        "1e-6 relative residual" % (steps, n),
                 mlups=cells * steps / t_total * 1e-6)
+
+
+def run_rb(lib, case_gpu, n, steps=3):
+    """The red-black SOR solver (src/solver-rb.c, its own stop test: carried
+    mean-square res < eps^2 or itermax; omega 1.9, eps 1e-3, itermax 1000) on
+    the state the headline run reached: steps/s, SOR iterations per step, the
+    reference's solver rate (iterations x interior cells / solve time,
+    src/profiler.c:61) and the colour pass's rate."""
+    c = A.Case(lib, options={"solver": "rb"}, **cavity_params(n))
+    c.set_fields(**case_gpu.fields())
+    trace = (C.c_double * (4 * steps))()
+    lib.nusifSolveTiming(C.byref(c.d), 1)
+    lib.nusifDeviceSynchronize()
+    t0 = time.perf_counter()
+    lib.nusifRunSteps(C.byref(c.d), C.byref(c.s), steps, trace)
+    lib.nusifDeviceSynchronize()
+    t1 = time.perf_counter()
+    lib.nusifSolveTiming(C.byref(c.d), 0)
+    ms, solves = C.c_double(), C.c_longlong()
+    lib.nusifSolveTimes(C.byref(c.d), C.byref(ms), C.byref(solves))
+    its = sum(trace[4 * q + 3] for q in range(steps))
+    # the iteration's kernel: one fused launch (rb_fused, 24 B/cell) or a
+    # launch per colour pass (rb_pass, 16 B/cell each)
+    for t in (b"rb_fused", b"rb_pass"):
+        lib.nusifTimerEnable(t)
+    lib.nusifTimerReset()
+    lib.nusifRunSteps(C.byref(c.d), C.byref(c.s), 1, trace)
+    lib.nusifDeviceSynchronize()
+    kern = None
+    for t in (b"rb_fused", b"rb_pass"):
+        launches = C.c_int(0)
+        ms_t = lib.nusifTimerRead(t, C.byref(launches))
+        if launches.value and kern is None:
+            us = 1e3 * ms_t / launches.value
+            kern = {"kernel": t.decode(), "us": us,
+                    "GBs": lib.nusifKernelBytes(C.byref(c.d), t) / (us * 1e-6) / 1e9}
+    lib.nusifTimerDisableAll()
+    c.close()
+    cells = float(n) ** 3
+    out = {"value": steps / (t1 - t0), "unit": "time steps/s", "steps": steps,
+           "iterations_per_step": its / steps,
+           "solver_mlups": its * cells / (ms.value * 1e-3) * 1e-6,
+           "solve_ms_per_step": ms.value / max(solves.value, 1),
+           "stop_test": "reference: carried mean-square in-sweep residual < eps^2 = 1e-6, or itermax 1000"}
+    if kern:
+        out["iteration_kernel"] = kern
+    log("[bench] rb: %d steps in %.3f s, %.0f iterations/step" % (steps, t1 - t0, its / steps))
+    return out
+
+
+def cpu_rb(case_gpu, n, iters=40):
+    """The oracle's red-black SOR solve on one host core, capped at `iters`
+    iterations, on the same state: the reference's solver rate"""
+    orc = A.load(A.oracle_lib_path("rb_fast"))
+    o = A.Case(orc, **dict(cavity_params(n), itermax=iters))
+    o.set_fields(**case_gpu.fields())
+    d = o.d
+    orc.computeTimestep(C.byref(d))
+    for fn in ("setBoundaryConditions", "setSpecialBoundaryCondition", "computeFG", "computeRHS"):
+        getattr(orc, fn)(C.byref(d))
+    t0 = time.perf_counter()
+    orc.solve(C.byref(o.s), d.p, d.rhs)
+    t = time.perf_counter() - t0
+    it = o.iterations()
+    o.close()
+    log("[bench] cpu rb: %d iterations in %.2f s" % (it, t))
+    return {"solver_mlups": it * float(n) ** 3 / t * 1e-6, "iterations": it, "seconds": t, "cores": 1,
+            "kind": "port", "sample": "one solve of the %d^3 cavity capped at %d iterations; oracle C "
+                                      "restatement (-O3 -ffast-math, one host core)" % (n, iters)}
 
 
 def main():
@@ -249,20 +318,26 @@ def main():
                           "GBs": round(v["bytes_per_launch"] / (v["us_per_launch"] * 1e-6) / 1e9, 1)}
                       for t, v in res["kernels"].items()}
     if not a.no_compare and a.mode != "fp64":
-        _, ref = run_mode(lib, "fp64", n, a.warmup, a.steps)
+        ref_case, ref = run_mode(lib, "fp64", n, a.warmup, a.steps)
+        ref_case.close()
         f64 = a.steps / ref["seconds"]
         out["fp64_mg"] = {"value": f64, "mlups": cells * f64 * 1e-6,
                           "cycles_per_step": sum(ref["cycles"]) / len(ref["cycles"])}
         out["fp64_mg"].update(solver_rate(ref, cells))
         out["speedup_vs_fp64_mg"] = sps / f64
         if a.mode == "learned":  # the same solver without the network
-            _, mx = run_mode(lib, "mixed", n, a.warmup, a.steps)
+            mx_case, mx = run_mode(lib, "mixed", n, a.warmup, a.steps)
+            mx_case.close()
             vm = a.steps / mx["seconds"]
             out["fp32_mg_without_warmstart"] = {
                 "value": vm, "mlups": cells * vm * 1e-6,
                 "cycles_per_step": sum(mx["cycles"]) / len(mx["cycles"]),
                 "speedup_vs_fp64_mg": vm / f64}
             out["fp32_mg_without_warmstart"].update(solver_rate(mx, cells))
+    if not a.no_compare:
+        out["rb_sor"] = run_rb(lib, case, n)
+        if not a.no_cpu:
+            out["rb_sor"]["cpu_baseline"] = cpu_rb(case, n)
     # what plain streaming kernels reach on this device (256 MiB arrays): the
     # practical ceiling under the 8 TB/s vendor peak
     ms = (C.c_double * 3)()

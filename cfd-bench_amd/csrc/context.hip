@@ -111,6 +111,7 @@ void nusifDefaultOptions(NusifOptions *o)
   o->mgInner     = 1;
   o->unetExtent  = 64;
   o->fgRhs       = 1;
+  o->rbFused     = 1;
   o->lid         = 1.0;
   o->uInflow     = 2.0;
   o->inflowNoise = 0.0;
@@ -226,6 +227,7 @@ extern "C" int nusifSetOption(Parameter *p, const char *key, const char *value)
   if (k == "mg_graph") return parseInt(v.c_str(), &o->mgGraph);
   if (k == "unet_extent") return parseInt(v.c_str(), &o->unetExtent);
   if (k == "fg_rhs") return parseInt(v.c_str(), &o->fgRhs);
+  if (k == "rb_fused") return parseInt(v.c_str(), &o->rbFused);
   if (k == "mg_speculate") return parseInt(v.c_str(), &o->mgSpeculate);
   if (k == "mg_inner") {
     int r = parseInt(v.c_str(), &o->mgInner);

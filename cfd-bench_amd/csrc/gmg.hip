@@ -254,11 +254,24 @@ __device__ inline void stP(__amdgpu_buffer_rsrc_t r, unsigned off, double v)
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, 0, 0);
 }
 
-template <typename T, int TJ, int MINW, bool ZX>
+/* RES (the reference's red-black SOR iteration, src/solver-rb.c:55-148): the
+ * sweep is iteration q of a batch; it returns at once when q >= *stopAt (the
+ * device-side stop test) and adds the in-sweep r^2 of the cells its tile owns
+ * (red and black, each r taken before that cell's update, as
+ * src/solver-rb.c:66-72) into partials[tile]. */
+struct SweepRes {
+  double *partials  = nullptr;
+  const int *stopAt = nullptr;
+  int q             = 0;
+};
+
+template <typename T, int TJ, int MINW, bool ZX, bool RES = false>
 __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ xo, T *__restrict__ xn,
-    const T *__restrict__ b, Geo g, Coef<T> cf, int kc, const int *__restrict__ stop)
+    const T *__restrict__ b, Geo g, Coef<T> cf, int kc, const int *__restrict__ stop,
+    SweepRes rs = SweepRes())
 {
   if (stop && *stop) return;
+  if (RES && rs.q >= *rs.stopAt) return;
   using S = SweepRBShape<T, TJ>;
   constexpr int NT = S::NT, XW = S::XW, RW = S::RW, HALF = S::HALF, NX = S::NX, NR = S::NR;
   __shared__ T X[4][2 * HALF];
@@ -310,6 +323,7 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
   const unsigned sjB  = (unsigned)sj * ES;
   const bool bjIn = bj <= nj;
   const bool bIn0 = bjIn && bi <= ni, bIn1 = bjIn && bi + 1 <= ni;
+  double acc = 0.0; /* RES: this thread's share of the sum of r^2 */
 
   T xr[NX];
   auto fetchX = [&](int k) {
@@ -360,6 +374,11 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
         T vNew = v - cf.factor * rr;
         bool in = kin && ((rIn[r] >> s) & 1);
         Yc[a - RW] = in ? vNew : v;
+        if (RES) { /* the tile's own red cells: columns 2..65, rows 2..TJ+1, planes [k0, k1) */
+          const int li = 2 * rP[r] + s;
+          if (in && k >= k0 && k < k1 && li >= 2 && li <= FT_I + 1 && rLj[r] >= 2 && rLj[r] <= TJ + 1)
+            acc += (double)rr * (double)rr;
+        }
       }
     }
   };
@@ -374,6 +393,7 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
     T rr   = bc - lap;
     T vB   = xc - cf.factor * rr;
     T vR   = Yc[a];
+    if (RES && (s ? bIn1 : bIn0)) acc += (double)rr * (double)rr;
     /* column 2p holds the black value when s = 0, the red one when s = 1 */
     T v0 = s ? vR : vB, v1 = s ? vB : vR;
     const __amdgpu_buffer_rsrc_t pl = planeRsrc(xn, g, k);
@@ -454,6 +474,11 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
     iterate(kk, xB, bB, bbB);
     if (kk + 1 <= k1) iterate(kk + 1, xA, bA, bbA);
   }
+  if constexpr (RES) {
+    __shared__ double lsum[NT / 64];
+    const double sum = blockSum(acc, lsum);
+    if (tid == 0) rs.partials[t] = sum;
+  }
 }
 
 /* Launch shape of the fused sweep on a level, from a sweep of tile heights
@@ -524,6 +549,42 @@ template <typename T> static void launchSweep(const LevelT<T> &L, SweepShape s, 
 
 /* true when a level's smoother is the fused sweep */
 template <typename T> static bool fusedLevel(const LevelT<T> &L) { return L.g.ni >= FUSED_MIN_EXTENT && L.g.nk >= 4; }
+
+/* ---- the reference's red-black SOR iteration as one fused sweep ----------
+ * Both colour passes of iteration q (src/solver-rb.c:55-79: pass 0 = odd
+ * cells first) and the Neumann ghost copy after them (src/solver-rb.c:129-148:
+ * the sweep writes the face ghosts of the new values; during both passes the
+ * stencil reads the old ghosts, as the reference does) in one launch, from
+ * xo into xn. Same operands in the same order as the colour-pass kernel, so
+ * the same bits; the in-sweep sum of r^2 goes to partials[tile]. */
+bool rbFusedFits(const Geo &g) { return g.ni >= FUSED_MIN_EXTENT && g.nk >= 4; }
+
+int rbFusedTiles(const Geo &g)
+{
+  const SweepShape s = sweepShape<double>(g, 0);
+  return (int)sweepTiles(g, s.tj, s.kc);
+}
+
+void launchRBFused(const double *xo, double *xn, const double *rhs, const Geo &g, const SweepCoef &sc,
+    double *partials, const int *stopAt, int q)
+{
+  const Coef<double> cf = {sc.idx2, sc.idy2, sc.idz2, sc.factor};
+  const SweepShape s    = sweepShape<double>(g, 0);
+  const dim3 grd        = fusedGrid(g, s);
+  SweepRes rs;
+  rs.partials = partials;
+  rs.stopAt   = stopAt;
+  rs.q        = q;
+  if (s.tj == 16)
+    NUSIF_TIMED("rb_fused", (kSweepRB<double, 16, 4, false, true><<<grd, SweepRBShape<double, 16>::NT, 0,
+                                nusifStream()>>>(xo, xn, rhs, g, cf, s.kc, nullptr, rs)));
+  else if (s.tj == 8)
+    NUSIF_TIMED("rb_fused", (kSweepRB<double, 8, 6, false, true><<<grd, SweepRBShape<double, 8>::NT, 0,
+                                nusifStream()>>>(xo, xn, rhs, g, cf, s.kc, nullptr, rs)));
+  else
+    NUSIF_TIMED("rb_fused", (kSweepRB<double, 4, 8, false, true><<<grd, SweepRBShape<double, 4>::NT, 0,
+                                nusifStream()>>>(xo, xn, rhs, g, cf, s.kc, nullptr, rs)));
+}
 
 /* b_c(I,J,K) = (1/8) sum over the 8 children of (b - A x); the coarse x is
  * zeroed (interior and face ghosts: the coarse cycle's initial guess). */

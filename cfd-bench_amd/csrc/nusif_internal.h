@@ -114,6 +114,7 @@ struct NusifOptions {
   char unetWeights[512];
   int unetExtent;  /* warm start: the network runs on the first level of extent <= this (64) */
   int fgRhs;       /* computeFG also forms rhs (1, default), computeRHS then skips its pass */
+  int rbFused;     /* rb: one launch per iteration where the grid allows (1, default) */
 };
 void nusifDefaultOptions(NusifOptions *o);
 /* Options registered for `p` (defaults when none were set). */
@@ -259,5 +260,11 @@ struct SweepCoef {
 };
 SweepCoef sweepCoef(const Grid *g, double omega);
 void launchPressureGhosts(const Geo &g, double *p, hipStream_t st);
+/* One red-black SOR iteration (both colours and the ghost copy) in one
+ * launch, xo -> xn, for grids the fused sweep covers (gmg.hip) */
+bool rbFusedFits(const Geo &g);
+int rbFusedTiles(const Geo &g);
+void launchRBFused(const double *xo, double *xn, const double *rhs, const Geo &g, const SweepCoef &sc,
+    double *partials, const int *stopAt, int q);
 
 #endif

@@ -128,15 +128,16 @@ __global__ __launch_bounds__(256) void kRBGhosts(double *p, Geo g, int q, const 
   ghostsAt(p, g, a, b);
 }
 
-/* Iteration q's fold: both passes' partials (pass 0 first) into the carried
- * residual (src/solver-rb.c:150-151), then the stop test (src/solver-rb.c:54). */
+/* Iteration q's fold: the partials of both passes (pass 0 first; one set
+ * when one launch ran the whole iteration) into the carried residual
+ * (src/solver-rb.c:150-151), then the stop test (src/solver-rb.c:54). */
 __global__ __launch_bounds__(256) void kRBFold(int q, RBState *st, const double *__restrict__ partials,
-    int nPart, double n, double epssq, int itermax)
+    int nPart, int sets, double n, double epssq, int itermax)
 {
   __shared__ double lds[4];
   if (q >= st->stopAt) return;
   double s0 = foldSum(partials, nPart, 1, 0, lds);
-  double s1 = foldSum(partials + nPart, nPart, 1, 0, lds);
+  double s1 = sets == 2 ? foldSum(partials + nPart, nPart, 1, 0, lds) : 0.0;
   if (threadIdx.x == 0) {
     double res = (st->res + (s0 + s1)) / n;
     int it     = q + 1;
@@ -230,7 +231,15 @@ double rbSolve(NusifSolverCtx *sc, double *p, const double *rhs, double **pOut)
   double epssq      = s->eps * s->eps;
   double n          = (double)(s->grid->imax * s->grid->jmax * s->grid->kmax);
   int itermax       = s->itermax;
-  if ((size_t)2 * L.nPart > c->nPartials) NUSIF_FATAL("partials buffer too small");
+  /* fused: iteration q reads buf[q & 1] and writes buf[(q + 1) & 1]; the
+   * partner starts as a copy of p (its edge and corner ghosts, which no
+   * iteration writes, stay equal to p's) */
+  const bool fused = c->opt.rbFused && rbFusedFits(g);
+  const int nPart  = fused ? rbFusedTiles(g) : L.nPart;
+  if ((size_t)(fused ? 1 : 2) * nPart > c->nPartials) NUSIF_FATAL("partials buffer too small");
+  double *buf[2] = {p, c->pAlt};
+  if (fused)
+    HIP_CHECK(hipMemcpyAsync(c->pAlt, p, (size_t)g.n * sizeof(double), hipMemcpyDeviceToDevice, nusifStream()));
   *pOut = p;
 
   int stopAt = (1.0 >= epssq && 0 < itermax) ? itermax : 0;
@@ -242,10 +251,15 @@ double rbSolve(NusifSolverCtx *sc, double *p, const double *rhs, double **pOut)
     int end = launched + batch;
     if (end > itermax) end = itermax;
     for (int q = launched; q < end; q++) {
+      if (fused) {
+        launchRBFused(buf[q & 1], buf[(q + 1) & 1], rhs, g, cf, c->partials, &stateBuf(c)->stopAt, q);
+        kRBFold<<<1, 256, 0, nusifStream()>>>(q, stateBuf(c), c->partials, nPart, 1, n, epssq, itermax);
+        continue;
+      }
       launchPass(c, g, L, p, rhs, cf, 1, q, c->partials, 1);
       launchPass(c, g, L, p, rhs, cf, 0, q, c->partials + L.nPart, 1);
       kRBGhosts<<<gg, dim3(64, 4), 0, nusifStream()>>>(p, g, q, stateBuf(c));
-      kRBFold<<<1, 256, 0, nusifStream()>>>(q, stateBuf(c), c->partials, L.nPart, n, epssq, itermax);
+      kRBFold<<<1, 256, 0, nusifStream()>>>(q, stateBuf(c), c->partials, L.nPart, 2, n, epssq, itermax);
     }
     launched = end;
     getState(c);
@@ -255,6 +269,7 @@ double rbSolve(NusifSolverCtx *sc, double *p, const double *rhs, double **pOut)
   getState(c);
   sc->lastIterations = stateHost(c)->it;
   sc->predicted      = sc->lastIterations;
+  if (fused) *pOut = buf[sc->lastIterations & 1];
   return stateHost(c)->res;
 }
 

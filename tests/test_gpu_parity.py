@@ -121,6 +121,29 @@ def test_gpu_run_steps_matches_step_calls(gpu_ok):
     b.close()
 
 
+@pytest.mark.parametrize("shape", [(32, 32, 32), (70, 37, 9), (48, 20, 4)])
+def test_gpu_rb_fused_iteration_matches_colour_passes(gpu_ok, shape):
+    """One launch per SOR iteration (rb_fused 1, the default) against one
+    launch per colour pass plus the ghost pass (rb_fused 0): the same p, u,
+    v, w bits and the same iteration counts over a run (ragged tiles and
+    k-chunks on the odd shapes; the result ends in either buffer)."""
+    prm = dict(G.TRAJ["c1_rb"][1], imax=shape[0], jmax=shape[1], kmax=shape[2])
+    a = A.Case(lib("gpu_rb"), options={"rb_fused": 1}, **prm)
+    b = A.Case(lib("gpu_rb"), options={"rb_fused": 0}, **prm)
+    its_a, its_b = [], []
+    for _ in range(8):
+        a.step()
+        b.step()
+        its_a.append(a.iterations())
+        its_b.append(b.iterations())
+    assert its_a == its_b
+    fa, fb = a.fields(), b.fields()
+    for k in A.FIELDS:
+        np.testing.assert_array_equal(fa[k], fb[k], err_msg=k)
+    a.close()
+    b.close()
+
+
 def test_gpu_solve_timing(gpu_ok):
     """Events around each solve of nusifRunSteps (70 steps: more than one
     pool of event pairs) count every solve, sum to less than the run's wall
