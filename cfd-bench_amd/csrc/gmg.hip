@@ -571,6 +571,18 @@ void launchRBFused(const double *xo, double *xn, const double *rhs, const Geo &g
   const Coef<double> cf = {sc.idx2, sc.idy2, sc.idz2, sc.factor};
   const SweepShape s    = sweepShape<double>(g, 0);
   const dim3 grd        = fusedGrid(g, s);
+  if (!partials) { /* a sweep without the residual (the reference MG's smoother) */
+    if (s.tj == 16)
+      NUSIF_TIMED("rb_fused", (kSweepRB<double, 16, 4, false><<<grd, SweepRBShape<double, 16>::NT, 0,
+                                  nusifStream()>>>(xo, xn, rhs, g, cf, s.kc, nullptr)));
+    else if (s.tj == 8)
+      NUSIF_TIMED("rb_fused", (kSweepRB<double, 8, 6, false><<<grd, SweepRBShape<double, 8>::NT, 0,
+                                  nusifStream()>>>(xo, xn, rhs, g, cf, s.kc, nullptr)));
+    else
+      NUSIF_TIMED("rb_fused", (kSweepRB<double, 4, 8, false><<<grd, SweepRBShape<double, 4>::NT, 0,
+                                  nusifStream()>>>(xo, xn, rhs, g, cf, s.kc, nullptr)));
+    return;
+  }
   SweepRes rs;
   rs.partials = partials;
   rs.stopAt   = stopAt;

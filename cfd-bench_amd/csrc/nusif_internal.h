@@ -114,7 +114,7 @@ struct NusifOptions {
   char unetWeights[512];
   int unetExtent;  /* warm start: the network runs on the first level of extent <= this (64) */
   int fgRhs;       /* computeFG also forms rhs (1, default), computeRHS then skips its pass */
-  int rbFused;     /* rb: one launch per iteration where the grid allows (1, default) */
+  int rbFused;     /* rb, mg: one launch per sweep where the grid allows (1, default) */
 };
 void nusifDefaultOptions(NusifOptions *o);
 /* Options registered for `p` (defaults when none were set). */
@@ -250,7 +250,7 @@ void launchAdaptUV(NusifCtx *c, const Discretization *d);
 
 /* Solvers */
 double rbSolve(NusifSolverCtx *sc, double *p, const double *rhs, double **pOut);
-double mgRefSolve(NusifSolverCtx *sc, double *p, const double *rhs);
+double mgRefSolve(NusifSolverCtx *sc, double *p, const double *rhs, double **pOut);
 double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhs);
 void gmgFree(GmgState *g);
 
@@ -261,7 +261,8 @@ struct SweepCoef {
 SweepCoef sweepCoef(const Grid *g, double omega);
 void launchPressureGhosts(const Geo &g, double *p, hipStream_t st);
 /* One red-black SOR iteration (both colours and the ghost copy) in one
- * launch, xo -> xn, for grids the fused sweep covers (gmg.hip) */
+ * launch, xo -> xn, for grids the fused sweep covers (gmg.hip); with
+ * partials null, without the in-sweep residual and the stop test */
 bool rbFusedFits(const Geo &g);
 int rbFusedTiles(const Geo &g);
 void launchRBFused(const double *xo, double *xn, const double *rhs, const Geo &g, const SweepCoef &sc,

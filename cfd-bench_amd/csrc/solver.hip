@@ -109,7 +109,14 @@ extern "C" double solve(Solver *s, double *p, const double *rhs)
       c->p    = pOut;
     }
   } break;
-  case SOLVER_MG: res = mgRefSolve(sc, c->p, c->rhs); break;
+  case SOLVER_MG: {
+    double *pOut = c->p;
+    res          = mgRefSolve(sc, c->p, c->rhs, &pOut);
+    if (pOut != c->p) { /* ping-pong ended on the partner buffer */
+      c->pAlt = c->p;
+      c->p    = pOut;
+    }
+  } break;
   case SOLVER_GMG: res = gmgSolve(sc, c->p, c->rhs); break;
   default: NUSIF_FATAL("unknown solver kind %d", sc->kind);
   }

@@ -16,6 +16,8 @@
 #include <limits.h>
 #include <math.h>
 
+#include <utility>
+
 #include "nusif_internal.h"
 #include "reduce.h"
 
@@ -276,7 +278,7 @@ double rbSolve(NusifSolverCtx *sc, double *p, const double *rhs, double **pOut)
 /* Observable behaviour of src/solver-mg.c:299-402 on p. The coarse levels'
  * work never reaches p or the returned residual (`correct` adds s->e[0],
  * which nothing writes; src/solver-mg.c:87,96,334), so it is not run. */
-double mgRefSolve(NusifSolverCtx *sc, double *p, const double *rhs)
+double mgRefSolve(NusifSolverCtx *sc, double *p, const double *rhs, double **pOut)
 {
   NusifCtx *c     = sc->dctx;
   const Solver *s = sc->owner;
@@ -286,7 +288,22 @@ double mgRefSolve(NusifSolverCtx *sc, double *p, const double *rhs)
   double n        = (double)(s->grid->imax * s->grid->jmax * s->grid->kmax);
   putState(c, 1.0, 0, INT_MAX);
   bool coarsest = (s->levels - 1 == 0) || g.ni < 2 || g.nj < 2 || g.nk < 2;
+  /* a sweep followed by the ghost copy is one fused launch into the partner
+   * buffer (which starts as a copy of p, so the edge and corner ghosts no
+   * sweep writes agree); without the ghost copy (the coarsest branch) it
+   * stays two colour passes in place */
+  const bool fused = c->opt.rbFused && rbFusedFits(g) && !coarsest;
+  if (fused)
+    HIP_CHECK(hipMemcpyAsync(c->pAlt, p, (size_t)g.n * sizeof(double), hipMemcpyDeviceToDevice, nusifStream()));
+  double *other = c->pAlt;
+  *pOut         = p;
   auto smooth = [&](bool ghosts) {
+    if (fused) {
+      launchRBFused(p, other, rhs, g, cf, nullptr, nullptr, 0);
+      std::swap(p, other);
+      *pOut = p;
+      return;
+    }
     launchPass(c, g, L, p, rhs, cf, 1, 0, c->partials, 0);
     launchPass(c, g, L, p, rhs, cf, 0, 0, c->partials, 0);
     if (ghosts) launchPressureGhosts(g, p, nusifStream());

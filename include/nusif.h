@@ -79,11 +79,11 @@ void nusifFreeSolver(Solver *s);
  *                         in (1, default; computeRHS recomputes when f, g, h,
  *                         rhs or dt changed in between); 0: computeRHS runs
  *                         its own pass. Same values either way
- *   rb_fused      0 | 1   rb solver: each iteration (both colour passes and the
- *                         ghost copy) in one launch where the grid allows
- *                         (imax >= 32, kmax >= 4; 1, default); 0: a launch per
- *                         colour pass. Same p either way; the sum of r^2 is
- *                         added in another order
+ *   rb_fused      0 | 1   rb and mg solvers: each sweep (both colour passes
+ *                         and the ghost copy) in one launch where the grid
+ *                         allows (imax >= 32, kmax >= 4; 1, default); 0: a
+ *                         launch per colour pass. Same p either way; rb's sum
+ *                         of r^2 is added in another order
  * Returns 0 on success, -1 for an unknown key, -2 for a bad value. */
 int nusifSetOption(Parameter *p, const char *key, const char *value);
 

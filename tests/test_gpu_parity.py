@@ -144,6 +144,26 @@ def test_gpu_rb_fused_iteration_matches_colour_passes(gpu_ok, shape):
     b.close()
 
 
+@pytest.mark.parametrize("shape", [(32, 32, 32), (70, 37, 9)])
+def test_gpu_mg_fused_sweeps_match_colour_passes(gpu_ok, shape):
+    """The reference MG's smoothing sweeps (sweep + ghost copy) as one fused
+    launch each against colour passes + ghost pass: the same bits, residual
+    included (V(2,2): the result ends on p after four sweeps, V(1,2) on the
+    partner buffer after three)."""
+    for pre, post in ((2, 2), (1, 2)):
+        prm = dict(G.TRAJ["c1_rb"][1], imax=shape[0], jmax=shape[1], kmax=shape[2], levels=3,
+                   presmooth=pre, postsmooth=post)
+        a = A.Case(lib("gpu_mg"), options={"rb_fused": 1}, **prm)
+        b = A.Case(lib("gpu_mg"), options={"rb_fused": 0}, **prm)
+        for _ in range(6):
+            assert a.step() == b.step()
+        fa, fb = a.fields(), b.fields()
+        for k in A.FIELDS:
+            np.testing.assert_array_equal(fa[k], fb[k], err_msg="%s V(%d,%d)" % (k, pre, post))
+        a.close()
+        b.close()
+
+
 def test_gpu_solve_timing(gpu_ok):
     """Events around each solve of nusifRunSteps (70 steps: more than one
     pool of event pairs) count every solve, sum to less than the run's wall
