@@ -374,7 +374,8 @@ void warmstartCoarseNormLaunch(WarmStart *w, const float *rFine, bool level1)
   const WsLevel &L = w->lv[w->Ln];
   dim3 g2 = sumGrid(L.g);
   kSumSq2<<<g2, dim3(WTX, WTY), 0, st>>>(src, src, L.g, w->partials);
-  kFoldSumSq2<<<1, 256, 0, st>>>(w->partials, (int)(g2.x * g2.y * g2.z), w->scal + 2);
+  /* one component into scal[2]: scal[3] keeps the fine ||r0||^2 of the input */
+  kFoldComp0<<<1, 256, 0, st>>>(w->partials, (int)(g2.x * g2.y * g2.z), w->scal + 2);
 }
 
 /* enqueue the copy of scal[0..3] ([0] = sum (R r0)^2, [2] = the last

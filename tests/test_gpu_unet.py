@@ -89,3 +89,47 @@ def test_gpu_unet_train_and_warm_start(tmp_path, gpu_ok):
     print("warm start used per step:", used)
     assert set(used) <= {0, 1}
     c.close()
+
+
+@pytest.mark.parametrize("n", [64, 128])
+def test_gpu_warmstart_residual_report(n, gpu_ok):
+    """nusifWarmstartResiduals after a warm-started solve: [0] is the fine
+    ||rhs - A p|| of the previous-step pressure, recomputed here with numpy
+    from the same p (Neumann face ghosts, as the solve sets them) and rhs;
+    the decision taken is the comparison of the restricted pair [3] < [2].
+    64^3: the network runs on the finest grid, 128^3: on the first coarse
+    level (input averaged onto it in the residual pass)."""
+    weights = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "cfd-bench_amd",
+                           "weights", "unet_cavity.bin")
+    prm = dict(name="dcavity", re=1000.0, imax=n, jmax=n, kmax=n, tau=0.5, te=1e9, levels=5,
+               presmooth=2, postsmooth=2)
+    opts = {"solver": "gmg", "precision": "mixed", "mg_tol": 1e-6, "mg_omega": 1.3,
+            "warmstart": "unet", "unet_weights": weights}
+    c = A.Case(lib("gpu_rb"), options=opts, **prm)
+    c.run(4)
+    d, L = c.d, c.lib
+    for fn in ("computeTimestep", "setBoundaryConditions", "setSpecialBoundaryCondition", "computeFG",
+               "computeRHS"):
+        c.call(fn)
+    f = c.fields()
+    p, rhs = f["p"].copy(), f["rhs"]
+    p[0, 1:-1, 1:-1], p[-1, 1:-1, 1:-1] = p[1, 1:-1, 1:-1], p[-2, 1:-1, 1:-1]
+    p[1:-1, 0, 1:-1], p[1:-1, -1, 1:-1] = p[1:-1, 1, 1:-1], p[1:-1, -2, 1:-1]
+    p[1:-1, 1:-1, 0], p[1:-1, 1:-1, -1] = p[1:-1, 1:-1, 1], p[1:-1, 1:-1, -2]
+    g = d.grid
+    idx2, idy2, idz2 = 1.0 / g.dx ** 2, 1.0 / g.dy ** 2, 1.0 / g.dz ** 2
+    pc = p[1:-1, 1:-1, 1:-1]
+    lap = ((p[1:-1, 1:-1, 2:] - 2.0 * pc + p[1:-1, 1:-1, :-2]) * idx2
+           + (p[1:-1, 2:, 1:-1] - 2.0 * pc + p[1:-1, :-2, 1:-1]) * idy2
+           + (p[2:, 1:-1, 1:-1] - 2.0 * pc + p[:-2, 1:-1, 1:-1]) * idz2)
+    r0 = float(np.sqrt(np.sum((rhs[1:-1, 1:-1, 1:-1] - lap) ** 2)))
+    L.solve(C.byref(c.s), d.p, d.rhs)
+    out = (C.c_double * 4)()
+    L.nusifWarmstartResiduals(C.byref(c.s), out)
+    used = L.nusifWarmstartUsed(C.byref(c.s))
+    print("n %d: fine r0 %.6e (numpy %.6e), fine r_guess %.6e, restricted %.6e / %.6e, used %d"
+          % (n, out[0], r0, out[1], out[2], out[3], used))
+    assert out[0] == pytest.approx(r0, rel=1e-9)
+    assert out[1] > 0.0 and out[2] > 0.0 and out[3] > 0.0
+    assert used == (1 if out[3] < out[2] else 0)
+    c.close()
