@@ -1819,7 +1819,8 @@ static int nextGroupAfter(const GmgState *G, int first, int cycles, double rel0,
   return std::max(1, std::min(next, G->maxCycles));
 }
 
-template <typename Body> static int runCycles(NusifCtx *c, GmgState *G, double &rel, double &nb, Body body)
+template <typename Body>
+static int runCycles(NusifCtx *c, GmgState *G, NusifSolverCtx *sc, double &rel, double &nb, Body body)
 {
   dim3 ng        = normGrid(c->geo);
   const int nbk  = (int)(ng.x * ng.y * ng.z);
@@ -1839,7 +1840,21 @@ template <typename Body> static int runCycles(NusifCtx *c, GmgState *G, double &
         kFoldNorms<<<1, 256, 0, st>>>(c->partials, nbk, c->scal->norm2, c->scalHostDev->norm2);
       });
       cycles += m;
-      double rn = readNorms(c, &nb);
+      double rn;
+      if (sc->afterFirst && sc->afterFirstState == 0) {
+        /* the group's p is in c->p (the bodies keep it there): queue the
+         * caller's work, then wait for the group's fold alone (it writes the
+         * norms into the pinned mirror), not for that work */
+        HIP_CHECK(hipEventRecord(G->ev[0], st));
+        sc->afterFirst(sc->afterFirstArg);
+        sc->afterFirstState = 1;
+        HIP_CHECK(hipEventSynchronize(G->ev[0]));
+        nb = sqrt(c->scalHost->norm2[1]);
+        rn = sqrt(c->scalHost->norm2[0]);
+      } else {
+        if (sc->afterFirst) sc->afterFirstState = 2;
+        rn = readNorms(c, &nb);
+      }
       rel       = rn / nb;
       if (cycles == m) relFirst = rel;
     }
@@ -1986,11 +2001,15 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
     double rn = residualNorm(c, G, L0.x, rhs, nullptr, &nb);
     rel       = nb > 0.0 ? rn / nb : 0.0;
     dim3 ng   = normGrid(g);
-    cycles    = runCycles(c, G, rel, nb, [&](int m) {
+    cycles    = runCycles(c, G, sc, rel, nb, [&](int m) {
       for (int q = 0; q < m; q++) vcycle<double>(G, G->L64, 0);
       NUSIF_TIMED("gmg_residual64", (kResidualLds<false><<<xcdGrid(ng), dim3(GTX, RTY), 0, nusifStream()>>>(
                                         L0.x, nullptr, rhs, g, G->cf0, nullptr, nullptr, nullptr, c->partials,
                                         ng.x, ng.y, ng.z, gCycleStop)));
+      if (L0.x != c->p) { /* keep the current iterate in c->p */
+        c->pAlt = c->p;
+        c->p    = L0.x;
+      }
     });
     if (L0.x != c->p) { /* the result sits in the partner buffer */
       c->pAlt = c->p;
@@ -2004,7 +2023,7 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
     double rn = haveInitial ? rnInit : residualNorm(c, G, p, rhs, L0.b, &nb, l0NeedsZero(G) ? L0.x : nullptr);
     rel       = nb > 0.0 ? rn / nb : 0.0;
     dim3 ng   = normGrid(g);
-    cycles    = runCycles(c, G, rel, nb, [&](int m) {
+    cycles    = runCycles(c, G, sc, rel, nb, [&](int m) {
       for (int q = 0; q < m; q++) vcycle<float>(G, G->L32, 0, q > 0);
       NUSIF_TIMED("gmg_correct_residual",
           (kResidualLds<true><<<xcdGrid(ng), dim3(GTX, RTY), 0, nusifStream()>>>(c->p, L0.x, rhs, g, G->cf0,

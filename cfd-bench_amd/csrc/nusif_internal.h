@@ -196,6 +196,14 @@ struct NusifSolverCtx {
   WarmStart *ws; /* learned warm start / training-pair capture (gmg only) */
   double wsRes[4]; /* last warm start: fine ||r0||, fine ||r_guess||, restricted both */
   long long wsTried, wsTaken; /* network guesses formed / accepted since init */
+  /* set by nusifRunSteps for one solve: the gmg cycle loop calls
+   * afterFirst(afterFirstArg) once the first group of cycles is enqueued,
+   * before the host waits for its norms, so the work queued there overlaps
+   * that round trip. afterFirstState: 0 not called, 1 called with p final,
+   * 2 called but later cycles changed p */
+  void (*afterFirst)(void *) = nullptr;
+  void *afterFirstArg        = nullptr;
+  int afterFirstState        = 0;
 };
 
 hipStream_t nusifStream(void);

@@ -184,7 +184,8 @@ extern "C" void nusifSolveTimes(Discretization *d, double *ms, long long *solves
 
 extern "C" int nusifRunSteps(Discretization *d, Solver *s, int nsteps, double *trace)
 {
-  NusifCtx *c = nusifCtxOf(d);
+  NusifCtx *c        = nusifCtxOf(d);
+  NusifSolverCtx *sc = nusifSolverCtxOf(s);
   /* with an adaptive dt, the CFL fold forms dt on the device and computeFG,
    * computeRHS and adaptUV read it there: the step is enqueued without
    * waiting for the maxima. The host learns dt (same bits as
@@ -207,9 +208,18 @@ extern "C" int nusifRunSteps(Discretization *d, Solver *s, int nsteps, double *t
       if (c->solveEvUsed == SOLVE_EV_POOL) foldSolveTimes(c);
       HIP_CHECK(hipEventRecord(c->solveEv[2 * c->solveEvUsed], nusifStream()));
     }
-    double res = solve(s, d->p, d->rhs);
+    /* gmg: adaptUV is queued as soon as the first cycle group is, so it runs
+     * while the host reads that group's norms; it runs again when the solve
+     * needed more cycles after it (it reads only f, g, h and p). Not with
+     * solve timing, whose events bracket the solve alone. */
+    sc->afterFirst      = timeSolve ? nullptr : [](void *a) { adaptUV((Discretization *)a); };
+    sc->afterFirstArg   = d;
+    sc->afterFirstState = 0;
+    double res          = solve(s, d->p, d->rhs);
+    const int early     = sc->afterFirstState;
+    sc->afterFirst      = nullptr;
     if (timeSolve) HIP_CHECK(hipEventRecord(c->solveEv[2 * c->solveEvUsed++ + 1], nusifStream()));
-    adaptUV(d);
+    if (early != 1) adaptUV(d);
     if (devDt) {
       c->dtOnDevice = false;
       HIP_CHECK(hipEventSynchronize(c->dtEvent));

@@ -10,9 +10,10 @@
  * Output y, rescaled: dp = y * s_r * (n h_Ln / pi)^2 (the inverse Laplacian of
  * the smoothest mode on the network grid of n cells), prolongated (trilinear) to the
  * finest grid; guess = p_prev + dp. The guess replaces p_prev only when its
- * residual ||rhs - A guess|| is smaller than ||r0||; otherwise the solve
- * starts from p_prev. Either way the multigrid iterates to the same
- * tolerance, so the answer's accuracy does not depend on the network.
+ * residual rhs - A guess, restricted to the network grid, has a smaller norm
+ * than r0 restricted there; otherwise the solve starts from p_prev. Either
+ * way the multigrid iterates to the same tolerance, so the answer's accuracy
+ * does not depend on the network.
  *
  * Training pairs come from the solver itself: with capture on, each solve
  * stores the input above and the target (p_solved - p_prev) restricted to
@@ -44,7 +45,7 @@ struct WarmStart {
   double h2;              /* output scale (n h_Ln / pi)^2, x spacing */
   WsLevel lv[12];
   uint16_t *x;            /* network input n^3 x 8 */
-  double *pPrev;          /* FP64 copy of p for capture */
+  double *pPrev;          /* FP64 copy of p for capture (capture only) */
   double *scal;           /* [0] sum r0^2, [1] sum b^2 on level Ln, [2] guess residual
                            * on level Ln, [3] sum r0^2 on level 0 */
   double *scalHost;       /* pinned mirror of scal */
@@ -278,7 +279,8 @@ WarmStart *warmstartCreate(NusifCtx *c, UnetModel *net, bool capture)
   HIP_CHECK(hipMalloc((void **)&w->scal, 4 * sizeof(double)));
   HIP_CHECK(hipHostMalloc((void **)&w->scalHost, 4 * sizeof(double), hipHostMallocDefault));
   HIP_CHECK(hipMalloc((void **)&w->partials, 65536 * sizeof(double)));
-  HIP_CHECK(hipMalloc((void **)&w->pPrev, (size_t)c->geo.n * sizeof(double)));
+  w->pPrev = nullptr;
+  if (capture) HIP_CHECK(hipMalloc((void **)&w->pPrev, (size_t)c->geo.n * sizeof(double)));
   return w;
 }
 
@@ -295,7 +297,7 @@ void warmstartFree(WarmStart *w)
   HIP_CHECK(hipFree(w->scal));
   HIP_CHECK(hipHostFree(w->scalHost));
   HIP_CHECK(hipFree(w->partials));
-  HIP_CHECK(hipFree(w->pPrev));
+  if (w->pPrev) HIP_CHECK(hipFree(w->pPrev));
   for (auto p : w->dsX) HIP_CHECK(hipFree(p));
   for (auto p : w->dsT) HIP_CHECK(hipFree(p));
   delete w;

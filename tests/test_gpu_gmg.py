@@ -206,6 +206,34 @@ def test_gpu_gmg_graph_replay_equals_eager_launches(precision, gpu_ok):
         c.close()
 
 
+@pytest.mark.parametrize("opts", [{"precision": "fp64", "mg_inner": 0}, {"precision": "mixed", "mg_inner": 0},
+                                  {"precision": "mixed", "mg_inner": 1}, {"precision": "mixed", "mg_speculate": 1}],
+                         ids=["fp64", "mixed", "mixed-inner1", "mixed-speculate"])
+def test_gpu_run_steps_early_adapt_equals_step_calls(opts, gpu_ok):
+    """nusifRunSteps queues adaptUV right after the first cycle group, before
+    the host reads its norms, and runs it again when later groups change p
+    (mg_inner 1: every solve needs more than its first group). Against the
+    same steps called one by one (adaptUV after the solve): every field and
+    the trace (t, dt, residual, cycles) equal bit for bit."""
+    print("run steps early adaptUV", opts, flush=True)
+    prm = dict(G.TRAJ["c1_rb"][1], imax=64, jmax=64, kmax=64, re=1000.0, levels=5, presmooth=2,
+               postsmooth=2)
+    o = dict({"solver": "gmg", "mg_tol": 1e-8}, **opts)
+    n = 8
+    a = A.Case(lib("gpu_rb"), options=o, **prm).run(n)
+    b = A.Case(lib("gpu_rb"), options=o, **prm)
+    tr = (C.c_double * (4 * n))()
+    assert b.lib.nusifRunSteps(C.byref(b.d), C.byref(b.s), n, tr) == n
+    t = np.array(tr).reshape(n, 4)
+    np.testing.assert_array_equal(t[:, :3], np.array(a.trace))
+    assert t[-1, 3] == a.iterations()
+    fa, fb = a.fields(), b.fields()
+    for k in A.FIELDS:
+        np.testing.assert_array_equal(fa[k], fb[k], err_msg=k)
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("shape,levels", [((128, 144, 240), 5), ((64, 12, 14), 2)])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_gpu_gmg_tall_tile_sweep_equals_default(shape, levels, precision, gpu_ok):
