@@ -103,13 +103,24 @@ __global__ void kReluMask(float *d, const uint16_t *act, size_t n)
   GRID_STRIDE(q, n) if (!(bf2f(act[q]) > 0.f)) d[q] = 0.f;
 }
 
-/* y[v] = hb + sum_c hw[c] act[v][c] */
+/* y[v] = hb + sum_c hw[c] act[v][c], the channels added in order; a voxel's
+ * channels are read eight at a time (one 16 B load) when c is a multiple of
+ * eight */
 __global__ void kHeadFwd(const uint16_t *act, int c, const float *hw, float *y, size_t nv)
 {
   GRID_STRIDE(v, nv)
   {
     float s = hw[c];
-    for (int k = 0; k < c; k++) s += hw[k] * bf2f(act[v * c + k]);
+    if ((c & 7) == 0) {
+      for (int k0 = 0; k0 < c; k0 += 8) {
+        const uint4 q      = *(const uint4 *)(act + v * c + k0);
+        const uint32_t h[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int u = 0; u < 8; u++) s += hw[k0 + u] * bf2f((uint16_t)(h[u >> 1] >> (16 * (u & 1))));
+      }
+    } else {
+      for (int k = 0; k < c; k++) s += hw[k] * bf2f(act[v * c + k]);
+    }
     y[v] = s;
   }
 }

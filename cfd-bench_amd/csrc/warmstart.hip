@@ -48,7 +48,8 @@ struct WarmStart {
   double *pPrev;          /* FP64 copy of p for capture (capture only) */
   double *scal;           /* [0] sum r0^2, [1] sum b^2 on level Ln, [2] guess residual
                            * on level Ln, [3] sum r0^2 on level 0 */
-  double *scalHost;       /* pinned mirror of scal */
+  double *scalHost;       /* pinned, host-coherent mirror of scal (the folds write both) */
+  double *scalHostDev;    /* its device address */
   double *partials;
   /* capture */
   bool capture;
@@ -68,6 +69,47 @@ __global__ __launch_bounds__(256) void kRestrictAvg(const Tin *__restrict__ f, G
   for (int e = 0; e < 8; e++)
     s += (double)f[gf.at(2 * I - 1 + (e & 1), 2 * J - 1 + ((e >> 1) & 1), 2 * K - 1 + (e >> 2))];
   c[gc.at(I, J, K)] = (float)(0.125 * s);
+}
+
+/* 8-child averages of one or two level arrays (f1 -> c1, and f2 -> c2 when
+ * f2 is not null), each summed as kRestrictAvg does; with partials, also each
+ * workgroup's sums of c1^2 and c2^2 over the cells it wrote (stride 2, as
+ * kSumSq2 lays them out), so the norms on the network grid need no pass of
+ * their own */
+__global__ __launch_bounds__(256) void kRestrictPair(const float *__restrict__ f1, const float *__restrict__ f2,
+    Geo gf, float *__restrict__ c1, float *__restrict__ c2, Geo gc, double *partials)
+{
+  __shared__ double lds[4];
+  const int I = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int J = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int K = blockIdx.z + 1;
+  double q1 = 0.0, q2 = 0.0;
+  if (I <= gc.ni && J <= gc.nj) {
+    const long long cc = gc.at(I, J, K);
+    double s = 0.0;
+    for (int e = 0; e < 8; e++)
+      s += (double)f1[gf.at(2 * I - 1 + (e & 1), 2 * J - 1 + ((e >> 1) & 1), 2 * K - 1 + (e >> 2))];
+    const float v1 = (float)(0.125 * s);
+    c1[cc]         = v1;
+    q1             = (double)v1 * v1;
+    if (f2) {
+      s = 0.0;
+      for (int e = 0; e < 8; e++)
+        s += (double)f2[gf.at(2 * I - 1 + (e & 1), 2 * J - 1 + ((e >> 1) & 1), 2 * K - 1 + (e >> 2))];
+      const float v2 = (float)(0.125 * s);
+      c2[cc]         = v2;
+      q2             = (double)v2 * v2;
+    }
+  }
+  if (partials) {
+    q1 = blockSum(q1, lds);
+    q2 = blockSum(q2, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      const unsigned bid    = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+      partials[2 * bid]     = q1;
+      partials[2 * bid + 1] = q2;
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void kCast(const double *__restrict__ a, float *__restrict__ b, Geo g)
@@ -110,13 +152,19 @@ static dim3 sumGrid(const Geo &g)
   return dim3((g.ni + 63) / 64, (g.nj + 3) / 4, (g.nk + SQ_KC - 1) / SQ_KC);
 }
 
-__global__ __launch_bounds__(256) void kFoldSumSq2(const double *partials, int nb, double *out)
+/* the folds below also write their sums into the pinned, host-coherent
+ * mirror (outHost: its device address), which the host reads once the
+ * stream has run them */
+__global__ __launch_bounds__(256) void kFoldSumSq2(const double *partials, int nb, double *out, double *outHost)
 {
   __shared__ double lds[4];
   double s0 = foldSum(partials, nb, 2, 0, lds), s1 = foldSum(partials, nb, 2, 1, lds);
   if (threadIdx.x == 0) {
-    out[0] = s0;
-    out[1] = s1;
+    out[0]     = s0;
+    out[1]     = s1;
+    outHost[0] = s0;
+    outHost[1] = s1;
+    __threadfence_system();
   }
 }
 
@@ -205,18 +253,26 @@ __global__ __launch_bounds__(256) void kInputFine(const double *__restrict__ p, 
 }
 
 /* component 0 of stride-2 partials (the input pass's r0^2) */
-__global__ __launch_bounds__(256) void kFoldComp0(const double *partials, int nb, double *out)
+__global__ __launch_bounds__(256) void kFoldComp0(const double *partials, int nb, double *out, double *outHost)
 {
   __shared__ double lds[4];
   double s = foldSum(partials, nb, 2, 0, lds);
-  if (threadIdx.x == 0) *out = s;
+  if (threadIdx.x == 0) {
+    *out     = s;
+    *outHost = s;
+    __threadfence_system();
+  }
 }
 
-__global__ __launch_bounds__(256) void kFoldSum1(const double *partials, int nb, double *out)
+__global__ __launch_bounds__(256) void kFoldSum1(const double *partials, int nb, double *out, double *outHost)
 {
   __shared__ double lds[4];
   double s = foldSum(partials, nb, 1, 0, lds);
-  if (threadIdx.x == 0) *out = s;
+  if (threadIdx.x == 0) {
+    *out     = s;
+    *outHost = s;
+    __threadfence_system();
+  }
 }
 
 /* ---- host ------------------------------------------------------------------ */
@@ -277,7 +333,8 @@ WarmStart *warmstartCreate(NusifCtx *c, UnetModel *net, bool capture)
   }
   HIP_CHECK(hipMalloc((void **)&w->x, (size_t)w->n * w->n * w->n * 8 * sizeof(uint16_t)));
   HIP_CHECK(hipMalloc((void **)&w->scal, 4 * sizeof(double)));
-  HIP_CHECK(hipHostMalloc((void **)&w->scalHost, 4 * sizeof(double), hipHostMallocDefault));
+  HIP_CHECK(hipHostMalloc((void **)&w->scalHost, 4 * sizeof(double), hipHostMallocCoherent));
+  HIP_CHECK(hipHostGetDevicePointer((void **)&w->scalHostDev, w->scalHost, 0));
   HIP_CHECK(hipMalloc((void **)&w->partials, 65536 * sizeof(double)));
   w->pPrev = nullptr;
   if (capture) HIP_CHECK(hipMalloc((void **)&w->pPrev, (size_t)c->geo.n * sizeof(double)));
@@ -321,20 +378,27 @@ void warmstartPrepare(WarmStart *w, const double *p, const double *rhs, double i
   if (w->Ln == 0) {
     dim3 gr = cells(g0);
     kInputFine<<<gr, dim3(WTX, WTY), 0, st>>>(p, rhs, g0, cf, w->lv[0].r, w->lv[0].b, w->partials);
-    kFoldSum1<<<1, 256, 0, st>>>(w->partials, (int)(gr.x * gr.y * gr.z), w->scal + 3);
+    kFoldSum1<<<1, 256, 0, st>>>(w->partials, (int)(gr.x * gr.y * gr.z), w->scal + 3, w->scalHostDev + 3);
   } else {
     /* r0 and rhs averaged onto level 1 in the residual pass itself */
     int nb = launchInputRestrict(p, rhs, g0, idx2, idy2, idz2, w->lv[1].r, w->lv[1].b, w->lv[1].g, w->partials);
-    kFoldComp0<<<1, 256, 0, st>>>(w->partials, nb, w->scal + 3);
-    for (int l = 2; l <= w->Ln; l++) {
-      kRestrictAvg<float><<<cells(w->lv[l].g), dim3(WTX, WTY), 0, st>>>(w->lv[l - 1].b, w->lv[l - 1].g, w->lv[l].b, w->lv[l].g);
-      kRestrictAvg<float><<<cells(w->lv[l].g), dim3(WTX, WTY), 0, st>>>(w->lv[l - 1].r, w->lv[l - 1].g, w->lv[l].r, w->lv[l].g);
-    }
+    kFoldComp0<<<1, 256, 0, st>>>(w->partials, nb, w->scal + 3, w->scalHostDev + 3);
   }
+  /* down to the network level, r and rhs in one launch per level; the last
+   * one also gives the sums of squares there (the partials above are folded
+   * already: the stream runs in order) */
   const WsLevel &L = w->lv[w->Ln];
-  dim3 g2 = sumGrid(L.g);
-  kSumSq2<<<g2, dim3(WTX, WTY), 0, st>>>(L.r, L.b, L.g, w->partials);
-  kFoldSumSq2<<<1, 256, 0, st>>>(w->partials, (int)(g2.x * g2.y * g2.z), w->scal);
+  if (w->Ln >= 2) {
+    for (int l = 2; l <= w->Ln; l++)
+      kRestrictPair<<<cells(w->lv[l].g), dim3(WTX, WTY), 0, st>>>(w->lv[l - 1].r, w->lv[l - 1].b, w->lv[l - 1].g,
+          w->lv[l].r, w->lv[l].b, w->lv[l].g, l == w->Ln ? w->partials : nullptr);
+    const dim3 gl = cells(L.g);
+    kFoldSumSq2<<<1, 256, 0, st>>>(w->partials, (int)(gl.x * gl.y * gl.z), w->scal, w->scalHostDev);
+  } else {
+    const dim3 g2 = sumGrid(L.g);
+    kSumSq2<<<g2, dim3(WTX, WTY), 0, st>>>(L.r, L.b, L.g, w->partials);
+    kFoldSumSq2<<<1, 256, 0, st>>>(w->partials, (int)(g2.x * g2.y * g2.z), w->scal, w->scalHostDev);
+  }
   kPack<<<1024, 256, 0, st>>>(L.r, L.b, L.g, w->scal, w->x);
 }
 
@@ -369,25 +433,28 @@ void warmstartCoarseNormLaunch(WarmStart *w, const float *rFine, bool level1)
 {
   hipStream_t st = nusifStream();
   const float *src = level1 ? w->lv[1].r : rFine;
-  for (int l = level1 ? 2 : 1; l <= w->Ln; l++) {
-    kRestrictAvg<float><<<cells(w->lv[l].g), dim3(WTX, WTY), 0, st>>>(src, w->lv[l - 1].g, w->lv[l].r, w->lv[l].g);
-    src = w->lv[l].r;
-  }
+  const int l0     = level1 ? 2 : 1;
   const WsLevel &L = w->lv[w->Ln];
-  dim3 g2 = sumGrid(L.g);
-  kSumSq2<<<g2, dim3(WTX, WTY), 0, st>>>(src, src, L.g, w->partials);
   /* one component into scal[2]: scal[3] keeps the fine ||r0||^2 of the input */
-  kFoldComp0<<<1, 256, 0, st>>>(w->partials, (int)(g2.x * g2.y * g2.z), w->scal + 2);
+  if (l0 <= w->Ln) { /* the last restriction also gives the sum of squares */
+    for (int l = l0; l <= w->Ln; l++) {
+      kRestrictPair<<<cells(w->lv[l].g), dim3(WTX, WTY), 0, st>>>(src, nullptr, w->lv[l - 1].g, w->lv[l].r, nullptr,
+          w->lv[l].g, l == w->Ln ? w->partials : nullptr);
+      src = w->lv[l].r;
+    }
+    const dim3 gl = cells(L.g);
+    kFoldComp0<<<1, 256, 0, st>>>(w->partials, (int)(gl.x * gl.y * gl.z), w->scal + 2, w->scalHostDev + 2);
+  } else {
+    const dim3 g2 = sumGrid(L.g);
+    kSumSq2<<<g2, dim3(WTX, WTY), 0, st>>>(src, src, L.g, w->partials);
+    kFoldComp0<<<1, 256, 0, st>>>(w->partials, (int)(g2.x * g2.y * g2.z), w->scal + 2, w->scalHostDev + 2);
+  }
 }
 
-/* enqueue the copy of scal[0..3] ([0] = sum (R r0)^2, [2] = the last
- * warmstartCoarseNormLaunch) into pinned host memory; valid after the
- * stream synchronises */
-const double *warmstartNormsAsync(WarmStart *w)
-{
-  HIP_CHECK(hipMemcpyAsync(w->scalHost, w->scal, 4 * sizeof(double), hipMemcpyDeviceToHost, nusifStream()));
-  return w->scalHost;
-}
+/* scal[0..3] ([0] = sum (R r0)^2, [2] = the last warmstartCoarseNormLaunch)
+ * in pinned host memory, which the folds write themselves: valid once the
+ * stream has run them (after the caller's next synchronisation) */
+const double *warmstartNormsAsync(WarmStart *w) { return w->scalHost; }
 
 /* capture, part 1 (before the solve, after warmstartPrepare): the input and
  * a copy of p */
