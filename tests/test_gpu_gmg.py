@@ -1,6 +1,7 @@
 """The GPU geometric multigrid (FP64 and FP32-in-FP64 defect correction)
 against the oracle's restatement, through the C ABI."""
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
@@ -206,9 +207,14 @@ def test_gpu_gmg_graph_replay_equals_eager_launches(precision, gpu_ok):
         c.close()
 
 
+WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "cfd-bench_amd", "weights",
+                       "unet_cavity.bin")
+
+
 @pytest.mark.parametrize("opts", [{"precision": "fp64", "mg_inner": 0}, {"precision": "mixed", "mg_inner": 0},
-                                  {"precision": "mixed", "mg_inner": 1}, {"precision": "mixed", "mg_speculate": 1}],
-                         ids=["fp64", "mixed", "mixed-inner1", "mixed-speculate"])
+                                  {"precision": "mixed", "mg_inner": 1}, {"precision": "mixed", "mg_speculate": 1},
+                                  {"precision": "mixed", "mg_inner": 0, "warmstart": "unet", "unet_weights": WEIGHTS}],
+                         ids=["fp64", "mixed", "mixed-inner1", "mixed-speculate", "learned"])
 def test_gpu_run_steps_early_adapt_equals_step_calls(opts, gpu_ok):
     """nusifRunSteps queues adaptUV right after the first cycle group, before
     the host reads its norms, and runs it again when later groups change p
