@@ -74,10 +74,11 @@ def measured_traffic(kernel, n):
     path = os.path.join(ROOT, "profiles", "sweep32_cavity256_hbm_traffic.md")
     if n != 256 or not os.path.exists(path):
         return None
-    for line in open(path):
-        parts = line.split()
-        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
-            return float(parts[2])
+    with open(path) as fh:
+        for line in fh:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                return float(parts[2])
     return None
 
 
