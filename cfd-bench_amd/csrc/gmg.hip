@@ -223,15 +223,18 @@ struct SweepRBShape {
   static constexpr int YS   = NRT;
 };
 
-/* Plane-relative addressing through buffer instructions: a plane's buffer
- * descriptor (base = the plane's first cell) is uniform and sits in scalar
- * registers, and each lane passes a 32-bit byte offset that stays fixed for
- * the whole march. Global loads with 64-bit addresses cost two 64-bit VALU
- * adds per access here (about a third of the sweep's VALU instructions);
- * buffer accesses cost none. Every offset is inside the padded box (the
- * callers clamp), so the descriptor's range (2^31 - 1 bytes) is no bound
- * check. 0x00020000: the descriptor's fourth word for gfx9-family GPUs
- * (32-bit data format, no swizzle). */
+/* Plane-relative addressing through buffer instructions: a buffer
+ * descriptor (base = the first cell of a plane) is uniform and sits in
+ * scalar registers, each lane passes a 32-bit byte offset within a plane
+ * that stays fixed for the whole march, and the plane is a uniform scalar
+ * offset. Global loads with 64-bit addresses cost two 64-bit VALU adds per
+ * access here (about a third of the sweep's VALU instructions); buffer
+ * accesses cost none, and one descriptor per array and workgroup (not one
+ * per plane and access) saves the scalar work of a 64-bit base each time
+ * (3.7% of the finest FP32 sweep at 256^3). Every offset is inside the
+ * padded box (the callers clamp), so the descriptor's range (2^31 - 1
+ * bytes) is no bound check. 0x00020000: the descriptor's fourth word for
+ * gfx9-family GPUs (32-bit data format, no swizzle). */
 template <typename T> __device__ inline __amdgpu_buffer_rsrc_t planeRsrc(const T *a, const Geo &g, int k)
 {
   return __builtin_amdgcn_make_buffer_rsrc((void *)(a + (long long)k * g.sk), (short)0, 0x7fffffff, 0x00020000);
@@ -252,6 +255,26 @@ __device__ inline void stP(__amdgpu_buffer_rsrc_t r, unsigned off, double v)
 {
   using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, 0, 0);
+}
+/* the same with a scalar (uniform) byte offset `so`: one descriptor per
+ * array serves every plane, so a plane step costs a multiply, not a new
+ * 64-bit base and descriptor per access */
+__device__ inline float ldPS(float *, __amdgpu_buffer_rsrc_t r, unsigned off, unsigned so)
+{
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, so, 0));
+}
+__device__ inline double ldPS(double *, __amdgpu_buffer_rsrc_t r, unsigned off, unsigned so)
+{
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, so, 0));
+}
+__device__ inline void stPS(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned so, float v)
+{
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, so, 0);
+}
+__device__ inline void stPS(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned so, double v)
+{
+  using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, so, 0);
 }
 
 /* RES (the reference's red-black SOR iteration, src/solver-rb.c:55-148): the
@@ -324,6 +347,15 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
   const bool bjIn = bj <= nj;
   const bool bIn0 = bjIn && bi <= ni, bIn1 = bjIn && bi + 1 <= ni;
   double acc = 0.0; /* RES: this thread's share of the sum of r^2 */
+  /* one descriptor per array, based at the lowest plane this workgroup
+   * touches (k0 - 2, clamped); plane k is the scalar offset (k - kBase)
+   * planes, at most kc + 3 of them (the launchers check that it fits) */
+  const int kBase = max(k0 - 2, 0);
+  const __amdgpu_buffer_rsrc_t rXo = planeRsrc(xo, g, kBase);
+  const __amdgpu_buffer_rsrc_t rB  = planeRsrc(b, g, kBase);
+  const __amdgpu_buffer_rsrc_t rXn = planeRsrc(xn, g, kBase);
+  const unsigned PB = (unsigned)sk * ES;
+  auto soff = [&](int k) { return (unsigned)(k - kBase) * PB; };
 
   T xr[NX];
   auto fetchX = [&](int k) {
@@ -332,9 +364,9 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
       for (int q = 0; q < NX; q++) xr[q] = (T)0;
       return;
     }
-    const __amdgpu_buffer_rsrc_t pl = planeRsrc(xo, g, min(max(k, 0), nk + 1));
+    const unsigned so = soff(min(max(k, 0), nk + 1));
 #pragma unroll
-    for (int q = 0; q < NX; q++) xr[q] = ldP((T *)nullptr, pl, xOff[q]);
+    for (int q = 0; q < NX; q++) xr[q] = ldPS((T *)nullptr, rXo, xOff[q], so);
   };
   auto storeX = [&](int k) {
     if (ZX) return;
@@ -343,10 +375,10 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
       if (q < NX - 1 || tid + NT * q < S::NXE) X[k & 3][xLds[q]] = xr[q];
   };
   auto fetchBR = [&](int k, T *out) { /* b under this thread's red cells of plane k */
-    const __amdgpu_buffer_rsrc_t pl = planeRsrc(b, g, min(max(k, 1), nk));
+    const unsigned so = soff(min(max(k, 1), nk));
     const int s0 = (k + 1) & 1;
 #pragma unroll
-    for (int r = 0; r < NR; r++) out[r] = ldP((T *)nullptr, pl, rOff[r] + ((s0 + rLj[r]) & 1) * ES);
+    for (int r = 0; r < NR; r++) out[r] = ldPS((T *)nullptr, rB, rOff[r] + ((s0 + rLj[r]) & 1) * ES, so);
   };
   auto redPlane = [&](int k, const T *bred) {
     const int hr = (k + 1) & 1;
@@ -396,37 +428,36 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
     if (RES && (s ? bIn1 : bIn0)) acc += (double)rr * (double)rr;
     /* column 2p holds the black value when s = 0, the red one when s = 1 */
     T v0 = s ? vR : vB, v1 = s ? vB : vR;
-    const __amdgpu_buffer_rsrc_t pl = planeRsrc(xn, g, k);
+    const unsigned so = soff(k);
     const unsigned c0 = bRow, c1 = bRow + ES;
     if (bIn0) {
-      stP(pl, c0, v0);
-      if (bi == 1) stP(pl, c0 - ES, v0);
-      if (bi == ni) stP(pl, c1, v0);
-      if (bj == 1) stP(pl, c0 - sjB, v0);
-      if (bj == nj) stP(pl, c0 + sjB, v0);
+      stPS(rXn, c0, so, v0);
+      if (bi == 1) stPS(rXn, c0 - ES, so, v0);
+      if (bi == ni) stPS(rXn, c1, so, v0);
+      if (bj == 1) stPS(rXn, c0 - sjB, so, v0);
+      if (bj == nj) stPS(rXn, c0 + sjB, so, v0);
     }
     if (bIn1) {
-      stP(pl, c1, v1);
-      if (bi + 1 == ni) stP(pl, c1 + ES, v1);
-      if (bj == 1) stP(pl, c1 - sjB, v1);
-      if (bj == nj) stP(pl, c1 + sjB, v1);
+      stPS(rXn, c1, so, v1);
+      if (bi + 1 == ni) stPS(rXn, c1 + ES, so, v1);
+      if (bj == 1) stPS(rXn, c1 - sjB, so, v1);
+      if (bj == nj) stPS(rXn, c1 + sjB, so, v1);
     }
-    if (k == 1 || k == nk) { /* the k-face ghost planes */
+    if (k == 1 || k == nk) { /* the k-face ghost planes (k == 1: kBase is 0) */
       if (k == 1) {
-        const __amdgpu_buffer_rsrc_t pg = planeRsrc(xn, g, 0);
-        if (bIn0) stP(pg, c0, v0);
-        if (bIn1) stP(pg, c1, v1);
+        if (bIn0) stPS(rXn, c0, soff(0), v0);
+        if (bIn1) stPS(rXn, c1, soff(0), v1);
       }
       if (k == nk) {
-        const __amdgpu_buffer_rsrc_t pg = planeRsrc(xn, g, nk + 1);
-        if (bIn0) stP(pg, c0, v0);
-        if (bIn1) stP(pg, c1, v1);
+        const unsigned sg = soff(nk + 1);
+        if (bIn0) stPS(rXn, c0, sg, v0);
+        if (bIn1) stPS(rXn, c1, sg, v1);
       }
     }
   };
   auto fetchBB = [&](int k) {
     int kk = min(max(k, 1), nk);
-    return ldP((T *)nullptr, planeRsrc(b, g, kk), bRow + ((kk + blj) & 1) * ES);
+    return ldPS((T *)nullptr, rB, bRow + ((kk + blj) & 1) * ES, soff(kk));
   };
 
   /* one plane step with register set (xs, bs, bbs): store plane kk+1 to LDS,
@@ -548,7 +579,13 @@ template <typename T> static void launchSweep(const LevelT<T> &L, SweepShape s, 
 }
 
 /* true when a level's smoother is the fused sweep */
-template <typename T> static bool fusedLevel(const LevelT<T> &L) { return L.g.ni >= FUSED_MIN_EXTENT && L.g.nk >= 4; }
+/* the fused sweep addresses at most FT_KC + 3 planes past its base with
+ * 32-bit scalar offsets (descriptor range 2^31 - 1 bytes) */
+static bool fusedPlanesFit(const Geo &g, size_t elem) { return (size_t)(FT_KC + 4) * g.sk * elem < ((size_t)1 << 31); }
+template <typename T> static bool fusedLevel(const LevelT<T> &L)
+{
+  return L.g.ni >= FUSED_MIN_EXTENT && L.g.nk >= 4 && fusedPlanesFit(L.g, sizeof(T));
+}
 
 /* ---- the reference's red-black SOR iteration as one fused sweep ----------
  * Both colour passes of iteration q (src/solver-rb.c:55-79: pass 0 = odd
@@ -557,7 +594,7 @@ template <typename T> static bool fusedLevel(const LevelT<T> &L) { return L.g.ni
  * stencil reads the old ghosts, as the reference does) in one launch, from
  * xo into xn. Same operands in the same order as the colour-pass kernel, so
  * the same bits; the in-sweep sum of r^2 goes to partials[tile]. */
-bool rbFusedFits(const Geo &g) { return g.ni >= FUSED_MIN_EXTENT && g.nk >= 4; }
+bool rbFusedFits(const Geo &g) { return g.ni >= FUSED_MIN_EXTENT && g.nk >= 4 && fusedPlanesFit(g, sizeof(double)); }
 
 int rbFusedTiles(const Geo &g)
 {
