@@ -163,6 +163,9 @@ def _declare_optional(lib):
         "orcGmgCoarseSolve": ([C.c_int] * 3 + [C.c_double] * 3 + [_DP, _DP], None),
         "orcUnetParamCount": ([C.POINTER(C.c_int)], C.c_long),
         "orcUnetLoss": ([C.POINTER(C.c_int), C.c_int, _DP, _DP, _DP, _DP, _DP, C.c_int], C.c_double),
+        "orcUnetLossBox": ([C.POINTER(C.c_int), C.c_int, C.c_int, C.c_int, _DP, _DP, _DP, _DP, _DP, C.c_int],
+                           C.c_double),
+        "orcUnetTransfer": ([C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _DP, _DP], None),
         "orcBf16": ([C.c_double], C.c_double),
         "orcGmgProlong": ([C.c_int] * 3 + [_DP, _DP], None),
         "orcGmgRestrict": ([C.c_int] * 3 + [C.c_double] * 3 + [_DP] * 3, None),

@@ -20,9 +20,15 @@
  * bfloat16 (round to nearest even) where the GPU stores them, so forward
  * outputs match the GPU up to FP32 accumulation order.
  *
+ * Tensors are boxes of nx x ny x nz voxels (x fastest), each extent
+ * divisible by 2^(L-1).
+ *
  * Pins (tests/test_oracle_unet.py): central finite differences of the loss
- * against the analytic gradient for every parameter block; a 1x1x1-kernel
- * special case against numpy matmuls; pool/upsample adjointness.
+ * against the analytic gradient for every parameter block; the whole forward
+ * at L = 1, 2 and 4 on cubes and boxes against an independent numpy
+ * restatement (reshape-mean pooling, np.repeat upsampling, concatenation
+ * order, zero-padded correlation); the pooling and upsampling backward passes
+ * against the adjoint identities <P x, y> = <x, P^T y>.
  */
 #include <math.h>
 #include <stdint.h>
@@ -46,46 +52,63 @@ static double bf16r(double x)
 }
 
 typedef struct {
-  int n, c; /* cube extent, channels */
+  int nx, ny, nz, c; /* box extents, channels */
   double *v;
 } Ten;
 
-static Ten ten(int n, int c)
+static size_t nvox(const Ten *t) { return (size_t)t->nx * t->ny * t->nz; }
+
+static Ten ten(int nx, int ny, int nz, int c)
 {
   Ten t;
-  t.n = n;
-  t.c = c;
-  t.v = calloc((size_t)n * n * n * c, sizeof(double));
+  t.nx = nx;
+  t.ny = ny;
+  t.nz = nz;
+  t.c  = c;
+  t.v  = calloc((size_t)nx * ny * nz * c, sizeof(double));
   return t;
 }
+
+/* same extents as t, c channels */
+static Ten like(const Ten *t, int c) { return ten(t->nx, t->ny, t->nz, c); }
+
+/* half / double the extents of t, c channels */
+static Ten half(const Ten *t, int c) { return ten(t->nx / 2, t->ny / 2, t->nz / 2, c); }
+static Ten twice(const Ten *t, int c) { return ten(2 * t->nx, 2 * t->ny, 2 * t->nz, c); }
 
 static void roundTen(Ten *t, int bf16)
 {
   if (!bf16) return;
-  size_t m = (size_t)t->n * t->n * t->n * t->c;
+  size_t m = nvox(t) * t->c;
   for (size_t q = 0; q < m; q++) t->v[q] = bf16r(t->v[q]);
+}
+
+/* voxel index of (x, y, z) in t, -1 outside (the zero padding) */
+static long vox(const Ten *t, int x, int y, int z)
+{
+  if (x < 0 || x >= t->nx || y < 0 || y >= t->ny || z < 0 || z >= t->nz) return -1;
+  return ((long)z * t->ny + y) * t->nx + x;
 }
 
 /* out = conv3(in0 ++ in1) + b, optional ReLU. W[cout][27][cin0+cin1] */
 static void convFwd(const Ten *in0, const Ten *in1, const double *W, const double *b, Ten *out,
     int relu)
 {
-  int n = out->n, c0 = in0->c, c1 = in1 ? in1->c : 0, cin = c0 + c1, co = out->c;
-  for (int z = 0; z < n; z++)
-    for (int y = 0; y < n; y++)
-      for (int x = 0; x < n; x++)
+  int c0 = in0->c, c1 = in1 ? in1->c : 0, cin = c0 + c1, co = out->c;
+  for (int z = 0; z < out->nz; z++)
+    for (int y = 0; y < out->ny; y++)
+      for (int x = 0; x < out->nx; x++)
         for (int o = 0; o < co; o++) {
           double s = b[o];
           for (int t = 0; t < 27; t++) {
-            int zz = z + t / 9 - 1, yy = y + (t / 3) % 3 - 1, xx = x + t % 3 - 1;
-            if (zz < 0 || zz >= n || yy < 0 || yy >= n || xx < 0 || xx >= n) continue;
-            size_t vox = ((size_t)zz * n + yy) * n + xx;
+            long v = vox(out, x + t % 3 - 1, y + (t / 3) % 3 - 1, z + t / 9 - 1);
+            if (v < 0) continue;
             const double *w = W + ((size_t)o * 27 + t) * cin;
-            for (int c = 0; c < c0; c++) s += w[c] * in0->v[vox * c0 + c];
-            for (int c = 0; c < c1; c++) s += w[c0 + c] * in1->v[vox * c1 + c];
+            for (int c = 0; c < c0; c++) s += w[c] * in0->v[v * c0 + c];
+            for (int c = 0; c < c1; c++) s += w[c0 + c] * in1->v[v * c1 + c];
           }
           if (relu && s < 0.0) s = 0.0;
-          out->v[(((size_t)z * n + y) * n + x) * co + o] = s;
+          out->v[vox(out, x, y, z) * co + o] = s;
         }
 }
 
@@ -93,27 +116,26 @@ static void convFwd(const Ten *in0, const Ten *in1, const double *W, const doubl
 static void convBwd(const Ten *in0, const Ten *in1, const double *W, const Ten *dOut, Ten *dIn0,
     Ten *dIn1, double *dW, double *db)
 {
-  int n = dOut->n, c0 = in0->c, c1 = in1 ? in1->c : 0, cin = c0 + c1, co = dOut->c;
-  for (int z = 0; z < n; z++)
-    for (int y = 0; y < n; y++)
-      for (int x = 0; x < n; x++)
+  int c0 = in0->c, c1 = in1 ? in1->c : 0, cin = c0 + c1, co = dOut->c;
+  for (int z = 0; z < dOut->nz; z++)
+    for (int y = 0; y < dOut->ny; y++)
+      for (int x = 0; x < dOut->nx; x++)
         for (int o = 0; o < co; o++) {
-          double g = dOut->v[(((size_t)z * n + y) * n + x) * co + o];
+          double g = dOut->v[vox(dOut, x, y, z) * co + o];
           if (g == 0.0) continue;
           db[o] += g;
           for (int t = 0; t < 27; t++) {
-            int zz = z + t / 9 - 1, yy = y + (t / 3) % 3 - 1, xx = x + t % 3 - 1;
-            if (zz < 0 || zz >= n || yy < 0 || yy >= n || xx < 0 || xx >= n) continue;
-            size_t vox = ((size_t)zz * n + yy) * n + xx;
+            long v = vox(dOut, x + t % 3 - 1, y + (t / 3) % 3 - 1, z + t / 9 - 1);
+            if (v < 0) continue;
             const double *w = W + ((size_t)o * 27 + t) * cin;
             double *dw      = dW + ((size_t)o * 27 + t) * cin;
             for (int c = 0; c < c0; c++) {
-              dw[c] += g * in0->v[vox * c0 + c];
-              if (dIn0) dIn0->v[vox * c0 + c] += g * w[c];
+              dw[c] += g * in0->v[v * c0 + c];
+              if (dIn0) dIn0->v[v * c0 + c] += g * w[c];
             }
             for (int c = 0; c < c1; c++) {
-              dw[c0 + c] += g * in1->v[vox * c1 + c];
-              if (dIn1) dIn1->v[vox * c1 + c] += g * w[c0 + c];
+              dw[c0 + c] += g * in1->v[v * c1 + c];
+              if (dIn1) dIn1->v[v * c1 + c] += g * w[c0 + c];
             }
           }
         }
@@ -121,58 +143,55 @@ static void convBwd(const Ten *in0, const Ten *in1, const double *W, const Ten *
 
 static void reluMask(Ten *d, const Ten *act)
 {
-  size_t m = (size_t)d->n * d->n * d->n * d->c;
+  size_t m = nvox(d) * d->c;
   for (size_t q = 0; q < m; q++)
     if (!(act->v[q] > 0.0)) d->v[q] = 0.0;
 }
 
+/* out = average of the 2x2x2 children in `in` (child q = (dz, dy, dx) bits) */
 static void poolFwd(const Ten *in, Ten *out)
 {
-  int n = out->n, c = in->c, N = in->n;
-  for (int z = 0; z < n; z++)
-    for (int y = 0; y < n; y++)
-      for (int x = 0; x < n; x++)
+  int c = in->c;
+  for (int z = 0; z < out->nz; z++)
+    for (int y = 0; y < out->ny; y++)
+      for (int x = 0; x < out->nx; x++)
         for (int k = 0; k < c; k++) {
           double s = 0.0;
-          for (int q = 0; q < 8; q++) {
-            int zz = 2 * z + (q >> 2), yy = 2 * y + ((q >> 1) & 1), xx = 2 * x + (q & 1);
-            s += in->v[(((size_t)zz * N + yy) * N + xx) * c + k];
-          }
-          out->v[(((size_t)z * n + y) * n + x) * c + k] = 0.125 * s;
+          for (int q = 0; q < 8; q++)
+            s += in->v[vox(in, 2 * x + (q & 1), 2 * y + ((q >> 1) & 1), 2 * z + (q >> 2)) * c + k];
+          out->v[vox(out, x, y, z) * c + k] = 0.125 * s;
         }
 }
 
 static void poolBwd(const Ten *dOut, Ten *dIn)
 {
-  int n = dOut->n, c = dOut->c, N = dIn->n;
-  for (int z = 0; z < N; z++)
-    for (int y = 0; y < N; y++)
-      for (int x = 0; x < N; x++)
+  int c = dOut->c;
+  for (int z = 0; z < dIn->nz; z++)
+    for (int y = 0; y < dIn->ny; y++)
+      for (int x = 0; x < dIn->nx; x++)
         for (int k = 0; k < c; k++)
-          dIn->v[(((size_t)z * N + y) * N + x) * c + k] +=
-              0.125 * dOut->v[(((size_t)(z / 2) * n + y / 2) * n + x / 2) * c + k];
+          dIn->v[vox(dIn, x, y, z) * c + k] += 0.125 * dOut->v[vox(dOut, x / 2, y / 2, z / 2) * c + k];
 }
 
+/* out = nearest x2 copy of in (each voxel takes its parent's value) */
 static void upFwd(const Ten *in, Ten *out)
 {
-  int N = out->n, n = in->n, c = in->c;
-  for (int z = 0; z < N; z++)
-    for (int y = 0; y < N; y++)
-      for (int x = 0; x < N; x++)
+  int c = in->c;
+  for (int z = 0; z < out->nz; z++)
+    for (int y = 0; y < out->ny; y++)
+      for (int x = 0; x < out->nx; x++)
         for (int k = 0; k < c; k++)
-          out->v[(((size_t)z * N + y) * N + x) * c + k] =
-              in->v[(((size_t)(z / 2) * n + y / 2) * n + x / 2) * c + k];
+          out->v[vox(out, x, y, z) * c + k] = in->v[vox(in, x / 2, y / 2, z / 2) * c + k];
 }
 
 static void upBwd(const Ten *dOut, Ten *dIn)
 {
-  int N = dOut->n, n = dIn->n, c = dOut->c;
-  for (int z = 0; z < N; z++)
-    for (int y = 0; y < N; y++)
-      for (int x = 0; x < N; x++)
+  int c = dOut->c;
+  for (int z = 0; z < dOut->nz; z++)
+    for (int y = 0; y < dOut->ny; y++)
+      for (int x = 0; x < dOut->nx; x++)
         for (int k = 0; k < c; k++)
-          dIn->v[(((size_t)(z / 2) * n + y / 2) * n + x / 2) * c + k] +=
-              dOut->v[(((size_t)z * N + y) * N + x) * c + k];
+          dIn->v[vox(dIn, x / 2, y / 2, z / 2) * c + k] += dOut->v[vox(dOut, x, y, z) * c + k];
 }
 
 /* cfg = {L, ch0, ch1, ch2, ch3, cinPad} */
@@ -225,9 +244,10 @@ static Net netOf(const int *cfg)
 
 long orcUnetParamCount(const int *cfg) { return (long)netOf(cfg).total; }
 
-/* Loss of one sample; grad (may be NULL) receives d loss / d params (added
- * to, not overwritten); pred (may be NULL) receives the n^3 prediction. */
-double orcUnetLoss(const int *cfg, int n, const double *params, const double *x,
+/* Loss of one sample on an nx x ny x nz box (each extent divisible by
+ * 2^(L-1)); grad (may be NULL) receives d loss / d params (added to, not
+ * overwritten); pred (may be NULL) receives the nx ny nz prediction. */
+double orcUnetLossBox(const int *cfg, int nx, int ny, int nz, const double *params, const double *x,
     const double *target, double *grad, double *pred, int bf16)
 {
   Net N = netOf(cfg);
@@ -238,46 +258,44 @@ double orcUnetLoss(const int *cfg, int n, const double *params, const double *x,
   if (bf16)
     for (int c = 0; c <= N.ch[0]; c++) P[N.headW + c] = (double)(float)params[N.headW + c];
 
-  Ten xin = ten(n, N.cin);
-  memcpy(xin.v, x, (size_t)n * n * n * N.cin * sizeof(double));
+  Ten xin = ten(nx, ny, nz, N.cin);
+  memcpy(xin.v, x, nvox(&xin) * N.cin * sizeof(double));
   roundTen(&xin, bf16);
-  Ten a[2 * MAXL], s[MAXL], pl[MAXL], up[MAXL], da[2 * MAXL], d[2 * MAXL];
-  int q = 0, nl = n;
+  Ten a[MAXL], s[MAXL], pl[MAXL], up[MAXL], da[MAXL], d[MAXL];
+  int q = 0;
   const Ten *cur = &xin;
   for (int l = 0; l < L; l++) {
-    a[l] = ten(nl, N.ch[l]);
+    a[l] = like(cur, N.ch[l]);
     convFwd(cur, NULL, P + N.wOff[q], P + N.bOff[q], &a[l], 1);
     roundTen(&a[l], bf16);
     q++;
-    s[l] = ten(nl, N.ch[l]);
+    s[l] = like(cur, N.ch[l]);
     convFwd(&a[l], NULL, P + N.wOff[q], P + N.bOff[q], &s[l], 1);
     roundTen(&s[l], bf16);
     q++;
     if (l < L - 1) {
-      pl[l] = ten(nl / 2, N.ch[l]);
+      pl[l] = half(&s[l], N.ch[l]);
       poolFwd(&s[l], &pl[l]);
       roundTen(&pl[l], bf16);
       cur = &pl[l];
-      nl /= 2;
     }
   }
   const Ten *prev = &s[L - 1];
   for (int l = L - 2; l >= 0; l--) {
-    nl *= 2;
-    up[l] = ten(nl, N.ch[l + 1]);
+    up[l] = twice(prev, N.ch[l + 1]);
     upFwd(prev, &up[l]);
-    da[l] = ten(nl, N.ch[l]);
+    da[l] = like(&s[l], N.ch[l]);
     convFwd(&up[l], &s[l], P + N.wOff[q], P + N.bOff[q], &da[l], 1);
     roundTen(&da[l], bf16);
     q++;
-    d[l] = ten(nl, N.ch[l]);
+    d[l] = like(&s[l], N.ch[l]);
     convFwd(&da[l], NULL, P + N.wOff[q], P + N.bOff[q], &d[l], 1);
     roundTen(&d[l], bf16);
     q++;
     prev = &d[l];
   }
   const Ten *top = L > 1 ? &d[0] : &s[0];
-  size_t nv = (size_t)n * n * n;
+  size_t nv = nvox(&xin);
   double loss = 0.0;
   double *y = malloc(nv * sizeof(double));
   for (size_t v = 0; v < nv; v++) {
@@ -291,7 +309,7 @@ double orcUnetLoss(const int *cfg, int n, const double *params, const double *x,
   if (pred) memcpy(pred, y, nv * sizeof(double));
 
   if (grad) {
-    Ten dtop = ten(n, N.ch[0]);
+    Ten dtop = like(&xin, N.ch[0]);
     for (size_t v = 0; v < nv; v++) {
       double g = 2.0 * (y[v] - target[v]) / (double)nv;
       grad[N.headB] += g;
@@ -303,7 +321,7 @@ double orcUnetLoss(const int *cfg, int n, const double *params, const double *x,
     /* decoder, reverse order */
     Ten dprev = dtop; /* gradient w.r.t. the current decoder output */
     Ten dS[MAXL];
-    for (int l = 0; l < L; l++) dS[l] = ten(s[l].n, s[l].c);
+    for (int l = 0; l < L; l++) dS[l] = like(&s[l], s[l].c);
     int qq = N.nconv;
     if (L == 1) {
       for (size_t m = 0; m < nv * N.ch[0]; m++) dS[0].v[m] = dtop.v[m];
@@ -313,20 +331,19 @@ double orcUnetLoss(const int *cfg, int n, const double *params, const double *x,
       int ib = qq - 1, ia = qq - 2;
       qq -= 2;
       reluMask(&dprev, &d[l]);
-      Ten dda = ten(da[l].n, da[l].c);
+      Ten dda = like(&da[l], da[l].c);
       convBwd(&da[l], NULL, P + N.wOff[ib], &dprev, &dda, NULL, grad + N.wOff[ib], grad + N.bOff[ib]);
       reluMask(&dda, &da[l]);
-      Ten dup = ten(up[l].n, up[l].c);
+      Ten dup = like(&up[l], up[l].c);
       convBwd(&up[l], &s[l], P + N.wOff[ia], &dda, &dup, &dS[l], grad + N.wOff[ia], grad + N.bOff[ia]);
       free(dda.v);
       free(dprev.v);
       /* up is the nearest-x2 copy of the level-(l+1) output */
-      Ten dnext = ten(up[l].n / 2, up[l].c);
+      Ten dnext = half(&up[l], up[l].c);
       upBwd(&dup, &dnext);
       free(dup.v);
       if (l + 1 == L - 1) {
-        for (size_t m = 0; m < (size_t)dnext.n * dnext.n * dnext.n * dnext.c; m++)
-          dS[L - 1].v[m] += dnext.v[m];
+        for (size_t m = 0; m < nvox(&dnext) * dnext.c; m++) dS[L - 1].v[m] += dnext.v[m];
         free(dnext.v);
       } else {
         dprev = dnext; /* gradient of d[l+1] */
@@ -339,11 +356,11 @@ double orcUnetLoss(const int *cfg, int n, const double *params, const double *x,
     for (int l = L - 1; l >= 0; l--) {
       int ia = 2 * l, ib = 2 * l + 1;
       reluMask(&dS[l], &s[l]);
-      Ten dA = ten(a[l].n, a[l].c);
+      Ten dA = like(&a[l], a[l].c);
       convBwd(&a[l], NULL, P + N.wOff[ib], &dS[l], &dA, NULL, grad + N.wOff[ib], grad + N.bOff[ib]);
       reluMask(&dA, &a[l]);
       const Ten *inl = l == 0 ? &xin : &pl[l - 1];
-      Ten dIn = ten(inl->n, inl->c);
+      Ten dIn = like(inl, inl->c);
       convBwd(inl, NULL, P + N.wOff[ia], &dA, l == 0 ? NULL : &dIn, NULL, grad + N.wOff[ia],
           grad + N.bOff[ia]);
       free(dA.v);
@@ -367,6 +384,28 @@ double orcUnetLoss(const int *cfg, int n, const double *params, const double *x,
     free(d[l].v);
   }
   return loss;
+}
+
+/* the same on an n^3 cube */
+double orcUnetLoss(const int *cfg, int n, const double *params, const double *x,
+    const double *target, double *grad, double *pred, int bf16)
+{
+  return orcUnetLossBox(cfg, n, n, n, params, x, target, grad, pred, bf16);
+}
+
+/* test hooks: the pooling and upsampling layers alone, on channels-last
+ * boxes; `fine` has extents (2 nx, 2 ny, 2 nz), `coarse` (nx, ny, nz).
+ * dir 0: coarse = pool(fine); 1: fine += pool^T(coarse);
+ *     2: fine = up(coarse);   3: coarse += up^T(fine). */
+void orcUnetTransfer(int dir, int nx, int ny, int nz, int c, double *fine, double *coarse)
+{
+  Ten f = {2 * nx, 2 * ny, 2 * nz, c, fine}, g = {nx, ny, nz, c, coarse};
+  switch (dir) {
+  case 0: poolFwd(&f, &g); break;
+  case 1: poolBwd(&g, &f); break;
+  case 2: upFwd(&g, &f); break;
+  default: upBwd(&f, &g); break;
+  }
 }
 
 /* round-to-nearest-even bfloat16 of x, as a double (test hook) */
