@@ -106,6 +106,7 @@ void nusifDefaultOptions(NusifOptions *o)
   o->mgProject   = 0;
   o->mgTail      = 1;
   o->mgGraph     = 0;
+  o->mgGraphCache = 64;
   o->mgTileRows  = 0;
   o->mgSpeculate = 0;
   o->mgInner     = 1;
@@ -117,6 +118,7 @@ void nusifDefaultOptions(NusifOptions *o)
   o->inflowNoise = 0.0;
   o->seed        = 1;
   o->warmstart   = 0;
+  o->verbose     = NUSIF_VERBOSE_DEFAULT;
 }
 
 static OptEntry *findEntry(const Parameter *p)
@@ -225,6 +227,16 @@ extern "C" int nusifSetOption(Parameter *p, const char *key, const char *value)
   if (k == "mg_project") return parseInt(v.c_str(), &o->mgProject);
   if (k == "mg_tail") return parseInt(v.c_str(), &o->mgTail);
   if (k == "mg_graph") return parseInt(v.c_str(), &o->mgGraph);
+  if (k == "mg_graph_cache") {
+    int r = parseInt(v.c_str(), &o->mgGraphCache);
+    if (r == 0 && o->mgGraphCache < 1) return -2;
+    return r;
+  }
+  if (k == "verbose") {
+    int r = parseInt(v.c_str(), &o->verbose);
+    if (r == 0 && (o->verbose < 0 || o->verbose > 2)) return -2;
+    return r;
+  }
   if (k == "unet_extent") return parseInt(v.c_str(), &o->unetExtent);
   if (k == "fg_rhs") return parseInt(v.c_str(), &o->fgRhs);
   if (k == "rb_fused") return parseInt(v.c_str(), &o->rbFused);

@@ -869,6 +869,8 @@ static void buildInflowProfile(NusifCtx *c)
 }
 
 /* src/discretization.c:49-117 */
+static void printConfiguration(const Discretization *s);
+
 extern "C" void initDiscretization(Discretization *s, Parameter *params)
 {
   s->problem  = params->name;
@@ -967,6 +969,34 @@ extern "C" void initDiscretization(Discretization *s, Parameter *params)
   double dx = s->grid.dx, dy = s->grid.dy, dz = s->grid.dz;
   double invSqrSum = 1.0 / (dx * dx) + 1.0 / (dy * dy) + 1.0 / (dz * dz);
   s->dtBound       = 0.5 * s->re * 1.0 / invSqrSum;
+  if (c->opt.verbose >= 1) printConfiguration(s);
+}
+
+/* What the reference's initDiscretization prints when built with -DVERBOSE
+ * (src/discretization.c:16-47, 114-116; the sequential communicator prints
+ * nothing more): the same lines, in the same formats. */
+static void printConfiguration(const Discretization *s)
+{
+  if (!commIsMaster((CommType *)&s->comm)) return;
+  const Grid &g = s->grid;
+  printf("Parameters for #%s#\n", s->problem);
+  printf("BC Left:%d Right:%d Bottom:%d Top:%d Front:%d Back:%d\n", s->bcLeft, s->bcRight, s->bcBottom,
+      s->bcTop, s->bcFront, s->bcBack);
+  printf("\tReynolds number: %.2f\n", s->re);
+  printf("\tGx Gy: %.2f %.2f %.2f\n", s->gx, s->gy, s->gz);
+  printf("Geometry data:\n");
+  printf("\tDomain box size (x, y, z): %.2f, %.2f, %.2f\n", g.xlength, g.ylength, g.zlength);
+  printf("\tCells (x, y, z): %d, %d, %d\n", g.imax, g.jmax, g.kmax);
+  printf("\tCell size (dx, dy, dz): %f, %f, %f\n", g.dx, g.dy, g.dz);
+  printf("Timestep parameters:\n");
+  printf("\tDefault stepsize: %.2f, Final time %.2f\n", s->dt, s->te);
+  printf("\tdt bound: %.6f\n", s->dtBound);
+  printf("\tTau factor: %.2f\n", s->tau);
+  printf("Iterative parameters:\n");
+  printf("\tMax iterations: %d\n", s->itermax);
+  printf("\tepsilon (stopping tolerance) : %f\n", s->eps);
+  printf("\tgamma factor: %f\n", s->gamma);
+  printf("\tomega (SOR relaxation): %f\n", s->omega);
 }
 
 extern "C" void nusifSyncHost(Discretization *d)

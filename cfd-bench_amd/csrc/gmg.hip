@@ -90,8 +90,9 @@ struct CycleGraph {
 struct CycleCtl;
 
 struct GmgState {
-  std::vector<CycleGraph> graphs; /* replayable cycles, at most GRAPH_CACHE */
+  std::vector<CycleGraph> graphs; /* replayable cycles, at most graphCache */
   int useGraphs;
+  int graphCache;
   int speculate;  /* enqueue cycles ahead of the convergence test */
   int predicted;  /* cycles of the previous solve: the first batch */
   int inner;      /* cycles before a solve's first stop test; 0: nextGroup */
@@ -1582,6 +1583,7 @@ GmgState *gmgCreate(NusifSolverCtx *sc, const Parameter *prm)
     G->L32[l] = LevelT<float>();
   }
   G->useGraphs = c->opt.mgGraph;
+  G->graphCache = c->opt.mgGraphCache;
   G->speculate = c->opt.mgSpeculate;
   G->inner     = c->opt.mgInner;
   G->nextGroup = 1;
@@ -1683,6 +1685,8 @@ GmgState *gmgCreate(NusifSolverCtx *sc, const Parameter *prm)
 
 void gmgFree(GmgState *G)
 {
+  /* no graph is destroyed, and no buffer freed, while a launch may be queued */
+  HIP_CHECK(hipStreamSynchronize(nusifStream()));
   HIP_CHECK(hipFree(G->dCtl));
   HIP_CHECK(hipHostFree(G->hRing));
   HIP_CHECK(hipHostFree(G->hInit));
@@ -1710,8 +1714,10 @@ void gmgFree(GmgState *G)
  * cycle alternates between a few arrangements) and replayed with one
  * hipGraphLaunch. The body contains kernels only (no memset or copy nodes).
  * Timers put events around launches, so cycles run eagerly while any timer
- * is enabled. */
-#define GRAPH_CACHE 8
+ * is enabled. At most graphCache graphs are kept; the oldest is destroyed
+ * to make room, only after the stream has drained, so no launch of it is
+ * still queued or running when it goes (HIP does not document destroying an
+ * executable graph with launches in flight as safe). */
 
 static std::vector<void *> cycleState(NusifCtx *c, GmgState *G)
 {
@@ -1768,7 +1774,8 @@ template <typename Body> static void runCycle(NusifCtx *c, GmgState *G, int key,
       setCycleState(c, G, e.after);
       return;
     }
-  if (G->graphs.size() >= GRAPH_CACHE) {
+  if ((int)G->graphs.size() >= G->graphCache) {
+    HIP_CHECK(hipStreamSynchronize(nusifStream()));
     HIP_CHECK(hipGraphExecDestroy(G->graphs.front().exec));
     G->graphs.erase(G->graphs.begin());
   }

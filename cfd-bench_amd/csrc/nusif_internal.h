@@ -91,6 +91,13 @@ __device__ inline bool xcdBlock(int gx, int gy, int gz, int &bx, int &by, int &b
   return true;
 }
 
+/* Default of the `verbose` option: the reference builds with -DVERBOSE
+ * (config.mk:13), so a program linked against this library prints what the
+ * reference's solve() and initDiscretization print unless told otherwise. */
+#ifndef NUSIF_VERBOSE_DEFAULT
+#define NUSIF_VERBOSE_DEFAULT 1
+#endif
+
 /* Options carried next to a Parameter (nusifSetOption / extra .par keys). */
 enum SolverKind { SOLVER_DEFAULT = 0, SOLVER_RB, SOLVER_MG, SOLVER_GMG };
 struct NusifOptions {
@@ -103,6 +110,7 @@ struct NusifOptions {
   int mgProject;
   int mgTail;      /* gmg: coarse levels in one workgroup (1) or per-level kernels (0) */
   int mgGraph;     /* gmg: replay each cycle from a captured HIP graph (1) or launch eagerly (0) */
+  int mgGraphCache;/* gmg: captured cycle graphs kept at most (oldest evicted first) */
   int mgTileRows;  /* gmg: fused-sweep tile height, 0 automatic, 4, 8 or 16 */
   int mgSpeculate; /* gmg: enqueue cycles ahead of the convergence test (1) or one at a time (0) */
   int mgInner;     /* gmg: cycles before a solve's first convergence test (0: sized from the previous solve) */
@@ -115,6 +123,9 @@ struct NusifOptions {
   int unetExtent;  /* warm start: the network runs on the first level of extent <= this (64) */
   int fgRhs;       /* computeFG also forms rhs (1, default), computeRHS then skips its pass */
   int rbFused;     /* rb, mg: one launch per sweep where the grid allows (1, default) */
+  int verbose;     /* 0 quiet; 1 the reference's -DVERBOSE output (configuration at
+                    * initDiscretization, one line per solve); 2 also -DPROFILING's
+                    * solver-time line after each red-black solve */
 };
 void nusifDefaultOptions(NusifOptions *o);
 /* Options registered for `p` (defaults when none were set). */

@@ -4,9 +4,12 @@
  * This library is built twice from the same sources, libnusif_rb.so and
  * libnusif_mg.so, whose default solver is NUSIF_DEFAULT_SOLVER; the added
  * option `solver` (rb | mg | gmg) overrides it per problem. */
+#include <math.h>
 #include <string.h>
 
 #include "nusif_internal.h"
+#include "profiler.h"
+#include "timing.h"
 #include "unet.h"
 #include "warmstart.h"
 
@@ -47,6 +50,10 @@ extern "C" void initSolver(Solver *s, Discretization *d, Parameter *p)
   if (sc->kind == SOLVER_GMG) sc->gmg = gmgCreate(sc, p);
   if (sc->dctx->opt.warmstart) {
     if (sc->kind != SOLVER_GMG) NUSIF_FATAL("warmstart unet needs solver gmg");
+    /* A kernel-traced run of this combination ended in a host segmentation
+     * fault inside the HIP runtime whose cause is not known; graph replay is
+     * no faster than launching the kernels, so the pair is refused. */
+    if (sc->dctx->opt.mgGraph) NUSIF_FATAL("warmstart unet does not combine with mg_graph 1");
     const char *path = sc->dctx->opt.unetWeights;
     /* a weights file that records its training extent sets the network level */
     int fileExtent = path[0] ? unetFileExtent(path) : 0;
@@ -70,6 +77,31 @@ extern "C" void nusifFreeSolver(Solver *s)
   }
   nusifUnregisterSolver(sc);
   delete sc;
+}
+
+/* The per-solve output of the reference's -DVERBOSE build: red-black SOR
+ * prints its iteration count and the root of its mean-square residual, then
+ * printProfile (src/solver-rb.c:163-168), which with -DPROFILING (verbose 2)
+ * reports the solver time accumulated in T[SOLVER] (src/profiler.c:49-74;
+ * here the wall time of solve(), which returns only after the device has
+ * finished the solve); the reference MG prints its returned residual
+ * (src/solver-mg.c:395-397). The gmg solver reports its V-cycles and
+ * relative residual in the red-black form. */
+static void printSolve(NusifSolverCtx *sc, Solver *s, double res, double tStart)
+{
+  const int verbose = sc->dctx->opt.verbose;
+  if (verbose >= 2) {
+    HIP_CHECK(hipStreamSynchronize(nusifStream()));
+    T[SOLVER] += getTimeStamp() - tStart;
+    C[SOLVER]++;
+  }
+  if (sc->kind == SOLVER_MG) {
+    if (commIsMaster(s->comm)) printf("Residuum: %.6f\n", res);
+    return;
+  }
+  if (commIsMaster(s->comm))
+    printf("Solver took %d iterations to reach %f\n", sc->lastIterations, sc->kind == SOLVER_RB ? sqrt(res) : res);
+  printProfile(s->comm, sc->lastIterations);
 }
 
 /* p and rhs are host pointers (src/main.c:71). When they are the arrays of a
@@ -100,6 +132,7 @@ extern "C" double solve(Solver *s, double *p, const double *rhs)
     nusifRhsSpareSync(c);
   }
   double res = 0.0;
+  const double tStart = c->opt.verbose >= 2 ? getTimeStamp() : 0.0;
   switch (sc->kind) {
   case SOLVER_RB: {
     double *pOut = c->p;
@@ -128,6 +161,7 @@ extern "C" double solve(Solver *s, double *p, const double *rhs)
     HIP_CHECK(hipStreamSynchronize(nusifStream()));
     c->cflValid = false;
   }
+  if (c->opt.verbose >= 1) printSolve(sc, s, res, tStart);
   return res;
 }
 

@@ -547,6 +547,7 @@ extern "C" int nusifUnetTrainCavityGrids(const char *weightsOut, const int *grid
     nusifSetOption(&prm, "solver", "gmg");
     nusifSetOption(&prm, "precision", "mixed");
     nusifSetOption(&prm, "mg_tol", "1e-6");
+    nusifSetOption(&prm, "verbose", "0");
     snprintf(buf, sizeof buf, "%.17g", lid);
     nusifSetOption(&prm, "lid", buf);
     snprintf(buf, sizeof buf, "%d", netExtent);

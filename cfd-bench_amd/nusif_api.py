@@ -202,6 +202,10 @@ class Case:
         self.lib = lib
         self.prm = make_parameter(**params)
         self.options = dict(options or {})
+        if hasattr(lib, "nusifSetOption"):
+            # the library prints the reference's -DVERBOSE lines by default;
+            # a caller driving it from Python reads the results instead
+            self.options.setdefault("verbose", 0)
         if self.options:
             if not hasattr(lib, "nusifSetOption"):
                 raise RuntimeError("library has no nusifSetOption")

@@ -48,7 +48,10 @@ void nusifFreeSolver(Solver *s);
  *   mg_tail       0 | 1   gmg: run the levels of <= 4096 cells in one workgroup
  *                         (1, default) or with one kernel per pass (0); same values
  *   mg_graph      0 | 1   gmg: replay each cycle from a captured HIP graph (1) or
- *                         launch its kernels one by one (0, default); same values
+ *                         launch its kernels one by one (0, default); same values.
+ *                         Not with warmstart unet (initSolver stops with an error)
+ *   mg_graph_cache int    gmg with mg_graph 1: captured cycle graphs kept, the
+ *                         oldest destroyed first once the stream has drained (64)
  *   mg_tile_rows  0|4|8|16 gmg: fused-sweep tile height (0: automatic, 16 on
  *                         levels with >= 128 rows, 8 with >= 64); same values
  *   mg_inner      int     gmg: cycles before a solve's first convergence test
@@ -84,6 +87,17 @@ void nusifFreeSolver(Solver *s);
  *                         allows (imax >= 32, kmax >= 4; 1, default); 0: a
  *                         launch per colour pass. Same p either way; rb's sum
  *                         of r^2 is added in another order
+ *   verbose       0 | 1 | 2  standard output of the reference's default build
+ *                         (config.mk:13-14): 1 = -DVERBOSE, i.e. the
+ *                         configuration at initDiscretization
+ *                         (discretization.c:16-47) and one line per solve:
+ *                         "Solver took %d iterations to reach %f" (rb and
+ *                         gmg, solver-rb.c:163-166) or "Residuum: %.6f" (mg,
+ *                         solver-mg.c:395-397); 2 = -DVERBOSE -DPROFILING,
+ *                         adding printProfile's solver-time and MLUPS line
+ *                         after each rb and gmg solve (profiler.c:49-74,
+ *                         solve() wall time); 0 = none. Default 1, as the
+ *                         reference's build; the Python binding sets 0
  * Returns 0 on success, -1 for an unknown key, -2 for a bad value. */
 int nusifSetOption(Parameter *p, const char *key, const char *value);
 

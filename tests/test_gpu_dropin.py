@@ -1,14 +1,20 @@
 """The drop-in, end to end: the reference's own main.c, compiled unchanged
 and linked against libnusif_<solver>.so (oracle/Makefile target `dropin`),
 runs a .par file on the GPU; the reference executable built from its own
-sources runs the same file on the CPU. Their per-step output (main.c:80-87
-TIME lines), residual.dat (progress.c:63-66) and the VTK result
-(comm.c:436-476 + vtkWriter-seq.c) must agree.
+sources runs the same file on the CPU. Their standard output must agree
+line for line: printParameter, the -DVERBOSE configuration of
+initDiscretization (discretization.c:16-47), each solve's line
+("Solver took %d iterations to reach %f", solver-rb.c:163-166, with equal
+iteration counts; "Residuum: %.6f", solver-mg.c:395-397) and the per-step
+TIME lines (main.c:80-87); only the two wall-clock lines may differ. So must
+residual.dat (progress.c:63-66) and the VTK result (comm.c:436-476 +
+vtkWriter-seq.c).
 
 Both executables live in oracle/_ref (built here from the reference sources,
 never committed); the test is skipped where they were not built.
 """
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -65,6 +71,34 @@ def _run(exe, workdir, par):
     return r.stdout
 
 
+WALLCLOCK = ("Solution took", "Result output took")
+
+
+def _compare_stdout(out_c, out_g):
+    """every line equal, except the wall-clock ones; a solve's residual
+    printed with %f may differ in its last digit (the sums run in another
+    order), its iteration count may not"""
+    lc = [l for l in out_c.splitlines() if not l.startswith(WALLCLOCK)]
+    lg = [l for l in out_g.splitlines() if not l.startswith(WALLCLOCK)]
+    assert len(lc) == len(lg), (len(lc), len(lg))
+    solves = 0
+    for a, b in zip(lc, lg):
+        ma = re.fullmatch(r"Solver took (\d+) iterations to reach (\S+)", a)
+        mr = re.fullmatch(r"Residuum: (\S+)", a)
+        if ma:
+            mb = re.fullmatch(r"Solver took (\d+) iterations to reach (\S+)", b)
+            assert mb and mb.group(1) == ma.group(1), (a, b)
+            assert abs(float(mb.group(2)) - float(ma.group(2))) <= 1.5e-6, (a, b)
+            solves += 1
+        elif mr:
+            mb = re.fullmatch(r"Residuum: (\S+)", b)
+            assert mb and abs(float(mb.group(1)) - float(mr.group(1))) <= 1.5e-6, (a, b)
+            solves += 1
+        else:
+            assert a == b
+    return solves
+
+
 def _vtk(path):
     with open(path) as fh:
         lines = fh.read().splitlines()
@@ -86,8 +120,9 @@ def test_reference_main_on_gpu_library(solver, n, re, te, levels, tmp_path, gpu_
     out_c = _run(cpu, str(tmp_path / "cpu"), par)
     out_g = _run(gpu, str(tmp_path / "gpu"), par)
     t_c = [l for l in out_c.splitlines() if l.startswith("TIME")]
-    t_g = [l for l in out_g.splitlines() if l.startswith("TIME")]
-    assert len(t_c) > 5 and t_c == t_g
+    assert len(t_c) > 5
+    assert _compare_stdout(out_c, out_g) == len(t_c)  # one line per solve
+    assert "Parameters for #dcavity" in out_g  # the -DVERBOSE configuration block
     assert os.path.exists(str(tmp_path / "cpu" / "dcavity.vtk")), os.listdir(str(tmp_path / "cpu"))
     res_c = np.loadtxt(str(tmp_path / "cpu" / "residual.dat"), delimiter=",", ndmin=2)
     res_g = np.loadtxt(str(tmp_path / "gpu" / "residual.dat"), delimiter=",", ndmin=2)
@@ -99,3 +134,25 @@ def test_reference_main_on_gpu_library(solver, n, re, te, levels, tmp_path, gpu_
     assert v_c.shape == v_g.shape == (4 * n ** 3,)
     # %f output: six decimals, so values equal to 1e-10 may differ by one unit
     np.testing.assert_allclose(v_g, v_c, rtol=0, atol=1.5e-6)
+
+
+@pytest.mark.gpu
+def test_reference_main_profiling_lines(tmp_path, gpu_ok):
+    """verbose 2 (the reference's -DVERBOSE -DPROFILING build, whose
+    sequential form the reference itself cannot run: it calls fclose(NULL) at
+    exit, profiler.c:104-108): after each red-black solve, printProfile's
+    line in its format (profiler.c:63-71), with the solver rate of the
+    iterations just printed"""
+    gpu = os.path.join(REF, "NusifSolver-gpu-rb")
+    if not os.path.exists(gpu):
+        pytest.skip("reference executables not built (oracle/Makefile dropin)")
+    par = PAR.format(n=32, re=10.0, te=0.01, levels=1) + "verbose 2 \n"
+    out = _run(gpu, str(tmp_path / "gpu"), par).splitlines()
+    idx = [q for q, l in enumerate(out) if l.startswith("Solver took")]
+    assert len(idx) >= 5
+    for q in idx:
+        m = re.fullmatch(r"Solver Time: (\d+\.\d\d) s Dataset size: (\d+\.\d\d) MB Performance: "
+                         r"(\d+\.\d\d) MLUPS/s Performance total: (\d+\.\d\d) MLUPS/s", out[q + 1])
+        assert m, out[q + 1]
+        assert float(m.group(2)) == round(34 ** 3 * 16e-6, 2)  # padded points x 16 B
+        assert m.group(3) == m.group(4)  # one rank

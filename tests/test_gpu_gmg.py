@@ -2,6 +2,8 @@
 against the oracle's restatement, through the C ABI."""
 import ctypes as C
 import os
+import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -175,22 +177,24 @@ def test_gpu_gmg_tail_cycle_equals_per_level_kernels(shape, levels, precision, g
         np.testing.assert_array_equal(o[0], out[0][0])
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_gpu_gmg_graph_replay_equals_eager_launches(precision, gpu_ok):
+@pytest.mark.parametrize("precision,speculate,cache", [("fp64", 1, 64), ("mixed", 1, 64), ("fp64", 0, 2),
+                                                     ("mixed", 0, 2), ("mixed", 1, 2)])
+def test_gpu_gmg_graph_replay_equals_eager_launches(precision, speculate, cache, gpu_ok):
     """Cycles replayed from captured HIP graphs (mg_graph 1) against the same
     cycles launched kernel by kernel (mg_graph 0): whole cavity steps, each
     with several cycles (replays of the cached graphs) and the step's own
     eager kernels and host synchronisations between them; every field equal
-    bit for bit after every step."""
-    print("gmg graph", precision, flush=True)
+    bit for bit after every step. Speculative cycles: the replayed fold
+    kernel must write the cycle-control slot of the cycle that runs, not of
+    the captured one. A cache of two graphs with groups sized from the
+    previous solve (mg_inner 0) evicts and recaptures throughout."""
+    print("gmg graph", precision, "speculate", speculate, "cache", cache, flush=True)
     prm = dict(G.TRAJ["c1_rb"][1], imax=64, jmax=64, kmax=64, re=1000.0, levels=5,
                presmooth=2, postsmooth=2)
     cases = []
     for graph in (0, 1):
-        # speculative cycles too: the replayed fold kernel must write the
-        # cycle-control slot of the cycle that runs, not of the captured one
         opts = {"solver": "gmg", "precision": precision, "mg_tol": 1e-8, "mg_graph": graph,
-                "mg_speculate": 1}
+                "mg_speculate": speculate, "mg_inner": 0, "mg_graph_cache": cache}
         cases.append(A.Case(lib("gpu_rb"), options=opts, **prm))
     total_cycles = 0
     for step in range(6):
@@ -209,6 +213,47 @@ def test_gpu_gmg_graph_replay_equals_eager_launches(precision, gpu_ok):
 
 WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "cfd-bench_amd", "weights",
                        "unet_cavity.bin")
+
+
+def test_gpu_gmg_graph_replay_run_steps_with_eviction(gpu_ok):
+    """nusifRunSteps (adaptUV queued behind the first cycle group, no host
+    synchronisation between steps but the norm read-backs) with graphs
+    evicted and recaptured from a cache of two: trace and fields equal to the
+    eager launches bit for bit."""
+    print("gmg graph run steps, cache 2", flush=True)
+    prm = dict(G.TRAJ["c1_rb"][1], imax=64, jmax=64, kmax=64, re=1000.0, levels=5,
+               presmooth=2, postsmooth=2)
+    n = 10
+    out = []
+    for graph in (0, 1):
+        opts = {"solver": "gmg", "precision": "mixed", "mg_tol": 1e-8, "mg_graph": graph, "mg_inner": 0,
+                "mg_graph_cache": 2}
+        c = A.Case(lib("gpu_rb"), options=opts, **prm)
+        tr = (C.c_double * (4 * n))()
+        assert c.lib.nusifRunSteps(C.byref(c.d), C.byref(c.s), n, tr) == n
+        out.append((np.array(tr), c.fields()))
+        c.close()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    for k in A.FIELDS:
+        np.testing.assert_array_equal(out[0][1][k], out[1][1][k], err_msg=k)
+
+
+def test_gpu_graph_replay_refused_with_warmstart(gpu_ok, tmp_path):
+    """mg_graph 1 together with warmstart unet stops in initSolver with a
+    message (the combination is not supported), in a child process"""
+    script = tmp_path / "refuse.py"
+    script.write_text("""
+import sys
+sys.path.insert(0, %r)
+import nusif_api as A
+lib = A.load(A.gpu_lib_path("rb"))
+A.Case(lib, options={"solver": "gmg", "precision": "mixed", "mg_graph": 1, "warmstart": "unet",
+                     "unet_weights": %r}, name="dcavity", imax=32, jmax=32, kmax=32, levels=4)
+print("initialised")
+""" % (os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "cfd-bench_amd"), WEIGHTS))
+    r = subprocess.run([sys.executable, str(script)], capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and "initialised" not in r.stdout
+    assert "does not combine with mg_graph 1" in r.stderr
 
 
 @pytest.mark.parametrize("opts", [{"precision": "fp64", "mg_inner": 0}, {"precision": "mixed", "mg_inner": 0},

@@ -264,3 +264,59 @@ def test_gpu_fused_rhs_equals_separate_pass(shape, gpu_ok):
     for q in range(2):
         for k in A.FIELDS:
             np.testing.assert_array_equal(runs[1][q][k], runs[0][q][k], err_msg="%d %s" % (q, k))
+
+
+# Grids whose fused SOR iteration runs the 64x16 tile (jmax >= 128) and the
+# 64x8 tile (jmax >= 64), the shapes of the 256^3 cavity's red-black solve;
+# 70 x 130 leaves partial tiles in i and j. omega 1.9 and eps 3e-2 let the
+# reference's stop test fire after ~350-1500 iterations (oracle counts).
+WIDE = [(64, 128, 24), (48, 64, 16), (70, 130, 10)]
+
+
+def _wide_prm(shape, itermax=3000):
+    return dict(G.TRAJ["c1_rb"][1], imax=shape[0], jmax=shape[1], kmax=shape[2], omg=1.9, eps=3e-2,
+                itermax=itermax)
+
+
+@pytest.mark.parametrize("shape", WIDE)
+def test_gpu_rb_fused_wide_tiles_match_colour_passes(gpu_ok, shape):
+    """The fused iteration with the in-sweep residual on 64x16 and 64x8
+    tiles against one launch per colour pass: p, u, v, w bit for bit and the
+    same iteration counts, with the stop test firing before itermax."""
+    print("rb fused wide", shape, flush=True)
+    a = A.Case(lib("gpu_rb"), options={"rb_fused": 1}, **_wide_prm(shape))
+    b = A.Case(lib("gpu_rb"), options={"rb_fused": 0}, **_wide_prm(shape))
+    its_a, its_b = [], []
+    for _ in range(4):
+        ra, rb = a.step(), b.step()
+        its_a.append(a.iterations())
+        its_b.append(b.iterations())
+        assert ra == pytest.approx(rb, rel=1e-12)
+    print("iterations", its_a, flush=True)
+    assert its_a == its_b
+    assert max(its_a) < 3000  # every solve stopped on the residual
+    fa, fb = a.fields(), b.fields()
+    for k in A.FIELDS:
+        np.testing.assert_array_equal(fa[k], fb[k], err_msg=k)
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("shape", WIDE)
+def test_gpu_rb_fused_wide_tiles_match_oracle(gpu_ok, shape):
+    """The same tiles against the oracle's red-black SOR (solver-rb.c:27-172)
+    with itermax capped at 20, so both run the same iterations: fields to
+    1e-10 relative, the carried mean-square residual to 1e-13 (only its
+    summation order differs)."""
+    print("rb fused wide vs oracle", shape, flush=True)
+    g = A.Case(lib("gpu_rb"), **_wide_prm(shape, itermax=20))
+    o = A.Case(lib("orc_rb"), **_wide_prm(shape, itermax=20))
+    for _ in range(3):
+        rg, ro = g.step(), o.step()
+        assert g.iterations() == o.iterations() == 20
+        assert rg == pytest.approx(ro, rel=1e-13)
+    fg, fo = g.fields(), o.fields()
+    for k in ("u", "v", "w", "p"):
+        assert A.rel_maxnorm(fg[k], fo[k]) <= 1e-10, k
+    g.close()
+    o.close()

@@ -169,3 +169,50 @@ def test_gpu_committed_network_saves_cycles(gpu_ok):
     # measured: 3.83 -> 3.00 cycles per step (0.78), every guess taken; the
     # counts are deterministic, the bound leaves room for retrained weights
     assert cycles["learned"] <= 0.9 * cycles["plain"]
+
+
+def _spoiled_weights(path_in, path_out, head_scale):
+    """a copy of a weights file whose 1x1 head weights are multiplied by
+    head_scale (file: magic, 6 ints, int64 count, FP32 parameters, the head
+    W[ch0] and b last; then the extent trailer)"""
+    raw = bytearray(open(path_in, "rb").read())
+    hdr = np.frombuffer(bytes(raw[10:34]), dtype=np.int32)
+    npar = int(np.frombuffer(bytes(raw[34:42]), dtype=np.int64)[0])
+    prm = np.frombuffer(bytes(raw[42:42 + 4 * npar]), dtype=np.float32).copy()
+    ch0 = int(hdr[1])
+    prm[npar - ch0 - 1:npar - 1] *= head_scale
+    raw[42:42 + 4 * npar] = prm.tobytes()
+    open(path_out, "wb").write(bytes(raw))
+
+
+@pytest.mark.parametrize("precision,n", [("mixed", 64), ("mixed", 128), ("fp64", 128)])
+def test_gpu_forced_fallback_equals_solve_without_network(precision, n, gpu_ok, tmp_path):
+    """A network whose guess is poor (the committed weights with the head
+    scaled by -20: the correction reversed and amplified) is rejected on
+    every step, and the steps are then those of the same solver without a
+    warm start, bit for bit: p, u, v, w and the cycle counts (the rejected
+    guess has overwritten the partner buffer, and in mixed mode the first
+    defect buffer, which the solve must not read). 64^3: the network on the
+    finest grid; 128^3: on the first coarse level."""
+    weights = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "cfd-bench_amd",
+                           "weights", "unet_cavity.bin")
+    bad = str(tmp_path / "spoiled.bin")
+    _spoiled_weights(weights, bad, -20.0)
+    prm = dict(name="dcavity", re=1000.0, imax=n, jmax=n, kmax=n, tau=0.5, te=1e9, levels=5,
+               presmooth=2, postsmooth=2)
+    base = {"solver": "gmg", "precision": precision, "mg_tol": 1e-6, "mg_omega": 1.3, "mg_inner": 0}
+    a = A.Case(lib("gpu_rb"), options=base, **prm)
+    b = A.Case(lib("gpu_rb"), options=dict(base, warmstart="unet", unet_weights=bad), **prm)
+    for step in range(6):
+        ra, rb = a.step(), b.step()
+        assert b.lib.nusifWarmstartUsed(C.byref(b.s)) == 0, step
+        assert ra == rb and a.iterations() == b.iterations(), step
+    out = (C.c_double * 4)()
+    b.lib.nusifWarmstartResiduals(C.byref(b.s), out)
+    print("restricted residual: previous p %.4e, spoiled guess %.4e" % (out[2], out[3]))
+    assert out[3] > out[2]
+    fa, fb = a.fields(), b.fields()
+    for k in ("u", "v", "w", "p"):
+        np.testing.assert_array_equal(fa[k], fb[k], err_msg=k)
+    a.close()
+    b.close()
