@@ -328,3 +328,109 @@ double mgRefSolve(NusifSolverCtx *sc, double *p, const double *rhs, double **pOu
   getState(c);
   return stateHost(c)->res;
 }
+
+/* ---- allocation-free red-black solve ---------------------------------------
+ * nusifSolveRBNoAlloc: the reference's red-black SOR solve
+ * (src/solver-rb.c:27-172) on device arrays the caller owns, with every
+ * buffer beyond p and rhs in `scratch` below (no hipMalloc, no hipFree).
+ * The kernels are those of solve() with rb_fused 0 (one launch per colour
+ * pass, the ghost copy, the fold), so p, the residual and the iteration
+ * count are those of that solve bit for bit.
+ *
+ * scratch layout (doubles): [0, 32) the RBState (res, it, stopAt; 256 B),
+ * then NOALLOC_PARTIALS partial sums of r^2, nPart per colour pass: 514 KiB
+ * in all, enough for every grid with imax, jmax, kmax <= 512 (nPart = 32768
+ * at 512^3). Each GPU holds one copy for the life of the process; every
+ * call on a GPU uses it, and the RBState is rewritten at the start of each
+ * call (the partials are written before they are read), so nothing carries
+ * over between calls. Calls that overlap on one GPU (two host threads) race
+ * on it. */
+#define NOALLOC_STATE 32
+#define NOALLOC_PARTIALS 65536
+#define NOALLOC_MAX_DEVICES 64
+static __device__ __align__(256) double scratch[NOALLOC_STATE + NOALLOC_PARTIALS];
+static double *gScratchDev[NOALLOC_MAX_DEVICES]; /* scratch's address on each device (set up on first use) */
+
+__global__ void kRBStateInit(RBState *st, double res, int it, int stopAt)
+{
+  if (threadIdx.x == 0) {
+    st->res    = res;
+    st->it     = it;
+    st->stopAt = stopAt;
+  }
+}
+
+/* the device address of scratch on the current device, looked up once per
+ * device with that device current; null (and a message) on failure */
+static double *noAllocScratch()
+{
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+    fprintf(stderr, "nusif: nusifSolveRBNoAlloc: no usable current device (%d)\n", dev);
+    return nullptr;
+  }
+  if (!gScratchDev[dev]) {
+    void *ptr      = nullptr;
+    hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (err != hipSuccess || !ptr) {
+      fprintf(stderr, "nusif: nusifSolveRBNoAlloc: hipGetSymbolAddress failed on device %d: %s\n", dev,
+          hipGetErrorString(err));
+      return nullptr;
+    }
+    gScratchDev[dev] = (double *)ptr;
+  }
+  return gScratchDev[dev];
+}
+
+extern "C" int nusifSolveRBNoAlloc(double *p, const double *rhs, int imax, int jmax, int kmax, double dx,
+    double dy, double dz, double omega, double eps, int itermax, double *res, int *iterations)
+{
+  if (!p || !rhs || imax < 1 || jmax < 1 || kmax < 1 || !(dx > 0.0) || !(dy > 0.0) || !(dz > 0.0)) {
+    fprintf(stderr, "nusif: nusifSolveRBNoAlloc: bad arguments (grid %d x %d x %d)\n", imax, jmax, kmax);
+    return -1;
+  }
+  const Geo g       = makeGeo(imax, jmax, kmax);
+  const SorLaunch L = sorShape(g);
+  if (2LL * L.nPart > NOALLOC_PARTIALS) {
+    fprintf(stderr,
+        "nusif: nusifSolveRBNoAlloc: grid %d x %d x %d needs %lld partial sums, more than the %d its fixed "
+        "scratch holds (every grid with imax, jmax, kmax <= 512 fits)\n",
+        imax, jmax, kmax, 2LL * L.nPart, NOALLOC_PARTIALS);
+    return -2;
+  }
+  double *base = noAllocScratch();
+  if (!base) return -3;
+  RBState *st      = (RBState *)base;
+  double *partials = base + NOALLOC_STATE;
+  Grid grid;
+  grid.dx = dx;
+  grid.dy = dy;
+  grid.dz = dz;
+  const SweepCoef cf = sweepCoef(&grid, omega);
+  const double epssq = eps * eps;
+  const double n     = (double)imax * jmax * kmax;
+  hipStream_t s      = nusifStream();
+  int stopAt         = (1.0 >= epssq && 0 < itermax) ? itermax : 0;
+  kRBStateInit<<<1, 64, 0, s>>>(st, 1.0, 0, stopAt);
+  const dim3 gg = ghostGrid(g);
+  RBState h     = {1.0, 0, stopAt};
+  int launched = 0;
+  while (launched < stopAt) {
+    int end = launched + 8 < itermax ? launched + 8 : itermax;
+    for (int q = launched; q < end; q++) {
+      kColourPass<<<L.grd, L.blk, 0, s>>>(p, rhs, g, cf, 1, q, st, partials, 1);
+      kColourPass<<<L.grd, L.blk, 0, s>>>(p, rhs, g, cf, 0, q, st, partials + L.nPart, 1);
+      kRBGhosts<<<gg, dim3(64, 4), 0, s>>>(p, g, q, st);
+      kRBFold<<<1, 256, 0, s>>>(q, st, partials, L.nPart, 2, n, epssq, itermax);
+    }
+    launched = end;
+    HIP_CHECK(hipMemcpyAsync(&h, st, sizeof(RBState), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    stopAt = h.stopAt;
+  }
+  HIP_CHECK(hipMemcpyAsync(&h, st, sizeof(RBState), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  if (res) *res = h.res;
+  if (iterations) *iterations = h.it;
+  return 0;
+}

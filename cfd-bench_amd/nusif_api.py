@@ -150,6 +150,8 @@ def _declare_optional(lib):
         "nusifRunSteps": ([C.POINTER(Discretization), C.POINTER(Solver), C.c_int,
                            C.POINTER(C.c_double)], C.c_int),
         "nusifSolveTiming": ([C.POINTER(Discretization), C.c_int], None),
+        "nusifSolveRBNoAlloc": ([C.c_void_p, C.c_void_p] + [C.c_int] * 3 + [C.c_double] * 5 + [C.c_int, _DP,
+                                C.POINTER(C.c_int)], C.c_int),
         "nusifSolveTimes": ([C.POINTER(Discretization), _DP, C.POINTER(C.c_longlong)], None),
         # oracle extras
         "orcSetProblemExtras": ([C.c_double, C.c_double], None),

@@ -193,6 +193,22 @@ void nusifRestrictResidualCheck(int ni, int nj, int nk, double dx, double dy, do
     const double *rhs, float *rc, float *bc, double *sumsq);
 void nusifProlongSetCheck(int nic, int njc, int nkc, const float *xc, float *xf);
 
+/* No counterpart in the reference: the red-black SOR solve of solve() with
+ * rb_fused 0 (src/solver-rb.c:27-172) for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc). p (in/out) and rhs are DEVICE arrays of the padded
+ * layout, (imax+2)(jmax+2)(kmax+2) doubles, i fastest; dx, dy, dz, omega,
+ * eps and itermax are the Solver's. It calls no hipMalloc or hipFree: its
+ * partial sums and stop state live in a fixed 514 KiB device array the
+ * library holds per GPU for the life of the process (every grid with imax,
+ * jmax, kmax <= 512 fits; larger ones return -2). The first call on a device
+ * looks the array up there (and creates the library stream if nothing did).
+ * Returns 0 and sets *res (the carried mean-square residual, as solve()
+ * returns it) and *iterations; -1 bad arguments, -2 grid too large, -3 no
+ * device. Calls that overlap on one GPU share the array and race. */
+int nusifSolveRBNoAlloc(double *p, const double *rhs, int imax, int jmax, int kmax, double dx, double dy,
+    double dz, double omega, double eps, int itermax, double *res, int *iterations);
+
 /* Runs the reference time loop (src/main.c:60-88) for `nsteps` steps entirely
  * inside the library: the same calls in the same order, without the Python
  * or host-driver overhead between them. trace (may be NULL) receives
