@@ -150,9 +150,27 @@ def solver_rate(res, cells):
             "solve_ms_per_step": s["ms"] / s["solves"]}
 
 
+def host_cpu():
+    """model name and logical CPU count of this host (/proc/cpuinfo)"""
+    model, count = "unknown", os.cpu_count()
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, count
+
+
 def cpu_baseline(case_gpu, n, budget_s=20.0):
     """The oracle (one host core) on the same workload: the state after the
-    GPU warmup, then whole steps with its FP64 multigrid to the same 1e-6."""
+    GPU warmup, then whole steps with its FP64 multigrid to the same 1e-6,
+    with the GPU modes' smoother (omega 1.3) and first stop test after as many
+    cycles as the previous solve needed (the GPU's mg_inner 0 grouping, whose
+    contraction-based shrinking it leaves out)."""
+    omega = float(mode_options("fp64")["mg_omega"])
     orc = A.load(A.oracle_lib_path("rb_fast"))
     gmg = A.load(A.oracle_lib_path("gmg_fast"))
     prm = cavity_params(n)
@@ -160,7 +178,7 @@ def cpu_baseline(case_gpu, n, budget_s=20.0):
     o.set_fields(**case_gpu.fields())
     d = o.d
     cells = n ** 3
-    steps, t_total = 0, 0.0
+    steps, t_total, group, cycles = 0, 0.0, 1, 0
     while t_total < budget_s and steps < 50:
         t0 = time.perf_counter()
         orc.computeTimestep(C.byref(d))
@@ -169,19 +187,24 @@ def cpu_baseline(case_gpu, n, budget_s=20.0):
             getattr(orc, fn)(C.byref(d))
         rel = C.c_double()
         g = d.grid
-        gmg.orcGmgSolve(g.imax, g.jmax, g.kmax, g.dx, g.dy, g.dz, d.p, d.rhs, prm["levels"],
-                        prm["presmooth"], prm["postsmooth"], 1.0, 1e-6, 100, 0, 0,
-                        C.byref(rel))
+        c = gmg.orcGmgSolveGrouped(g.imax, g.jmax, g.kmax, g.dx, g.dy, g.dz, d.p, d.rhs, prm["levels"],
+                                   prm["presmooth"], prm["postsmooth"], omega, 1e-6, 100, 0, 0, group,
+                                   C.byref(rel))
         orc.adaptUV(C.byref(d))
         t_total += time.perf_counter() - t0
         steps += 1
-        log("[bench] cpu baseline step %d: %.2f s" % (steps, t_total))
+        cycles += c
+        group = max(1, c)
+        log("[bench] cpu baseline step %d: %.2f s, %d cycles" % (steps, t_total, c))
     o.close()
+    model, ncpu = host_cpu()
     return dict(value=steps / t_total, unit="time steps/s", cores=1, kind="port",
+                host_cpu=model, host_logical_cpus=ncpu,
                 sample="%d step(s) of the %d^3 cavity from the GPU state after warmup; oracle "
-                       "C restatement (-O3, one host core), FP64 V(2,2) 6-level multigrid to "
-                       "1e-6 relative residual" % (steps, n),
-                mlups=cells * steps / t_total * 1e-6)
+                       "C restatement (-O3 -ffast-math, one core of %s), FP64 V(2,2) 6-level multigrid "
+                       "to 1e-6 relative residual, smoother omega %.1f as on the GPU, first stop test "
+                       "after the previous solve's cycle count" % (steps, n, model, omega),
+                cycles_per_step=cycles / steps, mlups=cells * steps / t_total * 1e-6)
 
 
 def run_rb(lib, case_gpu, n, steps=3):

@@ -58,3 +58,25 @@ def test_cpu_rb_baseline_kinds():
     if has_ref:
         ref = out[1]
         assert ref["kind"] == "reference" and ref["iterations"] == 12 and ref["solver_mlups"] > 0
+
+
+CPU_GMG = """
+import json, sys
+sys.path.insert(0, %r)
+import bench, nusif_api as A
+src = A.Case(A.load(A.oracle_lib_path("rb_fast")), **bench.cavity_params(32)).run(2)
+out = bench.cpu_baseline(src, 32, budget_s=0.05)
+src.close()
+print(json.dumps(out))
+"""
+
+
+def test_cpu_gmg_baseline():
+    """The headline CPU baseline (in a child process, as above): the oracle's
+    FP64 multigrid with the GPU modes' smoother omega, the host CPU named"""
+    r = subprocess.run([sys.executable, "-c", CPU_GMG % ROOT], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["kind"] == "port" and out["cores"] == 1 and out["value"] > 0
+    assert "omega 1.3" in out["sample"] and out["host_cpu"] in out["sample"]
+    assert out["cycles_per_step"] >= 1
