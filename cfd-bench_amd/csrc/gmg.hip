@@ -98,6 +98,7 @@ struct GmgState {
   int inner;      /* cycles before a solve's first stop test; 0: nextGroup */
   int nextGroup;  /* mg_inner 0: the next solve's first group (starts at 1) */
   int lastGroup;  /* the last solve's first group */
+  int groupCap;   /* mixed, mg_inner 0: longest useful first group (0: none) */
   CycleCtl *dCtl; /* device cycle control */
   CycleCtl *hRing;/* pinned copies of dCtl after each enqueued cycle */
   CycleCtl *hRingDev; /* their device address (written by kFoldCycle) */
@@ -1588,6 +1589,7 @@ GmgState *gmgCreate(NusifSolverCtx *sc, const Parameter *prm)
   G->inner     = c->opt.mgInner;
   G->nextGroup = 1;
   G->lastGroup = 0;
+  G->groupCap  = 0;
   G->predicted = 1;
   HIP_CHECK(hipMalloc((void **)&G->dCtl, sizeof(CycleCtl)));
   HIP_CHECK(hipHostMalloc((void **)&G->hRing, CTL_RING * sizeof(CycleCtl), hipHostMallocCoherent));
@@ -1834,8 +1836,9 @@ static double readNorms(NusifCtx *c, double *nb)
  * and, in mixed mode, the FP64 correction + residual pass between the
  * cycles. G->inner = 0 sizes the group from the previous solve
  * (nextGroupAfter): a group that did not converge gives the exact count the
- * solve needed; one that converged shrinks by one when its own contraction
- * says one cycle fewer would have reached the tolerance. A solve that needs
+ * solve needed (in mixed precision at most the length the FP32 bound on one
+ * correction leaves useful); one that converged shrinks by one when its own
+ * contraction says one cycle fewer would have reached the tolerance. A solve that needs
  * fewer cycles than its group still runs the whole group.
  *
  * Speculative mode: the host enqueues cycles before it knows whether the
@@ -1849,11 +1852,30 @@ static double readNorms(NusifCtx *c, double *nb)
 /* the next solve's first group under mg_inner 0, from this solve's: group
  * `first` took the relative residual from rel0 to relFirst; the solve ended
  * after `cycles` */
-static int nextGroupAfter(const GmgState *G, int first, int cycles, double rel0, double relFirst)
+static int nextGroupAfter(GmgState *G, int first, int cycles, double rel0, double relFirst, double relEnd)
 {
   int next = first;
   if (cycles > first) {
     next = cycles;
+    /* Mixed precision: the group's cycles solve for one correction of one
+     * FP64 defect in FP32, so the reduction a group can reach is bounded
+     * (FP32 rounding of the correction, amplified by the operator: about
+     * eps32 (n/pi)^2 for n cells along the longest side, a few percent at
+     * 768 cells). A group that reduced the residual less per cycle than the
+     * corrected single cycles after it had reached that bound: a longer one
+     * buys nothing. Its useful length is the single cycles that give the
+     * same reduction; it caps every later group (without the cap, each
+     * solve would lengthen the group by the cycles the bound forced after
+     * it, without end). */
+    if (G->mixed && first > 1 && rel0 > 0.0 && relFirst > 0.0 && relEnd > 0.0 && relEnd < relFirst) {
+      const double rhoGroup = pow(relFirst / rel0, 1.0 / first);
+      const double rhoOne   = pow(relEnd / relFirst, 1.0 / (cycles - first));
+      if (rhoGroup > rhoOne) {
+        int m = (int)ceil(log(relFirst / rel0) / log(rhoOne));
+        G->groupCap = std::max(1, std::min(m, first - 1));
+      }
+    }
+    if (G->groupCap > 0) next = std::min(next, G->groupCap);
   } else if (first > 1 && rel0 > 0.0 && relFirst > 0.0) {
     /* mean contraction per cycle over the group; the residual one cycle
      * earlier was about relFirst / rho */
@@ -1902,7 +1924,7 @@ static int runCycles(NusifCtx *c, GmgState *G, NusifSolverCtx *sc, double &rel, 
       rel       = rn / nb;
       if (cycles == m) relFirst = rel;
     }
-    if (G->inner == 0 && cycles > 0) G->nextGroup = nextGroupAfter(G, G->lastGroup, cycles, rel0, relFirst);
+    if (G->inner == 0 && cycles > 0) G->nextGroup = nextGroupAfter(G, G->lastGroup, cycles, rel0, relFirst, rel);
     return cycles;
   }
   if (!(nb > 0.0 && rel > G->tol && G->maxCycles > 0)) return 0;

@@ -58,8 +58,12 @@ extern "C" void initSolver(Solver *s, Discretization *d, Parameter *p)
     /* a weights file that records its training extent sets the network level */
     int fileExtent = path[0] ? unetFileExtent(path) : 0;
     if (fileExtent > 0) sc->dctx->opt.unetExtent = fileExtent;
-    sc->ws = warmstartCreate(sc->dctx, nullptr, false);
-    UnetModel *net   = path[0] ? unetLoad(path, warmstartExtent(sc->ws)) : nullptr;
+    const int levels = path[0] ? unetFileLevels(path) : 0;
+    if (levels < 1) NUSIF_FATAL("warmstart unet: cannot read weights '%s' (option unet_weights)", path);
+    sc->ws = warmstartCreate(sc->dctx, nullptr, false, levels);
+    int nx, ny, nz;
+    warmstartExtents(sc->ws, &nx, &ny, &nz);
+    UnetModel *net = unetLoad(path, nx, ny, nz);
     if (!net) NUSIF_FATAL("warmstart unet: cannot read weights '%s' (option unet_weights)", path);
     warmstartSetNet(sc->ws, net);
   }

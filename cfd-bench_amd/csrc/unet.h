@@ -7,7 +7,8 @@
 #define UNET_MAXL 4
 
 struct UnetModel;
-UnetModel *unetNew(int L, const int *ch, int cin, int n);
+/* a network on nx x ny x nz voxels (each divisible by 2^(L-1)) */
+UnetModel *unetNew(int L, const int *ch, int cin, int nx, int ny, int nz);
 void unetFree(UnetModel *m);
 void unetInit(UnetModel *m, unsigned seed);
 size_t unetParamCount(const UnetModel *m);
@@ -16,17 +17,18 @@ void unetGetParams(UnetModel *m, float *hostParams);
 void unetGetGrads(UnetModel *m, float *hostGrads);
 void unetZeroGrads(UnetModel *m);
 void unetSyncWeights(UnetModel *m);
-/* x: n^3 x cin bf16 (channels-last); the prediction stays on the device */
+/* x: nx ny nz x cin bf16 (channels-last); the prediction stays on the device */
 /* forTraining: also keep the upsampled decoder inputs unetBackward reads */
 void unetForward(UnetModel *m, const uint16_t *x, bool forTraining = false);
 const float *unetOutput(UnetModel *m);
-/* after unetForward: loss against t (n^3 FP32), gradients added into G */
+/* after unetForward: loss against t (nx ny nz FP32), gradients added into G */
 double unetBackward(UnetModel *m, const uint16_t *x, const float *t, bool wantLoss);
 double unetLastLoss(UnetModel *m);
 void unetAdam(UnetModel *m, float lr, int batch);
 int unetSave(UnetModel *m, const char *path);
-UnetModel *unetLoad(const char *path, int n);
+UnetModel *unetLoad(const char *path, int nx, int ny, int nz);
 int unetFileExtent(const char *path);
+int unetFileLevels(const char *path);
 int unetLevels(const UnetModel *m);
-int unetExtent(const UnetModel *m);
+void unetExtents(const UnetModel *m, int *nx, int *ny, int *nz);
 #endif

@@ -32,68 +32,73 @@ static inline unsigned blocksFor(size_t n) { return (unsigned)((n + 255) / 256 <
 __global__ void kFillF(float *a, size_t n, float v) { GRID_STRIDE(q, n) a[q] = v; }
 __global__ void kFillH(uint16_t *a, size_t n) { GRID_STRIDE(q, n) a[q] = 0; }
 
-/* out (extent m = n/2) = average of the 8 children, bf16 */
-__global__ void kPoolFwd(const uint16_t *in, uint16_t *out, int m, int c)
+/* Box extents of a tensor; the coarse partner of a fine box has half of
+ * each. Voxel v = (z * ny + y) * nx + x. */
+struct Ext {
+  int nx, ny, nz;
+  __host__ __device__ size_t voxels() const { return (size_t)nx * ny * nz; }
+  __host__ __device__ size_t at(int x, int y, int z) const { return ((size_t)z * ny + y) * nx + x; }
+  __host__ __device__ Ext half() const { return {nx / 2, ny / 2, nz / 2}; }
+};
+
+/* out (extents m) = average of the 8 children in `in` (extents 2 m), bf16 */
+__global__ void kPoolFwd(const uint16_t *in, uint16_t *out, Ext m, int c)
 {
-  size_t tot = (size_t)m * m * m * c;
-  int n      = 2 * m;
+  const Ext n  = {2 * m.nx, 2 * m.ny, 2 * m.nz};
+  size_t tot   = m.voxels() * c;
   GRID_STRIDE(q, tot)
   {
     int k = (int)(q % c);
     size_t v = q / c;
-    int x = (int)(v % m), y = (int)((v / m) % m), z = (int)(v / ((size_t)m * m));
+    int x = (int)(v % m.nx), y = (int)((v / m.nx) % m.ny), z = (int)(v / ((size_t)m.nx * m.ny));
     float s = 0.f;
-    for (int e = 0; e < 8; e++) {
-      int zz = 2 * z + (e >> 2), yy = 2 * y + ((e >> 1) & 1), xx = 2 * x + (e & 1);
-      s += bf2f(in[(((size_t)zz * n + yy) * n + xx) * c + k]);
-    }
+    for (int e = 0; e < 8; e++)
+      s += bf2f(in[n.at(2 * x + (e & 1), 2 * y + ((e >> 1) & 1), 2 * z + (e >> 2)) * c + k]);
     out[q] = f2bf(0.125f * s);
   }
 }
 
-/* dIn (extent n) += dOut(parent) / 8 */
-__global__ void kPoolBwd(const float *dOut, float *dIn, int n, int c)
+/* dIn (extents n) += dOut(parent) / 8 */
+__global__ void kPoolBwd(const float *dOut, float *dIn, Ext n, int c)
 {
-  size_t tot = (size_t)n * n * n * c;
-  int m      = n / 2;
+  const Ext m = n.half();
+  size_t tot  = n.voxels() * c;
   GRID_STRIDE(q, tot)
   {
     int k = (int)(q % c);
     size_t v = q / c;
-    int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((size_t)n * n));
-    dIn[q] += 0.125f * dOut[(((size_t)(z / 2) * m + y / 2) * m + x / 2) * c + k];
+    int x = (int)(v % n.nx), y = (int)((v / n.nx) % n.ny), z = (int)(v / ((size_t)n.nx * n.ny));
+    dIn[q] += 0.125f * dOut[m.at(x / 2, y / 2, z / 2) * c + k];
   }
 }
 
-/* out (extent n) = in(parent), bf16 */
-__global__ void kUpFwd(const uint16_t *in, uint16_t *out, int n, int c)
+/* out (extents n) = in(parent), bf16 */
+__global__ void kUpFwd(const uint16_t *in, uint16_t *out, Ext n, int c)
 {
-  size_t tot = (size_t)n * n * n * c;
-  int m      = n / 2;
+  const Ext m = n.half();
+  size_t tot  = n.voxels() * c;
   GRID_STRIDE(q, tot)
   {
     int k = (int)(q % c);
     size_t v = q / c;
-    int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((size_t)n * n));
-    out[q] = in[(((size_t)(z / 2) * m + y / 2) * m + x / 2) * c + k];
+    int x = (int)(v % n.nx), y = (int)((v / n.nx) % n.ny), z = (int)(v / ((size_t)n.nx * n.ny));
+    out[q] = in[m.at(x / 2, y / 2, z / 2) * c + k];
   }
 }
 
-/* dIn (extent m) = sum of the 8 children of dOut */
-__global__ void kUpBwd(const float *dOut, float *dIn, int m, int c)
+/* dIn (extents m) = sum of the 8 children of dOut (extents 2 m) */
+__global__ void kUpBwd(const float *dOut, float *dIn, Ext m, int c)
 {
-  size_t tot = (size_t)m * m * m * c;
-  int n      = 2 * m;
+  const Ext n = {2 * m.nx, 2 * m.ny, 2 * m.nz};
+  size_t tot  = m.voxels() * c;
   GRID_STRIDE(q, tot)
   {
     int k = (int)(q % c);
     size_t v = q / c;
-    int x = (int)(v % m), y = (int)((v / m) % m), z = (int)(v / ((size_t)m * m));
+    int x = (int)(v % m.nx), y = (int)((v / m.nx) % m.ny), z = (int)(v / ((size_t)m.nx * m.ny));
     float s = 0.f;
-    for (int e = 0; e < 8; e++) {
-      int zz = 2 * z + (e >> 2), yy = 2 * y + ((e >> 1) & 1), xx = 2 * x + (e & 1);
-      s += dOut[(((size_t)zz * n + yy) * n + xx) * c + k];
-    }
+    for (int e = 0; e < 8; e++)
+      s += dOut[n.at(2 * x + (e & 1), 2 * y + ((e >> 1) & 1), 2 * z + (e >> 2)) * c + k];
     dIn[q] = s;
   }
 }
@@ -220,19 +225,26 @@ __global__ void kAdam(float *p, const float *g, float *m, float *v, size_t n, fl
 }
 
 /* ---- model ---------------------------------------------------------------- */
+/* 16 x 4 x 4 output bricks of the LDS-staged forward convolution */
+static int convBricks(const Ext &e)
+{
+  return ((e.nx + CL_BX - 1) / CL_BX) * ((e.ny + CL_BY - 1) / CL_BY) * ((e.nz + CL_BZ - 1) / CL_BZ);
+}
+
 struct ConvL {
   int cin, cout;
   size_t wOff, bOff;
 };
 
 struct LevelBuf {
-  int n;                     /* extent */
+  Ext e;                     /* extents */
   uint16_t *a, *s, *pl, *up, *da, *d; /* activations (bf16) */
   float *gA, *gS, *gD, *gU, *gP;      /* gradients (fp32) */
 };
 
 struct UnetModel {
-  int L, ch[UNET_MAXL], cin, n;
+  int L, ch[UNET_MAXL], cin;
+  Ext e; /* extents of the finest level */
   int nconv;
   ConvL conv[4 * UNET_MAXL];
   size_t nParams, headW, headB;
@@ -245,8 +257,6 @@ struct UnetModel {
   float *split;    /* split-K partial sums of the small-cube forward convs */
   size_t splitCap; /* floats */
 };
-
-static size_t cube(int n) { return (size_t)n * n * n; }
 
 template <typename T> static T *dalloc(size_t n)
 {
@@ -281,11 +291,11 @@ static void layout(UnetModel *m)
 
 static void allocBuffers(UnetModel *m)
 {
-  int n = m->n;
-  for (int l = 0; l < m->L; l++, n /= 2) {
+  Ext e = m->e;
+  for (int l = 0; l < m->L; l++, e = e.half()) {
     LevelBuf &b = m->lv[l];
-    b.n         = n;
-    size_t v    = cube(n);
+    b.e         = e;
+    size_t v    = e.voxels();
     int c = m->ch[l], cn = l + 1 < m->L ? m->ch[l + 1] : 0, cp = l > 0 ? m->ch[l - 1] : m->cin;
     b.a  = dalloc<uint16_t>(v * c);
     b.s  = dalloc<uint16_t>(v * c);
@@ -299,18 +309,20 @@ static void allocBuffers(UnetModel *m)
     b.gU = cn ? dalloc<float>(v * cn) : nullptr;
     b.gP = dalloc<float>(v * cp);
   }
-  m->y        = dalloc<float>(cube(m->n));
-  m->dy       = dalloc<float>(cube(m->n));
+  m->y        = dalloc<float>(m->e.voxels());
+  m->dy       = dalloc<float>(m->e.voxels());
   m->partials = dalloc<double>(65536);
   m->lossDev  = dalloc<double>(1);
   HIP_CHECK(hipHostMalloc((void **)&m->lossHost, sizeof(double), hipHostMallocDefault));
 }
 
-UnetModel *unetNew(int L, const int *ch, int cin, int n)
+UnetModel *unetNew(int L, const int *ch, int cin, int nx, int ny, int nz)
 {
   if (L < 1 || L > UNET_MAXL) NUSIF_FATAL("unet: levels %d outside 1..%d", L, UNET_MAXL);
   if (cin % 8) NUSIF_FATAL("unet: input channels %d must be a multiple of 8", cin);
-  if (n % (1 << (L - 1))) NUSIF_FATAL("unet: extent %d not divisible by 2^(L-1)", n);
+  const int div = 1 << (L - 1);
+  if (nx < 1 || ny < 1 || nz < 1 || nx % div || ny % div || nz % div)
+    NUSIF_FATAL("unet: extents %d x %d x %d not all divisible by 2^(L-1) = %d", nx, ny, nz, div);
   for (int l = 0; l < L; l++)
     if (ch[l] % 16 || ch[l] > 128) NUSIF_FATAL("unet: channels must be multiples of 16 up to 128");
   if (ch[0] > 16) NUSIF_FATAL("unet: the head supports at most 16 channels on level 0");
@@ -318,7 +330,7 @@ UnetModel *unetNew(int L, const int *ch, int cin, int n)
   memset(m, 0, sizeof(*m));
   m->L   = L;
   m->cin = cin;
-  m->n   = n;
+  m->e   = {nx, ny, nz};
   for (int l = 0; l < L; l++) m->ch[l] = ch[l];
   layout(m);
   m->P  = dalloc<float>(m->nParams);
@@ -337,18 +349,18 @@ UnetModel *unetNew(int L, const int *ch, int cin, int n)
   {
     size_t cap = 0;
     int q      = 0;
-    auto need  = [&](int k, int nl) {
-      int bricks = ((nl + CL_BX - 1) / CL_BX) * ((nl + CL_BY - 1) / CL_BY) * ((nl + CL_BZ - 1) / CL_BZ);
+    auto need  = [&](int k, Ext e) {
+      int bricks = convBricks(e);
       int chunks = (m->conv[k].cin + CL_CC - 1) / CL_CC;
-      if (bricks < 256 && chunks > 1) cap = std::max(cap, (size_t)chunks * cube(nl) * m->conv[k].cout);
+      if (bricks < 256 && chunks > 1) cap = std::max(cap, (size_t)chunks * e.voxels() * m->conv[k].cout);
     };
     for (int l = 0; l < L; l++) {
-      need(q++, n >> l);
-      need(q++, n >> l);
+      need(q++, m->lv[l].e);
+      need(q++, m->lv[l].e);
     }
     for (int l = L - 2; l >= 0; l--) {
-      need(q++, n >> l);
-      need(q++, n >> l);
+      need(q++, m->lv[l].e);
+      need(q++, m->lv[l].e);
     }
     m->splitCap = cap;
     m->split    = cap ? dalloc<float>(cap) : nullptr;
@@ -439,7 +451,7 @@ void unetSyncWeights(UnetModel *m)
 /* ---- convolution launchers ---------------------------------------------- */
 template <typename InT, typename OutT> static void launchConv(const ConvArgs<InT, OutT> &a)
 {
-  int rows = a.r0 + a.r1, nvox = a.n * a.n * a.n;
+  int rows = a.r0 + a.r1, nvox = (int)a.voxels();
   const int MB = 2;
   dim3 blk(256);
   auto grid = [&](int NB) { return dim3((nvox + 4 * 16 * MB - 1) / (4 * 16 * MB), (rows + 16 * NB - 1) / (16 * NB)); };
@@ -457,12 +469,11 @@ template <typename InT, typename OutT> static void launchConv(const ConvArgs<InT
  * kConvSplitFinish, so the result does not depend on scheduling. */
 static void launchConvLds(UnetModel *m, const ConvArgs<uint16_t, uint16_t> &a)
 {
-  int rows = a.r0 + a.r1, n = a.n, cin = a.c0 + a.c1;
-  unsigned bricks = (unsigned)(((n + CL_BX - 1) / CL_BX) * ((n + CL_BY - 1) / CL_BY) * ((n + CL_BZ - 1) / CL_BZ));
+  int rows = a.r0 + a.r1, cin = a.c0 + a.c1;
+  unsigned bricks = (unsigned)convBricks({a.nx, a.ny, a.nz});
   int chunks      = (cin + CL_CC - 1) / CL_CC;
   int split       = 1;
-  if (bricks < 256 && chunks > 1 &&
-      (size_t)chunks * n * n * n * rows <= m->splitCap) split = chunks;
+  if (bricks < 256 && chunks > 1 && (size_t)chunks * a.voxels() * rows <= m->splitCap) split = chunks;
   auto grid = [&](int NB) { return dim3(bricks, (rows + 16 * NB - 1) / (16 * NB), split); };
   hipStream_t st = nusifStream();
   const int nb = rows >= 64 && bricks >= 512 ? 4 : (rows >= 32 && bricks >= 512 ? 2 : 1);
@@ -483,7 +494,7 @@ static void launchConvLds(UnetModel *m, const ConvArgs<uint16_t, uint16_t> &a)
   else if (nb == 2) withGb(std::integral_constant<int, 2>());
   else withGb(std::integral_constant<int, 1>());
   if (split > 1) {
-    size_t total = (size_t)n * n * n * rows;
+    size_t total = (size_t)a.voxels() * rows;
     unsigned nb  = (unsigned)std::min<size_t>((total + 255) / 256, 4096);
     kConvSplitFinish<uint16_t><<<nb, 256, 0, st>>>(a, m->split, split);
   }
@@ -491,7 +502,7 @@ static void launchConvLds(UnetModel *m, const ConvArgs<uint16_t, uint16_t> &a)
 
 /* forward conv k: out = relu(conv(in0 ++ in1) + b) */
 static void convForward(UnetModel *m, int k, const uint16_t *in0, int c0, const uint16_t *in1, int c1,
-    uint16_t *out, int n, bool half0 = false)
+    uint16_t *out, Ext e, bool half0 = false)
 {
   const ConvL &c = m->conv[k];
   ConvArgs<uint16_t, uint16_t> a;
@@ -505,7 +516,9 @@ static void convForward(UnetModel *m, int k, const uint16_t *in0, int c0, const 
   a.out1 = nullptr;
   a.r0 = c.cout;
   a.r1 = 0;
-  a.n = n;
+  a.nx = e.nx;
+  a.ny = e.ny;
+  a.nz = e.nz;
   a.tapSign = 1;
   a.relu = 1;
   a.accumulate = 0;
@@ -521,7 +534,7 @@ static void convForward(UnetModel *m, int k, const uint16_t *in0, int c0, const 
 /* backward of conv k given dOut (already ReLU-masked): dIn0 (= or +=),
  * dIn1 (+=) when requested, weight and bias gradients */
 static void convBackward(UnetModel *m, int k, const float *dOut, const uint16_t *in0, int c0,
-    const uint16_t *in1, int c1, float *dIn0, int acc0, float *dIn1, int n)
+    const uint16_t *in1, int c1, float *dIn0, int acc0, float *dIn1, Ext e)
 {
   const ConvL &c = m->conv[k];
   if (dIn0 || dIn1) {
@@ -529,7 +542,7 @@ static void convBackward(UnetModel *m, int k, const float *dOut, const uint16_t 
       /* split rows: dIn0 overwritten, dIn1 accumulated -> two launches */
       ConvArgs<float, float> a;
       a.in0 = dOut; a.in1 = nullptr; a.c0 = c.cout; a.c1 = 0;
-      a.A = m->Wt + c.wOff; a.bias = nullptr; a.n = n; a.tapSign = -1; a.relu = 0;
+      a.A = m->Wt + c.wOff; a.bias = nullptr; a.nx = e.nx; a.ny = e.ny; a.nz = e.nz; a.tapSign = -1; a.relu = 0;
       a.out0 = dIn0; a.r0 = c0; a.out1 = nullptr; a.r1 = 0; a.accumulate = 0;
       launchConv(a);
       a.A = m->Wt + c.wOff + (size_t)c0 * 27 * c.cout;
@@ -538,16 +551,16 @@ static void convBackward(UnetModel *m, int k, const float *dOut, const uint16_t 
     } else {
       ConvArgs<float, float> a;
       a.in0 = dOut; a.in1 = nullptr; a.c0 = c.cout; a.c1 = 0;
-      a.A = m->Wt + c.wOff; a.bias = nullptr; a.n = n; a.tapSign = -1; a.relu = 0;
+      a.A = m->Wt + c.wOff; a.bias = nullptr; a.nx = e.nx; a.ny = e.ny; a.nz = e.nz; a.tapSign = -1; a.relu = 0;
       a.out0 = dIn0; a.r0 = c0; a.out1 = dIn1; a.r1 = dIn1 ? c1 : 0; a.accumulate = acc0;
       launchConv(a);
     }
   }
-  int cin = c0 + c1, nvox = n * n * n;
+  int cin = c0 + c1, nvox = (int)e.voxels();
   int chunk = 2048;
   dim3 g((cin + 31) / 32, 27 * ((c.cout + 31) / 32), (nvox + chunk - 1) / chunk);
   NUSIF_TIMED("unet_wgrad", (kConvWgrad<float><<<g, 256, 0, nusifStream()>>>(dOut, c.cout, in0, in1, c0, c1,
-                                n, m->G + c.wOff, chunk)));
+                                e.nx, e.ny, e.nz, m->G + c.wOff, chunk)));
   int cb = c.cout;
   kBiasGrad<<<256, 256, 0, nusifStream()>>>(dOut, cb, (size_t)nvox, m->G + c.bOff);
 }
@@ -561,11 +574,11 @@ void unetForward(UnetModel *m, const uint16_t *x, bool forTraining)
   hipStream_t st = nusifStream();
   for (int l = 0; l < L; l++) {
     LevelBuf &b = m->lv[l];
-    convForward(m, q++, cur, ccur, nullptr, 0, b.a, b.n);
-    convForward(m, q++, b.a, m->ch[l], nullptr, 0, b.s, b.n);
+    convForward(m, q++, cur, ccur, nullptr, 0, b.a, b.e);
+    convForward(m, q++, b.a, m->ch[l], nullptr, 0, b.s, b.e);
     if (l + 1 < L) {
-      size_t nout = cube(b.n / 2) * m->ch[l];
-      kPoolFwd<<<blocksFor(nout), 256, 0, st>>>(b.s, b.pl, b.n / 2, m->ch[l]);
+      size_t nout = b.e.half().voxels() * m->ch[l];
+      kPoolFwd<<<blocksFor(nout), 256, 0, st>>>(b.s, b.pl, b.e.half(), m->ch[l]);
       cur  = b.pl;
       ccur = m->ch[l];
     }
@@ -573,18 +586,18 @@ void unetForward(UnetModel *m, const uint16_t *x, bool forTraining)
   const uint16_t *prev = m->lv[L - 1].s;
   for (int l = L - 2; l >= 0; l--) {
     LevelBuf &b = m->lv[l];
-    size_t nup  = cube(b.n) * m->ch[l + 1];
+    size_t nup  = b.e.voxels() * m->ch[l + 1];
     if (forTraining) { /* the backward pass reads the upsampled tensor */
-      kUpFwd<<<blocksFor(nup), 256, 0, st>>>(prev, b.up, b.n, m->ch[l + 1]);
-      convForward(m, q++, b.up, m->ch[l + 1], m->lv[l].s, m->ch[l], b.da, b.n);
+      kUpFwd<<<blocksFor(nup), 256, 0, st>>>(prev, b.up, b.e, m->ch[l + 1]);
+      convForward(m, q++, b.up, m->ch[l + 1], m->lv[l].s, m->ch[l], b.da, b.e);
     } else { /* the conv reads the coarse tensor through its parent index */
-      convForward(m, q++, prev, m->ch[l + 1], m->lv[l].s, m->ch[l], b.da, b.n, true);
+      convForward(m, q++, prev, m->ch[l + 1], m->lv[l].s, m->ch[l], b.da, b.e, true);
     }
-    convForward(m, q++, b.da, m->ch[l], nullptr, 0, b.d, b.n);
+    convForward(m, q++, b.da, m->ch[l], nullptr, 0, b.d, b.e);
     prev = b.d;
   }
   const uint16_t *top = L > 1 ? m->lv[0].d : m->lv[0].s;
-  kHeadFwd<<<blocksFor(cube(m->n)), 256, 0, st>>>(top, m->ch[0], m->P + m->headW, m->y, cube(m->n));
+  kHeadFwd<<<blocksFor(m->e.voxels()), 256, 0, st>>>(top, m->ch[0], m->P + m->headW, m->y, m->e.voxels());
 }
 
 const float *unetOutput(UnetModel *m) { return m->y; }
@@ -594,43 +607,46 @@ double unetBackward(UnetModel *m, const uint16_t *x, const float *t, bool wantLo
 {
   int L = m->L;
   hipStream_t st = nusifStream();
-  size_t nv = cube(m->n);
+  size_t nv = m->e.voxels();
   unsigned nb = blocksFor(nv) > 1024 ? 1024 : blocksFor(nv);
   kLossGrad<<<nb, 256, 0, st>>>(m->y, t, nv, m->dy, m->partials);
   kFoldLoss<<<1, 256, 0, st>>>(m->partials, (int)nb, (double)nv, m->lossDev);
   const uint16_t *top = L > 1 ? m->lv[0].d : m->lv[0].s;
   float *dTop         = L > 1 ? m->lv[0].gD : m->lv[0].gS;
   kHeadBwd<<<256, 256, 0, st>>>(top, m->ch[0], m->P + m->headW, m->dy, dTop, m->G + m->headW, nv);
-  for (int l = 0; l < L - 1; l++) kFillF<<<blocksFor(cube(m->lv[l].n) * m->ch[l]), 256, 0, st>>>(m->lv[l].gS, cube(m->lv[l].n) * m->ch[l], 0.f);
+  for (int l = 0; l < L - 1; l++) {
+    size_t n = m->lv[l].e.voxels() * m->ch[l];
+    kFillF<<<blocksFor(n), 256, 0, st>>>(m->lv[l].gS, n, 0.f);
+  }
   int q = m->nconv;
   /* decoder, top level first */
   for (int l = 0; l <= L - 2; l++) {
     LevelBuf &b = m->lv[l];
-    size_t v    = cube(b.n);
+    size_t v    = b.e.voxels();
     int ib = q - 1, ia = q - 2;
     q -= 2;
     kReluMask<<<blocksFor(v * m->ch[l]), 256, 0, st>>>(b.gD, b.d, v * m->ch[l]);
-    convBackward(m, ib, b.gD, b.da, m->ch[l], nullptr, 0, b.gA, 0, nullptr, b.n);
+    convBackward(m, ib, b.gD, b.da, m->ch[l], nullptr, 0, b.gA, 0, nullptr, b.e);
     kReluMask<<<blocksFor(v * m->ch[l]), 256, 0, st>>>(b.gA, b.da, v * m->ch[l]);
-    convBackward(m, ia, b.gA, b.up, m->ch[l + 1], b.s, m->ch[l], b.gU, 0, b.gS, b.n);
+    convBackward(m, ia, b.gA, b.up, m->ch[l + 1], b.s, m->ch[l], b.gU, 0, b.gS, b.e);
     LevelBuf &nx = m->lv[l + 1];
     float *dst   = (l + 1 == L - 1) ? nx.gS : nx.gD;
-    kUpBwd<<<blocksFor(cube(nx.n) * m->ch[l + 1]), 256, 0, st>>>(b.gU, dst, nx.n, m->ch[l + 1]);
+    kUpBwd<<<blocksFor(nx.e.voxels() * m->ch[l + 1]), 256, 0, st>>>(b.gU, dst, nx.e, m->ch[l + 1]);
   }
   /* encoder, deepest level first */
   for (int l = L - 1; l >= 0; l--) {
     LevelBuf &b = m->lv[l];
-    size_t v    = cube(b.n);
+    size_t v    = b.e.voxels();
     int ia = 2 * l, ib = 2 * l + 1;
     kReluMask<<<blocksFor(v * m->ch[l]), 256, 0, st>>>(b.gS, b.s, v * m->ch[l]);
-    convBackward(m, ib, b.gS, b.a, m->ch[l], nullptr, 0, b.gA, 0, nullptr, b.n);
+    convBackward(m, ib, b.gS, b.a, m->ch[l], nullptr, 0, b.gA, 0, nullptr, b.e);
     kReluMask<<<blocksFor(v * m->ch[l]), 256, 0, st>>>(b.gA, b.a, v * m->ch[l]);
     const uint16_t *inl = l == 0 ? x : m->lv[l - 1].pl;
     int cinl            = l == 0 ? m->cin : m->ch[l - 1];
-    convBackward(m, ia, b.gA, inl, cinl, nullptr, 0, l > 0 ? b.gP : nullptr, 0, nullptr, b.n);
+    convBackward(m, ia, b.gA, inl, cinl, nullptr, 0, l > 0 ? b.gP : nullptr, 0, nullptr, b.e);
     if (l > 0) {
       LevelBuf &pv = m->lv[l - 1];
-      kPoolBwd<<<blocksFor(cube(pv.n) * m->ch[l - 1]), 256, 0, st>>>(b.gP, pv.gS, pv.n, m->ch[l - 1]);
+      kPoolBwd<<<blocksFor(pv.e.voxels() * m->ch[l - 1]), 256, 0, st>>>(b.gP, pv.gS, pv.e, m->ch[l - 1]);
     }
   }
   if (!wantLoss) return 0.0;
@@ -670,8 +686,8 @@ int unetSave(UnetModel *m, const char *path)
   fwrite(hdr, sizeof(int), 6, f);
   fwrite(&np, sizeof(np), 1, f);
   fwrite(h.data(), sizeof(float), h.size(), f);
-  /* trailer: the network extent it was trained on */
-  int ext = m->n;
+  /* trailer: the largest extent of the grid it was trained on */
+  int ext = std::max(m->e.nx, std::max(m->e.ny, m->e.nz));
   fwrite("NUSIFEXT", 1, 8, f);
   fwrite(&ext, sizeof(int), 1, f);
   fclose(f);
@@ -697,7 +713,21 @@ int unetFileExtent(const char *path)
   return ext;
 }
 
-UnetModel *unetLoad(const char *path, int n)
+/* the number of levels of the network in a weights file, 0 if unreadable */
+int unetFileLevels(const char *path)
+{
+  FILE *f = fopen(path, "rb");
+  if (!f) return 0;
+  char magic[10];
+  int hdr[6] = {0, 0, 0, 0, 0, 0};
+  int L      = 0;
+  if (fread(magic, 1, 10, f) == 10 && memcmp(magic, "NUSIFUNET1", 10) == 0 && fread(hdr, sizeof(int), 6, f) == 6)
+    L = hdr[0];
+  fclose(f);
+  return L;
+}
+
+UnetModel *unetLoad(const char *path, int nx, int ny, int nz)
 {
   FILE *f = fopen(path, "rb");
   if (!f) return nullptr;
@@ -709,7 +739,7 @@ UnetModel *unetLoad(const char *path, int n)
     fclose(f);
     NUSIF_FATAL("unet: %s is not a weights file", path);
   }
-  UnetModel *m = unetNew(hdr[0], hdr + 1, hdr[5], n);
+  UnetModel *m = unetNew(hdr[0], hdr + 1, hdr[5], nx, ny, nz);
   if ((size_t)np != m->nParams) NUSIF_FATAL("unet: %s holds %lld parameters, expected %zu", path, np, m->nParams);
   std::vector<float> h(m->nParams);
   if (fread(h.data(), sizeof(float), h.size(), f) != h.size()) NUSIF_FATAL("unet: %s is truncated", path);
@@ -720,4 +750,9 @@ UnetModel *unetLoad(const char *path, int n)
 }
 
 int unetLevels(const UnetModel *m) { return m->L; }
-int unetExtent(const UnetModel *m) { return m->n; }
+void unetExtents(const UnetModel *m, int *nx, int *ny, int *nz)
+{
+  *nx = m->e.nx;
+  *ny = m->e.ny;
+  *nz = m->e.nz;
+}

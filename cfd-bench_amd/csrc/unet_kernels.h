@@ -1,7 +1,8 @@
 /* Kernels of the warm-start U-Net: BF16 implicit-GEMM 3x3x3 convolutions on
  * gfx950 MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulation), their data and
  * weight gradients, and the elementwise layers. Tensors are channels-last
- * ([z][y][x][c], c a multiple of 8), cubes of extent n, 'same' zero padding.
+ * ([z][y][x][c], c a multiple of 8), boxes of nx x ny x nz voxels, 'same'
+ * zero padding.
  *
  * MFMA 16x16x32 lane layout (checked on the device): lane l holds
  *   A[row l%16][k 8*(l/16) .. +7],  B[k 8*(l/16) .. +7][col l%16],
@@ -57,12 +58,13 @@ template <typename InT, typename OutT> struct ConvArgs {
   const float *bias;      /* [rows] or null */
   OutT *out0, *out1;      /* [vox][r0], [vox][r1] */
   int r0, r1;
-  int n;                  /* cube extent */
+  int nx, ny, nz;         /* box extents (voxels) */
   int tapSign;            /* +1 forward, -1 data gradient */
   int relu;               /* ReLU on the result */
   int accumulate;         /* out += result (FP32 outputs) */
-  int half0 = 0;          /* in0 is stored at extent n / 2 and read upsampled
-                             (nearest parent voxel); LDS forward conv only */
+  int half0 = 0;          /* in0 is stored at extents (nx, ny, nz) / 2 and read
+                             upsampled (nearest parent voxel); LDS forward conv only */
+  __host__ __device__ long long voxels() const { return (long long)nx * ny * nz; }
 };
 
 template <typename T> __device__ inline void storeOut(T *p, float v, int accumulate);
@@ -78,7 +80,7 @@ template <typename InT, typename OutT, int NB, int MB>
 __global__ __launch_bounds__(256) void kConvMfma(ConvArgs<InT, OutT> a)
 {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int n = a.n, nvox = n * n * n;
+  const int nx = a.nx, ny = a.ny, nz = a.nz, nvox = nx * ny * nz;
   const int cin = a.c0 + a.c1, g8 = cin >> 3, G = 27 * g8;
   const int rows = a.r0 + a.r1;
   const int row0 = blockIdx.y * (16 * NB);
@@ -92,9 +94,9 @@ __global__ __launch_bounds__(256) void kConvMfma(ConvArgs<InT, OutT> a)
     int v   = vox0 + mb * 16 + lr;
     vin[mb] = v < nvox;
     int vc  = vin[mb] ? v : 0;
-    vx[mb]  = vc % n;
-    vy[mb]  = (vc / n) % n;
-    vz[mb]  = vc / (n * n);
+    vx[mb]  = vc % nx;
+    vy[mb]  = (vc / nx) % ny;
+    vz[mb]  = vc / (nx * ny);
   }
   floatx4 acc[NB][MB];
 #pragma unroll
@@ -116,9 +118,9 @@ __global__ __launch_bounds__(256) void kConvMfma(ConvArgs<InT, OutT> a)
 #pragma unroll
     for (int mb = 0; mb < MB; mb++) {
       int z = vz[mb] + dz, y = vy[mb] + dy, x = vx[mb] + dx;
-      bool ok = gv && vin[mb] && z >= 0 && z < n && y >= 0 && y < n && x >= 0 && x < n;
+      bool ok = gv && vin[mb] && z >= 0 && z < nz && y >= 0 && y < ny && x >= 0 && x < nx;
       if (ok) {
-        size_t vv = ((size_t)z * n + y) * n + x;
+        size_t vv = ((size_t)z * ny + y) * nx + x;
         int cc    = c8 * 8;
         fb[mb]    = cc < a.c0 ? ld8(a.in0 + vv * a.c0 + cc) : ld8(a.in1 + vv * a.c1 + (cc - a.c0));
       } else {
@@ -176,7 +178,7 @@ template <int NB, int G8>
 __device__ inline void convChunk(const ConvArgs<uint16_t, uint16_t> &a, uint16_t *sIn, int cc0, int cin, int rows,
     int row0, int x0, int y0, int z0, int wave, int lr, int lg, floatx4 (&acc)[NB][4])
 {
-  const int tid = threadIdx.x, n = a.n;
+  const int tid = threadIdx.x, nx = a.nx, ny = a.ny, nz = a.nz;
   constexpr int G   = 27 * G8;
   constexpr int NIT = (G + 3) / 4; /* K-steps of four (tap, group) pairs */
   /* weight fragments run D K-steps ahead of the MFMAs in a register ring, so
@@ -223,12 +225,11 @@ __device__ inline void convChunk(const ConvArgs<uint16_t, uint16_t> &a, uint16_t
         int v = u / G8, q = u - v * G8;
         int hx = v % CL_HX, hy = (v / CL_HX) % CL_HY, hz = v / (CL_HX * CL_HY);
         int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
-        if (x >= 0 && x < n && y >= 0 && y < n && z >= 0 && z < n) {
-          size_t vv = ((size_t)z * n + y) * n + x;
+        if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz) {
+          size_t vv = ((size_t)z * ny + y) * nx + x;
           int c     = cc0 + q * 8;
           if (c < a.c0) {
-            const int h = n >> 1;
-            size_t v0   = a.half0 ? ((size_t)(z >> 1) * h + (y >> 1)) * h + (x >> 1) : vv;
+            size_t v0 = a.half0 ? ((size_t)(z >> 1) * (ny >> 1) + (y >> 1)) * (nx >> 1) + (x >> 1) : vv;
             val[r]      = *(const uint4 *)(a.in0 + v0 * a.c0 + c);
           } else {
             val[r] = *(const uint4 *)(a.in1 + vv * a.c1 + (c - a.c0));
@@ -292,8 +293,8 @@ __global__ __launch_bounds__(256) void kConvLds(ConvArgs<uint16_t, OutT> a, floa
 {
   __shared__ __attribute__((aligned(16))) uint16_t sIn[CL_HV * CL_CSTR];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int n = a.n;
-  const int nbx = (n + CL_BX - 1) / CL_BX, nby = (n + CL_BY - 1) / CL_BY;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int nbx = (nx + CL_BX - 1) / CL_BX, nby = (ny + CL_BY - 1) / CL_BY;
   const int x0 = (blockIdx.x % nbx) * CL_BX;
   const int y0 = ((blockIdx.x / nbx) % nby) * CL_BY;
   const int z0 = (blockIdx.x / (nbx * nby)) * CL_BZ;
@@ -323,16 +324,16 @@ __global__ __launch_bounds__(256) void kConvLds(ConvArgs<uint16_t, OutT> a, floa
     convTail<NB, GB>(a, sIn, nFull * CL_CC, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
   }
   const int x = x0 + lr, z = z0 + wave;
-  if (x >= n || z >= n) return;
+  if (x >= nx || z >= nz) return;
   /* each lane holds four consecutive rows (channels) of one voxel: rows are
    * multiples of 16, so they are one 16 B (FP32) or 8 B (bf16) store */
   if (gridDim.z > 1) {
-    float *sp = split + (size_t)blockIdx.z * n * n * n * rows;
+    float *sp = split + (size_t)blockIdx.z * a.voxels() * rows;
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) {
       int y = y0 + mb;
-      if (y >= n) continue;
-      size_t v = ((size_t)z * n + y) * n + x;
+      if (y >= ny) continue;
+      size_t v = ((size_t)z * ny + y) * nx + x;
 #pragma unroll
       for (int nb = 0; nb < NB; nb++) {
         int r = row0 + nb * 16 + 4 * lg;
@@ -344,8 +345,8 @@ __global__ __launch_bounds__(256) void kConvLds(ConvArgs<uint16_t, OutT> a, floa
 #pragma unroll
   for (int mb = 0; mb < 4; mb++) {
     int y = y0 + mb;
-    if (y >= n) continue;
-    size_t v = ((size_t)z * n + y) * n + x;
+    if (y >= ny) continue;
+    size_t v = ((size_t)z * ny + y) * nx + x;
 #pragma unroll
     for (int nb = 0; nb < NB; nb++) {
       int r = row0 + nb * 16 + 4 * lg;
@@ -379,7 +380,7 @@ __global__ __launch_bounds__(256) void kConvSplitFinish(ConvArgs<uint16_t, OutT>
     int chunks)
 {
   const int rows = a.r0 + a.r1;
-  const size_t total = (size_t)a.n * a.n * a.n * rows;
+  const size_t total = (size_t)a.voxels() * rows;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
     float val = 0.f;
     for (int z = 0; z < chunks; z++) val += split[(size_t)z * total + e];
@@ -399,13 +400,13 @@ __global__ __launch_bounds__(256) void kConvSplitFinish(ConvArgs<uint16_t, OutT>
  * block; partial sums go to dW with FP32 atomics (split over voxel chunks). */
 template <typename GT>
 __global__ __launch_bounds__(256) void kConvWgrad(const GT *__restrict__ dOut, int cout,
-    const uint16_t *__restrict__ in0, const uint16_t *__restrict__ in1, int c0, int c1, int n,
-    float *__restrict__ dW, int voxPerChunk)
+    const uint16_t *__restrict__ in0, const uint16_t *__restrict__ in1, int c0, int c1, int nx, int ny,
+    int nz, float *__restrict__ dW, int voxPerChunk)
 {
   __shared__ __attribute__((aligned(16))) uint16_t sA[32][40]; /* [cout][vox], padded rows */
   __shared__ __attribute__((aligned(16))) uint16_t sB[32][40]; /* [cin][vox] */
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int cin = c0 + c1, nvox = n * n * n;
+  const int cin = c0 + c1, nvox = nx * ny * nz;
   const int t   = blockIdx.y % 27;
   const int nT  = blockIdx.y / 27;          /* cout tile */
   const int cT  = blockIdx.x;               /* cin tile */
@@ -427,9 +428,9 @@ __global__ __launch_bounds__(256) void kConvWgrad(const GT *__restrict__ dOut, i
       uint16_t gb = 0;
       int ci = cT * 32 + ch;
       if (v < v1 && ci < cin) {
-        int x = v % n + dx, y = (v / n) % n + dy, z = v / (n * n) + dz;
-        if (z >= 0 && z < n && y >= 0 && y < n && x >= 0 && x < n) {
-          size_t w = ((size_t)z * n + y) * n + x;
+        int x = v % nx + dx, y = (v / nx) % ny + dy, z = v / (nx * ny) + dz;
+        if (z >= 0 && z < nz && y >= 0 && y < ny && x >= 0 && x < nx) {
+          size_t w = ((size_t)z * ny + y) * nx + x;
           gb       = ci < c0 ? in0[w * c0 + ci] : in1[w * c1 + (ci - c0)];
         }
       }

@@ -9,10 +9,12 @@
 
 struct WarmStart;
 struct UnetModel;
-WarmStart *warmstartCreate(NusifCtx *c, UnetModel *net, bool capture);
+/* netLevels: pooling levels of the network it will run (its grid must
+ * halve netLevels - 1 times) */
+WarmStart *warmstartCreate(NusifCtx *c, UnetModel *net, bool capture, int netLevels);
 void warmstartFree(WarmStart *w);
 int warmstartLevel(const WarmStart *w);
-int warmstartExtent(const WarmStart *w);
+void warmstartExtents(const WarmStart *w, int *nx, int *ny, int *nz);
 void warmstartSetNet(WarmStart *w, UnetModel *net);
 UnetModel *warmstartNet(WarmStart *w);
 bool warmstartCapturing(const WarmStart *w);

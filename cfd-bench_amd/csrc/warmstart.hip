@@ -2,13 +2,15 @@
  * reference, whose initial guess is the previous step's p: p persists
  * across steps, reference src/main.c:71).
  *
- * Inputs, on the network grid (multigrid level Ln, the first whose extent is
- * <= 64; restriction = average of the 8 children):
+ * Inputs, on the network grid (multigrid level Ln: the first whose extents
+ * are all <= unet_extent (64), or the last before that whose extents the
+ * network's pooling still divides; restriction = average of the 8 children):
  *   channel 0: r0 / s_r,  r0 = rhs - A p_prev (the previous pressure against
  *              the new right-hand side), s_r = rms(r0)
  *   channel 1: rhs / s_b, s_b = rms(rhs)
- * Output y, rescaled: dp = y * s_r * (n h_Ln / pi)^2 (the inverse Laplacian of
- * the smoothest mode on the network grid of n cells), prolongated (trilinear) to the
+ * Output y, rescaled: dp = y * s_r * (l / pi)^2, l = max(nx dx, ny dy, nz dz)
+ * on the network grid (the inverse Laplacian of the smoothest mode of the
+ * domain's longest side; l = n h on a cube), prolongated (trilinear) to the
  * finest grid; guess = p_prev + dp. The guess replaces p_prev only when its
  * residual rhs - A guess, restricted to the network grid, has a smaller norm
  * than r0 restricted there; otherwise the solve starts from p_prev. Either
@@ -41,10 +43,10 @@ struct WsLevel {
 struct WarmStart {
   UnetModel *net;
   int Ln;                 /* network level */
-  int n;                  /* network extent */
-  double h2;              /* output scale (n h_Ln / pi)^2, x spacing */
+  int nx, ny, nz;         /* network grid extents */
+  double h2;              /* output scale (l / pi)^2, l the domain's longest side */
   WsLevel lv[12];
-  uint16_t *x;            /* network input n^3 x 8 */
+  uint16_t *x;            /* network input, nx ny nz voxels x 8 channels */
   double *pPrev;          /* FP64 copy of p for capture (capture only) */
   double *scal;           /* [0] sum r0^2, [1] sum b^2 on level Ln, [2] guess residual
                            * on level Ln, [3] sum r0^2 on level 0 */
@@ -276,6 +278,8 @@ __global__ __launch_bounds__(256) void kFoldSum1(const double *partials, int nb,
 }
 
 /* ---- host ------------------------------------------------------------------ */
+static size_t netVoxels(const WarmStart *w) { return (size_t)w->nx * w->ny * w->nz; }
+
 static dim3 cells(const Geo &g) { return dim3((g.ni + WTX - 1) / WTX, (g.nj + WTY - 1) / WTY, g.nk); }
 
 static float *falloc(long long n)
@@ -286,11 +290,15 @@ static float *falloc(long long n)
   return p;
 }
 
-/* first level whose largest extent is <= 64 (extents must stay even) */
-static int networkLevel(const Geo &g0, int maxExtent)
+/* the first level whose extents are all <= maxExtent, halving only while
+ * the halved extents stay divisible by div (the network's 2^(L-1)), so a
+ * box such as 768 x 192 x 192 stops at 96 x 24 x 24 rather than at the
+ * 48 x 12 x 12 that four levels of pooling cannot halve */
+static int networkLevel(const Geo &g0, int maxExtent, int div)
 {
   int ni = g0.ni, nj = g0.nj, nk = g0.nk, l = 0;
-  while ((ni > maxExtent || nj > maxExtent || nk > maxExtent) && ni % 2 == 0 && nj % 2 == 0 && nk % 2 == 0) {
+  while ((ni > maxExtent || nj > maxExtent || nk > maxExtent) && ni % (2 * div) == 0 && nj % (2 * div) == 0 &&
+         nk % (2 * div) == 0) {
     ni /= 2;
     nj /= 2;
     nk /= 2;
@@ -299,29 +307,33 @@ static int networkLevel(const Geo &g0, int maxExtent)
   return l;
 }
 
-WarmStart *warmstartCreate(NusifCtx *c, UnetModel *net, bool capture)
+WarmStart *warmstartCreate(NusifCtx *c, UnetModel *net, bool capture, int netLevels)
 {
   WarmStart *w = new WarmStart();
   w->net       = net;
   w->capture   = capture;
-  w->Ln        = networkLevel(c->geo, c->opt.unetExtent > 0 ? c->opt.unetExtent : 64);
-  Geo g        = c->geo;
-  double h     = c->owner->grid.dx;
+  const int div = 1 << (netLevels > 0 ? netLevels - 1 : 0);
+  w->Ln        = networkLevel(c->geo, c->opt.unetExtent > 0 ? c->opt.unetExtent : 64, div);
+  Geo g = c->geo;
   for (int l = 0; l <= w->Ln; l++) {
     w->lv[l].g = g;
     w->lv[l].r = falloc(g.n);
     w->lv[l].b = falloc(g.n);
     w->lv[l].d = falloc(g.n);
-    if (l < w->Ln) {
-      g = makeGeo(g.ni / 2, g.nj / 2, g.nk / 2);
-      h *= 2.0;
-    }
+    if (l < w->Ln) g = makeGeo(g.ni / 2, g.nj / 2, g.nk / 2);
   }
   const Geo &gn = w->lv[w->Ln].g;
-  if (gn.ni != gn.nj || gn.nj != gn.nk)
-    NUSIF_FATAL("warm start: the network needs a cubic grid on its level (got %dx%dx%d)", gn.ni, gn.nj, gn.nk);
-  w->n  = gn.ni;
-  w->h2 = (gn.ni * h / M_PI) * (gn.ni * h / M_PI);
+  if (gn.ni % div || gn.nj % div || gn.nk % div)
+    NUSIF_FATAL("warm start: network grid %dx%dx%d (level %d) is not divisible by %d, which %d pooling levels need",
+        gn.ni, gn.nj, gn.nk, w->Ln, div, netLevels);
+  w->nx = gn.ni;
+  w->ny = gn.nj;
+  w->nz = gn.nk;
+  {
+    const Grid &gr = c->owner->grid; /* the domain's side lengths, on every level */
+    const double l = std::max(gr.imax * gr.dx, std::max(gr.jmax * gr.dy, gr.kmax * gr.dz));
+    w->h2          = (l / M_PI) * (l / M_PI);
+  }
   {
     /* partials of the first input pass (two per workgroup when the network
      * is on a coarser level, one on the finest) and of the norms */
@@ -331,7 +343,7 @@ WarmStart *warmstartCreate(NusifCtx *c, UnetModel *net, bool capture)
       NUSIF_FATAL("warm start: grid %dx%dx%d too large for the network input pass (coarsen to extents <= %d)",
           c->geo.ni, c->geo.nj, c->geo.nk, c->opt.unetExtent > 0 ? c->opt.unetExtent : 64);
   }
-  HIP_CHECK(hipMalloc((void **)&w->x, (size_t)w->n * w->n * w->n * 8 * sizeof(uint16_t)));
+  HIP_CHECK(hipMalloc((void **)&w->x, netVoxels(w) * 8 * sizeof(uint16_t)));
   HIP_CHECK(hipMalloc((void **)&w->scal, 4 * sizeof(double)));
   HIP_CHECK(hipHostMalloc((void **)&w->scalHost, 4 * sizeof(double), hipHostMallocCoherent));
   HIP_CHECK(hipHostGetDevicePointer((void **)&w->scalHostDev, w->scalHost, 0));
@@ -361,7 +373,12 @@ void warmstartFree(WarmStart *w)
 }
 
 int warmstartLevel(const WarmStart *w) { return w->Ln; }
-int warmstartExtent(const WarmStart *w) { return w->n; }
+void warmstartExtents(const WarmStart *w, int *nx, int *ny, int *nz)
+{
+  *nx = w->nx;
+  *ny = w->ny;
+  *nz = w->nz;
+}
 void warmstartSetNet(WarmStart *w, UnetModel *net) { w->net = net; }
 UnetModel *warmstartNet(WarmStart *w) { return w->net; }
 bool warmstartCapturing(const WarmStart *w) { return w->capture; }
@@ -463,7 +480,7 @@ void warmstartCaptureBegin(WarmStart *w, const double *p, const Geo &g0)
   if (!w->capture) return;
   HIP_CHECK(hipMemcpyAsync(w->pPrev, p, (size_t)g0.n * sizeof(double), hipMemcpyDeviceToDevice, nusifStream()));
   uint16_t *x = nullptr;
-  size_t nv   = (size_t)w->n * w->n * w->n;
+  size_t nv   = netVoxels(w);
   HIP_CHECK(hipMalloc((void **)&x, nv * 8 * sizeof(uint16_t)));
   HIP_CHECK(hipMemcpyAsync(x, w->x, nv * 8 * sizeof(uint16_t), hipMemcpyDeviceToDevice, nusifStream()));
   w->dsX.push_back(x);
@@ -477,7 +494,7 @@ void warmstartCaptureEnd(WarmStart *w, const double *p)
   kDiff<<<cells(w->lv[0].g), dim3(WTX, WTY), 0, st>>>(p, w->pPrev, w->lv[0].g, w->lv[0].d);
   for (int l = 1; l <= w->Ln; l++)
     kRestrictAvg<float><<<cells(w->lv[l].g), dim3(WTX, WTY), 0, st>>>(w->lv[l - 1].d, w->lv[l - 1].g, w->lv[l].d, w->lv[l].g);
-  size_t nv = (size_t)w->n * w->n * w->n;
+  size_t nv = netVoxels(w);
   float *t  = nullptr;
   HIP_CHECK(hipMalloc((void **)&t, nv * sizeof(float)));
   kTarget<<<1024, 256, 0, st>>>(w->lv[w->Ln].d, w->lv[w->Ln].g, w->scal, w->h2, t);
@@ -495,100 +512,99 @@ void warmstartTakeSamples(WarmStart *w, std::vector<uint16_t *> &xs, std::vector
   w->dsT.clear();
 }
 
-/* ---- C ABI: training driver ------------------------------------------------ */
+/* ---- C ABI: training drivers ----------------------------------------------- */
 static const int kUnetLevels = 4;
 static const int kUnetChannels[4] = {16, 32, 64, 128};
 
-extern "C" int nusifUnetTrainCavityGrids(const char *weightsOut, const int *grids, int ngrids, int netExtent,
-    int runs, int stepsPerRun, int skipSteps, int epochs, int batch, double lr, int seed, double *lossPerEpoch);
+/* xorshift64* uniform in [0, 1), seeded per training call */
+struct Uniform {
+  unsigned long long s;
+  explicit Uniform(int seed) : s(0x243F6A8885A308D3ull ^ ((unsigned long long)seed * 0x9E3779B97F4A7C15ull)) {}
+  double operator()()
+  {
+    s ^= s >> 12;
+    s ^= s << 25;
+    s ^= s >> 27;
+    return ((s * 0x2545F4914F6CDD1Dull) >> 11) * (1.0 / 9007199254740992.0);
+  }
+};
 
-extern "C" int nusifUnetTrainCavity(const char *weightsOut, int grid, int runs, int stepsPerRun,
-    int skipSteps, int epochs, int batch, double lr, int seed, double *lossPerEpoch)
-{
-  return nusifUnetTrainCavityGrids(weightsOut, &grid, 1, 64, runs, stepsPerRun, skipSteps, epochs, batch, lr,
-      seed, lossPerEpoch);
-}
-
-/* as nusifUnetTrainCavity, run r on a grids[r % ngrids]^3 cavity, with the
- * network on the first multigrid level of extent <= netExtent (all grids
- * must give the same network extent) */
-extern "C" int nusifUnetTrainCavityGrids(const char *weightsOut, const int *grids, int ngrids, int netExtent,
-    int runs, int stepsPerRun, int skipSteps, int epochs, int batch, double lr, int seed, double *lossPerEpoch)
-{
-  if (ngrids < 1) NUSIF_FATAL("unet training: no grid given");
+/* Training pairs captured from solver runs, all on one network grid. */
+struct PairSet {
   std::vector<uint16_t *> xs;
   std::vector<float *> ts;
-  unsigned long long rng = 0x243F6A8885A308D3ull ^ ((unsigned long long)seed * 0x9E3779B97F4A7C15ull);
-  auto uni = [&]() {
-    rng ^= rng >> 12;
-    rng ^= rng << 25;
-    rng ^= rng >> 27;
-    return ((rng * 0x2545F4914F6CDD1Dull) >> 11) * (1.0 / 9007199254740992.0);
-  };
-  int netN = 0;
-  for (int r = 0; r < runs; r++) {
-    const int grid = grids[r % ngrids];
-    double re = 10.0 + 990.0 * uni(), lid = 0.5 + 1.5 * uni();
-    Parameter prm;
-    initParameter(&prm);
-    prm.imax = prm.jmax = prm.kmax = grid;
-    prm.xlength = prm.ylength = prm.zlength = 1.0;
-    prm.re = re;
-    prm.tau = 0.5;
-    prm.te = 1e9;
-    prm.dt = 0.02;
-    prm.gx = prm.gy = prm.gz = 0.0;
-    prm.name = (char *)"dcavity";
-    prm.bcLeft = prm.bcRight = prm.bcBottom = prm.bcTop = prm.bcFront = prm.bcBack = NOSLIP;
-    prm.u_init = prm.v_init = prm.w_init = prm.p_init = 0.0;
-    prm.levels = 6;
-    prm.presmooth = prm.postsmooth = 2;
-    char buf[64];
-    nusifSetOption(&prm, "solver", "gmg");
-    nusifSetOption(&prm, "precision", "mixed");
-    nusifSetOption(&prm, "mg_tol", "1e-6");
-    nusifSetOption(&prm, "verbose", "0");
-    snprintf(buf, sizeof buf, "%.17g", lid);
-    nusifSetOption(&prm, "lid", buf);
-    snprintf(buf, sizeof buf, "%d", netExtent);
-    nusifSetOption(&prm, "unet_extent", buf);
-    Discretization d;
-    Solver s;
-    memset(&d, 0, sizeof d);
-    commInit(&d.comm, 0, nullptr);
-    commPartition(&d.comm, grid, grid, grid);
-    initDiscretization(&d, &prm);
-    initSolver(&s, &d, &prm);
-    NusifSolverCtx *sc = nusifSolverCtxOf(&s);
-    if (skipSteps > 0) nusifRunSteps(&d, &s, skipSteps, nullptr);
-    sc->ws = warmstartCreate(sc->dctx, nullptr, true);
-    if (netN && sc->ws->n != netN)
-      NUSIF_FATAL("unet training: grid %d gives network extent %d, earlier grids %d", grid, sc->ws->n, netN);
-    netN = sc->ws->n;
-    nusifRunSteps(&d, &s, stepsPerRun, nullptr);
-    warmstartTakeSamples(sc->ws, xs, ts);
-    nusifFreeSolver(&s);
-    nusifFreeDiscretization(&d);
-    printf("[unet] run %d/%d: %d^3, Re %.1f lid %.2f, %zu pairs so far\n", r + 1, runs, grid, re, lid, ts.size());
-    fflush(stdout);
+  int nx = 0, ny = 0, nz = 0;
+  ~PairSet()
+  {
+    for (auto p : xs) HIP_CHECK(hipFree(p));
+    for (auto p : ts) HIP_CHECK(hipFree(p));
   }
-  if (ts.empty()) return 0;
-  UnetModel *m = unetNew(kUnetLevels, kUnetChannels, 8, netN);
-  unetInit(m, (unsigned)seed);
-  unetSyncWeights(m);
+};
+
+/* One solver run on `prm` (gmg, mixed precision, 1e-6, quiet): skipSteps
+ * steps of spin-up, then stepsPerRun steps whose solves are captured as
+ * pairs on the network grid the warm start would use (netExtent). */
+static void capturePairs(Parameter *prm, int netExtent, int skipSteps, int stepsPerRun, PairSet &ps)
+{
+  char buf[64];
+  nusifSetOption(prm, "solver", "gmg");
+  nusifSetOption(prm, "precision", "mixed");
+  nusifSetOption(prm, "mg_tol", "1e-6");
+  nusifSetOption(prm, "verbose", "0");
+  snprintf(buf, sizeof buf, "%d", netExtent);
+  nusifSetOption(prm, "unet_extent", buf);
+  Discretization d;
+  Solver s;
+  memset(&d, 0, sizeof d);
+  commInit(&d.comm, 0, nullptr);
+  commPartition(&d.comm, prm->kmax, prm->jmax, prm->imax);
+  initDiscretization(&d, prm);
+  initSolver(&s, &d, prm);
+  NusifSolverCtx *sc = nusifSolverCtxOf(&s);
+  if (skipSteps > 0) nusifRunSteps(&d, &s, skipSteps, nullptr);
+  sc->ws = warmstartCreate(sc->dctx, nullptr, true, kUnetLevels);
+  WarmStart *w = sc->ws;
+  if (ps.nx && (w->nx != ps.nx || w->ny != ps.ny || w->nz != ps.nz))
+    NUSIF_FATAL("unet training: grid %dx%dx%d gives network grid %dx%dx%d, earlier runs %dx%dx%d", prm->imax,
+        prm->jmax, prm->kmax, w->nx, w->ny, w->nz, ps.nx, ps.ny, ps.nz);
+  ps.nx = w->nx;
+  ps.ny = w->ny;
+  ps.nz = w->nz;
+  nusifRunSteps(&d, &s, stepsPerRun, nullptr);
+  warmstartTakeSamples(w, ps.xs, ps.ts);
+  nusifFreeSolver(&s);
+  nusifFreeDiscretization(&d);
+}
+
+/* Adam over the pairs: `epochs` shuffled passes, cosine-decayed lr, an
+ * update per `batch` samples; from weightsIn when given (fine-tuning), else
+ * from a seeded He initialisation. Writes weightsOut. */
+static void trainOnPairs(PairSet &ps, const char *weightsIn, const char *weightsOut, int epochs, int batch,
+    double lr, int seed, double *lossPerEpoch)
+{
+  Uniform uni(seed ^ 0x5bd1e995);
+  UnetModel *m = nullptr;
+  if (weightsIn && weightsIn[0]) {
+    m = unetLoad(weightsIn, ps.nx, ps.ny, ps.nz);
+    if (!m) NUSIF_FATAL("unet training: cannot read %s", weightsIn);
+  } else {
+    m = unetNew(kUnetLevels, kUnetChannels, 8, ps.nx, ps.ny, ps.nz);
+    unetInit(m, (unsigned)seed);
+    unetSyncWeights(m);
+  }
   unetZeroGrads(m);
-  std::vector<int> order(ts.size());
+  std::vector<int> order(ps.ts.size());
   for (size_t q = 0; q < order.size(); q++) order[q] = (int)q;
   for (int e = 0; e < epochs; e++) {
     for (size_t q = order.size() - 1; q > 0; q--) std::swap(order[q], order[(size_t)(uni() * (q + 1)) % (q + 1)]);
     double sum = 0.0;
     float lrE  = (float)(lr * 0.5 * (1.0 + cos(M_PI * e / (double)epochs))); /* cosine decay */
     for (size_t q = 0; q < order.size(); q++) {
-      unetForward(m, xs[order[q]], true);
+      unetForward(m, ps.xs[order[q]], true);
       bool last = (q + 1) % batch == 0 || q + 1 == order.size();
-      sum += unetBackward(m, xs[order[q]], ts[order[q]], true);
+      sum += unetBackward(m, ps.xs[order[q]], ps.ts[order[q]], true);
       if (last) unetAdam(m, lrE, batch);
-      if ((q + 1) % 256 == 0) {
+      if ((q + 1) % 1024 == 0) {
         printf("[unet] epoch %d: %zu/%zu samples, running loss %.5g\n", e + 1, q + 1, order.size(), sum / (q + 1));
         fflush(stdout);
       }
@@ -599,24 +615,113 @@ extern "C" int nusifUnetTrainCavityGrids(const char *weightsOut, const int *grid
   }
   int rc = unetSave(m, weightsOut);
   unetFree(m);
-  for (auto p : xs) HIP_CHECK(hipFree(p));
-  for (auto p : ts) HIP_CHECK(hipFree(p));
   if (rc != 0) NUSIF_FATAL("cannot write %s", weightsOut);
-  return (int)ts.size();
+}
+
+/* the driver loop's parameters shared by the training runs */
+static void trainingParameter(Parameter *prm, int imax, int jmax, int kmax, double re)
+{
+  initParameter(prm);
+  prm->imax = imax;
+  prm->jmax = jmax;
+  prm->kmax = kmax;
+  prm->re   = re;
+  prm->tau  = 0.5;
+  prm->te   = 1e9;
+  prm->dt   = 0.02;
+  prm->gx = prm->gy = prm->gz = 0.0;
+  prm->u_init = prm->v_init = prm->w_init = prm->p_init = 0.0;
+  prm->levels    = 6;
+  prm->presmooth = prm->postsmooth = 2;
+}
+
+extern "C" int nusifUnetTrainCavity(const char *weightsOut, int grid, int runs, int stepsPerRun,
+    int skipSteps, int epochs, int batch, double lr, int seed, double *lossPerEpoch)
+{
+  return nusifUnetTrainCavityGrids(weightsOut, &grid, 1, 64, runs, stepsPerRun, skipSteps, epochs, batch, lr,
+      seed, lossPerEpoch);
+}
+
+/* as nusifUnetTrainCavity, run r on a grids[r % ngrids]^3 cavity, with the
+ * network on the first multigrid level of extent <= netExtent (all grids
+ * must give the same network grid) */
+extern "C" int nusifUnetTrainCavityGrids(const char *weightsOut, const int *grids, int ngrids, int netExtent,
+    int runs, int stepsPerRun, int skipSteps, int epochs, int batch, double lr, int seed, double *lossPerEpoch)
+{
+  if (ngrids < 1) NUSIF_FATAL("unet training: no grid given");
+  Uniform uni(seed);
+  PairSet ps;
+  for (int r = 0; r < runs; r++) {
+    const int grid = grids[r % ngrids];
+    double re = 10.0 + 990.0 * uni(), lid = 0.5 + 1.5 * uni();
+    Parameter prm;
+    trainingParameter(&prm, grid, grid, grid, re);
+    prm.xlength = prm.ylength = prm.zlength = 1.0;
+    prm.name = (char *)"dcavity";
+    prm.bcLeft = prm.bcRight = prm.bcBottom = prm.bcTop = prm.bcFront = prm.bcBack = NOSLIP;
+    char buf[64];
+    snprintf(buf, sizeof buf, "%.17g", lid);
+    nusifSetOption(&prm, "lid", buf);
+    capturePairs(&prm, netExtent, skipSteps, stepsPerRun, ps);
+    printf("[unet] run %d/%d: %d^3, Re %.1f lid %.2f, %zu pairs so far\n", r + 1, runs, grid, re, lid, ps.ts.size());
+    fflush(stdout);
+  }
+  if (ps.ts.empty()) return 0;
+  trainOnPairs(ps, nullptr, weightsOut, epochs, batch, lr, seed, lossPerEpoch);
+  return (int)ps.ts.size();
+}
+
+/* Channel pairs (the canal problem with uniform inflow u = 1 perturbed by a
+ * seeded 1% uniform noise, zero-gradient outflow, no-slip walls, domain
+ * xlength x 1 x 1; Re ~ U[100, 1000], an inflow seed per run; the rhs mean
+ * removed), then training from weightsIn (fine-tuning; NULL or "" trains
+ * from scratch). */
+extern "C" int nusifUnetTrainChannel(const char *weightsIn, const char *weightsOut, int imax, int jmax, int kmax,
+    double xlength, int netExtent, int runs, int stepsPerRun, int skipSteps, int epochs, int batch, double lr,
+    int seed, double *lossPerEpoch)
+{
+  Uniform uni(seed);
+  PairSet ps;
+  for (int r = 0; r < runs; r++) {
+    double re = 100.0 + 900.0 * uni();
+    Parameter prm;
+    trainingParameter(&prm, imax, jmax, kmax, re);
+    prm.xlength = xlength;
+    prm.ylength = prm.zlength = 1.0;
+    prm.u_init  = 1.0;
+    prm.gamma   = 0.9;
+    prm.name    = (char *)"canal";
+    prm.bcLeft = prm.bcRight = OUTFLOW;
+    prm.bcBottom = prm.bcTop = prm.bcFront = prm.bcBack = NOSLIP;
+    char buf[64];
+    nusifSetOption(&prm, "u_inflow", "1.0");
+    nusifSetOption(&prm, "inflow_noise", "0.01");
+    snprintf(buf, sizeof buf, "%d", seed * 1000 + r + 1);
+    nusifSetOption(&prm, "seed", buf);
+    nusifSetOption(&prm, "mg_project", "1");
+    capturePairs(&prm, netExtent, skipSteps, stepsPerRun, ps);
+    printf("[unet] channel run %d/%d: %dx%dx%d, Re %.1f, %zu pairs so far\n", r + 1, runs, imax, jmax, kmax, re,
+        ps.ts.size());
+    fflush(stdout);
+  }
+  if (ps.ts.empty()) return 0;
+  trainOnPairs(ps, weightsIn, weightsOut, epochs, batch, lr, seed, lossPerEpoch);
+  return (int)ps.ts.size();
 }
 
 /* ---- C ABI: one forward/backward for gradient checks ------------------------ */
-/* a model with the oracle's flat parameters and x (n^3 x cin, channels-last)
- * uploaded as bf16 */
-static UnetModel *checkModel(const int *cfg, int n, const double *params, const double *x, uint16_t **dx)
+/* a model with the oracle's flat parameters and x (nx ny nz x cin,
+ * channels-last) uploaded as bf16 */
+static UnetModel *checkModel(const int *cfg, int nx, int ny, int nz, const double *params, const double *x,
+    uint16_t **dx)
 {
-  UnetModel *m = unetNew(cfg[0], cfg + 1, cfg[5], n);
+  UnetModel *m = unetNew(cfg[0], cfg + 1, cfg[5], nx, ny, nz);
   size_t np    = unetParamCount(m);
   std::vector<float> hp(np);
   for (size_t q = 0; q < np; q++) hp[q] = (float)params[q];
   unetSetParams(m, hp.data());
   unetSyncWeights(m);
-  size_t nv = (size_t)n * n * n;
+  size_t nv = (size_t)nx * ny * nz;
   std::vector<uint16_t> hx(nv * cfg[5]);
   for (size_t q = 0; q < hx.size(); q++) hx[q] = f2bf((float)x[q]);
   HIP_CHECK(hipMalloc((void **)dx, hx.size() * sizeof(uint16_t)));
@@ -624,9 +729,8 @@ static UnetModel *checkModel(const int *cfg, int n, const double *params, const 
   return m;
 }
 
-static void readPrediction(UnetModel *m, int n, double *pred)
+static void readPrediction(UnetModel *m, size_t nv, double *pred)
 {
-  size_t nv = (size_t)n * n * n;
   std::vector<float> hy(nv);
   /* the forward was queued on the (non-blocking) library stream */
   HIP_CHECK(hipStreamSynchronize(nusifStream()));
@@ -634,16 +738,16 @@ static void readPrediction(UnetModel *m, int n, double *pred)
   for (size_t q = 0; q < nv; q++) pred[q] = hy[q];
 }
 
-/* cfg = {L, ch0, ch1, ch2, ch3, cin}; x: n^3 x cin (channels-last), target n^3;
- * params in the oracle's flat layout. Returns the loss; grad and pred (may be
- * NULL) receive the gradient and the prediction. */
-extern "C" double nusifUnetLossGrad(const int *cfg, int n, const double *params, const double *x,
+/* cfg = {L, ch0, ch1, ch2, ch3, cin}; x: nx ny nz x cin (channels-last),
+ * target nx ny nz; params in the oracle's flat layout. Returns the loss;
+ * grad and pred (may be NULL) receive the gradient and the prediction. */
+extern "C" double nusifUnetLossGrad(const int *cfg, int nx, int ny, int nz, const double *params, const double *x,
     const double *target, double *grad, double *pred)
 {
   uint16_t *dx = nullptr;
-  UnetModel *m = checkModel(cfg, n, params, x, &dx);
+  UnetModel *m = checkModel(cfg, nx, ny, nz, params, x, &dx);
   unetZeroGrads(m);
-  size_t nv = (size_t)n * n * n, np = unetParamCount(m);
+  size_t nv = (size_t)nx * ny * nz, np = unetParamCount(m);
   std::vector<float> ht(nv);
   for (size_t q = 0; q < nv; q++) ht[q] = (float)target[q];
   float *dt = nullptr;
@@ -651,7 +755,7 @@ extern "C" double nusifUnetLossGrad(const int *cfg, int n, const double *params,
   HIP_CHECK(hipMemcpy(dt, ht.data(), nv * sizeof(float), hipMemcpyHostToDevice));
   unetForward(m, dx, true);
   double loss = unetBackward(m, dx, dt, true);
-  if (pred) readPrediction(m, n, pred);
+  if (pred) readPrediction(m, nv, pred);
   if (grad) {
     std::vector<float> hg(np);
     unetGetGrads(m, hg.data());
@@ -664,13 +768,14 @@ extern "C" double nusifUnetLossGrad(const int *cfg, int n, const double *params,
 }
 
 /* The inference forward (the one the warm start runs: decoder inputs read
- * upsampled in place) on the same arguments; pred receives n^3 values. */
-extern "C" void nusifUnetPredict(const int *cfg, int n, const double *params, const double *x, double *pred)
+ * upsampled in place) on the same arguments; pred receives nx ny nz values. */
+extern "C" void nusifUnetPredict(const int *cfg, int nx, int ny, int nz, const double *params, const double *x,
+    double *pred)
 {
   uint16_t *dx = nullptr;
-  UnetModel *m = checkModel(cfg, n, params, x, &dx);
+  UnetModel *m = checkModel(cfg, nx, ny, nz, params, x, &dx);
   unetForward(m, dx, false);
-  readPrediction(m, n, pred);
+  readPrediction(m, (size_t)nx * ny * nz, pred);
   HIP_CHECK(hipFree(dx));
   unetFree(m);
 }

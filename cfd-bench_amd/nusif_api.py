@@ -142,8 +142,10 @@ def _declare_optional(lib):
         "nusifUnetTrainCavity": ([C.c_char_p] + [C.c_int] * 6 + [C.c_double, C.c_int, _DP], C.c_int),
         "nusifUnetTrainCavityGrids": ([C.c_char_p, C.POINTER(C.c_int)] + [C.c_int] * 7 +
                                       [C.c_double, C.c_int, _DP], C.c_int),
-        "nusifUnetLossGrad": ([C.POINTER(C.c_int), C.c_int, _DP, _DP, _DP, _DP, _DP], C.c_double),
-        "nusifUnetPredict": ([C.POINTER(C.c_int), C.c_int, _DP, _DP, _DP], None),
+        "nusifUnetTrainChannel": ([C.c_char_p, C.c_char_p] + [C.c_int] * 3 + [C.c_double] + [C.c_int] * 6 +
+                                  [C.c_double, C.c_int, _DP], C.c_int),
+        "nusifUnetLossGrad": ([C.POINTER(C.c_int)] + [C.c_int] * 3 + [_DP, _DP, _DP, _DP, _DP], C.c_double),
+        "nusifUnetPredict": ([C.POINTER(C.c_int)] + [C.c_int] * 3 + [_DP, _DP, _DP], None),
         "nusifRestrictResidualCheck": ([C.c_int] * 3 + [C.c_double] * 3 + [_DP, _DP, C.POINTER(C.c_float),
                                        C.POINTER(C.c_float), _DP], None),
         "nusifProlongSetCheck": ([C.c_int] * 3 + [C.POINTER(C.c_float), C.POINTER(C.c_float)], None),

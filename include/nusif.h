@@ -171,17 +171,28 @@ int nusifUnetTrainCavity(const char *weightsOut, int grid, int runs, int stepsPe
 int nusifUnetTrainCavityGrids(const char *weightsOut, const int *grids, int ngrids, int netExtent,
     int runs, int stepsPerRun, int skipSteps, int epochs, int batch, double lr, int seed,
     double *lossPerEpoch);
+/* Pairs from `runs` channels of imax x jmax x kmax cells (the canal problem
+ * on a domain xlength x 1 x 1: inflow u = 1 with a seeded 1% uniform
+ * perturbation, a new seed per run, zero-gradient outflow, no-slip walls,
+ * Re ~ U[100, 1000], the rhs mean removed), the network on the first level
+ * of extents <= netExtent that four pooling levels still divide; then
+ * training as above, starting from the weights in weightsIn (fine-tuning;
+ * NULL or "" starts from a seeded initialisation). */
+int nusifUnetTrainChannel(const char *weightsIn, const char *weightsOut, int imax, int jmax, int kmax,
+    double xlength, int netExtent, int runs, int stepsPerRun, int skipSteps, int epochs, int batch, double lr,
+    int seed, double *lossPerEpoch);
 
 /* One forward and backward pass of the U-Net on the GPU, for gradient
- * checks: cfg = {levels, ch0, ch1, ch2, ch3, input channels}, x is n^3 x
- * cin channels-last, params in the flat layout of the weights file. Returns
- * the loss; grad and pred (may be NULL) receive the results. */
-double nusifUnetLossGrad(const int *cfg, int n, const double *params, const double *x,
+ * checks: cfg = {levels, ch0, ch1, ch2, ch3, input channels}, x is
+ * nx ny nz x cin channels-last (each extent divisible by 2^(levels-1)),
+ * params in the flat layout of the weights file. Returns the loss; grad and
+ * pred (may be NULL) receive the results. */
+double nusifUnetLossGrad(const int *cfg, int nx, int ny, int nz, const double *params, const double *x,
     const double *target, double *grad, double *pred);
 /* The inference forward on the same arguments (the warm start's: decoder
  * inputs read upsampled in place, nothing kept for a backward pass); pred
- * receives n^3 values, bit-identical to nusifUnetLossGrad's. */
-void nusifUnetPredict(const int *cfg, int n, const double *params, const double *x, double *pred);
+ * receives nx ny nz values, bit-identical to nusifUnetLossGrad's. */
+void nusifUnetPredict(const int *cfg, int nx, int ny, int nz, const double *params, const double *x, double *pred);
 
 /* Checks of the warm start's grid transfers on host arrays (padded layouts,
  * for tests): r0 = rhs - A p (FP64, the solvers' 7-point Neumann Laplacian

@@ -25,11 +25,15 @@ def cfg_arr(L, ch, cin):
     return (C.c_int * 6)(L, *(list(ch) + [0] * (4 - len(ch))), cin)
 
 
-# n >= 32 runs the forward convolutions on the LDS-staged brick kernel
-@pytest.mark.parametrize("L,ch,n", [(2, (16, 32), 8), (4, (16, 32, 64, 128), 16),
-                                    (2, (16, 32), 32), (3, (16, 32, 64), 48)])
-def test_gpu_unet_forward_and_gradient_match_oracle(L, ch, n, gpu_ok):
-    print("unet", L, ch, n, flush=True)
+# extents (nx, ny, nz): cubes, and boxes with every extent different; the
+# last is the channel's network grid (96 x 24 x 24, fewer than 256 bricks:
+# split-K forward convolutions on its finest level)
+@pytest.mark.parametrize("L,ch,ext", [(2, (16, 32), (8, 8, 8)), (4, (16, 32, 64, 128), (16, 16, 16)),
+                                      (2, (16, 32), (32, 32, 32)), (3, (16, 32, 64), (48, 48, 48)),
+                                      (2, (16, 32), (40, 12, 20)), (4, (16, 32, 64, 128), (96, 24, 24))])
+def test_gpu_unet_forward_and_gradient_match_oracle(L, ch, ext, gpu_ok):
+    print("unet", L, ch, ext, flush=True)
+    nx, ny, nz = ext
     cin = 8
     cfg = cfg_arr(L, ch, cin)
     total = lib("orc_unet").orcUnetParamCount(cfg)
@@ -37,23 +41,23 @@ def test_gpu_unet_forward_and_gradient_match_oracle(L, ch, n, gpu_ok):
     params = np.zeros(total)
     # He-like init per conv, small head
     params[:] = rng.normal(0, 0.08, total)
-    x = np.zeros((n, n, n, cin))
-    x[..., :2] = rng.uniform(-1, 1, (n, n, n, 2))
-    t = rng.uniform(-1, 1, (n, n, n))
+    x = np.zeros((nz, ny, nx, cin))
+    x[..., :2] = rng.uniform(-1, 1, (nz, ny, nx, 2))
+    t = rng.uniform(-1, 1, (nz, ny, nx))
     g_gpu = np.zeros(total)
-    p_gpu = np.zeros((n, n, n))
-    loss_gpu = lib("gpu_rb").nusifUnetLossGrad(cfg, n, params.ctypes.data_as(DP), x.ctypes.data_as(DP),
+    p_gpu = np.zeros((nz, ny, nx))
+    loss_gpu = lib("gpu_rb").nusifUnetLossGrad(cfg, nx, ny, nz, params.ctypes.data_as(DP), x.ctypes.data_as(DP),
                                                t.ctypes.data_as(DP), g_gpu.ctypes.data_as(DP),
                                                p_gpu.ctypes.data_as(DP))
     g_orc = np.zeros(total)
-    p_orc = np.zeros((n, n, n))
-    loss_orc = lib("orc_unet").orcUnetLoss(cfg, n, params.ctypes.data_as(DP), x.ctypes.data_as(DP),
-                                          t.ctypes.data_as(DP), g_orc.ctypes.data_as(DP),
-                                          p_orc.ctypes.data_as(DP), 1)
+    p_orc = np.zeros((nz, ny, nx))
+    loss_orc = lib("orc_unet").orcUnetLossBox(cfg, nx, ny, nz, params.ctypes.data_as(DP), x.ctypes.data_as(DP),
+                                             t.ctypes.data_as(DP), g_orc.ctypes.data_as(DP),
+                                             p_orc.ctypes.data_as(DP), 1)
     # the inference forward (decoder inputs upsampled inside the conv) gives
     # the training forward's prediction bit for bit
-    p_inf = np.zeros((n, n, n))
-    lib("gpu_rb").nusifUnetPredict(cfg, n, params.ctypes.data_as(DP), x.ctypes.data_as(DP),
+    p_inf = np.zeros((nz, ny, nx))
+    lib("gpu_rb").nusifUnetPredict(cfg, nx, ny, nz, params.ctypes.data_as(DP), x.ctypes.data_as(DP),
                                    p_inf.ctypes.data_as(DP))
     assert np.array_equal(p_inf, p_gpu)
     perr = np.linalg.norm(p_gpu - p_orc) / np.linalg.norm(p_orc)
