@@ -111,6 +111,7 @@ void nusifDefaultOptions(NusifOptions *o)
   o->mgSpeculate = 0;
   o->mgInner     = 1;
   o->unetExtent  = 64;
+  o->unetAcceptLevel = 2;
   o->fgRhs       = 1;
   o->rbFused     = 1;
   o->lid         = 1.0;
@@ -238,6 +239,11 @@ extern "C" int nusifSetOption(Parameter *p, const char *key, const char *value)
     return r;
   }
   if (k == "unet_extent") return parseInt(v.c_str(), &o->unetExtent);
+  if (k == "unet_accept_level") {
+    int r = parseInt(v.c_str(), &o->unetAcceptLevel);
+    if (r == 0 && o->unetAcceptLevel < 0) return -2;
+    return r;
+  }
   if (k == "fg_rhs") return parseInt(v.c_str(), &o->fgRhs);
   if (k == "rb_fused") return parseInt(v.c_str(), &o->rbFused);
   if (k == "mg_speculate") return parseInt(v.c_str(), &o->mgSpeculate);

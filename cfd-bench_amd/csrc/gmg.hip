@@ -2034,7 +2034,7 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
       const double *wn = warmstartNormsAsync(ws);
       double nbg       = 0.0;
       double rng       = readNorms(c, &nbg);
-      double rc0 = sqrt(wn[0]), rcg = sqrt(wn[2]);
+      double rc0 = sqrt(warmstartReferenceNorm2(ws)), rcg = sqrt(wn[2]);
       /* the starting residuals are compared on the network grid (restricted
        * fine residuals): a correction interpolated from that grid carries
        * interpolation error whose fine-grid residual the first smoothing

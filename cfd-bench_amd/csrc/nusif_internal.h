@@ -121,6 +121,8 @@ struct NusifOptions {
   int warmstart;
   char unetWeights[512];
   int unetExtent;  /* warm start: the network runs on the first level of extent <= this (64) */
+  int unetAcceptLevel; /* warm start: the acceptance test compares residuals restricted this
+                        * many levels below the network grid (2; 0: on it) */
   int fgRhs;       /* computeFG also forms rhs (1, default), computeRHS then skips its pass */
   int rbFused;     /* rb, mg: one launch per sweep where the grid allows (1, default) */
   int verbose;     /* 0 quiet; 1 the reference's -DVERBOSE output (configuration at

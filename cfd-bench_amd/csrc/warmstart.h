@@ -23,10 +23,11 @@ float *warmstartResidualBuffer(WarmStart *w);
 /* level l's residual buffer and geometry when 1 <= l <= the network level,
  * else null */
 float *warmstartLevelResidual(WarmStart *w, int l, Geo *g);
-/* sum of squares of the residual restricted to the network grid, from the
- * fine FP32 residual, or from level 1 when it is already there (level1) */
+/* sum of squares of the residual restricted to the acceptance level, from
+ * the fine FP32 residual, or from level 1 when it is already there (level1) */
 void warmstartCoarseNormLaunch(WarmStart *w, const float *rFine, bool level1);
 const double *warmstartNormsAsync(WarmStart *w);
+double warmstartReferenceNorm2(const WarmStart *w);
 void warmstartPrepare(WarmStart *w, const double *p, const double *rhs, double idx2, double idy2, double idz2);
 const float *warmstartCorrection(WarmStart *w);
 void warmstartCaptureBegin(WarmStart *w, const double *p, const Geo &g0);

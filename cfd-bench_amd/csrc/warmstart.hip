@@ -12,8 +12,9 @@
  * on the network grid (the inverse Laplacian of the smoothest mode of the
  * domain's longest side; l = n h on a cube), prolongated (trilinear) to the
  * finest grid; guess = p_prev + dp. The guess replaces p_prev only when its
- * residual rhs - A guess, restricted to the network grid, has a smaller norm
- * than r0 restricted there; otherwise the solve starts from p_prev. Either
+ * residual rhs - A guess, restricted to the acceptance level (unet_accept_level
+ * halvings below the network grid, 2 by default), has a smaller norm than r0
+ * restricted there; otherwise the solve starts from p_prev. Either
  * way the multigrid iterates to the same tolerance, so the answer's accuracy
  * does not depend on the network.
  *
@@ -28,6 +29,7 @@
 
 #include "nusif_internal.h"
 #include "reduce.h"
+#include "timing.h"
 #include "unet.h"
 #include "unet_kernels.h"
 #include "warmstart.h"
@@ -43,13 +45,14 @@ struct WsLevel {
 struct WarmStart {
   UnetModel *net;
   int Ln;                 /* network level */
+  int Lc;                 /* level whose restricted residuals the acceptance compares (>= Ln) */
   int nx, ny, nz;         /* network grid extents */
   double h2;              /* output scale (l / pi)^2, l the domain's longest side */
   WsLevel lv[12];
   uint16_t *x;            /* network input, nx ny nz voxels x 8 channels */
   double *pPrev;          /* FP64 copy of p for capture (capture only) */
   double *scal;           /* [0] sum r0^2, [1] sum b^2 on level Ln, [2] guess residual
-                           * on level Ln, [3] sum r0^2 on level 0 */
+                           * on level Lc, [3] sum r0^2 on level 0, [4] sum r0^2 on level Lc */
   double *scalHost;       /* pinned, host-coherent mirror of scal (the folds write both) */
   double *scalHostDev;    /* its device address */
   double *partials;
@@ -322,6 +325,18 @@ WarmStart *warmstartCreate(NusifCtx *c, UnetModel *net, bool capture, int netLev
     w->lv[l].d = falloc(g.n);
     if (l < w->Ln) g = makeGeo(g.ni / 2, g.nj / 2, g.nk / 2);
   }
+  /* the acceptance test's level: unet_accept_level halvings below the
+   * network grid, as far as the extents stay even */
+  w->Lc = w->Ln;
+  for (int q = 0; q < c->opt.unetAcceptLevel && w->Lc + 1 < 12; q++) {
+    const Geo &gl = w->lv[w->Lc].g;
+    if (gl.ni % 2 || gl.nj % 2 || gl.nk % 2 || gl.ni < 2 || gl.nj < 2 || gl.nk < 2) break;
+    w->Lc++;
+    w->lv[w->Lc].g = makeGeo(gl.ni / 2, gl.nj / 2, gl.nk / 2);
+    w->lv[w->Lc].r = falloc(w->lv[w->Lc].g.n);
+    w->lv[w->Lc].b = nullptr;
+    w->lv[w->Lc].d = nullptr;
+  }
   const Geo &gn = w->lv[w->Ln].g;
   if (gn.ni % div || gn.nj % div || gn.nk % div)
     NUSIF_FATAL("warm start: network grid %dx%dx%d (level %d) is not divisible by %d, which %d pooling levels need",
@@ -344,8 +359,8 @@ WarmStart *warmstartCreate(NusifCtx *c, UnetModel *net, bool capture, int netLev
           c->geo.ni, c->geo.nj, c->geo.nk, c->opt.unetExtent > 0 ? c->opt.unetExtent : 64);
   }
   HIP_CHECK(hipMalloc((void **)&w->x, netVoxels(w) * 8 * sizeof(uint16_t)));
-  HIP_CHECK(hipMalloc((void **)&w->scal, 4 * sizeof(double)));
-  HIP_CHECK(hipHostMalloc((void **)&w->scalHost, 4 * sizeof(double), hipHostMallocCoherent));
+  HIP_CHECK(hipMalloc((void **)&w->scal, 8 * sizeof(double)));
+  HIP_CHECK(hipHostMalloc((void **)&w->scalHost, 8 * sizeof(double), hipHostMallocCoherent));
   HIP_CHECK(hipHostGetDevicePointer((void **)&w->scalHostDev, w->scalHost, 0));
   HIP_CHECK(hipMalloc((void **)&w->partials, 65536 * sizeof(double)));
   w->pPrev = nullptr;
@@ -357,10 +372,10 @@ void warmstartFree(WarmStart *w)
 {
   if (!w) return;
   HIP_CHECK(hipStreamSynchronize(nusifStream()));
-  for (int l = 0; l <= w->Ln; l++) {
+  for (int l = 0; l <= w->Lc; l++) {
     HIP_CHECK(hipFree(w->lv[l].r));
-    HIP_CHECK(hipFree(w->lv[l].b));
-    HIP_CHECK(hipFree(w->lv[l].d));
+    if (w->lv[l].b) HIP_CHECK(hipFree(w->lv[l].b));
+    if (w->lv[l].d) HIP_CHECK(hipFree(w->lv[l].d));
   }
   HIP_CHECK(hipFree(w->x));
   HIP_CHECK(hipFree(w->scal));
@@ -417,6 +432,14 @@ void warmstartPrepare(WarmStart *w, const double *p, const double *rhs, double i
     kFoldSumSq2<<<1, 256, 0, st>>>(w->partials, (int)(g2.x * g2.y * g2.z), w->scal, w->scalHostDev);
   }
   kPack<<<1024, 256, 0, st>>>(L.r, L.b, L.g, w->scal, w->x);
+  /* r0 on down to the acceptance level, its sum of squares there (scal[4]) */
+  if (w->Lc > w->Ln) {
+    for (int l = w->Ln + 1; l <= w->Lc; l++)
+      kRestrictPair<<<cells(w->lv[l].g), dim3(WTX, WTY), 0, st>>>(w->lv[l - 1].r, nullptr, w->lv[l - 1].g, w->lv[l].r,
+          nullptr, w->lv[l].g, l == w->Lc ? w->partials : nullptr);
+    const dim3 gc = cells(w->lv[w->Lc].g);
+    kFoldComp0<<<1, 256, 0, st>>>(w->partials, (int)(gc.x * gc.y * gc.z), w->scal + 4, w->scalHostDev + 4);
+  }
 }
 
 /* The network's correction on the finest grid (FP32, interior and face
@@ -451,12 +474,12 @@ void warmstartCoarseNormLaunch(WarmStart *w, const float *rFine, bool level1)
   hipStream_t st = nusifStream();
   const float *src = level1 ? w->lv[1].r : rFine;
   const int l0     = level1 ? 2 : 1;
-  const WsLevel &L = w->lv[w->Ln];
+  const WsLevel &L = w->lv[w->Lc];
   /* one component into scal[2]: scal[3] keeps the fine ||r0||^2 of the input */
-  if (l0 <= w->Ln) { /* the last restriction also gives the sum of squares */
-    for (int l = l0; l <= w->Ln; l++) {
+  if (l0 <= w->Lc) { /* the last restriction also gives the sum of squares */
+    for (int l = l0; l <= w->Lc; l++) {
       kRestrictPair<<<cells(w->lv[l].g), dim3(WTX, WTY), 0, st>>>(src, nullptr, w->lv[l - 1].g, w->lv[l].r, nullptr,
-          w->lv[l].g, l == w->Ln ? w->partials : nullptr);
+          w->lv[l].g, l == w->Lc ? w->partials : nullptr);
       src = w->lv[l].r;
     }
     const dim3 gl = cells(L.g);
@@ -468,10 +491,14 @@ void warmstartCoarseNormLaunch(WarmStart *w, const float *rFine, bool level1)
   }
 }
 
-/* scal[0..3] ([0] = sum (R r0)^2, [2] = the last warmstartCoarseNormLaunch)
+/* scal[0..4] ([0] = sum (R r0)^2, [2] = the last warmstartCoarseNormLaunch)
  * in pinned host memory, which the folds write themselves: valid once the
  * stream has run them (after the caller's next synchronisation) */
 const double *warmstartNormsAsync(WarmStart *w) { return w->scalHost; }
+
+/* the previous pressure's restricted sum of squares the acceptance test
+ * compares with the guess's (scal[2]): on the acceptance level */
+double warmstartReferenceNorm2(const WarmStart *w) { return w->scalHost[w->Lc > w->Ln ? 4 : 0]; }
 
 /* capture, part 1 (before the solve, after warmstartPrepare): the input and
  * a copy of p */
@@ -576,6 +603,43 @@ static void capturePairs(Parameter *prm, int netExtent, int skipSteps, int steps
   nusifFreeDiscretization(&d);
 }
 
+/* sum of t^2 over a flat array: per-workgroup partials */
+__global__ __launch_bounds__(256) void kSumSqFlat(const float *t, size_t n, double *partials)
+{
+  __shared__ double lds[4];
+  double a = 0.0;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    a += (double)t[q] * t[q];
+  a = blockSum(a, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = a;
+}
+
+/* the loss of a network that predicts zero: the mean over the pairs of the
+ * targets' mean square, the scale the training loss is read against */
+static double zeroPredictionLoss(const PairSet &ps)
+{
+  const size_t nv = (size_t)ps.nx * ps.ny * ps.nz;
+  const int nb    = 256;
+  double *part = nullptr, *out = nullptr, *outDev = nullptr;
+  HIP_CHECK(hipMalloc((void **)&part, nb * sizeof(double)));
+  HIP_CHECK(hipMalloc((void **)&outDev, sizeof(double)));
+  HIP_CHECK(hipHostMalloc((void **)&out, sizeof(double), hipHostMallocCoherent));
+  double *outHostDev = nullptr;
+  HIP_CHECK(hipHostGetDevicePointer((void **)&outHostDev, out, 0));
+  hipStream_t st = nusifStream();
+  double sum     = 0.0;
+  for (float *t : ps.ts) {
+    kSumSqFlat<<<nb, 256, 0, st>>>(t, nv, part);
+    kFoldSum1<<<1, 256, 0, st>>>(part, nb, outDev, outHostDev);
+    HIP_CHECK(hipStreamSynchronize(st));
+    sum += *out / (double)nv;
+  }
+  HIP_CHECK(hipFree(part));
+  HIP_CHECK(hipFree(outDev));
+  HIP_CHECK(hipHostFree(out));
+  return ps.ts.empty() ? 0.0 : sum / ps.ts.size();
+}
+
 /* Adam over the pairs: `epochs` shuffled passes, cosine-decayed lr, an
  * update per `batch` samples; from weightsIn when given (fine-tuning), else
  * from a seeded He initialisation. Writes weightsOut. */
@@ -593,6 +657,10 @@ static void trainOnPairs(PairSet &ps, const char *weightsIn, const char *weights
     unetSyncWeights(m);
   }
   unetZeroGrads(m);
+  const double zeroLoss = zeroPredictionLoss(ps);
+  const double t0       = getTimeStamp();
+  printf("[unet] %zu pairs on %dx%dx%d; loss of a zero prediction %.5g\n", ps.ts.size(), ps.nx, ps.ny, ps.nz,
+      zeroLoss);
   std::vector<int> order(ps.ts.size());
   for (size_t q = 0; q < order.size(); q++) order[q] = (int)q;
   for (int e = 0; e < epochs; e++) {
@@ -610,12 +678,14 @@ static void trainOnPairs(PairSet &ps, const char *weightsIn, const char *weights
       }
     }
     if (lossPerEpoch) lossPerEpoch[e] = sum / order.size();
-    printf("[unet] epoch %d/%d: mean loss %.5g (lr %.3g)\n", e + 1, epochs, sum / order.size(), lrE);
+    printf("[unet] epoch %d/%d: mean loss %.5g, %.4f of a zero prediction's (lr %.3g), %.0f s\n", e + 1,
+        epochs, sum / order.size(), zeroLoss > 0.0 ? sum / order.size() / zeroLoss : 0.0, lrE,
+        getTimeStamp() - t0);
     fflush(stdout);
+    /* the weights after every epoch, so an interrupted run keeps its last one */
+    if (unetSave(m, weightsOut) != 0) NUSIF_FATAL("cannot write %s", weightsOut);
   }
-  int rc = unetSave(m, weightsOut);
   unetFree(m);
-  if (rc != 0) NUSIF_FATAL("cannot write %s", weightsOut);
 }
 
 /* the driver loop's parameters shared by the training runs */
@@ -649,6 +719,7 @@ extern "C" int nusifUnetTrainCavityGrids(const char *weightsOut, const int *grid
     int runs, int stepsPerRun, int skipSteps, int epochs, int batch, double lr, int seed, double *lossPerEpoch)
 {
   if (ngrids < 1) NUSIF_FATAL("unet training: no grid given");
+  const double tStart = getTimeStamp();
   Uniform uni(seed);
   PairSet ps;
   for (int r = 0; r < runs; r++) {
@@ -663,7 +734,8 @@ extern "C" int nusifUnetTrainCavityGrids(const char *weightsOut, const int *grid
     snprintf(buf, sizeof buf, "%.17g", lid);
     nusifSetOption(&prm, "lid", buf);
     capturePairs(&prm, netExtent, skipSteps, stepsPerRun, ps);
-    printf("[unet] run %d/%d: %d^3, Re %.1f lid %.2f, %zu pairs so far\n", r + 1, runs, grid, re, lid, ps.ts.size());
+    printf("[unet] run %d/%d: %d^3, Re %.1f lid %.2f, %zu pairs so far, %.0f s\n", r + 1, runs, grid, re, lid,
+        ps.ts.size(), getTimeStamp() - tStart);
     fflush(stdout);
   }
   if (ps.ts.empty()) return 0;

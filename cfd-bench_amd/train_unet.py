@@ -68,17 +68,24 @@ def evaluate(lib, weights, grid, re, lid, warm, steps, force=False, extent=64):
 warmup = 10
 
 
-def evaluate_channel(lib, weights, ext, re, seed, warm, steps, extent=64):
+def evaluate_channel(lib, weights, ext, re, seed, warm, steps, extent=64, force=False):
     prm = dict(name="canal", xlength=4.0, ylength=1.0, zlength=1.0, bcLeft=3, bcRight=3, bcBottom=1, bcTop=1,
                bcFront=1, bcBack=1, u_init=1.0, re=re, imax=ext[0], jmax=ext[1], kmax=ext[2], tau=0.5, te=1e9,
                gamma=0.9, levels=6, presmooth=2, postsmooth=2)
     opts = {"solver": "gmg", "precision": "mixed", "mg_tol": 1e-6, "mg_project": 1, "u_inflow": 1.0,
             "inflow_noise": 0.01, "seed": seed, "mg_omega": 1.3}
     if warm:
-        opts.update({"warmstart": "unet", "unet_weights": weights, "unet_extent": extent})
+        opts.update({"warmstart": "unet_force" if force else "unet", "unet_weights": weights,
+                     "unet_extent": extent})
     case = A.Case(lib, options=opts, **prm)
     tr = (C.c_double * (4 * (warmup + steps)))()
     lib.nusifRunSteps(C.byref(case.d), C.byref(case.s), warmup, tr)
+    ratio = None
+    if warm:  # the guess's restricted and fine residual against the previous p's, one step
+        lib.nusifRunSteps(C.byref(case.d), C.byref(case.s), 1, tr)
+        res = (C.c_double * 4)()
+        lib.nusifWarmstartResiduals(C.byref(case.s), res)
+        ratio = {"fine": res[1] / res[0], "restricted": res[3] / res[2]}
     tried0, taken0, tried, taken = C.c_longlong(), C.c_longlong(), C.c_longlong(), C.c_longlong()
     lib.nusifWarmstartCounts(C.byref(case.s), C.byref(tried0), C.byref(taken0))
     lib.nusifDeviceSynchronize()
@@ -91,6 +98,7 @@ def evaluate_channel(lib, weights, ext, re, seed, warm, steps, extent=64):
     out = dict(cycles_per_step=sum(tr[4 * q + 3] for q in range(steps)) / steps, ms_per_step=1e3 * dt / steps)
     if warm:
         out["guess_used_fraction"] = (taken.value - taken0.value) / max(tried.value - tried0.value, 1)
+        out["guess_over_prev_residual"] = ratio
     return out
 
 
@@ -130,6 +138,8 @@ def main():
             if w and os.path.exists(w):
                 report[name] = evaluate_channel(lib, w, a.channel, 500.0, 99991, True, a.eval_steps,
                                                 extent=a.extent)
+                report[name + "_always"] = evaluate_channel(lib, w, a.channel, 500.0, 99991, True, a.eval_steps,
+                                                            extent=a.extent, force=True)
         report["without"] = evaluate_channel(lib, a.out, a.channel, 500.0, 99991, False, a.eval_steps)
         print(json.dumps(report), flush=True)
         return

@@ -75,8 +75,16 @@ void nusifFreeSolver(Solver *s);
  *                 (unet_force: always take the network's guess, diagnostic)
  *   unet_weights  path    trained U-Net weights (nusifUnetSave format)
  *   unet_extent   int     the network runs on the first multigrid level whose
- *                         extents are all <= this (64); a weights file that
+ *                         extents are all <= this (64), or the last that its
+ *                         pooling levels still divide; a weights file that
  *                         records its training extent overrides it
+ *   unet_accept_level int the guess is taken when its residual, restricted
+ *                         this many levels below the network grid (2; 0: on
+ *                         it), is smaller than the previous pressure's: the
+ *                         trilinear correction's interpolation error leaves
+ *                         residual at the network grid's own scale that the
+ *                         first sweeps remove, which the coarser comparison
+ *                         no longer counts
  *   fg_rhs        0 | 1   computeFG also forms the right-hand side in the same
  *                         pass, into a spare buffer that computeRHS then swaps
  *                         in (1, default; computeRHS recomputes when f, g, h,
@@ -116,8 +124,9 @@ double nusifSolverRelResidual(Solver *s);
  * previous-step pressure, -1 no network. */
 int nusifWarmstartUsed(Solver *s);
 /* Residual norms of the last warm start: fine ||r0|| (previous-step p),
- * fine ||r_guess||, and both restricted to the network grid (the acceptance
- * test compares the restricted pair). */
+ * fine ||r_guess||, and both restricted to the acceptance level
+ * (unet_accept_level below the network grid; the acceptance test compares
+ * the restricted pair). */
 void nusifWarmstartResiduals(Solver *s, double *out4);
 /* Network guesses formed and accepted by the solves of `s` since
  * initSolver (the acceptance rate of the warm start over a run). */

@@ -124,8 +124,10 @@ def test_gpu_channel768_learned_vs_fp64(gpu_ok):
     the 96 x 24 x 24 level, the first whose extents are <= 64 or that four
     pooling levels no longer divide) with FP32 V-cycles in FP64 defect
     correction, against plain FP64 multigrid, both to 1e-6 relative residual
-    per step: every solve reaches the tolerance, the guesses are counted, and
-    the per-field difference is printed and bounded."""
+    per step, 16 steps from rest: every solve reaches the tolerance, the
+    network's guess is taken on most of the last six (the start-up transient
+    of the first ten lies outside what the network was tuned on), and the
+    per-field difference is printed and bounded."""
     weights = next(w for w in CHANNEL_WEIGHTS if os.path.exists(w))
     c = dict(CHANNELS["channel768"])
     noise = {k: c.pop(k) for k in ("inflow_noise", "seed")}
@@ -140,24 +142,33 @@ def test_gpu_channel768_learned_vs_fp64(gpu_ok):
     for name, opts in modes.items():
         print("channel768", name, os.path.basename(weights), flush=True)
         g = A.Case(lib("gpu_rb"), options=opts, **prm)
-        n = 0
-        for step in range(6):
+        n, taken = 0, 0
+        for step in range(16):
             g.step()
             rel = g.lib.nusifSolverRelResidual(C.byref(g.s))
             cyc = g.lib.nusifSolverIterations(C.byref(g.s))
-            n += cyc
-            print(name, "step", step, "cycles", cyc, "rel residual %.3e" % rel, flush=True)
+            used = g.lib.nusifWarmstartUsed(C.byref(g.s))
+            if step >= 10:
+                n += cyc
+                taken += max(used, 0)
+            print(name, "step", step, "cycles", cyc, "rel residual %.3e" % rel, "guess", used, flush=True)
             assert rel <= 1e-6
         if name == "learned":
-            tried, taken = C.c_longlong(), C.c_longlong()
-            g.lib.nusifWarmstartCounts(C.byref(g.s), C.byref(tried), C.byref(taken))
-            print("network guesses taken %d of %d" % (taken.value, tried.value))
-            assert tried.value == 6
+            print("network guesses taken %d of the last 6" % taken)
+            if os.path.basename(weights) == "unet_channel.bin":
+                assert taken >= 3
         cycles[name] = n / 6.0
         fields[name] = g.fields()
         g.close()
     print("V-cycles per step: fp64 %.2f, learned %.2f" % (cycles["fp64"], cycles["learned"]))
+    # p is determined up to a constant (all-Neumann pressure; the reference
+    # re-centres it only every 100 steps), and an accepted guess may shift
+    # that constant: p is compared with its interior mean removed
     for k in ("u", "v", "w", "p"):
-        err = A.rel_maxnorm(fields["learned"][k], fields["fp64"][k])
+        a, b = fields["learned"][k], fields["fp64"][k]
+        if k == "p":
+            a = a[1:-1, 1:-1, 1:-1] - a[1:-1, 1:-1, 1:-1].mean()
+            b = b[1:-1, 1:-1, 1:-1] - b[1:-1, 1:-1, 1:-1].mean()
+        err = A.rel_maxnorm(a, b)
         print("channel768 learned", k, "rel err vs FP64 %.3e" % err)
-        assert err <= (1e-3 if k == "p" else 1e-5), k
+        assert err <= 1e-5, k

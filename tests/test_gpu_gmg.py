@@ -110,13 +110,15 @@ def test_gpu_gmg_cavity_steps(precision, inner, gpu_ok):
     solve: the oracle is given the group the library reports, the cycle
     counts agree step by step, and each group follows from the previous
     solve (its count when the group did not converge, else the same group or
-    one less)."""
+    one less; in mixed precision a group that stopped reducing at the FP32
+    bound of one correction caps the later groups below itself)."""
     print("gmg cavity", precision, inner, flush=True)
     prm = dict(G.TRAJ["c1_rb"][1], re=1000.0, levels=4, presmooth=2, postsmooth=2)
     opts = {"solver": "gmg", "precision": precision, "mg_tol": 1e-8, "mg_inner": inner}
     g = A.Case(lib("gpu_rb"), options=opts, **prm)
     o = A.Case(lib("orc_rb"), **prm)
     prev = None  # (group, cycles) of the previous solve
+    cap = None   # mixed precision: the group cap the library set
     for step in range(6):
         g.step()
         d = o.d
@@ -137,7 +139,11 @@ def test_gpu_gmg_cavity_steps(precision, inner, gpu_ok):
             if prev is None:
                 assert first == 1
             elif prev[1] > prev[0]:
-                assert first == prev[1], (step, prev, first)
+                if first != prev[1]:
+                    # set below the stalled group, or kept from an earlier stall
+                    assert precision == "mixed" and (first < prev[0] or first == cap), (step, prev, first, cap)
+                    cap = first
+                    print("step %d: group capped at %d after %s" % (step, first, prev))
             else:
                 assert first in (prev[0], max(1, prev[0] - 1)), (step, prev, first)
         prev = (first, cyc)
