@@ -255,6 +255,54 @@ def run_rb(lib, case_gpu, n, steps=3):
     return out
 
 
+def channel_params():
+    """the combined configuration at scale: channel 768 x 192 x 192 (domain
+    4 x 1 x 1), Re 500, inflow u = 1 with a seeded 1% uniform perturbation,
+    zero-gradient outflow, no-slip walls"""
+    return dict(name="canal", xlength=4.0, ylength=1.0, zlength=1.0, bcLeft=3, bcRight=3, bcBottom=1, bcTop=1,
+                bcFront=1, bcBack=1, u_init=1.0, re=500.0, imax=768, jmax=192, kmax=192, tau=0.5, te=1e9,
+                gamma=0.9, levels=6, presmooth=2, postsmooth=2)
+
+
+def run_channel(lib, warmup=10, steps=40):
+    """The channel at scale: the learned warm start (channel-tuned network)
+    with FP32 V-cycles in FP64 defect correction against plain FP64 multigrid,
+    both to 1e-6 relative residual per step, from the same start."""
+    base = {"solver": "gmg", "mg_tol": 1e-6, "mg_project": 1, "u_inflow": 1.0, "inflow_noise": 0.01, "seed": 7,
+            "mg_omega": 1.3, "mg_inner": 0}
+    weights = os.path.join(ROOT, "cfd-bench_amd", "weights", "unet_channel.bin")
+    modes = {"fp64": dict(base, precision="fp64"),
+             "learned": dict(base, precision="mixed", warmstart="unet", unet_weights=weights)}
+    out = {}
+    cells = 768.0 * 192 * 192
+    for name, opts in modes.items():
+        c = A.Case(lib, options=opts, **channel_params())
+        tr = (C.c_double * (4 * max(warmup, steps)))()
+        lib.nusifRunSteps(C.byref(c.d), C.byref(c.s), warmup, tr)
+        tried0, taken0 = C.c_longlong(), C.c_longlong()
+        lib.nusifWarmstartCounts(C.byref(c.s), C.byref(tried0), C.byref(taken0))
+        lib.nusifDeviceSynchronize()
+        t0 = time.perf_counter()
+        lib.nusifRunSteps(C.byref(c.d), C.byref(c.s), steps, tr)
+        lib.nusifDeviceSynchronize()
+        dt = time.perf_counter() - t0
+        tried, taken = C.c_longlong(), C.c_longlong()
+        lib.nusifWarmstartCounts(C.byref(c.s), C.byref(tried), C.byref(taken))
+        c.close()
+        out[name] = {"value": steps / dt, "mlups": cells * steps / dt * 1e-6,
+                     "cycles_per_step": sum(tr[4 * q + 3] for q in range(steps)) / steps,
+                     "max_rel_residual": max(tr[4 * q + 2] for q in range(steps))}
+        if name == "learned":
+            out[name]["warmstart_guess_taken_fraction"] = ((taken.value - taken0.value)
+                                                           / max(tried.value - tried0.value, 1))
+        log("[bench] channel 768x192x192 %s: %d steps in %.3f s" % (name, steps, dt))
+    out["speedup_learned_vs_fp64_mg"] = out["learned"]["value"] / out["fp64"]["value"]
+    out["unit"] = "time steps/s"
+    out["workload"] = ("channel 768x192x192, Re 500, inflow u=1 with seeded 1%% noise, pressure to 1e-6 relative "
+                       "residual per step, V(2,2) 6 levels, omega 1.3; %d steps after %d" % (steps, warmup))
+    return out
+
+
 def cpu_rb(case_gpu, n, gpu_iters, itermax, iters=40):
     """Red-black SOR on one host core on the same state, one solve capped at
     `iters` iterations: the reference's solver rate. The reference itself
@@ -368,6 +416,7 @@ def main():
                 "speedup_vs_fp64_mg": vm / f64}
             out["fp32_mg_without_warmstart"].update(solver_rate(mx, cells))
     if not a.no_compare:
+        out["channel_768"] = run_channel(lib)
         out["rb_sor"] = run_rb(lib, case, n)
         if not a.no_cpu:
             out["rb_sor"]["cpu_baseline"] = cpu_rb(case, n, out["rb_sor"]["iterations_per_step"],

@@ -1363,7 +1363,7 @@ static dim3 normGrid(const Geo &g)
  * partials of r0^2, rhs^2 (slots 2 b, 2 b + 1). Returns the partial count.
  * Extents must be even. */
 int launchInputRestrict(const double *p, const double *rhs, const Geo &g, double idx2, double idy2, double idz2,
-    float *rc, float *bc, const Geo &gc, double *partials)
+    float *rc, float *bc, const Geo &gc, double *partials, float *r32, float *eZero)
 {
   const Coef<double> cf = {idx2, idy2, idz2, 0.0};
   ResidualRestrict ro;
@@ -1372,8 +1372,13 @@ int launchInputRestrict(const double *p, const double *rhs, const Geo &g, double
   ro.gc   = gc;
   dim3 ng = normGrid(g);
   kResidualLds<false, true><<<xcdGrid(ng), dim3(GTX, RTY), 0, nusifStream()>>>(p, nullptr, rhs, g, cf, nullptr,
-      nullptr, nullptr, partials, ng.x, ng.y, ng.z, nullptr, ro);
+      r32, eZero, partials, ng.x, ng.y, ng.z, nullptr, ro);
   return (int)(ng.x * ng.y * ng.z);
+}
+
+void launchFoldNorms(const double *partials, int nb, double *out, double *outHost)
+{
+  kFoldNorms<<<1, 256, 0, nusifStream()>>>(partials, nb, out, outHost);
 }
 
 int inputRestrictPartials(const Geo &g)
@@ -2005,7 +2010,43 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
   WarmStart *ws = sc->ws;
   bool haveInitial = false; /* mixed: residual of c->p already in L0.b, its norms in rnInit/nb */
   double rnInit = 0.0;
-  if (ws) {
+  bool fromGuess = false; /* mixed: the first cycle starts from the network's correction in L0.x */
+  if (ws && G->mixed && warmstartNet(ws) && warmstartLevel(ws) > 0) {
+    /* Mixed precision: the network's correction is the starting value of
+     * the FP32 error equation A e = r0, not a new FP64 iterate. The input
+     * pass (r0 = rhs - A p_prev restricted for the network) also writes r0
+     * as the solve's first FP32 defect with its norms, and the acceptance
+     * test runs on the coarse grids; an accepted correction is prolongated
+     * straight into the first cycle's e. No fine-grid pass beyond the one
+     * the solve makes without a network. (With the network on the finest
+     * grid, small problems, the general path below runs.) */
+    LevelT<float> &L0 = G->L32[0];
+    if (warmstartPrepare(ws, p, rhs, G->cf0.idx2, G->cf0.idy2, G->cf0.idz2, L0.b,
+            l0NeedsZero(G) ? L0.x : nullptr, c->scal->norm2, c->scalHostDev->norm2)) {
+      warmstartCaptureBegin(ws, p, g);
+      warmstartForward(ws);
+      warmstartCoarseAcceptLaunch(ws);
+      const double *wn = warmstartNormsAsync(ws);
+      double nb0       = 0.0;
+      double rn0       = readNorms(c, &nb0); /* one read-back: also the folds above */
+      double rc0 = sqrt(warmstartReferenceNorm2(ws)), rcg = sqrt(wn[2]);
+      sc->wsRes[0] = rn0;
+      sc->wsRes[1] = -1.0; /* the guess's fine-grid residual is not formed */
+      sc->wsRes[2] = rc0;
+      sc->wsRes[3] = rcg;
+      sc->warmUsed = (c->opt.warmstart == 2 || rcg < rc0) ? 1 : 0;
+      sc->wsTried++;
+      sc->wsTaken += sc->warmUsed;
+      if (sc->warmUsed) {
+        warmstartProlongCorrection(ws, L0.x);
+        fromGuess = true;
+      }
+      haveInitial = true;
+      rnInit      = rn0;
+      nb          = nb0;
+    }
+  }
+  if (ws && !haveInitial) {
     /* the network input: r0 = rhs - A p_prev and rhs, restricted to the
      * network grid, and their norms */
     warmstartPrepare(ws, p, rhs, G->cf0.idx2, G->cf0.idy2, G->cf0.idz2);
@@ -2090,7 +2131,8 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
     rel       = nb > 0.0 ? rn / nb : 0.0;
     dim3 ng   = normGrid(g);
     cycles    = runCycles(c, G, sc, rel, nb, [&](int m) {
-      for (int q = 0; q < m; q++) vcycle<float>(G, G->L32, 0, q > 0);
+      for (int q = 0; q < m; q++) vcycle<float>(G, G->L32, 0, q > 0 || fromGuess);
+      fromGuess = false;
       NUSIF_TIMED("gmg_correct_residual",
           (kResidualLds<true><<<xcdGrid(ng), dim3(GTX, RTY), 0, nusifStream()>>>(c->p, L0.x, rhs, g, G->cf0,
               c->pAlt, L0.b, l0NeedsZero(G) ? L0.xAlt : nullptr, c->partials, ng.x, ng.y, ng.z, gCycleStop)));

@@ -257,10 +257,13 @@ void launchSpecialBC(NusifCtx *c, const Discretization *d);
 void launchComputeFG(NusifCtx *c, const Discretization *d);
 /* gmg.hip kernels the warm start also runs: r0 = rhs - A p restricted to
  * the next level with rhs (returns the partial count; partials hold r0^2
- * and rhs^2 per workgroup), its partial count for a grid, and the FP32
+ * and rhs^2 per workgroup; optionally r0 as FP32 on the fine grid, r32, and
+ * an FP32 array zeroed, eZero), its partial count for a grid, the fold of
+ * such partials into a norm pair (and its host mirror), and the FP32
  * trilinear prolongation x_f = P x_c */
 int launchInputRestrict(const double *p, const double *rhs, const Geo &g, double idx2, double idy2, double idz2,
-    float *rc, float *bc, const Geo &gc, double *partials);
+    float *rc, float *bc, const Geo &gc, double *partials, float *r32 = nullptr, float *eZero = nullptr);
+void launchFoldNorms(const double *partials, int nb, double *out, double *outHost);
 int inputRestrictPartials(const Geo &g);
 void launchProlongSet32(float *xf, const Geo &gf, const float *xc, const Geo &gc);
 /* copy rhs into the spare rhs buffer (after a host upload of rhs) */

@@ -28,8 +28,12 @@ float *warmstartLevelResidual(WarmStart *w, int l, Geo *g);
 void warmstartCoarseNormLaunch(WarmStart *w, const float *rFine, bool level1);
 const double *warmstartNormsAsync(WarmStart *w);
 double warmstartReferenceNorm2(const WarmStart *w);
-void warmstartPrepare(WarmStart *w, const double *p, const double *rhs, double idx2, double idy2, double idz2);
+bool warmstartPrepare(WarmStart *w, const double *p, const double *rhs, double idx2, double idy2, double idz2,
+    float *r32 = nullptr, float *eZero = nullptr, double *norm2 = nullptr, double *norm2Host = nullptr);
 const float *warmstartCorrection(WarmStart *w);
+void warmstartForward(WarmStart *w);
+void warmstartProlongCorrection(WarmStart *w, float *fine);
+void warmstartCoarseAcceptLaunch(WarmStart *w);
 void warmstartCaptureBegin(WarmStart *w, const double *p, const Geo &g0);
 void warmstartCaptureEnd(WarmStart *w, const double *p);
 size_t warmstartSamples(const WarmStart *w);

@@ -48,6 +48,7 @@ struct WarmStart {
   int Lc;                 /* level whose restricted residuals the acceptance compares (>= Ln) */
   int nx, ny, nz;         /* network grid extents */
   double h2;              /* output scale (l / pi)^2, l the domain's longest side */
+  double cn[3];           /* 1 / h^2 per direction on the network grid */
   WsLevel lv[12];
   uint16_t *x;            /* network input, nx ny nz voxels x 8 channels */
   double *pPrev;          /* FP64 copy of p for capture (capture only) */
@@ -345,6 +346,13 @@ WarmStart *warmstartCreate(NusifCtx *c, UnetModel *net, bool capture, int netLev
   w->ny = gn.nj;
   w->nz = gn.nk;
   {
+    const Grid &gr = c->owner->grid;
+    const double s = (double)(1 << w->Ln);
+    w->cn[0]       = 1.0 / ((gr.dx * s) * (gr.dx * s));
+    w->cn[1]       = 1.0 / ((gr.dy * s) * (gr.dy * s));
+    w->cn[2]       = 1.0 / ((gr.dz * s) * (gr.dz * s));
+  }
+  {
     const Grid &gr = c->owner->grid; /* the domain's side lengths, on every level */
     const double l = std::max(gr.imax * gr.dx, std::max(gr.jmax * gr.dy, gr.kmax * gr.dz));
     w->h2          = (l / M_PI) * (l / M_PI);
@@ -401,9 +409,15 @@ bool warmstartCapturing(const WarmStart *w) { return w->capture; }
 /* Network input from p and rhs: r0 = rhs - A p and rhs restricted to the
  * network level (one fused pass to level 1, then 8-child averages), their
  * sums of squares there (scal[0], scal[1]) and of r0 on the finest grid
- * (scal[3]), packed into x. */
-void warmstartPrepare(WarmStart *w, const double *p, const double *rhs, double idx2, double idy2, double idz2)
+ * (scal[3]), packed into x. With r32 and the network below the finest grid,
+ * the same fine-grid pass also writes r0 as FP32 into r32 (a mixed-precision
+ * solve's first defect), zeroes eZero when given, and folds ||r0||^2 and
+ * ||rhs||^2 into norm2[0..1] (and the host mirror norm2Host); returns
+ * whether it did. */
+bool warmstartPrepare(WarmStart *w, const double *p, const double *rhs, double idx2, double idy2, double idz2,
+    float *r32, float *eZero, double *norm2, double *norm2Host)
 {
+  bool defect = false;
   hipStream_t st = nusifStream();
   Lap cf         = {idx2, idy2, idz2};
   const Geo &g0  = w->lv[0].g;
@@ -413,8 +427,13 @@ void warmstartPrepare(WarmStart *w, const double *p, const double *rhs, double i
     kFoldSum1<<<1, 256, 0, st>>>(w->partials, (int)(gr.x * gr.y * gr.z), w->scal + 3, w->scalHostDev + 3);
   } else {
     /* r0 and rhs averaged onto level 1 in the residual pass itself */
-    int nb = launchInputRestrict(p, rhs, g0, idx2, idy2, idz2, w->lv[1].r, w->lv[1].b, w->lv[1].g, w->partials);
+    int nb = launchInputRestrict(p, rhs, g0, idx2, idy2, idz2, w->lv[1].r, w->lv[1].b, w->lv[1].g, w->partials,
+        r32, r32 ? eZero : nullptr);
     kFoldComp0<<<1, 256, 0, st>>>(w->partials, nb, w->scal + 3, w->scalHostDev + 3);
+    if (r32) {
+      launchFoldNorms(w->partials, nb, norm2, norm2Host);
+      defect = true;
+    }
   }
   /* down to the network level, r and rhs in one launch per level; the last
    * one also gives the sums of squares there (the partials above are folded
@@ -440,6 +459,7 @@ void warmstartPrepare(WarmStart *w, const double *p, const double *rhs, double i
     const dim3 gc = cells(w->lv[w->Lc].g);
     kFoldComp0<<<1, 256, 0, st>>>(w->partials, (int)(gc.x * gc.y * gc.z), w->scal + 4, w->scalHostDev + 4);
   }
+  return defect;
 }
 
 /* The network's correction on the finest grid (FP32, interior and face
@@ -448,12 +468,68 @@ void warmstartPrepare(WarmStart *w, const double *p, const double *rhs, double i
  * correction (the caller forms it together with its residual). */
 const float *warmstartCorrection(WarmStart *w)
 {
-  hipStream_t st = nusifStream();
+  warmstartForward(w);
+  warmstartProlongCorrection(w, nullptr);
+  return w->lv[0].d;
+}
+
+/* forward and unpack: the correction d on the network grid (lv[Ln].d) */
+void warmstartForward(WarmStart *w)
+{
   NUSIF_TIMED("unet_forward", unetForward(w->net, w->x));
   const WsLevel &L = w->lv[w->Ln];
-  kUnpack<<<1024, 256, 0, st>>>(unetOutput(w->net), L.g, w->scal, w->h2, L.d);
-  for (int l = w->Ln; l > 0; l--) launchProlongSet32(w->lv[l - 1].d, w->lv[l - 1].g, w->lv[l].d, w->lv[l].g);
-  return w->lv[0].d;
+  kUnpack<<<1024, 256, 0, nusifStream()>>>(unetOutput(w->net), L.g, w->scal, w->h2, L.d);
+}
+
+/* d prolongated trilinearly from the network grid to the finest, written
+ * into `fine` (FP32 padded, face ghosts) or, when null, into lv[0].d */
+void warmstartProlongCorrection(WarmStart *w, float *fine)
+{
+  for (int l = w->Ln; l > 0; l--)
+    launchProlongSet32(l == 1 && fine ? fine : w->lv[l - 1].d, w->lv[l - 1].g, w->lv[l].d, w->lv[l].g);
+}
+
+/* r - A d on the network grid (7-point Neumann Laplacian with that grid's
+ * spacings; d carries its face ghosts), FP32 */
+__global__ __launch_bounds__(256) void kCoarseDefect(const float *__restrict__ r, const float *__restrict__ d, Geo g,
+    double cx, double cy, double cz, float *__restrict__ out)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int k = blockIdx.z + 1;
+  if (i > g.ni || j > g.nj) return;
+  long long c = g.at(i, j, k);
+  double dc   = d[c];
+  double lap  = ((double)d[c + 1] - 2.0 * dc + (double)d[c - 1]) * cx +
+               ((double)d[c + g.sj] - 2.0 * dc + (double)d[c - g.sj]) * cy +
+               ((double)d[c + g.sk] - 2.0 * dc + (double)d[c - g.sk]) * cz;
+  out[c] = (float)((double)r[c] - lap);
+}
+
+/* The acceptance test of a mixed-precision solve, on coarse grids only: the
+ * guess's residual estimated on the network grid as R r0 - A_n d (A_n the
+ * network grid's operator, which stands in for R A P there), restricted to
+ * the acceptance level, its sum of squares into scal[2]; compared with
+ * warmstartReferenceNorm2 (R r0 there). No fine-grid pass. */
+void warmstartCoarseAcceptLaunch(WarmStart *w)
+{
+  hipStream_t st = nusifStream();
+  const WsLevel &N = w->lv[w->Ln];
+  kCoarseDefect<<<cells(N.g), dim3(WTX, WTY), 0, st>>>(N.r, N.d, N.g, w->cn[0], w->cn[1], w->cn[2], N.b);
+  if (w->Lc > w->Ln) {
+    const float *src = N.b;
+    for (int l = w->Ln + 1; l <= w->Lc; l++) {
+      kRestrictPair<<<cells(w->lv[l].g), dim3(WTX, WTY), 0, st>>>(src, nullptr, w->lv[l - 1].g, w->lv[l].r, nullptr,
+          w->lv[l].g, l == w->Lc ? w->partials : nullptr);
+      src = w->lv[l].r;
+    }
+    const dim3 gc = cells(w->lv[w->Lc].g);
+    kFoldComp0<<<1, 256, 0, st>>>(w->partials, (int)(gc.x * gc.y * gc.z), w->scal + 2, w->scalHostDev + 2);
+  } else {
+    const dim3 g2 = sumGrid(N.g);
+    kSumSq2<<<g2, dim3(WTX, WTY), 0, st>>>(N.b, N.b, N.g, w->partials);
+    kFoldComp0<<<1, 256, 0, st>>>(w->partials, (int)(g2.x * g2.y * g2.z), w->scal + 2, w->scalHostDev + 2);
+  }
 }
 
 float *warmstartResidualBuffer(WarmStart *w) { return w->lv[0].r; }

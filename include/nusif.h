@@ -124,7 +124,10 @@ double nusifSolverRelResidual(Solver *s);
  * previous-step pressure, -1 no network. */
 int nusifWarmstartUsed(Solver *s);
 /* Residual norms of the last warm start: fine ||r0|| (previous-step p),
- * fine ||r_guess||, and both restricted to the acceptance level
+ * fine ||r_guess|| (-1 in mixed precision with the network below the finest
+ * grid, whose guess is judged on the coarse grids: ||R r0 - A_n d|| against
+ * ||R r0||, A_n the network grid's operator), and both restricted to the
+ * acceptance level
  * (unet_accept_level below the network grid; the acceptance test compares
  * the restricted pair). */
 void nusifWarmstartResiduals(Solver *s, double *out4);

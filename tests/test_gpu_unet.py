@@ -101,8 +101,10 @@ def test_gpu_warmstart_residual_report(n, gpu_ok):
     ||rhs - A p|| of the previous-step pressure, recomputed here with numpy
     from the same p (Neumann face ghosts, as the solve sets them) and rhs;
     the decision taken is the comparison of the restricted pair [3] < [2].
-    64^3: the network runs on the finest grid, 128^3: on the first coarse
-    level (input averaged onto it in the residual pass)."""
+    64^3: the network runs on the finest grid and [1] is the guess's fine
+    residual; 128^3: on the first coarse level (input averaged onto it in the
+    residual pass, which is also the mixed solve's first defect), the guess is
+    judged on the coarse grids alone and [1] is -1 (not formed)."""
     weights = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "cfd-bench_amd",
                            "weights", "unet_cavity.bin")
     prm = dict(name="dcavity", re=1000.0, imax=n, jmax=n, kmax=n, tau=0.5, te=1e9, levels=5,
@@ -134,7 +136,7 @@ def test_gpu_warmstart_residual_report(n, gpu_ok):
     print("n %d: fine r0 %.6e (numpy %.6e), fine r_guess %.6e, restricted %.6e / %.6e, used %d"
           % (n, out[0], r0, out[1], out[2], out[3], used))
     assert out[0] == pytest.approx(r0, rel=1e-9)
-    assert out[1] > 0.0 and out[2] > 0.0 and out[3] > 0.0
+    assert (out[1] > 0.0 if n == 64 else out[1] == -1.0) and out[2] > 0.0 and out[3] > 0.0
     assert used == (1 if out[3] < out[2] else 0)
     c.close()
 
