@@ -99,6 +99,7 @@ struct GmgState {
   int nextGroup;  /* mg_inner 0: the next solve's first group (starts at 1) */
   int lastGroup;  /* the last solve's first group */
   int groupCap;   /* mixed, mg_inner 0: longest useful first group (0: none) */
+  int groupHeld;  /* the last solve needed no cycle after its first group */
   CycleCtl *dCtl; /* device cycle control */
   CycleCtl *hRing;/* pinned copies of dCtl after each enqueued cycle */
   CycleCtl *hRingDev; /* their device address (written by kFoldCycle) */
@@ -1595,6 +1596,7 @@ GmgState *gmgCreate(NusifSolverCtx *sc, const Parameter *prm)
   G->nextGroup = 1;
   G->lastGroup = 0;
   G->groupCap  = 0;
+  G->groupHeld = 1;
   G->predicted = 1;
   HIP_CHECK(hipMalloc((void **)&G->dCtl, sizeof(CycleCtl)));
   HIP_CHECK(hipHostMalloc((void **)&G->hRing, CTL_RING * sizeof(CycleCtl), hipHostMallocCoherent));
@@ -1912,10 +1914,12 @@ static int runCycles(NusifCtx *c, GmgState *G, NusifSolverCtx *sc, double &rel, 
       });
       cycles += m;
       double rn;
-      if (sc->afterFirst && sc->afterFirstState == 0) {
+      if (sc->afterFirst && sc->afterFirstState == 0 && G->groupHeld) {
         /* the group's p is in c->p (the bodies keep it there): queue the
          * caller's work, then wait for the group's fold alone (it writes the
-         * norms into the pinned mirror), not for that work */
+         * norms into the pinned mirror), not for that work. Only while the
+         * solves finish within their first group: after one that did not,
+         * the work would likely have to run again after the later cycles. */
         HIP_CHECK(hipEventRecord(G->ev[0], st));
         sc->afterFirst(sc->afterFirstArg);
         sc->afterFirstState = 1;
@@ -1930,6 +1934,7 @@ static int runCycles(NusifCtx *c, GmgState *G, NusifSolverCtx *sc, double &rel, 
       if (cycles == m) relFirst = rel;
     }
     if (G->inner == 0 && cycles > 0) G->nextGroup = nextGroupAfter(G, G->lastGroup, cycles, rel0, relFirst, rel);
+    G->groupHeld = cycles <= G->lastGroup ? 1 : 0;
     return cycles;
   }
   if (!(nb > 0.0 && rel > G->tol && G->maxCycles > 0)) return 0;
