@@ -266,12 +266,14 @@ def channel_params():
 
 def run_channel(lib, warmup=10, steps=40):
     """The channel at scale: the learned warm start (channel-tuned network)
-    with FP32 V-cycles in FP64 defect correction against plain FP64 multigrid,
-    both to 1e-6 relative residual per step, from the same start."""
+    with FP32 V-cycles in FP64 defect correction against plain FP64 multigrid
+    and against the same FP32 solver without the network, each to 1e-6
+    relative residual per step, from the same start."""
     base = {"solver": "gmg", "mg_tol": 1e-6, "mg_project": 1, "u_inflow": 1.0, "inflow_noise": 0.01, "seed": 7,
             "mg_omega": 1.3, "mg_inner": 0}
     weights = os.path.join(ROOT, "cfd-bench_amd", "weights", "unet_channel.bin")
     modes = {"fp64": dict(base, precision="fp64"),
+             "mixed": dict(base, precision="mixed"),
              "learned": dict(base, precision="mixed", warmstart="unet", unet_weights=weights)}
     out = {}
     cells = 768.0 * 192 * 192
@@ -297,6 +299,7 @@ def run_channel(lib, warmup=10, steps=40):
                                                            / max(tried.value - tried0.value, 1))
         log("[bench] channel 768x192x192 %s: %d steps in %.3f s" % (name, steps, dt))
     out["speedup_learned_vs_fp64_mg"] = out["learned"]["value"] / out["fp64"]["value"]
+    out["speedup_mixed_vs_fp64_mg"] = out["mixed"]["value"] / out["fp64"]["value"]
     out["unit"] = "time steps/s"
     out["workload"] = ("channel 768x192x192, Re 500, inflow u=1 with seeded 1%% noise, pressure to 1e-6 relative "
                        "residual per step, V(2,2) 6 levels, omega 1.3; %d steps after %d" % (steps, warmup))
