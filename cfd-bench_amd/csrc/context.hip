@@ -239,6 +239,7 @@ extern "C" int nusifSetOption(Parameter *p, const char *key, const char *value)
     return r;
   }
   if (k == "unet_extent") return parseInt(v.c_str(), &o->unetExtent);
+  if (k == "unet_graph") return parseInt(v.c_str(), &o->unetGraph);
   if (k == "unet_accept_level") {
     int r = parseInt(v.c_str(), &o->unetAcceptLevel);
     if (r == 0 && o->unetAcceptLevel < 0) return -2;
@@ -404,6 +405,7 @@ extern "C" int nusifSyncHostIfOwned(const double *hostPtr)
 }
 
 extern "C" int nusifStepLimit(Parameter *p) { return nusifOptionsFor(p)->nsteps; }
+extern "C" int nusifVerboseLevel(Parameter *p) { return nusifOptionsFor(p)->verbose; }
 
 /* Algorithmic HBM bytes per launch: every array element the kernel must
  * touch, read once and/or written once, on the finest grid (N interior

@@ -121,6 +121,7 @@ struct NusifOptions {
   int warmstart;
   char unetWeights[512];
   int unetExtent;  /* warm start: the network runs on the first level of extent <= this (64) */
+  int unetGraph;       /* warm start: replay the network's forward from a captured HIP graph */
   int unetAcceptLevel; /* warm start: the acceptance test compares residuals restricted this
                         * many levels below the network grid (2; 0: on it) */
   int fgRhs;       /* computeFG also forms rhs (1, default), computeRHS then skips its pass */

@@ -2,7 +2,10 @@
  * the reference driver's semantics (reference src/main.c:18-154): the same
  * call order per step, residual.dat trace, cell-centred VTK output.
  * Added: the optional .par key `nsteps` stops the loop after that many steps
- * (the reference loop only ends at t > te). */
+ * (the reference loop only ends at t > te), and a closing line with steps/s,
+ * MLUP/s and the solver's iteration total. With `verbose` 1 or 2 (1 unless
+ * set) the loop prints the reference's -DVERBOSE step lines
+ * (main.c:80-83), otherwise its progress bar (main.c:49-51, 84-86). */
 #include <stdio.h>
 #include <stdlib.h>
 
@@ -41,7 +44,8 @@ int main(int argc, char **argv)
   char dev[256];
   nusifDeviceName(dev, sizeof dev);
   printf("Device: %s\n", dev);
-  initProgress(d.te);
+  const int verbose = nusifVerboseLevel(&p);
+  if (!verbose) initProgress(d.te);
 
   double t = 0.0, res = 0.0;
   int nt   = 0;
@@ -60,11 +64,12 @@ int main(int argc, char **argv)
     writeResidual(fp, t, res);
     t += d.dt;
     nt++;
-    printProgress(t);
+    if (verbose) printf("TIME %f , TIMESTEP %f\n", t, d.dt);
+    else printProgress(t);
   }
   nusifDeviceSynchronize();
   double t1 = getTimeStamp();
-  stopProgress();
+  if (!verbose) stopProgress();
   double cells = (double)d.grid.imax * d.grid.jmax * d.grid.kmax;
   printf("Solution took %.2fs\n", t1 - t0);
   printf("Steps: %d  steps/s: %.3f  MLUP/s (step): %.2f  solver iterations: %lld\n", nt,

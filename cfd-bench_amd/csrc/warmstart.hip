@@ -49,6 +49,8 @@ struct WarmStart {
   int nx, ny, nz;         /* network grid extents */
   double h2;              /* output scale (l / pi)^2, l the domain's longest side */
   double cn[3];           /* 1 / h^2 per direction on the network grid */
+  bool useGraph;          /* replay the forward + unpack from a captured HIP graph */
+  hipGraphExec_t fwdExec; /* that graph once captured (null before) */
   WsLevel lv[12];
   uint16_t *x;            /* network input, nx ny nz voxels x 8 channels */
   double *pPrev;          /* FP64 copy of p for capture (capture only) */
@@ -316,6 +318,8 @@ WarmStart *warmstartCreate(NusifCtx *c, UnetModel *net, bool capture, int netLev
   WarmStart *w = new WarmStart();
   w->net       = net;
   w->capture   = capture;
+  w->useGraph  = c->opt.unetGraph != 0;
+  w->fwdExec   = nullptr;
   const int div = 1 << (netLevels > 0 ? netLevels - 1 : 0);
   w->Ln        = networkLevel(c->geo, c->opt.unetExtent > 0 ? c->opt.unetExtent : 64, div);
   Geo g = c->geo;
@@ -380,6 +384,7 @@ void warmstartFree(WarmStart *w)
 {
   if (!w) return;
   HIP_CHECK(hipStreamSynchronize(nusifStream()));
+  if (w->fwdExec) HIP_CHECK(hipGraphExecDestroy(w->fwdExec));
   for (int l = 0; l <= w->Lc; l++) {
     HIP_CHECK(hipFree(w->lv[l].r));
     if (w->lv[l].b) HIP_CHECK(hipFree(w->lv[l].b));
@@ -402,7 +407,15 @@ void warmstartExtents(const WarmStart *w, int *nx, int *ny, int *nz)
   *ny = w->ny;
   *nz = w->nz;
 }
-void warmstartSetNet(WarmStart *w, UnetModel *net) { w->net = net; }
+void warmstartSetNet(WarmStart *w, UnetModel *net)
+{
+  if (w->fwdExec) { /* captured with the previous network's buffers */
+    HIP_CHECK(hipStreamSynchronize(nusifStream()));
+    HIP_CHECK(hipGraphExecDestroy(w->fwdExec));
+    w->fwdExec = nullptr;
+  }
+  w->net = net;
+}
 UnetModel *warmstartNet(WarmStart *w) { return w->net; }
 bool warmstartCapturing(const WarmStart *w) { return w->capture; }
 
@@ -473,12 +486,35 @@ const float *warmstartCorrection(WarmStart *w)
   return w->lv[0].d;
 }
 
-/* forward and unpack: the correction d on the network grid (lv[Ln].d) */
+/* forward and unpack: the correction d on the network grid (lv[Ln].d).
+ * With unet_graph 1 the ~40 launches (convolutions, split-K finishes,
+ * pooling, head, unpack) are captured once and replayed with one
+ * hipGraphLaunch: they always run on the same buffers (input, weights,
+ * activations, scratch, output), and the scale they read comes from device
+ * memory at run time. Kernels only: no memset or copy is captured. Eager
+ * while kernel timers are on (their events bracket single launches). */
+extern bool nusifTimersActive();
 void warmstartForward(WarmStart *w)
 {
-  NUSIF_TIMED("unet_forward", unetForward(w->net, w->x));
   const WsLevel &L = w->lv[w->Ln];
-  kUnpack<<<1024, 256, 0, nusifStream()>>>(unetOutput(w->net), L.g, w->scal, w->h2, L.d);
+  hipStream_t st   = nusifStream();
+  auto body        = [&]() {
+    NUSIF_TIMED("unet_forward", unetForward(w->net, w->x));
+    kUnpack<<<1024, 256, 0, st>>>(unetOutput(w->net), L.g, w->scal, w->h2, L.d);
+  };
+  if (!w->useGraph || nusifTimersActive()) {
+    body();
+    return;
+  }
+  if (!w->fwdExec) {
+    hipGraph_t graph;
+    HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    body();
+    HIP_CHECK(hipStreamEndCapture(st, &graph));
+    HIP_CHECK(hipGraphInstantiate(&w->fwdExec, graph, nullptr, nullptr, 0));
+    HIP_CHECK(hipGraphDestroy(graph));
+  }
+  HIP_CHECK(hipGraphLaunch(w->fwdExec, st));
 }
 
 /* d prolongated trilinearly from the network grid to the finest, written

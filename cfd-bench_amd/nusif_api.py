@@ -122,6 +122,7 @@ def _declare_optional(lib):
         "nusifFreeDiscretization": ([C.POINTER(Discretization)], None),
         "nusifFreeSolver": ([C.POINTER(Solver)], None),
         "nusifSetOption": ([C.POINTER(Parameter), C.c_char_p, C.c_char_p], C.c_int),
+        "nusifVerboseLevel": ([C.POINTER(Parameter)], C.c_int),
         "nusifSolverIterations": ([C.POINTER(Solver)], C.c_int),
         "nusifSolverFirstGroup": ([C.POINTER(Solver)], C.c_int),
         "nusifSolverRelResidual": ([C.POINTER(Solver)], C.c_double),

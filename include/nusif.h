@@ -78,6 +78,9 @@ void nusifFreeSolver(Solver *s);
  *                         extents are all <= this (64), or the last that its
  *                         pooling levels still divide; a weights file that
  *                         records its training extent overrides it
+ *   unet_graph    0 | 1   warm start: the network's forward (~40 launches) is
+ *                         replayed from a HIP graph captured at its first
+ *                         use (1) or launched kernel by kernel (0); same values
  *   unet_accept_level int the guess is taken when its residual, restricted
  *                         this many levels below the network grid (2; 0: on
  *                         it), is smaller than the previous pressure's: the
@@ -111,6 +114,8 @@ int nusifSetOption(Parameter *p, const char *key, const char *value);
 
 /* The `nsteps` option of `p` (0 when unset). */
 int nusifStepLimit(Parameter *p);
+/* The `verbose` option of `p` (1 when unset, as the reference's build). */
+int nusifVerboseLevel(Parameter *p);
 
 /* Iterations (rb), sweeps (mg) or V-cycles (gmg) of the last solve(). */
 int nusifSolverIterations(Solver *s);

@@ -222,3 +222,35 @@ def test_gpu_forced_fallback_equals_solve_without_network(precision, n, gpu_ok, 
         np.testing.assert_array_equal(fa[k], fb[k], err_msg=k)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("n", [128, 64])
+def test_gpu_unet_graph_replay_equals_eager(n, gpu_ok):
+    """The network's forward replayed from its captured HIP graph
+    (unet_graph 1) against the same launches one by one: ten warm-started
+    steps (the graph replayed on each, with the step's eager kernels and host
+    read-backs between replays), every field and the trace bit for bit.
+    128^3: the mixed-precision path (network on 64^3); 64^3: the network on
+    the finest grid."""
+    weights = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "cfd-bench_amd",
+                           "weights", "unet_cavity.bin")
+    prm = dict(name="dcavity", re=1000.0, imax=n, jmax=n, kmax=n, tau=0.5, te=1e9, levels=5,
+               presmooth=2, postsmooth=2)
+    out = []
+    for graph in (0, 1):
+        opts = {"solver": "gmg", "precision": "mixed", "mg_tol": 1e-6, "mg_omega": 1.3, "mg_inner": 0,
+                "warmstart": "unet", "unet_weights": weights, "unet_graph": graph}
+        c = A.Case(lib("gpu_rb"), options=opts, **prm)
+        tr = (C.c_double * 40)()
+        assert c.lib.nusifRunSteps(C.byref(c.d), C.byref(c.s), 6, tr) == 6
+        for _ in range(4):
+            c.step()
+        tried, taken = C.c_longlong(), C.c_longlong()
+        c.lib.nusifWarmstartCounts(C.byref(c.s), C.byref(tried), C.byref(taken))
+        out.append((np.array(tr[:24]), c.trace, c.fields(), taken.value))
+        c.close()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1] and out[0][3] == out[1][3]
+    print("guesses taken", out[1][3], "of 10")
+    for k in A.FIELDS:
+        np.testing.assert_array_equal(out[0][2][k], out[1][2][k], err_msg=k)

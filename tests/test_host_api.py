@@ -252,7 +252,8 @@ def test_set_option_accepts_documented_keys():
              "mg_graph": "0", "mg_tile_rows": "8", "mg_inner": "0", "mg_speculate": "0",
              "lid": "1.5", "u_inflow": "1.0", "inflow_noise": "0.01", "seed": "3",
              "warmstart": "none", "unet_weights": "/tmp/w.bin", "unet_extent": "32", "fg_rhs": "0",
-             "rb_fused": "0", "mg_graph_cache": "4", "verbose": "2", "unet_accept_level": "1"}
+             "rb_fused": "0", "mg_graph_cache": "4", "verbose": "2", "unet_accept_level": "1",
+             "unet_graph": "1"}
     assert keys == set(valid), sorted(keys ^ set(valid))
     prm = A.Parameter()
     lib.initParameter(C.byref(prm))
