@@ -195,7 +195,10 @@ __device__ inline void ldsBarrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_ba
  * redundantly, so no workgroup reads another's new values. Every cell gets
  * exactly the values of the two-pass sweep (same operands, same order); face
  * ghosts are written with the new values. X and Y rotate over 4 plane slots:
- * loads of plane k+2 never touch a slot still being read.
+ * loads of plane k+2 never touch a slot still being read. Each thread reads
+ * the b of its black cell and of that cell's red partner (the column pair
+ * 2p, 2p+1) in one load and writes the pair's two new values in one store;
+ * the red ring cells of the tile are a second, partial round of tasks.
  *
  * LDS holds every plane split by the parity of (i + j): half h, row lj,
  * position p = li / 2 of the local column li (tile origin minus 2). On plane
@@ -222,7 +225,6 @@ struct SweepRBShape {
    * in disjoint banks */
   static constexpr int HALF = XH * RW + ((16 - (XH * RW) % 32) + 32) % 32;
   static constexpr int NRT  = RW * (TJ + 2);  /* red tasks per plane */
-  static constexpr int NR   = (NRT + NT - 1) / NT;
   static constexpr int YS   = NRT;
 };
 
@@ -280,6 +282,42 @@ __device__ inline void stPS(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned so,
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, so, 0);
 }
 
+/* two adjacent cells in one access (multi-dword buffer accesses need only
+ * dword alignment): b under a black thread's column pair, and the pair's two
+ * new values */
+__device__ inline void ldPair(float *, __amdgpu_buffer_rsrc_t r, unsigned off, unsigned so, float &a, float &b)
+{
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, so, 0);
+  /* through plain scalars: a bit cast of a vector element itself reads
+   * element 0 with this compiler */
+  const unsigned e0 = v[0], e1 = v[1];
+  a = __builtin_bit_cast(float, e0);
+  b = __builtin_bit_cast(float, e1);
+}
+__device__ inline void ldPair(double *, __amdgpu_buffer_rsrc_t r, unsigned off, unsigned so, double &a, double &b)
+{
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, so, 0);
+  a = __builtin_bit_cast(double, (unsigned long long)v[0] | ((unsigned long long)v[1] << 32));
+  b = __builtin_bit_cast(double, (unsigned long long)v[2] | ((unsigned long long)v[3] << 32));
+}
+__device__ inline void stPair(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned so, float a, float b)
+{
+  decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)) v;
+  v[0] = __builtin_bit_cast(unsigned, a);
+  v[1] = __builtin_bit_cast(unsigned, b);
+  __builtin_amdgcn_raw_buffer_store_b64(v, r, off, so, 0);
+}
+__device__ inline void stPair(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned so, double a, double b)
+{
+  decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0, 0, 0)) v;
+  const unsigned long long ua = __builtin_bit_cast(unsigned long long, a), ub = __builtin_bit_cast(unsigned long long, b);
+  v[0] = (unsigned)ua;
+  v[1] = (unsigned)(ua >> 32);
+  v[2] = (unsigned)ub;
+  v[3] = (unsigned)(ub >> 32);
+  __builtin_amdgcn_raw_buffer_store_b128(v, r, off, so, 0);
+}
+
 /* RES (the reference's red-black SOR iteration, src/solver-rb.c:55-148): the
  * sweep is iteration q of a batch; it returns at once when q >= *stopAt (the
  * device-side stop test) and adds the in-sweep r^2 of the cells its tile owns
@@ -299,10 +337,14 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
   if (stop && *stop) return;
   if (RES && rs.q >= *rs.stopAt) return;
   using S = SweepRBShape<T, TJ>;
-  constexpr int NT = S::NT, XW = S::XW, RW = S::RW, HALF = S::HALF, NX = S::NX, NR = S::NR;
+  constexpr int NT = S::NT, XW = S::XW, RW = S::RW, HALF = S::HALF, NX = S::NX;
+  /* red tasks: round 0 the red cell of this thread's black pair, round 1 the
+   * ring (rows 1 and TJ+2, and positions 0 and RW-1 of the rows between) */
+  constexpr int NR = 2, NRING = 2 * RW + 2 * TJ;
+  static_assert(NRING <= NT, "ring tasks fit one round");
   __shared__ T X[4][2 * HALF];
   __shared__ T Y[4][S::YS];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, wave0 = tid & ~63;
   const int gxT = (g.ni + FT_I - 1) / FT_I, gyT = (g.nj + TJ - 1) / TJ;
   const int nTiles = gxT * gyT * ((g.nk + kc - 1) / kc);
   const int perXcd = (nTiles + NUM_XCD - 1) / NUM_XCD;
@@ -327,28 +369,49 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
     xOff[q] = (unsigned)(min(max(jb + lj, 0), nj + 1) * sj + min(max(ib + li, 0), ni + 1)) * ES;
     xLds[q] = ((li + lj) & 1) * HALF + lj * RW + (li >> 1);
   }
-  /* red tasks: position p, row lj in 1..TJ+2; the column is 2p + s with
-   * s = (k + 1 + lj) & 1 on plane k. rIn bit s: that cell is interior in (i, j). */
-  int rP[NR], rLj[NR], rIn[NR];
-  unsigned rOff[NR]; /* byte offset of column 2p within a plane */
-#pragma unroll
-  for (int r = 0; r < NR; r++) {
-    int task = min(tid + NT * r, S::NRT - 1);
-    int p = task % RW, lj = 1 + task / RW;
-    int j = jb + lj, i = ib + 2 * p;
-    rP[r] = p; rLj[r] = lj;
-    rOff[r] = (unsigned)(min(max(j, 0), nj + 1) * sj + min(max(i, 0), ni)) * ES;
-    bool jin = j >= 1 && j <= nj;
-    rIn[r] = (jin && i >= 1 && i <= ni ? 1 : 0) | (jin && i + 1 >= 1 && i + 1 <= ni ? 2 : 0);
-  }
   /* black task: position p in 1..32, row lj in 2..TJ+1; black column 2p + s,
-   * s = (k + lj) & 1, its red partner 2p + 1 - s. */
+   * s = (k + lj) & 1, its red partner 2p + 1 - s. The pair's b comes in one
+   * load and serves both. */
   const int bp = 1 + (tid & 31), blj = 2 + (tid >> 5);
   const int bj = jb + blj, bi = ib + 2 * bp; /* global i of column 2p */
   const unsigned bRow = (unsigned)(min(bj, nj + 1) * sj + min(bi, ni)) * ES; /* bytes */
   const unsigned sjB  = (unsigned)sj * ES;
   const bool bjIn = bj <= nj;
   const bool bIn0 = bjIn && bi <= ni, bIn1 = bjIn && bi + 1 <= ni;
+  /* workgroup-uniform: the tile holds cells next to an i or j face, whose
+   * Neumann ghosts its black pass writes */
+  const bool faceTile = i0 == 1 || i0 + FT_I > ni || j0 == 1 || j0 + TJ > nj;
+  /* red tasks (position p, row lj; the column is 2p + s with s = (k + 1 + lj)
+   * & 1 on plane k; rIn bit s: that cell is interior in (i, j)). Round 0 is
+   * (bp, blj); round 1 ring task q: row 1 (q < RW), row TJ+2 (q < 2 RW), then
+   * positions 0 and RW-1 of rows 2..TJ+1. */
+  int rP[NR], rLj[NR], rIn[NR];
+  unsigned rOff[NR]; /* byte offset of column 2p within a plane */
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    int p, lj;
+    if (r == 0) {
+      p  = bp;
+      lj = blj;
+    } else {
+      const int q = min(tid, NRING - 1);
+      if (q < RW) {
+        p  = q;
+        lj = 1;
+      } else if (q < 2 * RW) {
+        p  = q - RW;
+        lj = TJ + 2;
+      } else {
+        p  = ((q - 2 * RW) & 1) ? RW - 1 : 0;
+        lj = 2 + ((q - 2 * RW) >> 1);
+      }
+    }
+    int j = jb + lj, i = ib + 2 * p;
+    rP[r] = p; rLj[r] = lj;
+    rOff[r] = (unsigned)(min(max(j, 0), nj + 1) * sj + min(max(i, 0), ni)) * ES;
+    bool jin = j >= 1 && j <= nj;
+    rIn[r] = (jin && i >= 1 && i <= ni ? 1 : 0) | (jin && i + 1 >= 1 && i + 1 <= ni ? 2 : 0);
+  }
   double acc = 0.0; /* RES: this thread's share of the sum of r^2 */
   /* one descriptor per array, based at the lowest plane this workgroup
    * touches (k0 - 2, clamped); plane k is the scalar offset (k - kBase)
@@ -369,7 +432,8 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
     }
     const unsigned so = soff(min(max(k, 0), nk + 1));
 #pragma unroll
-    for (int q = 0; q < NX; q++) xr[q] = ldPS((T *)nullptr, rXo, xOff[q], so);
+    for (int q = 0; q < NX; q++)
+      xr[q] = q < NX - 1 || wave0 + NT * q < S::NXE ? ldPS((T *)nullptr, rXo, xOff[q], so) : (T)0;
   };
   auto storeX = [&](int k) {
     if (ZX) return;
@@ -377,11 +441,13 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
     for (int q = 0; q < NX; q++)
       if (q < NX - 1 || tid + NT * q < S::NXE) X[k & 3][xLds[q]] = xr[q];
   };
-  auto fetchBR = [&](int k, T *out) { /* b under this thread's red cells of plane k */
-    const unsigned so = soff(min(max(k, 1), nk));
+  auto fetchPair = [&](int k, T &pa, T &pb) { /* b under this thread's pair of plane k */
+    ldPair((T *)nullptr, rB, bRow, soff(min(max(k, 1), nk)), pa, pb);
+  };
+  auto fetchRing = [&](int k) { /* b under this thread's ring task of plane k */
+    if (wave0 >= NRING) return (T)0;
     const int s0 = (k + 1) & 1;
-#pragma unroll
-    for (int r = 0; r < NR; r++) out[r] = ldPS((T *)nullptr, rB, rOff[r] + ((s0 + rLj[r]) & 1) * ES, so);
+    return ldPS((T *)nullptr, rB, rOff[1] + ((s0 + rLj[1]) & 1) * ES, soff(min(max(k, 1), nk)));
   };
   auto redPlane = [&](int k, const T *bred) {
     const int hr = (k + 1) & 1;
@@ -391,7 +457,7 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
     const bool kin = k >= 1 && k <= nk;
 #pragma unroll
     for (int r = 0; r < NR; r++) {
-      if (r < NR - 1 || tid + NT * r < S::NRT) {
+      if (r == 0 || tid < NRING) {
         const int s = (hr + rLj[r]) & 1;
         const int a = rLj[r] * RW + rP[r];
         T v, lap;
@@ -433,56 +499,54 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
     T v0 = s ? vR : vB, v1 = s ? vB : vR;
     const unsigned so = soff(k);
     const unsigned c0 = bRow, c1 = bRow + ES;
-    if (bIn0) {
-      stPS(rXn, c0, so, v0);
-      if (bi == 1) stPS(rXn, c0 - ES, so, v0);
-      if (bi == ni) stPS(rXn, c1, so, v0);
-      if (bj == 1) stPS(rXn, c0 - sjB, so, v0);
-      if (bj == nj) stPS(rXn, c0 + sjB, so, v0);
-    }
-    if (bIn1) {
-      stPS(rXn, c1, so, v1);
-      if (bi + 1 == ni) stPS(rXn, c1 + ES, so, v1);
-      if (bj == 1) stPS(rXn, c1 - sjB, so, v1);
-      if (bj == nj) stPS(rXn, c1 + sjB, so, v1);
-    }
-    if (k == 1 || k == nk) { /* the k-face ghost planes (k == 1: kBase is 0) */
-      if (k == 1) {
-        if (bIn0) stPS(rXn, c0, soff(0), v0);
-        if (bIn1) stPS(rXn, c1, soff(0), v1);
+    auto put = [&](unsigned sp) {
+      if (bIn0 && bIn1) {
+        stPair(rXn, c0, sp, v0, v1);
+      } else {
+        if (bIn0) stPS(rXn, c0, sp, v0);
+        if (bIn1) stPS(rXn, c1, sp, v1);
       }
-      if (k == nk) {
-        const unsigned sg = soff(nk + 1);
-        if (bIn0) stPS(rXn, c0, sg, v0);
-        if (bIn1) stPS(rXn, c1, sg, v1);
+    };
+    put(so);
+    if (faceTile) { /* Neumann ghosts of the i and j faces */
+      if (bIn0) {
+        if (bi == 1) stPS(rXn, c0 - ES, so, v0);
+        if (bi == ni) stPS(rXn, c1, so, v0);
+        if (bj == 1) stPS(rXn, c0 - sjB, so, v0);
+        if (bj == nj) stPS(rXn, c0 + sjB, so, v0);
+      }
+      if (bIn1) {
+        if (bi + 1 == ni) stPS(rXn, c1 + ES, so, v1);
+        if (bj == 1) stPS(rXn, c1 - sjB, so, v1);
+        if (bj == nj) stPS(rXn, c1 + sjB, so, v1);
       }
     }
+    if (k == 1) put(soff(0)); /* the k-face ghost planes (k == 1: kBase is 0) */
+    if (k == nk) put(soff(nk + 1));
   };
-  auto fetchBB = [&](int k) {
-    int kk = min(max(k, 1), nk);
-    return ldPS((T *)nullptr, rB, bRow + ((kk + blj) & 1) * ES, soff(kk));
-  };
-
-  /* one plane step with register set (xs, bs, bbs): store plane kk+1 to LDS,
-   * issue the loads of plane kk+3 (x), kk+2 (red b) and kk+1 (black b), then
-   * red(kk) and black(kk-1). Loads for planes past the chunk's last use (x:
-   * k1+1, red b: k1, black b: k1-1) are skipped. */
-  auto iterate = [&](int kk, T(&xs)[NX], T(&bs)[NR], T &bbs) {
+  /* one plane step with register set (xs, pa/pb, rg): store plane kk+1 to
+   * LDS, issue the loads of plane kk+3 (x) and kk+2 (b of the pair and the
+   * ring task), then red(kk) and black(kk-1). The pair of plane kk gives
+   * red(kk) its b now and black(kk) its b at the next step (bbCarry). Loads
+   * for planes past the chunk's last use (x: k1+1, b: k1) are skipped. */
+  auto iterate = [&](int kk, T(&xs)[NX], T &pa, T &pb, T &rg, T &bbCarry) {
 #pragma unroll
     for (int q = 0; q < NX; q++) xr[q] = xs[q];
     storeX(kk + 1);
+    const bool sR = ((kk + 1 + blj) & 1) != 0; /* the red cell is column 2p + 1 */
     T bUse[NR];
-#pragma unroll
-    for (int r = 0; r < NR; r++) bUse[r] = bs[r];
-    T bbUse = bbs;
+    bUse[0]    = sR ? pb : pa;
+    bUse[1]    = rg;
+    T bbUse    = bbCarry;
+    bbCarry    = sR ? pa : pb;
     if (kk + 3 <= k1 + 1) {
       fetchX(kk + 3);
 #pragma unroll
       for (int q = 0; q < NX; q++) xs[q] = xr[q];
     }
     if (kk + 2 <= k1) {
-      fetchBR(kk + 2, bs);
-      bbs = fetchBB(kk + 1);
+      fetchPair(kk + 2, pa, pb);
+      rg = fetchRing(kk + 2);
     }
     ldsBarrier();
     redPlane(kk, bUse);
@@ -494,19 +558,21 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
   storeX(k0 - 2 + 4);
   fetchX(k0 - 1);
   storeX(k0 - 1);
-  T xA[NX], xB[NX], bA[NR], bB[NR], bbA = (T)0, bbB = (T)0;
+  T xA[NX], xB[NX], paA, pbA, paB, pbB, rgA, rgB, bbC = (T)0;
   fetchX(k0);
 #pragma unroll
   for (int q = 0; q < NX; q++) xA[q] = xr[q];
   fetchX(k0 + 1);
 #pragma unroll
   for (int q = 0; q < NX; q++) xB[q] = xr[q];
-  fetchBR(k0 - 1, bA);
-  fetchBR(k0, bB);
-  iterate(k0 - 1, xA, bA, bbA);
+  fetchPair(k0 - 1, paA, pbA);
+  rgA = fetchRing(k0 - 1);
+  fetchPair(k0, paB, pbB);
+  rgB = fetchRing(k0);
+  iterate(k0 - 1, xA, paA, pbA, rgA, bbC);
   for (int kk = k0; kk <= k1; kk += 2) {
-    iterate(kk, xB, bB, bbB);
-    if (kk + 1 <= k1) iterate(kk + 1, xA, bA, bbA);
+    iterate(kk, xB, paB, pbB, rgB, bbC);
+    if (kk + 1 <= k1) iterate(kk + 1, xA, paA, pbA, rgA, bbC);
   }
   if constexpr (RES) {
     __shared__ double lsum[NT / 64];
