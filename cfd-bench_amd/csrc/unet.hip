@@ -480,6 +480,12 @@ static void launchConvLds(UnetModel *m, const ConvArgs<uint16_t, uint16_t> &a)
   const int gb = (cin % CL_CC) / 8; /* groups of the partial chunk */
   auto launch = [&](auto nbC, auto gbC) {
     constexpr int NB = decltype(nbC)::value, GB = decltype(gbC)::value;
+    if constexpr (GB > 0) {
+      if (cin < CL_CC) {
+        NUSIF_TIMED("unet_conv", (kConvLds<uint16_t, NB, GB, false><<<grid(NB), 256, 0, st>>>(a, m->split)));
+        return;
+      }
+    }
     NUSIF_TIMED("unet_conv", (kConvLds<uint16_t, NB, GB><<<grid(NB), 256, 0, st>>>(a, m->split)));
   };
   auto withGb = [&](auto nbC) {

@@ -174,7 +174,7 @@ __global__ __launch_bounds__(256) void kConvMfma(ConvArgs<InT, OutT> a)
  * brick's halo into LDS, then K-steps of four (tap, group) pairs. G8 is a
  * compile-time constant so the (tap, group) split of the K index and the
  * tap's 3-D offset are shifts and constant multiplies, not divisions. */
-template <int NB, int G8>
+template <int NB, int G8, int CSTR = CL_CSTR>
 __device__ inline void convChunk(const ConvArgs<uint16_t, uint16_t> &a, uint16_t *sIn, int cc0, int cin, int rows,
     int row0, int x0, int y0, int z0, int wave, int lr, int lg, floatx4 (&acc)[NB][4])
 {
@@ -242,7 +242,7 @@ __device__ inline void convChunk(const ConvArgs<uint16_t, uint16_t> &a, uint16_t
       const int u = tid + 256 * (r0 + r);
       if (r0 + r < NST && u < CL_HV * G8) {
         int v = u / G8, q = u - v * G8;
-        *(uint4 *)&sIn[v * CL_CSTR + q * 8] = val[r];
+        *(uint4 *)&sIn[v * CSTR + q * 8] = val[r];
       }
     }
   }
@@ -260,7 +260,7 @@ __device__ inline void convChunk(const ConvArgs<uint16_t, uint16_t> &a, uint16_t
     if (it + D < NIT) loadA(4 * (it + D), ring[it % D]);
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) {
-      uint4 raw = *(const uint4 *)&sIn[(hvBase[mb] + off) * CL_CSTR + c8 * 8];
+      uint4 raw = *(const uint4 *)&sIn[(hvBase[mb] + off) * CSTR + c8 * 8];
       fb[mb]    = gv ? __builtin_bit_cast(bf16x8, raw) : zero8();
     }
 #pragma unroll
@@ -276,22 +276,27 @@ __device__ inline void convChunk(const ConvArgs<uint16_t, uint16_t> &a, uint16_t
  * every chunk is full). Full chunks take four groups. Only the chunk shapes a
  * layer has are instantiated, which keeps the register count (and so the
  * number of workgroups per CU) at what one chunk loop needs. */
-template <int NB, int GB>
+template <int NB, int GB, int CSTR = CL_CSTR>
 __device__ inline void convTail(const ConvArgs<uint16_t, uint16_t> &a, uint16_t *sIn, int cc0, int cin, int rows,
     int row0, int x0, int y0, int z0, int wave, int lr, int lg, floatx4 (&acc)[NB][4])
 {
   if constexpr (GB == 3) { /* 24 channels: as groups of 2 and 1 */
-    convChunk<NB, 2>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
-    convChunk<NB, 1>(a, sIn, cc0 + 16, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
+    convChunk<NB, 2, CSTR>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
+    convChunk<NB, 1, CSTR>(a, sIn, cc0 + 16, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
   } else if constexpr (GB > 0) {
-    convChunk<NB, GB>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
+    convChunk<NB, GB, CSTR>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
   }
 }
 
-template <typename OutT, int NB, int GB>
+template <typename OutT, int NB, int GB, bool FULL = true>
 __global__ __launch_bounds__(256) void kConvLds(ConvArgs<uint16_t, OutT> a, float *__restrict__ split)
 {
-  __shared__ __attribute__((aligned(16))) uint16_t sIn[CL_HV * CL_CSTR];
+  /* voxel stride in LDS (bf16): 32 channels + 8 of padding; a layer with
+   * only a partial chunk of 8 or 16 channels stages 8 or 16 + 8 (16 B or 48 B:
+   * 16 lanes' 16 B reads still fall in distinct banks), so more workgroups
+   * fit a CU */
+  constexpr int CSTR = FULL || GB == 3 ? CL_CSTR : (GB == 2 ? 24 : 8);
+  __shared__ __attribute__((aligned(16))) uint16_t sIn[CL_HV * CSTR];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int nbx = (nx + CL_BX - 1) / CL_BX, nby = (ny + CL_BY - 1) / CL_BY;
@@ -311,17 +316,21 @@ __global__ __launch_bounds__(256) void kConvLds(ConvArgs<uint16_t, OutT> a, floa
   /* split-K: with gridDim.z > 1, workgroup z takes channel chunk z only and
    * writes its FP32 partial sums to split[z][vox][rows] (kConvSplitFinish
    * adds the chunks in order) */
-  const int nFull = cin / CL_CC;
+  /* FULL: the layer has full 32-channel chunks (cin >= 32); without them
+   * only the partial chunk's loop is compiled, which keeps the register count
+   * of the narrow layers at what that loop needs */
+  const int nFull = FULL ? cin / CL_CC : 0;
   if (gridDim.z > 1) {
     const int cc0 = blockIdx.z * CL_CC;
-    if ((int)blockIdx.z < nFull)
+    if (FULL && (int)blockIdx.z < nFull)
       convChunk<NB, 4>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
     else
-      convTail<NB, GB>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
+      convTail<NB, GB, CSTR>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
   } else {
-    for (int q = 0; q < nFull; q++)
-      convChunk<NB, 4>(a, sIn, q * CL_CC, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
-    convTail<NB, GB>(a, sIn, nFull * CL_CC, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
+    if constexpr (FULL)
+      for (int q = 0; q < nFull; q++)
+        convChunk<NB, 4>(a, sIn, q * CL_CC, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
+    convTail<NB, GB, CSTR>(a, sIn, nFull * CL_CC, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
   }
   const int x = x0 + lr, z = z0 + wave;
   if (x >= nx || z >= nz) return;
