@@ -114,13 +114,18 @@ static int reductionBlocks(long long n)
 }
 
 /* Maxima over the ghost shell of the padded box: the planes k = 0, nk+1,
- * the rows j = 0, nj+1 and the cells i = 0, ni+1 of the planes between. */
+ * the rows j = 0, nj+1 and the cells i = 0, ni+1 of the planes between.
+ * Each workgroup also folds its share of the nInterior per-workgroup maxima
+ * adaptUV left (interior), so the final fold reads one partial per shell
+ * workgroup instead of one per adaptUV workgroup (a maximum does not depend
+ * on the order it is taken in). */
 #define SHELL_BLOCKS 1024
 /* 32-bit index arithmetic (the shell of any grid this library allocates
  * has far fewer than 2^31 cells; checked at launch): 64-bit divisions were
  * most of this kernel's time */
 __global__ __launch_bounds__(256) void kShellMax3(const double *__restrict__ u,
-    const double *__restrict__ v, const double *__restrict__ w, Geo g, double *__restrict__ partials)
+    const double *__restrict__ v, const double *__restrict__ w, Geo g, double *__restrict__ partials,
+    const double *__restrict__ interior, int nInterior)
 {
   __shared__ double lds[4];
   const int plane = (int)g.sk, sj = g.sj, rowsPerPlane = 2 * g.sj + 2 * g.nj;
@@ -148,6 +153,14 @@ __global__ __launch_bounds__(256) void kShellMax3(const double *__restrict__ u,
     mv = mv > b ? mv : b;
     mw = mw > d ? mw : d;
   }
+  const int per = (nInterior + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int qEnd = min(nInterior, ((int)blockIdx.x + 1) * per);
+  for (int q = (int)blockIdx.x * per + (int)threadIdx.x; q < qEnd; q += (int)blockDim.x) {
+    const double a = interior[3 * q], b = interior[3 * q + 1], d = interior[3 * q + 2];
+    mu = mu > a ? mu : a;
+    mv = mv > b ? mv : b;
+    mw = mw > d ? mw : d;
+  }
   mu = blockMax(mu, lds);
   mv = blockMax(mv, lds);
   mw = blockMax(mw, lds);
@@ -167,9 +180,9 @@ void launchAbsMax3(NusifCtx *c, const Discretization *d)
   if (d) da = {1, d->dtBound, d->grid.dx, d->grid.dy, d->grid.dz, d->tau};
   if (c->cflValid) {
     double *shell = c->cflPartials + 3LL * c->cflInteriorBlocks;
-    kShellMax3<<<SHELL_BLOCKS, 256, 0, nusifStream()>>>(c->u, c->v, c->w, c->geo, shell);
-    kFoldMax3<<<1, 1024, 0, nusifStream()>>>(c->cflPartials, c->cflInteriorBlocks + SHELL_BLOCKS, c->scal,
-        c->scalHostDev, da);
+    kShellMax3<<<SHELL_BLOCKS, 256, 0, nusifStream()>>>(c->u, c->v, c->w, c->geo, shell, c->cflPartials,
+        c->cflInteriorBlocks);
+    kFoldMax3<<<1, 1024, 0, nusifStream()>>>(shell, SHELL_BLOCKS, c->scal, c->scalHostDev, da);
     return;
   }
   int nb = reductionBlocks(c->geo.n);
