@@ -112,6 +112,7 @@ void nusifDefaultOptions(NusifOptions *o)
   o->mgInner     = 1;
   o->unetExtent  = 64;
   o->unetAcceptLevel = 2;
+  o->warmstartSkip   = 0;
   o->fgRhs       = 1;
   o->rbFused     = 1;
   o->lid         = 1.0;
@@ -243,6 +244,11 @@ extern "C" int nusifSetOption(Parameter *p, const char *key, const char *value)
   if (k == "unet_accept_level") {
     int r = parseInt(v.c_str(), &o->unetAcceptLevel);
     if (r == 0 && o->unetAcceptLevel < 0) return -2;
+    return r;
+  }
+  if (k == "warmstart_skip") {
+    int r = parseInt(v.c_str(), &o->warmstartSkip);
+    if (r == 0 && o->warmstartSkip < 0) return -2;
     return r;
   }
   if (k == "fg_rhs") return parseInt(v.c_str(), &o->fgRhs);

@@ -2079,6 +2079,13 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
   int cycles = 0;
   double rel = 0.0;
   WarmStart *ws = sc->ws;
+  /* warmstart_skip: after a solve of at most that many cycles this one
+   * starts from the previous-step pressure without the network */
+  if (ws && warmstartNet(ws) && c->opt.warmstartSkip > 0 && sc->lastIterations > 0 &&
+      sc->lastIterations <= c->opt.warmstartSkip) {
+    ws           = nullptr;
+    sc->warmUsed = 0;
+  }
   bool haveInitial = false; /* mixed: residual of c->p already in L0.b, its norms in rnInit/nb */
   double rnInit = 0.0;
   bool fromGuess = false; /* mixed: the first cycle starts from the network's correction in L0.x */

@@ -124,6 +124,8 @@ struct NusifOptions {
   int unetGraph;       /* warm start: replay the network's forward from a captured HIP graph */
   int unetAcceptLevel; /* warm start: the acceptance test compares residuals restricted this
                         * many levels below the network grid (2; 0: on it) */
+  int warmstartSkip;   /* warm start: no network for a solve whose predecessor took at most
+                        * this many cycles (0: never skipped) */
   int fgRhs;       /* computeFG also forms rhs (1, default), computeRHS then skips its pass */
   int rbFused;     /* rb, mg: one launch per sweep where the grid allows (1, default) */
   int verbose;     /* 0 quiet; 1 the reference's -DVERBOSE output (configuration at

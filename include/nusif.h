@@ -88,6 +88,10 @@ void nusifFreeSolver(Solver *s);
  *                         residual at the network grid's own scale that the
  *                         first sweeps remove, which the coarser comparison
  *                         no longer counts
+ *   warmstart_skip int    warm start: a solve whose predecessor took at most
+ *                         this many V-cycles starts from the previous-step
+ *                         pressure without running the network (0, default:
+ *                         the network runs for every solve)
  *   fg_rhs        0 | 1   computeFG also forms the right-hand side in the same
  *                         pass, into a spare buffer that computeRHS then swaps
  *                         in (1, default; computeRHS recomputes when f, g, h,

@@ -253,7 +253,7 @@ def test_set_option_accepts_documented_keys():
              "lid": "1.5", "u_inflow": "1.0", "inflow_noise": "0.01", "seed": "3",
              "warmstart": "none", "unet_weights": "/tmp/w.bin", "unet_extent": "32", "fg_rhs": "0",
              "rb_fused": "0", "mg_graph_cache": "4", "verbose": "2", "unet_accept_level": "1",
-             "unet_graph": "1"}
+             "unet_graph": "1", "warmstart_skip": "2"}
     assert keys == set(valid), sorted(keys ^ set(valid))
     prm = A.Parameter()
     lib.initParameter(C.byref(prm))
@@ -262,7 +262,7 @@ def test_set_option_accepts_documented_keys():
     assert lib.nusifStepLimit(C.byref(prm)) == 7
     bad = {"solver": "cg", "precision": "fp16", "mg_tile_rows": "5", "mg_inner": "-1",
            "warmstart": "maybe", "mg_tol": "tight", "verbose": "3", "mg_graph_cache": "0",
-           "unet_accept_level": "-1"}
+           "unet_accept_level": "-1", "warmstart_skip": "-1"}
     for k, v in bad.items():
         assert lib.nusifSetOption(C.byref(prm), k.encode(), v.encode()) == -2, k
     assert lib.nusifSetOption(C.byref(prm), b"no_such_key", b"1") == -1
