@@ -554,11 +554,16 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
     if (kk - 1 >= k0) blackPlane(kk - 1, bbUse);
   };
 
+  /* the chunk's first loads all at once (planes k0-2 .. k0+1 of x, the b of
+   * k0-1 and k0), so the prologue waits one memory latency, not two: it is
+   * a large part of a short chunk's time on the coarser levels */
+  T xP[NX], xQ[NX], xA[NX], xB[NX], paA, pbA, paB, pbB, rgA, rgB, bbC = (T)0;
   fetchX(k0 - 2);
-  storeX(k0 - 2 + 4);
+#pragma unroll
+  for (int q = 0; q < NX; q++) xP[q] = xr[q];
   fetchX(k0 - 1);
-  storeX(k0 - 1);
-  T xA[NX], xB[NX], paA, pbA, paB, pbB, rgA, rgB, bbC = (T)0;
+#pragma unroll
+  for (int q = 0; q < NX; q++) xQ[q] = xr[q];
   fetchX(k0);
 #pragma unroll
   for (int q = 0; q < NX; q++) xA[q] = xr[q];
@@ -569,6 +574,12 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
   rgA = fetchRing(k0 - 1);
   fetchPair(k0, paB, pbB);
   rgB = fetchRing(k0);
+#pragma unroll
+  for (int q = 0; q < NX; q++) xr[q] = xP[q];
+  storeX(k0 - 2 + 4);
+#pragma unroll
+  for (int q = 0; q < NX; q++) xr[q] = xQ[q];
+  storeX(k0 - 1);
   iterate(k0 - 1, xA, paA, pbA, rgA, bbC);
   for (int kk = k0; kk <= k1; kk += 2) {
     iterate(kk, xB, paB, pbB, rgB, bbC);
