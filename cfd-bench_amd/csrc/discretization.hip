@@ -769,6 +769,7 @@ void launchNormalize(NusifCtx *c)
   kFoldMean<<<1, 256, 0, nusifStream()>>>(c->partials, (int)(g2.x * g2.y), count, c->scal);
   dim3 g3(g2.x, g2.y, g.nk);
   kSubtractMean<<<xcdGrid(g3), blk, 0, nusifStream()>>>(c->p, g, c->scal, g3.x, g3.y);
+  c->pGhostsOk = false; /* the interior moved, its ghosts did not */
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1041,7 +1042,8 @@ extern "C" void nusifSyncDevice(Discretization *d)
     HIP_CHECK(hipMemcpyAsync(dev[q], c->host[q], bytes, hipMemcpyHostToDevice, nusifStream()));
   nusifRhsSpareSync(c);
   HIP_CHECK(hipStreamSynchronize(nusifStream()));
-  c->cflValid = false;
+  c->cflValid  = false;
+  c->pGhostsOk = false;
 }
 
 extern "C" void nusifFreeDiscretization(Discretization *d)

@@ -1337,11 +1337,18 @@ __global__ __launch_bounds__(GTX * RTY, RESTR ? 6 : 8) void kResidualLds(const d
 
 /* out: device norm pair; outHost (optional): device address of a pinned,
  * host-coherent copy the host reads after the stream synchronises */
-__global__ __launch_bounds__(256) void kFoldNorms(const double *partials, int nb, double *out, double *outHost)
+/* extra0 / extra0Host (optional): component 0 once more, e.g. the warm
+ * start's copy of ||r0||^2 (the same partials, so no second fold) */
+__global__ __launch_bounds__(256) void kFoldNorms(const double *partials, int nb, double *out, double *outHost,
+    double *extra0 = nullptr, double *extra0Host = nullptr)
 {
   __shared__ double lds[4];
   double s0, s1;
   foldSum2(partials, nb, lds, s0, s1);
+  if (threadIdx.x == 0) {
+    if (extra0) *extra0 = s0;
+    if (extra0Host) *extra0Host = s0;
+  }
   if (outHost && threadIdx.x == 0) {
     outHost[0] = s0;
     outHost[1] = s1;
@@ -1454,9 +1461,10 @@ int launchInputRestrict(const double *p, const double *rhs, const Geo &g, double
   return (int)(ng.x * ng.y * ng.z);
 }
 
-void launchFoldNorms(const double *partials, int nb, double *out, double *outHost)
+void launchFoldNorms(const double *partials, int nb, double *out, double *outHost, double *extra0,
+    double *extra0Host)
 {
-  kFoldNorms<<<1, 256, 0, nusifStream()>>>(partials, nb, out, outHost);
+  kFoldNorms<<<1, 256, 0, nusifStream()>>>(partials, nb, out, outHost, extra0, extra0Host);
 }
 
 int inputRestrictPartials(const Geo &g)
@@ -2085,7 +2093,9 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
     kSubtractRhsMean<<<cellGrid(g), dim3(GTX, GTY), 0, nusifStream()>>>(
         rhs, g, c->scal, (double)g.ni * g.nj * g.nk);
   }
-  launchPressureGhosts(g, p, nusifStream());
+  /* the Neumann face ghosts of the starting p (the last solve's already
+   * holds them) */
+  if (!c->pGhostsOk || p != c->p) launchPressureGhosts(g, p, nusifStream());
   double nb = 0.0;
   int cycles = 0;
   double rel = 0.0;
@@ -2235,6 +2245,10 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
     p = c->p;
   }
   if (ws) warmstartCaptureEnd(ws, c->p);
+  /* every path above leaves c->p with its face ghosts written: the last
+   * smoothing sweep (FP64) or correction pass (mixed) writes them, and a
+   * solve that ran no cycle keeps the ghosts set at its start */
+  c->pGhostsOk       = true;
   sc->lastIterations = cycles;
   sc->lastRel        = rel;
   return rel;

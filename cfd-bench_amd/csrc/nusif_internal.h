@@ -180,6 +180,11 @@ struct NusifCtx {
    * with dt rhsDt); cleared by anything else that writes f, g, h or rhs */
   bool rhsReady;
   double rhsDt;
+  /* p's face ghosts hold the Neumann copies of its interior: true after a
+   * gmg solve (its last pass writes them), cleared by anything else that
+   * writes p (normalizePressure, uploads, other solvers, staged arrays), so
+   * the next gmg solve skips its ghost pass */
+  bool pGhostsOk;
   /* while nusifRunSteps drives the loop: computeFG, computeRHS and adaptUV
    * read dt from scal->dt (formed on the device by the CFL fold), so no
    * step waits for a host round trip; dtEvent follows that fold */
@@ -266,7 +271,8 @@ void launchComputeFG(NusifCtx *c, const Discretization *d);
  * trilinear prolongation x_f = P x_c */
 int launchInputRestrict(const double *p, const double *rhs, const Geo &g, double idx2, double idy2, double idz2,
     float *rc, float *bc, const Geo &gc, double *partials, float *r32 = nullptr, float *eZero = nullptr);
-void launchFoldNorms(const double *partials, int nb, double *out, double *outHost);
+void launchFoldNorms(const double *partials, int nb, double *out, double *outHost, double *extra0 = nullptr,
+    double *extra0Host = nullptr);
 int inputRestrictPartials(const Geo &g);
 void launchProlongSet32(float *xf, const Geo &gf, const float *xc, const Geo &gc);
 /* copy rhs into the spare rhs buffer (after a host upload of rhs) */

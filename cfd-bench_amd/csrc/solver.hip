@@ -121,6 +121,8 @@ extern "C" double solve(Solver *s, double *p, const double *rhs)
   NusifCtx *cr = nusifCtxOfHost(rhs, &ir);
   bool staged  = !(cp == c && ip == 0 && cr == c && ir == 1);
   size_t bytes = (size_t)c->geo.n * sizeof(double);
+  /* only a gmg solve of the context's own p leaves its ghosts known */
+  if (staged || sc->kind != SOLVER_GMG) c->pGhostsOk = false;
   if (staged) {
     /* foreign arrays: the context's own p and rhs are set aside, the foreign
      * ones solved in their place and copied out, then p and rhs restored
@@ -163,7 +165,8 @@ extern "C" double solve(Solver *s, double *p, const double *rhs)
     HIP_CHECK(hipMemcpyAsync(c->rhs, c->saveRhs, bytes, hipMemcpyDeviceToDevice, nusifStream()));
     nusifRhsSpareSync(c);
     HIP_CHECK(hipStreamSynchronize(nusifStream()));
-    c->cflValid = false;
+    c->cflValid  = false;
+    c->pGhostsOk = false; /* p restored from its saved copy */
   }
   if (c->opt.verbose >= 1) printSolve(sc, s, res, tStart);
   return res;

@@ -18,8 +18,12 @@ void unetGetGrads(UnetModel *m, float *hostGrads);
 void unetZeroGrads(UnetModel *m);
 void unetSyncWeights(UnetModel *m);
 /* x: nx ny nz x cin bf16 (channels-last); the prediction stays on the device */
-/* forTraining: also keep the upsampled decoder inputs unetBackward reads */
-void unetForward(UnetModel *m, const uint16_t *x, bool forTraining = false);
+/* forTraining: also keep the upsampled decoder inputs unetBackward reads.
+ * corr (optional): the output also written, scaled by
+ * (float)(sqrt(corrScal[0] / voxels) * corrH2), into a padded
+ * (nx+2)(ny+2)(nz+2) FP32 array with its face ghosts. */
+void unetForward(UnetModel *m, const uint16_t *x, bool forTraining = false, float *corr = nullptr,
+    const double *corrScal = nullptr, double corrH2 = 0.0);
 const float *unetOutput(UnetModel *m);
 /* after unetForward: loss against t (nx ny nz FP32), gradients added into G */
 double unetBackward(UnetModel *m, const uint16_t *x, const float *t, bool wantLoss);

@@ -130,6 +130,30 @@ __global__ void kHeadFwd(const uint16_t *act, int c, const float *hw, float *y, 
   }
 }
 
+/* head output scaled into a padded (nx+2)(ny+2)(nz+2) FP32 array with its
+ * face ghosts: y * (float)(sqrt(scal[0] / voxels) * h2) (the warm start's
+ * correction; the conv epilogue does the same where it runs the head) */
+__global__ void kUnpackHead(const float *yv, Ext e, const double *scal, double h2, float *d)
+{
+  const size_t nv = e.voxels();
+  const float sr  = (float)sqrt(scal[0] / (double)nv);
+  const float f   = (float)(sr * h2);
+  const long long sj = e.nx + 2, sk = (long long)(e.nx + 2) * (e.ny + 2);
+  GRID_STRIDE(v, nv)
+  {
+    const int x = (int)(v % e.nx), y = (int)((v / e.nx) % e.ny), z = (int)(v / ((size_t)e.nx * e.ny));
+    const long long c = (z + 1) * sk + (y + 1) * sj + x + 1;
+    const float val   = yv[v] * f;
+    d[c] = val;
+    if (x == 0) d[c - 1] = val;
+    if (x == e.nx - 1) d[c + 1] = val;
+    if (y == 0) d[c - sj] = val;
+    if (y == e.ny - 1) d[c + sj] = val;
+    if (z == 0) d[c - sk] = val;
+    if (z == e.nz - 1) d[c + sk] = val;
+  }
+}
+
 /* dy = 2 (y - t) / nv; per-block partial sums of (y - t)^2 */
 __global__ __launch_bounds__(256) void kLossGrad(const float *y, const float *t, size_t nv, float *dy, double *partials)
 {
@@ -464,51 +488,76 @@ template <typename InT, typename OutT> static void launchConv(const ConvArgs<InT
 }
 
 /* Forward convolutions on LDS-staged bricks; row tiles chosen so the grid
- * has enough workgroups. Small cubes (few bricks) split K over the 32-channel
- * chunks (grid.z) into the model's FP32 scratch, summed in chunk order by
- * kConvSplitFinish, so the result does not depend on scheduling. */
-static void launchConvLds(UnetModel *m, const ConvArgs<uint16_t, uint16_t> &a)
+ * has enough workgroups. On small cubes (fewer than 256 bricks) with
+ * several 32-channel chunks K is split over the grid (grid.z) into the
+ * model's FP32 scratch, summed in chunk order by kConvSplitFinish, and
+ * layers that still have few workgroups give each brick two K-groups of
+ * four waves that split every chunk's K-steps (sums added in LDS, group
+ * order): the result does not depend on scheduling. Returns false when the
+ * conv ran split over the grid: its epilogue extras (pool, head) were then
+ * not applied. */
+static bool launchConvLds(UnetModel *m, const ConvArgs<uint16_t, uint16_t> &a)
 {
   int rows = a.r0 + a.r1, cin = a.c0 + a.c1;
   unsigned bricks = (unsigned)convBricks({a.nx, a.ny, a.nz});
   int chunks      = (cin + CL_CC - 1) / CL_CC;
   int split       = 1;
   if (bricks < 256 && chunks > 1 && (size_t)chunks * a.voxels() * rows <= m->splitCap) split = chunks;
-  auto grid = [&](int NB) { return dim3(bricks, (rows + 16 * NB - 1) / (16 * NB), split); };
-  hipStream_t st = nusifStream();
   const int nb = rows >= 64 && bricks >= 512 ? 4 : (rows >= 32 && bricks >= 512 ? 2 : 1);
   const int gb = (cin % CL_CC) / 8; /* groups of the partial chunk */
-  auto launch = [&](auto nbC, auto gbC) {
-    constexpr int NB = decltype(nbC)::value, GB = decltype(gbC)::value;
+  /* two K-groups where the layer has few workgroups (measured faster below
+   * ~384 workgroups with full chunks; slower for the wider grids and the
+   * partial-chunk layers) */
+  const unsigned wgs = bricks * (unsigned)((rows + 15) / 16) * (unsigned)split;
+  const bool ks2     = bricks < 256 && nb == 1 && cin >= CL_CC && wgs < 384;
+  if (a.headW && rows > 16 * nb) NUSIF_FATAL("unet: the head needs every channel of a voxel in one workgroup");
+  auto grid = [&](int NB) { return dim3(bricks, (rows + 16 * NB - 1) / (16 * NB), split); };
+  hipStream_t st = nusifStream();
+  auto launch = [&](auto nbC, auto gbC, auto ksC) {
+    constexpr int NB = decltype(nbC)::value, GB = decltype(gbC)::value, KS = decltype(ksC)::value;
     if constexpr (GB > 0) {
       if (cin < CL_CC) {
-        NUSIF_TIMED("unet_conv", (kConvLds<uint16_t, NB, GB, false><<<grid(NB), 256, 0, st>>>(a, m->split)));
+        NUSIF_TIMED("unet_conv", (kConvLds<uint16_t, NB, GB, false, KS><<<grid(NB), 256 * KS, 0, st>>>(a, m->split)));
         return;
       }
     }
-    NUSIF_TIMED("unet_conv", (kConvLds<uint16_t, NB, GB><<<grid(NB), 256, 0, st>>>(a, m->split)));
+    NUSIF_TIMED("unet_conv", (kConvLds<uint16_t, NB, GB, true, KS><<<grid(NB), 256 * KS, 0, st>>>(a, m->split)));
   };
-  auto withGb = [&](auto nbC) {
+  auto withGb = [&](auto nbC, auto ksC) {
     switch (gb) {
-    case 0: launch(nbC, std::integral_constant<int, 0>()); break;
-    case 1: launch(nbC, std::integral_constant<int, 1>()); break;
-    case 2: launch(nbC, std::integral_constant<int, 2>()); break;
-    default: launch(nbC, std::integral_constant<int, 3>()); break;
+    case 0: launch(nbC, std::integral_constant<int, 0>(), ksC); break;
+    case 1: launch(nbC, std::integral_constant<int, 1>(), ksC); break;
+    case 2: launch(nbC, std::integral_constant<int, 2>(), ksC); break;
+    default: launch(nbC, std::integral_constant<int, 3>(), ksC); break;
     }
   };
-  if (nb == 4) withGb(std::integral_constant<int, 4>());
-  else if (nb == 2) withGb(std::integral_constant<int, 2>());
-  else withGb(std::integral_constant<int, 1>());
+  if (ks2) withGb(std::integral_constant<int, 1>(), std::integral_constant<int, 2>());
+  else if (nb == 4) withGb(std::integral_constant<int, 4>(), std::integral_constant<int, 1>());
+  else if (nb == 2) withGb(std::integral_constant<int, 2>(), std::integral_constant<int, 1>());
+  else withGb(std::integral_constant<int, 1>(), std::integral_constant<int, 1>());
   if (split > 1) {
     size_t total = (size_t)a.voxels() * rows;
-    unsigned nb  = (unsigned)std::min<size_t>((total + 255) / 256, 4096);
-    kConvSplitFinish<uint16_t><<<nb, 256, 0, st>>>(a, m->split, split);
+    unsigned nbk = (unsigned)std::min<size_t>((total + 255) / 256, 4096);
+    kConvSplitFinish<uint16_t><<<nbk, 256, 0, st>>>(a, m->split, split);
   }
+  return split == 1;
 }
 
-/* forward conv k: out = relu(conv(in0 ++ in1) + b) */
+/* epilogue requests of one forward conv */
+struct ConvEpilogue {
+  uint16_t *pool = nullptr;     /* 2x2x2 average of the result (extents / 2) */
+  bool head = false;            /* the model's 1x1 head into m->y */
+  bool storeAct = true;         /* keep the activation itself */
+  float *corr = nullptr;        /* head output scaled into a padded correction array */
+  const double *corrScal = nullptr;
+  double corrH2 = 0.0;
+};
+
+/* forward conv k: out = relu(conv(in0 ++ in1) + b), with the epilogue's
+ * pooling and head fused where the conv runs in one pass (otherwise as
+ * separate kernels, same values) */
 static void convForward(UnetModel *m, int k, const uint16_t *in0, int c0, const uint16_t *in1, int c1,
-    uint16_t *out, Ext e, bool half0 = false)
+    uint16_t *out, Ext e, bool half0 = false, const ConvEpilogue &ep = ConvEpilogue())
 {
   const ConvL &c = m->conv[k];
   ConvArgs<uint16_t, uint16_t> a;
@@ -529,12 +578,38 @@ static void convForward(UnetModel *m, int k, const uint16_t *in0, int c0, const 
   a.relu = 1;
   a.accumulate = 0;
   a.half0 = half0 ? 1 : 0;
-  if ((c0 + c1) % 8 == 0)
-    launchConvLds(m, a);
-  else if (half0)
+  bool fused = false;
+  if ((c0 + c1) % 8 == 0) {
+    ConvArgs<uint16_t, uint16_t> f = a;
+    f.pool = ep.pool;
+    if (ep.head) {
+      f.headW    = m->P + m->headW;
+      f.headY    = m->y;
+      f.corr     = ep.corr;
+      f.corrScal = ep.corrScal;
+      f.corrH2   = ep.corrH2;
+      f.storeAct = ep.storeAct ? 1 : 0;
+    }
+    /* a grid split of K skips the epilogue (its finishing pass stores the
+     * activation); the stand-alone kernels below follow */
+    fused = launchConvLds(m, f);
+  } else if (half0) {
     NUSIF_FATAL("unet: an upsampled input needs channels in multiples of 8");
-  else
+  } else {
     launchConv(a);
+  }
+  if (!fused) {
+    hipStream_t st = nusifStream();
+    if (ep.pool) {
+      size_t nout = e.half().voxels() * c.cout;
+      kPoolFwd<<<blocksFor(nout), 256, 0, st>>>(out, ep.pool, e.half(), c.cout);
+    }
+    if (ep.head) {
+      kHeadFwd<<<blocksFor(e.voxels()), 256, 0, st>>>(out, c.cout, m->P + m->headW, m->y, e.voxels());
+      if (ep.corr)
+        kUnpackHead<<<blocksFor(e.voxels()), 256, 0, st>>>(m->y, e, ep.corrScal, ep.corrH2, ep.corr);
+    }
+  }
 }
 
 /* backward of conv k given dOut (already ReLU-masked): dIn0 (= or +=),
@@ -572,7 +647,8 @@ static void convBackward(UnetModel *m, int k, const float *dOut, const uint16_t 
 }
 
 /* ---- forward / backward --------------------------------------------------- */
-void unetForward(UnetModel *m, const uint16_t *x, bool forTraining)
+void unetForward(UnetModel *m, const uint16_t *x, bool forTraining, float *corr, const double *corrScal,
+    double corrH2)
 {
   int L = m->L;
   const uint16_t *cur = x;
@@ -581,10 +657,16 @@ void unetForward(UnetModel *m, const uint16_t *x, bool forTraining)
   for (int l = 0; l < L; l++) {
     LevelBuf &b = m->lv[l];
     convForward(m, q++, cur, ccur, nullptr, 0, b.a, b.e);
-    convForward(m, q++, b.a, m->ch[l], nullptr, 0, b.s, b.e);
+    ConvEpilogue ep;
+    if (l + 1 < L) ep.pool = b.pl; /* pooled for the next level in the same pass */
+    if (L == 1) {
+      ep.head = true;
+      ep.corr = corr;
+      ep.corrScal = corrScal;
+      ep.corrH2 = corrH2;
+    }
+    convForward(m, q++, b.a, m->ch[l], nullptr, 0, b.s, b.e, false, ep);
     if (l + 1 < L) {
-      size_t nout = b.e.half().voxels() * m->ch[l];
-      kPoolFwd<<<blocksFor(nout), 256, 0, st>>>(b.s, b.pl, b.e.half(), m->ch[l]);
       cur  = b.pl;
       ccur = m->ch[l];
     }
@@ -599,11 +681,17 @@ void unetForward(UnetModel *m, const uint16_t *x, bool forTraining)
     } else { /* the conv reads the coarse tensor through its parent index */
       convForward(m, q++, prev, m->ch[l + 1], m->lv[l].s, m->ch[l], b.da, b.e, true);
     }
-    convForward(m, q++, b.da, m->ch[l], nullptr, 0, b.d, b.e);
+    ConvEpilogue ep;
+    if (l == 0) { /* the head (and the scaled correction) in the last conv's pass */
+      ep.head     = true;
+      ep.storeAct = forTraining; /* the backward pass reads the last activation */
+      ep.corr     = corr;
+      ep.corrScal = corrScal;
+      ep.corrH2   = corrH2;
+    }
+    convForward(m, q++, b.da, m->ch[l], nullptr, 0, b.d, b.e, false, ep);
     prev = b.d;
   }
-  const uint16_t *top = L > 1 ? m->lv[0].d : m->lv[0].s;
-  kHeadFwd<<<blocksFor(m->e.voxels()), 256, 0, st>>>(top, m->ch[0], m->P + m->headW, m->y, m->e.voxels());
 }
 
 const float *unetOutput(UnetModel *m) { return m->y; }

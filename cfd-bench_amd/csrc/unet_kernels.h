@@ -64,6 +64,18 @@ template <typename InT, typename OutT> struct ConvArgs {
   int accumulate;         /* out += result (FP32 outputs) */
   int half0 = 0;          /* in0 is stored at extents (nx, ny, nz) / 2 and read
                              upsampled (nearest parent voxel); LDS forward conv only */
+  /* Epilogue extras of the LDS forward conv (one K chunk set per workgroup,
+   * no split-K): */
+  uint16_t *pool = nullptr;     /* 2x2x2 average of the bf16 result at extents / 2,
+                                   r0 channels (the next encoder level's input) */
+  const float *headW = nullptr; /* 1x1 head on the bf16 result (all r0 <= 16 NB rows in
+                                   one workgroup): y[v] = headW[r0] + sum_r headW[r] act[v][r] */
+  float *headY = nullptr;       /* head output [vox] */
+  float *corr = nullptr;        /* optional: y scaled into a padded (nx+2)(ny+2)(nz+2) FP32
+                                   array with its face ghosts (the warm start's correction) */
+  const double *corrScal = nullptr; /* corrScal[0] = sum of squares of the input's r channel */
+  double corrH2 = 0.0;              /* correction = y * (float)(sqrt(corrScal[0] / voxels) * corrH2) */
+  int storeAct = 1;             /* write out0 (off: only the head reads the result) */
   __host__ __device__ long long voxels() const { return (long long)nx * ny * nz; }
 };
 
@@ -173,17 +185,69 @@ __global__ __launch_bounds__(256) void kConvMfma(ConvArgs<InT, OutT> a)
 /* One channel chunk of G8 8-channel groups (G8 = 4, 2 or 1): stage the
  * brick's halo into LDS, then K-steps of four (tap, group) pairs. G8 is a
  * compile-time constant so the (tap, group) split of the K index and the
- * tap's 3-D offset are shifts and constant multiplies, not divisions. */
-template <int NB, int G8, int CSTR = CL_CSTR>
-__device__ inline void convChunk(const ConvArgs<uint16_t, uint16_t> &a, uint16_t *sIn, int cc0, int cin, int rows,
-    int row0, int x0, int y0, int z0, int wave, int lr, int lg, floatx4 (&acc)[NB][4])
+ * tap's 3-D offset are shifts and constant multiplies, not divisions.
+ * With KS K-groups of four waves (256 KS threads) all threads stage the
+ * brick and group g runs the g-th of KS consecutive ranges of K-steps (a
+ * chain of K-steps KS times shorter per wave; the caller adds the groups'
+ * sums). */
+template <int NB> struct ConvRing {
+  static constexpr int D = NB == 1 ? 8 : (NB == 2 ? 4 : 2); /* K-steps the weights run ahead */
+};
+
+template <int NB, int G8, int CSTR, int KS, int KG>
+__device__ inline void convSteps(const ConvArgs<uint16_t, uint16_t> &a, const uint16_t *sIn, const size_t (&aRow)[NB],
+    const bool (&rowOk)[NB], const int (&hvBase)[4], int cin, int lg, floatx4 (&acc)[NB][4], int phase,
+    bf16x8 (&ring)[ConvRing<NB>::D][NB])
 {
-  const int tid = threadIdx.x, nx = a.nx, ny = a.ny, nz = a.nz;
-  constexpr int G   = 27 * G8;
-  constexpr int NIT = (G + 3) / 4; /* K-steps of four (tap, group) pairs */
-  /* weight fragments run D K-steps ahead of the MFMAs in a register ring, so
-   * their global (L2) latency is paid once per chunk, not once per K-step */
-  constexpr int D = NB == 1 ? 8 : (NB == 2 ? 4 : 2);
+  constexpr int G    = 27 * G8;
+  constexpr int NIT  = (G + 3) / 4; /* K-steps of four (tap, group) pairs */
+  constexpr int NITG = (NIT + KS - 1) / KS;
+  constexpr int IB   = KG * NITG;
+  constexpr int IE   = IB + NITG < NIT ? IB + NITG : NIT;
+  constexpr int D    = ConvRing<NB>::D;
+  auto loadA = [&](int s, bf16x8 *fa) {
+    int gi  = s + lg;
+    bool gv = gi < G;
+    int t = gv ? gi / G8 : 0, c8 = gv ? gi % G8 : 0;
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++)
+      fa[nb] = (gv && rowOk[nb]) ? ld8(a.A + aRow[nb] + (size_t)t * cin + c8 * 8) : zero8();
+  };
+  if (phase == 0) { /* the first D K-steps' weight fragments, before the staging */
+#pragma unroll
+    for (int p = 0; p < D; p++)
+      if (IB + p < IE) loadA(4 * (IB + p), ring[p]);
+    return;
+  }
+#pragma unroll
+  for (int it = IB; it < IE; it++) {
+    const int s = 4 * it;
+    int gi      = s + lg;
+    bool gv     = gi < G;
+    int t = gv ? gi / G8 : 0, c8 = gv ? gi % G8 : 0;
+    int off = ((t / 9) * CL_HY + (t / 3) % 3) * CL_HX + t % 3;
+    bf16x8 fa[NB], fb[4];
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) fa[nb] = ring[(it - IB) % D][nb];
+    if (it + D < IE) loadA(4 * (it + D), ring[(it - IB) % D]);
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++) {
+      uint4 raw = *(const uint4 *)&sIn[(hvBase[mb] + off) * CSTR + c8 * 8];
+      fb[mb]    = gv ? __builtin_bit_cast(bf16x8, raw) : zero8();
+    }
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+      for (int mb = 0; mb < 4; mb++)
+        acc[nb][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[nb], fb[mb], acc[nb][mb], 0, 0, 0);
+  }
+}
+
+template <int NB, int G8, int CSTR = CL_CSTR, int KS = 1>
+__device__ inline void convChunk(const ConvArgs<uint16_t, uint16_t> &a, uint16_t *sIn, int cc0, int cin, int rows,
+    int row0, int x0, int y0, int z0, int wave, int lr, int lg, floatx4 (&acc)[NB][4], int tid, int kg)
+{
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
   /* per-lane bases: the A row offsets and the LDS voxel of each block at tap
    * offset (-1, -1, -1); a tap adds (dz * HY + dy) * HX + dx */
   size_t aRow[NB];
@@ -197,29 +261,32 @@ __device__ inline void convChunk(const ConvArgs<uint16_t, uint16_t> &a, uint16_t
   int hvBase[4];
 #pragma unroll
   for (int mb = 0; mb < 4; mb++) hvBase[mb] = (wave * CL_HY + mb) * CL_HX + lr;
-  auto loadA = [&](int s, bf16x8 *fa) {
-    int gi  = s + lg;
-    bool gv = gi < G;
-    int t = gv ? gi / G8 : 0, c8 = gv ? gi % G8 : 0;
-#pragma unroll
-    for (int nb = 0; nb < NB; nb++)
-      fa[nb] = (gv && rowOk[nb]) ? ld8(a.A + aRow[nb] + (size_t)t * cin + c8 * 8) : zero8();
+  /* weight fragments run D K-steps ahead of the MFMAs in a register ring, so
+   * their global (L2) latency is paid once per chunk, not once per K-step;
+   * the first D are requested before the staging */
+  bf16x8 ring[ConvRing<NB>::D][NB];
+  auto steps = [&](int phase) {
+    if constexpr (KS == 1) {
+      convSteps<NB, G8, CSTR, 1, 0>(a, sIn, aRow, rowOk, hvBase, cin, lg, acc, phase, ring);
+    } else {
+      static_assert(KS == 2, "two K-groups at most");
+      if (kg == 0) convSteps<NB, G8, CSTR, 2, 0>(a, sIn, aRow, rowOk, hvBase, cin, lg, acc, phase, ring);
+      else convSteps<NB, G8, CSTR, 2, 1>(a, sIn, aRow, rowOk, hvBase, cin, lg, acc, phase, ring);
+    }
   };
-  bf16x8 ring[D][NB];
-#pragma unroll
-  for (int p = 0; p < D; p++)
-    if (p < NIT) loadA(4 * p, ring[p]);
+  steps(0);
   /* stage the brick and its halo (this chunk's channels) into LDS: every
    * thread issues all its global loads before its first LDS store, so the
    * staging costs one memory latency, not one per element */
-  constexpr int NST = (CL_HV * G8 + 255) / 256;
+  constexpr int NT  = 256 * KS;
+  constexpr int NST = (CL_HV * G8 + NT - 1) / NT;
   constexpr int SB  = 6; /* loads in flight per thread (register budget) */
 #pragma unroll
   for (int r0 = 0; r0 < NST; r0 += SB) {
     uint4 val[SB];
 #pragma unroll
     for (int r = 0; r < SB; r++) {
-      const int u = tid + 256 * (r0 + r);
+      const int u = tid + NT * (r0 + r);
       val[r]      = make_uint4(0, 0, 0, 0);
       if (r0 + r < NST && u < CL_HV * G8) {
         int v = u / G8, q = u - v * G8;
@@ -239,7 +306,7 @@ __device__ inline void convChunk(const ConvArgs<uint16_t, uint16_t> &a, uint16_t
     }
 #pragma unroll
     for (int r = 0; r < SB; r++) {
-      const int u = tid + 256 * (r0 + r);
+      const int u = tid + NT * (r0 + r);
       if (r0 + r < NST && u < CL_HV * G8) {
         int v = u / G8, q = u - v * G8;
         *(uint4 *)&sIn[v * CSTR + q * 8] = val[r];
@@ -247,28 +314,7 @@ __device__ inline void convChunk(const ConvArgs<uint16_t, uint16_t> &a, uint16_t
     }
   }
   __syncthreads();
-#pragma unroll
-  for (int it = 0; it < NIT; it++) {
-    const int s = 4 * it;
-    int gi      = s + lg;
-    bool gv     = gi < G;
-    int t = gv ? gi / G8 : 0, c8 = gv ? gi % G8 : 0;
-    int off = ((t / 9) * CL_HY + (t / 3) % 3) * CL_HX + t % 3;
-    bf16x8 fa[NB], fb[4];
-#pragma unroll
-    for (int nb = 0; nb < NB; nb++) fa[nb] = ring[it % D][nb];
-    if (it + D < NIT) loadA(4 * (it + D), ring[it % D]);
-#pragma unroll
-    for (int mb = 0; mb < 4; mb++) {
-      uint4 raw = *(const uint4 *)&sIn[(hvBase[mb] + off) * CSTR + c8 * 8];
-      fb[mb]    = gv ? __builtin_bit_cast(bf16x8, raw) : zero8();
-    }
-#pragma unroll
-    for (int nb = 0; nb < NB; nb++)
-#pragma unroll
-      for (int mb = 0; mb < 4; mb++)
-        acc[nb][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[nb], fb[mb], acc[nb][mb], 0, 0, 0);
-  }
+  steps(1);
   __syncthreads();
 }
 
@@ -276,28 +322,45 @@ __device__ inline void convChunk(const ConvArgs<uint16_t, uint16_t> &a, uint16_t
  * every chunk is full). Full chunks take four groups. Only the chunk shapes a
  * layer has are instantiated, which keeps the register count (and so the
  * number of workgroups per CU) at what one chunk loop needs. */
-template <int NB, int GB, int CSTR = CL_CSTR>
+template <int NB, int GB, int CSTR = CL_CSTR, int KS = 1>
 __device__ inline void convTail(const ConvArgs<uint16_t, uint16_t> &a, uint16_t *sIn, int cc0, int cin, int rows,
-    int row0, int x0, int y0, int z0, int wave, int lr, int lg, floatx4 (&acc)[NB][4])
+    int row0, int x0, int y0, int z0, int wave, int lr, int lg, floatx4 (&acc)[NB][4], int tid, int kg)
 {
   if constexpr (GB == 3) { /* 24 channels: as groups of 2 and 1 */
-    convChunk<NB, 2, CSTR>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
-    convChunk<NB, 1, CSTR>(a, sIn, cc0 + 16, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
+    convChunk<NB, 2, CSTR, KS>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc, tid, kg);
+    convChunk<NB, 1, CSTR, KS>(a, sIn, cc0 + 16, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc, tid, kg);
   } else if constexpr (GB > 0) {
-    convChunk<NB, GB, CSTR>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
+    convChunk<NB, GB, CSTR, KS>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc, tid, kg);
   }
 }
 
-template <typename OutT, int NB, int GB, bool FULL = true>
-__global__ __launch_bounds__(256) void kConvLds(ConvArgs<uint16_t, OutT> a, float *__restrict__ split)
+/* KS: K-groups of four waves per workgroup (256 KS threads). All threads
+ * stage each chunk; group g runs the g-th range of its K-steps; the groups'
+ * sums are added in group order through LDS and group 0 writes the result.
+ * For the coarse levels, where a layer has too few bricks to fill the
+ * device: twice the waves per brick, each with half the chain of K-steps.
+ *
+ * Epilogue (one workgroup has all K, gridDim.z == 1): the brick's bf16
+ * results also go to LDS, from where the workgroup forms the 2x2x2 average
+ * pool (a.pool) or the 1x1 head (a.headW, with the optional scaled
+ * correction a.corr) in the same order as the stand-alone kernels
+ * (kPoolFwd, kHeadFwd, kUnpack), so the values are the same bits. */
+template <typename OutT, int NB, int GB, bool FULL = true, int KS = 1>
+__global__ __launch_bounds__(256 * KS) void kConvLds(ConvArgs<uint16_t, OutT> a, float *__restrict__ split)
 {
   /* voxel stride in LDS (bf16): 32 channels + 8 of padding; a layer with
    * only a partial chunk of 8 or 16 channels stages 8 or 16 + 8 (16 B or 48 B:
    * 16 lanes' 16 B reads still fall in distinct banks), so more workgroups
    * fit a CU */
   constexpr int CSTR = FULL || GB == 3 ? CL_CSTR : (GB == 2 ? 24 : 8);
-  __shared__ __attribute__((aligned(16))) uint16_t sIn[CL_HV * CSTR];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int SIN  = CL_HV * CSTR;          /* the staging area (bf16) */
+  constexpr int SA   = 16 * NB + 4;           /* bf16 per voxel of the epilogue's result brick */
+  constexpr int SACT = 256 * SA;
+  constexpr int SRED = (KS - 1) * 256 * 16 * NB * 2; /* the groups' FP32 sums, in bf16 units */
+  constexpr int SL0  = SIN > SACT ? SIN : SACT;
+  constexpr int SLDS = SL0 > SRED ? SL0 : SRED;
+  __shared__ __attribute__((aligned(16))) uint16_t sIn[SLDS];
+  const int tid = threadIdx.x & 255, kg = threadIdx.x >> 8, lane = tid & 63, wave = tid >> 6;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int nbx = (nx + CL_BX - 1) / CL_BX, nby = (ny + CL_BY - 1) / CL_BY;
   const int x0 = (blockIdx.x % nbx) * CL_BX;
@@ -320,23 +383,46 @@ __global__ __launch_bounds__(256) void kConvLds(ConvArgs<uint16_t, OutT> a, floa
    * only the partial chunk's loop is compiled, which keeps the register count
    * of the narrow layers at what that loop needs */
   const int nFull = FULL ? cin / CL_CC : 0;
+  const int tidAll = threadIdx.x; /* staging: all 256 KS threads */
   if (gridDim.z > 1) {
     const int cc0 = blockIdx.z * CL_CC;
     if (FULL && (int)blockIdx.z < nFull)
-      convChunk<NB, 4>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
+      convChunk<NB, 4, CL_CSTR, KS>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc, tidAll, kg);
     else
-      convTail<NB, GB, CSTR>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
+      convTail<NB, GB, CSTR, KS>(a, sIn, cc0, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc, tidAll, kg);
   } else {
     if constexpr (FULL)
       for (int q = 0; q < nFull; q++)
-        convChunk<NB, 4>(a, sIn, q * CL_CC, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
-    convTail<NB, GB, CSTR>(a, sIn, nFull * CL_CC, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc);
+        convChunk<NB, 4, CL_CSTR, KS>(a, sIn, q * CL_CC, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc, tidAll, kg);
+    convTail<NB, GB, CSTR, KS>(a, sIn, nFull * CL_CC, cin, rows, row0, x0, y0, z0, wave, lr, lg, acc, tidAll, kg);
+  }
+  if constexpr (KS > 1) {
+    /* the chunk loops end at a barrier: the staging areas are free */
+    float *sRed = (float *)sIn;
+    if (kg > 0) {
+      float *dst = sRed + ((size_t)(kg - 1) * 256 + tid) * (16 * NB);
+#pragma unroll
+      for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++) *(floatx4 *)(dst + (nb * 4 + mb) * 4) = acc[nb][mb];
+    }
+    __syncthreads();
+    if (kg == 0)
+      for (int g = 1; g < KS; g++) {
+        const float *src = sRed + ((size_t)(g - 1) * 256 + tid) * (16 * NB);
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+          for (int mb = 0; mb < 4; mb++) acc[nb][mb] += *(const floatx4 *)(src + (nb * 4 + mb) * 4);
+      }
+    __syncthreads();
   }
   const int x = x0 + lr, z = z0 + wave;
-  if (x >= nx || z >= nz) return;
+  const bool own = kg == 0 && x < nx && z < nz; /* this lane's voxels exist and it holds their sums */
   /* each lane holds four consecutive rows (channels) of one voxel: rows are
    * multiples of 16, so they are one 16 B (FP32) or 8 B (bf16) store */
   if (gridDim.z > 1) {
+    if (!own) return;
     float *sp = split + (size_t)blockIdx.z * a.voxels() * rows;
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) {
@@ -351,15 +437,16 @@ __global__ __launch_bounds__(256) void kConvLds(ConvArgs<uint16_t, OutT> a, floa
     }
     return;
   }
+  const bool epi  = a.pool != nullptr || a.headW != nullptr;
+  uint16_t *sAct  = sIn; /* [brick voxel][SA] bf16 results for the epilogue */
 #pragma unroll
   for (int mb = 0; mb < 4; mb++) {
     int y = y0 + mb;
-    if (y >= ny) continue;
     size_t v = ((size_t)z * ny + y) * nx + x;
 #pragma unroll
     for (int nb = 0; nb < NB; nb++) {
       int r = row0 + nb * 16 + 4 * lg;
-      if (r >= rows) continue;
+      if (kg != 0 || r >= rows) continue;
       float val[4];
 #pragma unroll
       for (int q = 0; q < 4; q++) {
@@ -370,14 +457,58 @@ __global__ __launch_bounds__(256) void kConvLds(ConvArgs<uint16_t, OutT> a, floa
         uint2 pk;
         pk.x = (uint32_t)f2bf(val[0]) | ((uint32_t)f2bf(val[1]) << 16);
         pk.y = (uint32_t)f2bf(val[2]) | ((uint32_t)f2bf(val[3]) << 16);
-        *(uint2 *)(a.out0 + v * a.r0 + r) = pk;
-      } else {
+        if (own && y < ny && a.storeAct) *(uint2 *)(a.out0 + v * a.r0 + r) = pk;
+        if (epi) *(uint2 *)&sAct[((wave * 4 + mb) * 16 + lr) * SA + nb * 16 + 4 * lg] = pk;
+      } else if (own && y < ny) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
           int rq = r + q;
           if (rq < a.r0) storeOut<OutT>(a.out0 + v * a.r0 + rq, val[q], a.accumulate);
           else storeOut<OutT>(a.out1 + v * a.r1 + (rq - a.r0), val[q], a.accumulate);
         }
+      }
+    }
+  }
+  if (!epi) return;
+  __syncthreads();
+  if (a.pool) {
+    /* the brick's 8 x 2 x 2 parents (extents are even, so a parent's eight
+     * children are all inside the box or all outside) */
+    const int mx = nx >> 1, my = ny >> 1, mz = nz >> 1, R = 16 * NB;
+    for (int e = threadIdx.x; e < 32 * R; e += blockDim.x) {
+      const int row = e % R, pc = e / R;
+      const int px = pc & 7, py = (pc >> 3) & 1, pz = pc >> 4;
+      const int gx = (x0 >> 1) + px, gy = (y0 >> 1) + py, gz = (z0 >> 1) + pz, rr = row0 + row;
+      if (gx >= mx || gy >= my || gz >= mz || rr >= a.r0) continue;
+      float s = 0.f;
+      for (int q = 0; q < 8; q++) {
+        const int lx = 2 * px + (q & 1), ly = 2 * py + ((q >> 1) & 1), lz = 2 * pz + (q >> 2);
+        s += bf2f(sAct[((lz * 4 + ly) * 16 + lx) * SA + row]);
+      }
+      a.pool[(((size_t)gz * my + gy) * mx + gx) * a.r0 + rr] = f2bf(0.125f * s);
+    }
+  }
+  if (a.headW && threadIdx.x < 256) {
+    /* every row of the voxel is in this workgroup (gridDim.y == 1) */
+    const int t = threadIdx.x, gx = x0 + (t & 15), gy = y0 + ((t >> 4) & 3), gz = z0 + (t >> 6);
+    if (gx < nx && gy < ny && gz < nz) {
+      float s = a.headW[a.r0];
+      for (int k = 0; k < a.r0; k++) s += a.headW[k] * bf2f(sAct[t * SA + k]);
+      const size_t v = ((size_t)gz * ny + gy) * nx + gx;
+      if (a.headY) a.headY[v] = s;
+      if (a.corr) {
+        const float sr  = (float)sqrt(a.corrScal[0] / (double)a.voxels());
+        const float f   = (float)(sr * a.corrH2);
+        const float val = s * f;
+        const long long sj = nx + 2, sk = (long long)(nx + 2) * (ny + 2);
+        const long long c  = (gz + 1) * sk + (gy + 1) * sj + gx + 1;
+        a.corr[c] = val;
+        if (gx == 0) a.corr[c - 1] = val;
+        if (gx == nx - 1) a.corr[c + 1] = val;
+        if (gy == 0) a.corr[c - sj] = val;
+        if (gy == ny - 1) a.corr[c + sj] = val;
+        if (gz == 0) a.corr[c - sk] = val;
+        if (gz == nz - 1) a.corr[c + sk] = val;
       }
     }
   }

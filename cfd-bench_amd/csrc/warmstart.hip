@@ -120,6 +120,49 @@ __global__ __launch_bounds__(256) void kRestrictPair(const float *__restrict__ f
   }
 }
 
+/* R R f of one FP32 array in one pass: each thread takes one cell of the
+ * level two below gf (gc) and its 8 children on the level between (gm),
+ * each of those the 8-child average of gf; every average is summed as
+ * kRestrictPair sums it, so gm and gc receive the values of two
+ * kRestrictPair launches. Partials: each workgroup's sum of squares of its
+ * gc cells (component 0, stride 2, as kRestrictPair lays them out). */
+__global__ __launch_bounds__(256) void kRestrictChain2(const float *__restrict__ f, Geo gf, float *__restrict__ m,
+    Geo gm, float *__restrict__ c, Geo gc, double *partials)
+{
+  __shared__ double lds[4];
+  const int I = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int J = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int K = blockIdx.z + 1;
+  double q = 0.0;
+  if (I <= gc.ni && J <= gc.nj) {
+    float vm[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int Im = 2 * I - 1 + (e & 1), Jm = 2 * J - 1 + ((e >> 1) & 1), Km = 2 * K - 1 + (e >> 2);
+      double s = 0.0;
+#pragma unroll
+      for (int d = 0; d < 8; d++)
+        s += (double)f[gf.at(2 * Im - 1 + (d & 1), 2 * Jm - 1 + ((d >> 1) & 1), 2 * Km - 1 + (d >> 2))];
+      vm[e]                = (float)(0.125 * s);
+      m[gm.at(Im, Jm, Km)] = vm[e];
+    }
+    double sc = 0.0;
+#pragma unroll
+    for (int e = 0; e < 8; e++) sc += (double)vm[e];
+    const float v    = (float)(0.125 * sc);
+    c[gc.at(I, J, K)] = v;
+    q = (double)v * v;
+  }
+  if (partials) {
+    q = blockSum(q, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      const unsigned bid    = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+      partials[2 * bid]     = q;
+      partials[2 * bid + 1] = 0.0;
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void kCast(const double *__restrict__ a, float *__restrict__ b, Geo g)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -190,21 +233,6 @@ __global__ void kPack(const float *r, const float *b, Geo g, const double *scal,
     o.x = (uint32_t)f2bf(r[c] * ir) | ((uint32_t)f2bf(b[c] * ib) << 16);
     o.y = o.z = o.w = 0;
     *(uint4 *)(x + v * 8) = o;
-  }
-}
-
-/* network output -> correction on level Ln (interior + face ghosts) */
-__global__ void kUnpack(const float *y, Geo g, const double *scal, double h2, float *d)
-{
-  size_t nv = (size_t)g.ni * g.nj * g.nk;
-  float sr  = (float)sqrt(scal[0] / (double)nv);
-  float f   = (float)(sr * h2);
-  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (size_t)gridDim.x * blockDim.x) {
-    int i = (int)(v % g.ni) + 1, j = (int)((v / g.ni) % g.nj) + 1, k = (int)(v / ((size_t)g.ni * g.nj)) + 1;
-    long long c = g.at(i, j, k);
-    float val   = y[v] * f;
-    d[c]        = val;
-    faceGhosts(d, g, i, j, k, c, val);
   }
 }
 
@@ -419,6 +447,31 @@ void warmstartSetNet(WarmStart *w, UnetModel *net)
 UnetModel *warmstartNet(WarmStart *w) { return w->net; }
 bool warmstartCapturing(const WarmStart *w) { return w->capture; }
 
+/* src (FP32 on level l0 < Lc) restricted level by level to the acceptance
+ * level Lc (lv[l].r as the levels' arrays), two levels per launch where it
+ * can; its sum of squares there folded into scal[slot] and its host mirror */
+static void restrictToAcceptance(WarmStart *w, const float *src, int l0, int slot)
+{
+  hipStream_t st = nusifStream();
+  int l = l0;
+  while (l < w->Lc) {
+    const bool last2 = l + 2 == w->Lc, last1 = l + 1 == w->Lc;
+    if (l + 2 <= w->Lc) {
+      kRestrictChain2<<<cells(w->lv[l + 2].g), dim3(WTX, WTY), 0, st>>>(src, w->lv[l].g, w->lv[l + 1].r,
+          w->lv[l + 1].g, w->lv[l + 2].r, w->lv[l + 2].g, last2 ? w->partials : nullptr);
+      l += 2;
+    } else {
+      kRestrictPair<<<cells(w->lv[l + 1].g), dim3(WTX, WTY), 0, st>>>(src, nullptr, w->lv[l].g, w->lv[l + 1].r,
+          nullptr, w->lv[l + 1].g, last1 ? w->partials : nullptr);
+      l += 1;
+    }
+    src = w->lv[l].r;
+  }
+  const dim3 gc = cells(w->lv[w->Lc].g);
+  kFoldComp0<<<1, 256, 0, st>>>(w->partials, (int)(gc.x * gc.y * gc.z), w->scal + slot, w->scalHostDev + slot);
+}
+
+
 /* Network input from p and rhs: r0 = rhs - A p and rhs restricted to the
  * network level (one fused pass to level 1, then 8-child averages), their
  * sums of squares there (scal[0], scal[1]) and of r0 on the finest grid
@@ -442,10 +495,11 @@ bool warmstartPrepare(WarmStart *w, const double *p, const double *rhs, double i
     /* r0 and rhs averaged onto level 1 in the residual pass itself */
     int nb = launchInputRestrict(p, rhs, g0, idx2, idy2, idz2, w->lv[1].r, w->lv[1].b, w->lv[1].g, w->partials,
         r32, r32 ? eZero : nullptr);
-    kFoldComp0<<<1, 256, 0, st>>>(w->partials, nb, w->scal + 3, w->scalHostDev + 3);
-    if (r32) {
-      launchFoldNorms(w->partials, nb, norm2, norm2Host);
+    if (r32) { /* ||r0||^2 and ||rhs||^2 for the solve, ||r0||^2 also into scal[3]: one fold */
+      launchFoldNorms(w->partials, nb, norm2, norm2Host, w->scal + 3, w->scalHostDev + 3);
       defect = true;
+    } else {
+      kFoldComp0<<<1, 256, 0, st>>>(w->partials, nb, w->scal + 3, w->scalHostDev + 3);
     }
   }
   /* down to the network level, r and rhs in one launch per level; the last
@@ -465,13 +519,7 @@ bool warmstartPrepare(WarmStart *w, const double *p, const double *rhs, double i
   }
   kPack<<<1024, 256, 0, st>>>(L.r, L.b, L.g, w->scal, w->x);
   /* r0 on down to the acceptance level, its sum of squares there (scal[4]) */
-  if (w->Lc > w->Ln) {
-    for (int l = w->Ln + 1; l <= w->Lc; l++)
-      kRestrictPair<<<cells(w->lv[l].g), dim3(WTX, WTY), 0, st>>>(w->lv[l - 1].r, nullptr, w->lv[l - 1].g, w->lv[l].r,
-          nullptr, w->lv[l].g, l == w->Lc ? w->partials : nullptr);
-    const dim3 gc = cells(w->lv[w->Lc].g);
-    kFoldComp0<<<1, 256, 0, st>>>(w->partials, (int)(gc.x * gc.y * gc.z), w->scal + 4, w->scalHostDev + 4);
-  }
+  if (w->Lc > w->Ln) restrictToAcceptance(w, w->lv[w->Ln].r, w->Ln, 4);
   return defect;
 }
 
@@ -487,8 +535,8 @@ const float *warmstartCorrection(WarmStart *w)
 }
 
 /* forward and unpack: the correction d on the network grid (lv[Ln].d).
- * With unet_graph 1 the ~40 launches (convolutions, split-K finishes,
- * pooling, head, unpack) are captured once and replayed with one
+ * With unet_graph 1 the forward's launches (convolutions with their fused
+ * pooling and head, any split-K finishes) are captured once and replayed with one
  * hipGraphLaunch: they always run on the same buffers (input, weights,
  * activations, scratch, output), and the scale they read comes from device
  * memory at run time. Kernels only: no memset or copy is captured. Eager
@@ -498,10 +546,9 @@ void warmstartForward(WarmStart *w)
 {
   const WsLevel &L = w->lv[w->Ln];
   hipStream_t st   = nusifStream();
-  auto body        = [&]() {
-    NUSIF_TIMED("unet_forward", unetForward(w->net, w->x));
-    kUnpack<<<1024, 256, 0, st>>>(unetOutput(w->net), L.g, w->scal, w->h2, L.d);
-  };
+  /* the last convolution's pass also runs the head and writes the scaled
+   * correction (interior and face ghosts) */
+  auto body = [&]() { NUSIF_TIMED("unet_forward", unetForward(w->net, w->x, false, L.d, w->scal, w->h2)); };
   if (!w->useGraph || nusifTimersActive()) {
     body();
     return;
@@ -553,14 +600,7 @@ void warmstartCoarseAcceptLaunch(WarmStart *w)
   const WsLevel &N = w->lv[w->Ln];
   kCoarseDefect<<<cells(N.g), dim3(WTX, WTY), 0, st>>>(N.r, N.d, N.g, w->cn[0], w->cn[1], w->cn[2], N.b);
   if (w->Lc > w->Ln) {
-    const float *src = N.b;
-    for (int l = w->Ln + 1; l <= w->Lc; l++) {
-      kRestrictPair<<<cells(w->lv[l].g), dim3(WTX, WTY), 0, st>>>(src, nullptr, w->lv[l - 1].g, w->lv[l].r, nullptr,
-          w->lv[l].g, l == w->Lc ? w->partials : nullptr);
-      src = w->lv[l].r;
-    }
-    const dim3 gc = cells(w->lv[w->Lc].g);
-    kFoldComp0<<<1, 256, 0, st>>>(w->partials, (int)(gc.x * gc.y * gc.z), w->scal + 2, w->scalHostDev + 2);
+    restrictToAcceptance(w, N.b, w->Ln, 2);
   } else {
     const dim3 g2 = sumGrid(N.g);
     kSumSq2<<<g2, dim3(WTX, WTY), 0, st>>>(N.b, N.b, N.g, w->partials);
@@ -589,13 +629,7 @@ void warmstartCoarseNormLaunch(WarmStart *w, const float *rFine, bool level1)
   const WsLevel &L = w->lv[w->Lc];
   /* one component into scal[2]: scal[3] keeps the fine ||r0||^2 of the input */
   if (l0 <= w->Lc) { /* the last restriction also gives the sum of squares */
-    for (int l = l0; l <= w->Lc; l++) {
-      kRestrictPair<<<cells(w->lv[l].g), dim3(WTX, WTY), 0, st>>>(src, nullptr, w->lv[l - 1].g, w->lv[l].r, nullptr,
-          w->lv[l].g, l == w->Lc ? w->partials : nullptr);
-      src = w->lv[l].r;
-    }
-    const dim3 gl = cells(L.g);
-    kFoldComp0<<<1, 256, 0, st>>>(w->partials, (int)(gl.x * gl.y * gl.z), w->scal + 2, w->scalHostDev + 2);
+    restrictToAcceptance(w, src, l0 - 1, 2);
   } else {
     const dim3 g2 = sumGrid(L.g);
     kSumSq2<<<g2, dim3(WTX, WTY), 0, st>>>(src, src, L.g, w->partials);
