@@ -806,18 +806,10 @@ __global__ __launch_bounds__(256) void kProlongMarch(T *__restrict__ xf, Geo gf,
   }
 }
 
-/* Coarsest level, exact: the Neumann (ghost-copy) Laplacian is diagonal in
- * the separable DCT-II basis C_n[m][i] = cos(pi m (i + 1/2) / n) with
- * eigenvalues -4/h^2 sin^2(pi m / (2n)), so x = A^+ b is three 1-D forward
- * transforms, a division by the eigenvalue sums (the constant mode, the
- * null space, set to 0 — this is the rhs-mean projection), and three inverse
- * transforms. One workgroup, both work arrays in LDS; every output element
- * is one thread's ascending-index sum. Tables (host-built):
- *   fwd  Ci[ni*ni] Cj[nj*nj] Ck[nk*nk]          C[m][i]
- *   inv  Di Dj Dk (same shapes)                  s_m C[m][i], s_0 = 1/n, s_m = 2/n
- *   lam  Li[ni] Lj[nj] Lk[nk]                    eigenvalues */
+/* the same solve for any element count per thread (the decompositions
+ * recomputed in every phase) */
 template <typename T>
-__device__ void coarseDct(T *__restrict__ x, const T *__restrict__ b, const Geo &g,
+__device__ void coarseDctGeneric(T *__restrict__ x, const T *__restrict__ b, const Geo &g,
     const T *__restrict__ tab, const DctTables &L, unsigned char *smem)
 {
   const int ni = g.ni, nj = g.nj, nk = g.nk, N = ni * nj * nk;
@@ -878,6 +870,101 @@ __device__ void coarseDct(T *__restrict__ x, const T *__restrict__ b, const Geo 
   }
 }
 
+/* Coarsest level, exact: the Neumann (ghost-copy) Laplacian is diagonal in
+ * the separable DCT-II basis C_n[m][i] = cos(pi m (i + 1/2) / n) with
+ * eigenvalues -4/h^2 sin^2(pi m / (2n)), so x = A^+ b is three 1-D forward
+ * transforms, a division by the eigenvalue sums (the constant mode, the
+ * null space, set to 0 — this is the rhs-mean projection), and three inverse
+ * transforms. One workgroup, both work arrays in LDS; every output element
+ * is one thread's ascending-index sum. Tables (host-built):
+ *   fwd  Ci[ni*ni] Cj[nj*nj] Ck[nk*nk]          C[m][i]
+ *   inv  Di Dj Dk (same shapes)                  s_m C[m][i], s_0 = 1/n, s_m = 2/n
+ *   lam  Li[ni] Lj[nj] Lk[nk]                    eigenvalues */
+template <typename T>
+__device__ void coarseDct(T *__restrict__ x, const T *__restrict__ b, const Geo &g,
+    const T *__restrict__ tab, const DctTables &L, unsigned char *smem)
+{
+  const int ni = g.ni, nj = g.nj, nk = g.nk, N = ni * nj * nk;
+  T *W1 = (T *)smem;
+  T *W2 = W1 + N;
+  const T *Ci = tab + L.oCi, *Cj = tab + L.oCj, *Ck = tab + L.oCk;
+  const T *Di = tab + L.oDi, *Dj = tab + L.oDj, *Dk = tab + L.oDk;
+  const T *Li = tab + L.oLi, *Lj = tab + L.oLj, *Lk = tab + L.oLk;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  /* each element's (a, b, c) = (q % ni, q / ni % nj, q / (ni nj)), found
+   * once (integer divisions by the runtime extents were most of the
+   * phases' instructions); a thread's elements are q = tid + u nt */
+  constexpr int MAXQ = 2; /* COARSE_MAX_CELLS / 1024 */
+  if (N > MAXQ * nt) {
+    coarseDctGeneric(x, b, g, tab, L, smem);
+    return;
+  }
+  int qa[MAXQ], qb[MAXQ], qc[MAXQ];
+#pragma unroll
+  for (int u = 0; u < MAXQ; u++) {
+    const int q = tid + u * nt;
+    qa[u] = q % ni;
+    qb[u] = (q / ni) % nj;
+    qc[u] = q / (ni * nj);
+  }
+#define DCT_EACH(...)                                                                                           \
+  _Pragma("unroll") for (int u = 0; u < MAXQ; u++)                                                              \
+  {                                                                                                             \
+    const int q = tid + u * nt;                                                                                 \
+    if (q < N) {                                                                                                \
+      const int ea = qa[u], eb = qb[u], ec = qc[u];                                                             \
+      __VA_ARGS__                                                                                               \
+    }                                                                                                           \
+  }
+  DCT_EACH(W1[q] = b[g.at(ea + 1, eb + 1, ec + 1)];)
+  __syncthreads();
+  DCT_EACH({ /* along i */
+    const int m = ea, row = q - m;
+    T s = 0;
+    for (int i = 0; i < ni; i++) s += Ci[m * ni + i] * W1[row + i];
+    W2[q] = s;
+  })
+  __syncthreads();
+  DCT_EACH({ /* along j */
+    const int mi = ea, m = eb, k = ec;
+    T s = 0;
+    for (int j = 0; j < nj; j++) s += Cj[m * nj + j] * W2[(k * nj + j) * ni + mi];
+    W1[q] = s;
+  })
+  __syncthreads();
+  DCT_EACH({ /* along k, then divide */
+    const int mi = ea, mj = eb, m = ec;
+    T s = 0;
+    for (int k = 0; k < nk; k++) s += Ck[m * nk + k] * W1[(k * nj + mj) * ni + mi];
+    T lam = Li[mi] + Lj[mj] + Lk[m];
+    W2[q] = (q == 0) ? (T)0 : s / lam;
+  })
+  __syncthreads();
+  DCT_EACH({ /* inverse along k */
+    const int mi = ea, mj = eb, k = ec;
+    T s = 0;
+    for (int m = 0; m < nk; m++) s += Dk[m * nk + k] * W2[(m * nj + mj) * ni + mi];
+    W1[q] = s;
+  })
+  __syncthreads();
+  DCT_EACH({ /* inverse along j */
+    const int mi = ea, j = eb, k = ec;
+    T s = 0;
+    for (int m = 0; m < nj; m++) s += Dj[m * nj + j] * W1[(k * nj + m) * ni + mi];
+    W2[q] = s;
+  })
+  __syncthreads();
+  DCT_EACH({ /* inverse along i, store with face ghosts */
+    const int i = ea, row = q - i, j = eb, k = ec;
+    T s = 0;
+    for (int m = 0; m < ni; m++) s += Di[m * ni + i] * W2[row + m];
+    long long c = g.at(i + 1, j + 1, k + 1);
+    x[c]        = s;
+    faceGhosts(x, g, i + 1, j + 1, k + 1, c, s);
+  })
+#undef DCT_EACH
+}
+
 template <typename T>
 __global__ __launch_bounds__(1024) void kCoarseDct(T *__restrict__ x, const T *__restrict__ b, Geo g,
     const T *__restrict__ tab, DctTables L, const int *__restrict__ stop)
@@ -923,6 +1010,17 @@ template <typename T> struct TailArgs {
 
 __device__ inline void tailSync() { __syncthreads(); }
 
+/* q / d for 0 <= q < 2^22 and 1 <= d < 2^12 (the tail's index ranges): a
+ * float estimate, corrected once in each direction, so exact; a few
+ * instructions instead of an integer division by a runtime extent */
+__device__ inline int tdiv(int q, int d)
+{
+  int t = (int)((float)q * __builtin_amdgcn_rcpf((float)d));
+  if (t * d > q) t--;
+  if ((t + 1) * d <= q) t++;
+  return t;
+}
+
 /* one colour pass over all cell pairs (the kSmooth work of every thread) */
 template <typename T>
 __device__ void tailPass(T *__restrict__ x, const T *__restrict__ b, const Geo &g, const Coef<T> &cf,
@@ -941,8 +1039,9 @@ __device__ void tailPass(T *__restrict__ x, const T *__restrict__ b, const Geo &
       int q   = base + u * TAIL_THREADS;
       live[u] = q < total;
       int qq  = live[u] ? q : 0;
-      int tx = qq % np, r = qq / np;
-      int j = r % g.nj + 1, k = r / g.nj + 1, i1 = 2 * tx + 1;
+      int r = tdiv(qq, np), tx = qq - r * np;
+      int k0 = tdiv(r, g.nj);
+      int j = r - k0 * g.nj + 1, k = k0 + 1, i1 = 2 * tx + 1;
       int i  = (((1 + j + k) & 1) == parity) ? i1 : i1 + 1;
       ii[u]  = i;
       io[u]  = (i == i1) ? i1 + 1 : i1;
@@ -1004,9 +1103,10 @@ template <typename T> __device__ bool tailPlan(const Geo &g, TailPlan<T> &P)
     int q     = threadIdx.x + u * TAIL_THREADS;
     P.live[u] = q < total;
     int qq    = P.live[u] ? q : 0;
-    int tx = qq % np, r = qq / np;
-    P.j[u]  = r % g.nj + 1;
-    P.k[u]  = r / g.nj + 1;
+    int r = tdiv(qq, np), tx = qq - r * np;
+    int k0  = tdiv(r, g.nj);
+    P.j[u]  = r - k0 * g.nj + 1;
+    P.k[u]  = k0 + 1;
     P.i1[u] = 2 * tx + 1;
     P.c1[u] = (int)g.at(P.i1[u], P.j[u], P.k[u]);
   }
@@ -1024,6 +1124,9 @@ __device__ void tailPassPlan(T *__restrict__ x, const T *__restrict__ b, const G
   int ii[B], io[B];
 #pragma unroll
   for (int u = 0; u < B; u++) {
+    /* slots past the level's pairs load nothing (on a level with fewer
+     * pairs than slots the whole workgroup skips them together) */
+    if (!P.live[u]) continue;
     const int i1 = P.i1[u];
     const int i  = (((1 + P.j[u] + P.k[u]) & 1) == parity) ? i1 : i1 + 1;
     ii[u] = i;
@@ -1067,7 +1170,8 @@ __device__ void tailRestrict(const T *__restrict__ x, const T *__restrict__ b, c
 {
   const int total = gc.ni * gc.nj * gc.nk;
   for (int q = threadIdx.x; q < total; q += TAIL_THREADS) {
-    int I = q % gc.ni + 1, J = (q / gc.ni) % gc.nj + 1, K = q / (gc.ni * gc.nj) + 1;
+    const int r = tdiv(q, gc.ni), k0 = tdiv(r, gc.nj);
+    int I = q - r * gc.ni + 1, J = r - k0 * gc.nj + 1, K = k0 + 1;
     T s = 0;
 #pragma unroll
     for (int dk = 0; dk < 2; dk++)
@@ -1098,7 +1202,8 @@ __device__ void tailProlong(T *__restrict__ xf, const Geo &gf, const T *__restri
       int q   = base + u * TAIL_THREADS;
       live[u] = q < total;
       int qv  = live[u] ? q : 0;
-      int i = qv % gf.ni + 1, j = (qv / gf.ni) % gf.nj + 1, k = qv / (gf.ni * gf.nj) + 1;
+      const int rr = tdiv(qv, gf.ni), k0 = tdiv(rr, gf.nj);
+      int i = qv - rr * gf.ni + 1, j = rr - k0 * gf.nj + 1, k = k0 + 1;
       ii[u] = i;
       jj[u] = j;
       kk[u] = k;
@@ -1129,6 +1234,25 @@ template <typename T> __device__ void tailZero(T *__restrict__ x, long long n)
   for (long long q = threadIdx.x; q < n; q += TAIL_THREADS) x[q] = (T)0;
 }
 
+/* dst[0..n) = src[0..n), global to LDS, eight loads per thread in flight */
+template <typename T> __device__ void tailCopyToLds(T *__restrict__ dst, const T *__restrict__ src, long long n)
+{
+  constexpr int PER = 8;
+  for (long long base = threadIdx.x; base < n; base += PER * TAIL_THREADS) {
+    T v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const long long q = base + (long long)u * TAIL_THREADS;
+      v[u]              = q < n ? src[q] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const long long q = base + (long long)u * TAIL_THREADS;
+      if (q < n) dst[q] = v[u];
+    }
+  }
+}
+
 /* RES: which level arrays live in LDS for the whole cycle (0: none, the
  * arrays stay in global memory; 1: every x; 2: every x and b). Every phase
  * then reads and writes LDS instead of L2, and a phase costs an LDS round
@@ -1145,19 +1269,25 @@ __global__ __launch_bounds__(TAIL_THREADS) void kTailCycle(TailArgs<T> a)
   auto B = [&](int l) -> T * { return RES == 2 ? (T *)(smem + a.bo[l]) : a.b[l]; };
   unsigned char *work = RES >= 1 ? smem + a.wo : smem;
   const T *tab        = a.tab;
+  /* the global loads of the prologue (DCT tables, the first level's b) are
+   * all issued before their LDS stores: one memory latency, not one per
+   * element a thread copies */
   if (RES >= 1 && a.tabLds) {
     T *t = (T *)(smem + a.to);
-    for (int q = threadIdx.x; q < a.dct.total; q += TAIL_THREADS) t[q] = a.tab[q];
+    tailCopyToLds(t, a.tab, (long long)a.dct.total);
     tab = t;
   }
   for (int l = 0; l < a.n; l++) tailZero(X(l), a.g[l].n);
-  if (RES == 2)
-    for (long long q = threadIdx.x; q < a.g[0].n; q += TAIL_THREADS) B(0)[q] = a.b[0][q];
+  if (RES == 2) tailCopyToLds(B(0), a.b[0], a.g[0].n);
   tailSync();
   /* nu colour-pass pairs on level l, on a plan when the level allows one */
+  /* the first level's plan serves both its smoothing blocks */
+  TailPlan<T> P0;
+  const bool planned0 = tailPlan(a.g[0], P0);
   auto smoothLevel = [&](int l, int nu) {
     TailPlan<T> P;
-    const bool planned = tailPlan(a.g[l], P);
+    const bool planned = l == 0 ? planned0 : tailPlan(a.g[l], P);
+    if (l == 0) P = P0;
     for (int s = 0; s < nu; s++) {
       if (planned) tailPassPlan(X(l), B(l), a.g[l], a.cf[l], 1, 0, P);
       else tailPass(X(l), B(l), a.g[l], a.cf[l], 1, 0);
