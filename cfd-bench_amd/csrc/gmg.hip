@@ -988,8 +988,10 @@ __global__ __launch_bounds__(1024) void kCoarseDct(T *__restrict__ x, const T *_
 #define TAIL_MAX_CELLS 4096
 #define MAXTAIL 8
 #define TAIL_THREADS 1024
-/* cells per thread with loads in flight: 4 (FP32) or 2 (FP64) fit 128 VGPRs */
-template <typename T> struct TailB { static constexpr int value = sizeof(T) == 8 ? 2 : 4; };
+/* cell pairs (and prolongation cells) per thread with loads in flight: 2,
+ * as many as a first tail level of at most 4096 cells has per thread (more
+ * slots are code that every launch fetches and no level uses) */
+template <typename T> struct TailB { static constexpr int value = 2; };
 
 template <typename T> struct TailArgs {
   int n;          /* levels in the tail; level 0 of the tail is the finest of them */
@@ -1297,17 +1299,33 @@ __global__ __launch_bounds__(TAIL_THREADS) void kTailCycle(TailArgs<T> a)
       tailSync();
     }
   };
-  for (int l = 0; l < a.n - 1; l++) {
-    smoothLevel(l, a.nu1);
-    tailRestrict(X(l), B(l), a.g[l], a.cf[l], B(l + 1), a.g[l + 1]);
+  /* down and up as one loop over the 2 (n - 1) smoothing blocks, so the
+   * smoothing code exists once (the kernel is fetched into the instruction
+   * cache anew at every launch: a second copy of a block costs its fetch
+   * again): block s smooths level s on the way down, then restricts (and
+   * after the last one solves the coarsest level); on the way up it first
+   * prolongates onto the level */
+  const int nb = 2 * (a.n - 1);
+  if (nb == 0) {
+    coarseDct(X(0), B(0), a.g[0], tab, a.dct, work);
     tailSync();
   }
-  coarseDct(X(a.n - 1), B(a.n - 1), a.g[a.n - 1], tab, a.dct, work);
-  tailSync();
-  for (int l = a.n - 2; l >= 0; l--) {
-    tailProlong(X(l), a.g[l], X(l + 1), a.g[l + 1]);
-    tailSync();
-    smoothLevel(l, a.nu2);
+  for (int s = 0; s < nb; s++) {
+    const bool down = s < a.n - 1;
+    const int l     = down ? s : nb - 1 - s;
+    if (!down) {
+      tailProlong(X(l), a.g[l], X(l + 1), a.g[l + 1]);
+      tailSync();
+    }
+    smoothLevel(l, down ? a.nu1 : a.nu2);
+    if (down) {
+      tailRestrict(X(l), B(l), a.g[l], a.cf[l], B(l + 1), a.g[l + 1]);
+      tailSync();
+      if (l == a.n - 2) {
+        coarseDct(X(a.n - 1), B(a.n - 1), a.g[a.n - 1], tab, a.dct, work);
+        tailSync();
+      }
+    }
   }
   if (RES >= 1) /* the first level's x (every padded cell) back to global */
     for (long long q = threadIdx.x; q < a.g[0].n; q += TAIL_THREADS) a.x[0][q] = X(0)[q];

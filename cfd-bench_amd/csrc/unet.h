@@ -21,9 +21,13 @@ void unetSyncWeights(UnetModel *m);
 /* forTraining: also keep the upsampled decoder inputs unetBackward reads.
  * corr (optional): the output also written, scaled by
  * (float)(sqrt(corrScal[0] / voxels) * corrH2), into a padded
- * (nx+2)(ny+2)(nz+2) FP32 array with its face ghosts. */
+ * (nx+2)(ny+2)(nz+2) FP32 array with its face ghosts. packR/packB
+ * (optional, x then unused): the 8-channel input formed inside the first
+ * layer from two padded FP32 arrays, scaled by 1 / sqrt(corrScal[0..1] /
+ * voxels), channels 2..7 zero. */
 void unetForward(UnetModel *m, const uint16_t *x, bool forTraining = false, float *corr = nullptr,
-    const double *corrScal = nullptr, double corrH2 = 0.0);
+    const double *corrScal = nullptr, double corrH2 = 0.0, const float *packR = nullptr,
+    const float *packB = nullptr);
 const float *unetOutput(UnetModel *m);
 /* after unetForward: loss against t (nx ny nz FP32), gradients added into G */
 double unetBackward(UnetModel *m, const uint16_t *x, const float *t, bool wantLoss);

@@ -557,7 +557,8 @@ struct ConvEpilogue {
  * pooling and head fused where the conv runs in one pass (otherwise as
  * separate kernels, same values) */
 static void convForward(UnetModel *m, int k, const uint16_t *in0, int c0, const uint16_t *in1, int c1,
-    uint16_t *out, Ext e, bool half0 = false, const ConvEpilogue &ep = ConvEpilogue())
+    uint16_t *out, Ext e, bool half0 = false, const ConvEpilogue &ep = ConvEpilogue(),
+    const float *packR = nullptr, const float *packB = nullptr, const double *packScal = nullptr)
 {
   const ConvL &c = m->conv[k];
   ConvArgs<uint16_t, uint16_t> a;
@@ -582,6 +583,12 @@ static void convForward(UnetModel *m, int k, const uint16_t *in0, int c0, const 
   if ((c0 + c1) % 8 == 0) {
     ConvArgs<uint16_t, uint16_t> f = a;
     f.pool = ep.pool;
+    if (packR) {
+      if (c0 != 8 || c1 != 0) NUSIF_FATAL("unet: the packed input needs 8 input channels");
+      f.packR    = packR;
+      f.packB    = packB;
+      f.packScal = packScal;
+    }
     if (ep.head) {
       f.headW    = m->P + m->headW;
       f.headY    = m->y;
@@ -648,7 +655,7 @@ static void convBackward(UnetModel *m, int k, const float *dOut, const uint16_t 
 
 /* ---- forward / backward --------------------------------------------------- */
 void unetForward(UnetModel *m, const uint16_t *x, bool forTraining, float *corr, const double *corrScal,
-    double corrH2)
+    double corrH2, const float *packR, const float *packB)
 {
   int L = m->L;
   const uint16_t *cur = x;
@@ -656,7 +663,10 @@ void unetForward(UnetModel *m, const uint16_t *x, bool forTraining, float *corr,
   hipStream_t st = nusifStream();
   for (int l = 0; l < L; l++) {
     LevelBuf &b = m->lv[l];
-    convForward(m, q++, cur, ccur, nullptr, 0, b.a, b.e);
+    if (l == 0 && packR) /* the first layer reads the FP32 input arrays (scaled by corrScal's norms) */
+      convForward(m, q++, cur, ccur, nullptr, 0, b.a, b.e, false, ConvEpilogue(), packR, packB, corrScal);
+    else
+      convForward(m, q++, cur, ccur, nullptr, 0, b.a, b.e);
     ConvEpilogue ep;
     if (l + 1 < L) ep.pool = b.pl; /* pooled for the next level in the same pass */
     if (L == 1) {

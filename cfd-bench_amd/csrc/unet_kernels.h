@@ -76,6 +76,13 @@ template <typename InT, typename OutT> struct ConvArgs {
   const double *corrScal = nullptr; /* corrScal[0] = sum of squares of the input's r channel */
   double corrH2 = 0.0;              /* correction = y * (float)(sqrt(corrScal[0] / voxels) * corrH2) */
   int storeAct = 1;             /* write out0 (off: only the head reads the result) */
+  /* First layer only (8 input channels): instead of in0, stage channel 0
+   * = packR * (1 / sqrt(packScal[0] / voxels)) and channel 1 = packB *
+   * (1 / sqrt(packScal[1] / voxels)) (a zero norm gives 0), rounded to bf16,
+   * channels 2..7 zero, read from padded (nx+2)(ny+2)(nz+2) FP32 arrays:
+   * the network input without a packing pass */
+  const float *packR = nullptr, *packB = nullptr;
+  const double *packScal = nullptr;
   __host__ __device__ long long voxels() const { return (long long)nx * ny * nz; }
 };
 
@@ -281,6 +288,43 @@ __device__ inline void convChunk(const ConvArgs<uint16_t, uint16_t> &a, uint16_t
   constexpr int NT  = 256 * KS;
   constexpr int NST = (CL_HV * G8 + NT - 1) / NT;
   constexpr int SB  = 6; /* loads in flight per thread (register budget) */
+  if constexpr (G8 == 1) {
+    if (a.packR) { /* the network input straight from the FP32 level arrays */
+      const double cnt = (double)a.voxels();
+      const float sr = (float)sqrt(a.packScal[0] / cnt), sb = (float)sqrt(a.packScal[1] / cnt);
+      const float ir = sr > 0.f ? 1.f / sr : 0.f, ib = sb > 0.f ? 1.f / sb : 0.f;
+      const long long psj = nx + 2, psk = (long long)(nx + 2) * (ny + 2);
+      float rv[NST], bv[NST];
+#pragma unroll
+      for (int r = 0; r < NST; r++) {
+        const int v = tid + NT * r;
+        rv[r] = bv[r] = 0.f;
+        if (v < CL_HV) {
+          int hx = v % CL_HX, hy = (v / CL_HX) % CL_HY, hz = v / (CL_HX * CL_HY);
+          int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
+          if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz) {
+            const long long c = (z + 1) * psk + (y + 1) * psj + x + 1;
+            rv[r] = a.packR[c];
+            bv[r] = a.packB[c];
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < NST; r++) {
+        const int v = tid + NT * r;
+        if (v < CL_HV) {
+          uint4 o;
+          o.x = (uint32_t)f2bf(rv[r] * ir) | ((uint32_t)f2bf(bv[r] * ib) << 16);
+          o.y = o.z = o.w = 0;
+          *(uint4 *)&sIn[v * CSTR] = o;
+        }
+      }
+      __syncthreads();
+      steps(1);
+      __syncthreads();
+      return;
+    }
+  }
 #pragma unroll
   for (int r0 = 0; r0 < NST; r0 += SB) {
     uint4 val[SB];

@@ -517,7 +517,9 @@ bool warmstartPrepare(WarmStart *w, const double *p, const double *rhs, double i
     kSumSq2<<<g2, dim3(WTX, WTY), 0, st>>>(L.r, L.b, L.g, w->partials);
     kFoldSumSq2<<<1, 256, 0, st>>>(w->partials, (int)(g2.x * g2.y * g2.z), w->scal, w->scalHostDev);
   }
-  kPack<<<1024, 256, 0, st>>>(L.r, L.b, L.g, w->scal, w->x);
+  /* the packed input tensor only for captured training pairs: the forward
+   * forms it in its first layer */
+  if (w->capture) kPack<<<1024, 256, 0, st>>>(L.r, L.b, L.g, w->scal, w->x);
   /* r0 on down to the acceptance level, its sum of squares there (scal[4]) */
   if (w->Lc > w->Ln) restrictToAcceptance(w, w->lv[w->Ln].r, w->Ln, 4);
   return defect;
@@ -548,7 +550,12 @@ void warmstartForward(WarmStart *w)
   hipStream_t st   = nusifStream();
   /* the last convolution's pass also runs the head and writes the scaled
    * correction (interior and face ghosts) */
-  auto body = [&]() { NUSIF_TIMED("unet_forward", unetForward(w->net, w->x, false, L.d, w->scal, w->h2)); };
+  auto body = [&]() {
+    if (w->capture)
+      NUSIF_TIMED("unet_forward", unetForward(w->net, w->x, false, L.d, w->scal, w->h2));
+    else
+      NUSIF_TIMED("unet_forward", unetForward(w->net, nullptr, false, L.d, w->scal, w->h2, L.r, L.b));
+  };
   if (!w->useGraph || nusifTimersActive()) {
     body();
     return;
