@@ -921,6 +921,7 @@ __device__ void coarseDct(T *__restrict__ x, const T *__restrict__ b, const Geo 
   DCT_EACH({ /* along i */
     const int m = ea, row = q - m;
     T s = 0;
+    _Pragma("unroll 8")
     for (int i = 0; i < ni; i++) s += Ci[m * ni + i] * W1[row + i];
     W2[q] = s;
   })
@@ -928,6 +929,7 @@ __device__ void coarseDct(T *__restrict__ x, const T *__restrict__ b, const Geo 
   DCT_EACH({ /* along j */
     const int mi = ea, m = eb, k = ec;
     T s = 0;
+    _Pragma("unroll 8")
     for (int j = 0; j < nj; j++) s += Cj[m * nj + j] * W2[(k * nj + j) * ni + mi];
     W1[q] = s;
   })
@@ -935,6 +937,7 @@ __device__ void coarseDct(T *__restrict__ x, const T *__restrict__ b, const Geo 
   DCT_EACH({ /* along k, then divide */
     const int mi = ea, mj = eb, m = ec;
     T s = 0;
+    _Pragma("unroll 8")
     for (int k = 0; k < nk; k++) s += Ck[m * nk + k] * W1[(k * nj + mj) * ni + mi];
     T lam = Li[mi] + Lj[mj] + Lk[m];
     W2[q] = (q == 0) ? (T)0 : s / lam;
@@ -943,6 +946,7 @@ __device__ void coarseDct(T *__restrict__ x, const T *__restrict__ b, const Geo 
   DCT_EACH({ /* inverse along k */
     const int mi = ea, mj = eb, k = ec;
     T s = 0;
+    _Pragma("unroll 8")
     for (int m = 0; m < nk; m++) s += Dk[m * nk + k] * W2[(m * nj + mj) * ni + mi];
     W1[q] = s;
   })
@@ -950,6 +954,7 @@ __device__ void coarseDct(T *__restrict__ x, const T *__restrict__ b, const Geo 
   DCT_EACH({ /* inverse along j */
     const int mi = ea, j = eb, k = ec;
     T s = 0;
+    _Pragma("unroll 8")
     for (int m = 0; m < nj; m++) s += Dj[m * nj + j] * W1[(k * nj + m) * ni + mi];
     W2[q] = s;
   })
@@ -957,6 +962,7 @@ __device__ void coarseDct(T *__restrict__ x, const T *__restrict__ b, const Geo 
   DCT_EACH({ /* inverse along i, store with face ghosts */
     const int i = ea, row = q - i, j = eb, k = ec;
     T s = 0;
+    _Pragma("unroll 8")
     for (int m = 0; m < ni; m++) s += Di[m * ni + i] * W2[row + m];
     long long c = g.at(i + 1, j + 1, k + 1);
     x[c]        = s;
