@@ -419,16 +419,6 @@ __global__ __launch_bounds__(256 * KS) void kConvLds(ConvArgs<uint16_t, OutT> a,
   for (int nb = 0; nb < NB; nb++)
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) acc[nb][mb] = floatx4{0.f, 0.f, 0.f, 0.f};
-  /* this lane's four bias values, requested now so their latency passes
-   * under the staging (the epilogue adds them) */
-  float bias[NB][4];
-#pragma unroll
-  for (int nb = 0; nb < NB; nb++)
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int r = row0 + nb * 16 + 4 * lg + q;
-      bias[nb][q] = a.bias && r < rows ? a.bias[r] : 0.f;
-    }
 
   /* split-K: with gridDim.z > 1, workgroup z takes channel chunk z only and
    * writes its FP32 partial sums to split[z][vox][rows] (kConvSplitFinish
@@ -504,7 +494,7 @@ __global__ __launch_bounds__(256 * KS) void kConvLds(ConvArgs<uint16_t, OutT> a,
       float val[4];
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        val[q] = acc[nb][mb][q] + bias[nb][q];
+        val[q] = acc[nb][mb][q] + (a.bias ? a.bias[r + q] : 0.f);
         if (a.relu) val[q] = val[q] > 0.f ? val[q] : 0.f;
       }
       if (sizeof(OutT) == 2 && !a.accumulate && r + 3 < a.r0) {
