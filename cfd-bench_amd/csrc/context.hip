@@ -108,6 +108,7 @@ void nusifDefaultOptions(NusifOptions *o)
   o->mgGraph     = 0;
   o->mgGraphCache = 64;
   o->mgTileRows  = 0;
+  o->mgFuseProlong = 1;
   o->mgSpeculate = 0;
   o->mgInner     = 1;
   o->unetExtent  = 64;
@@ -228,6 +229,7 @@ extern "C" int nusifSetOption(Parameter *p, const char *key, const char *value)
   if (k == "mg_omega") return parseReal(v.c_str(), &o->mgOmega);
   if (k == "mg_project") return parseInt(v.c_str(), &o->mgProject);
   if (k == "mg_tail") return parseInt(v.c_str(), &o->mgTail);
+  if (k == "mg_fuse_prolong") return parseInt(v.c_str(), &o->mgFuseProlong);
   if (k == "mg_graph") return parseInt(v.c_str(), &o->mgGraph);
   if (k == "mg_graph_cache") {
     int r = parseInt(v.c_str(), &o->mgGraphCache);
@@ -436,6 +438,10 @@ extern "C" double nusifKernelBytes(Discretization *d, const char *kernel)
       {"gmg_smooth32_l0", 12.0, 0},  /* one red-black sweep, FP32 */
       {"gmg_smooth64z_l0", 16.0, 0}, /* first sweep from x = 0, FP64: rd b, wr x */
       {"gmg_smooth32z_l0", 8.0, 0},  /* first sweep from x = 0, FP32 */
+      {"gmg_smooth64p_l0", 25.0, 0}, /* sweep from x + P x_coarse: + rd x_coarse (1/8) */
+      {"gmg_smooth32p_l0", 12.5, 0},
+      {"gmg_smooth64s_l0", 17.0, 0}, /* sweep from P x_coarse: rd b, x_coarse (1/8), wr x */
+      {"gmg_smooth32s_l0", 8.5, 0},
       {"gmg_restrict64_l0", 17.0, 0},/* rd x, b; wr b_coarse (1/8) */
       {"gmg_restrict32_l0", 8.5, 0},
       {"gmg_prolong64_l0", 17.0, 0}, /* rd/wr x, rd x_coarse (1/8) */

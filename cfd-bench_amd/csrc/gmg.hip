@@ -112,6 +112,7 @@ struct GmgState {
   int nu1, nu2, maxCycles, project, mixed;
   int tail;     /* first level of the one-workgroup tail cycle, -1: none */
   int tileRows; /* fused-sweep tile height: 0 automatic, 4, 8 or 16 */
+  int fuseProlong; /* prolongation inside the first post-smoothing sweep */
   double tol, omega;
   LevelT<double> L64[MAXLEV];
   LevelT<float> L32[MAXLEV];
@@ -329,10 +330,22 @@ struct SweepRes {
   int q             = 0;
 };
 
-template <typename T, int TJ, int MINW, bool ZX, bool RES = false>
+/* PRO (the first post-smoothing sweep of a level): the sweep starts from
+ * x + P xc (PRO 1) or from P xc alone (PRO 2, x not read) instead of x, P
+ * the cell-centred trilinear prolongation of kProlongMarch below, evaluated
+ * with the same operands in the same order; the face ghosts of the start
+ * are those of its interior neighbours, as kProlongMarch leaves them. So the
+ * sweep gives the bits of kProlongMarch followed by a plain sweep, without
+ * the pass that writes and reads x + P xc once more. */
+template <typename T> struct SweepPro {
+  const T *xc = nullptr; /* the coarse level's x (face ghosts not read) */
+  Geo gc{};
+};
+
+template <typename T, int TJ, int MINW, bool ZX, bool RES = false, int PRO = 0>
 __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ xo, T *__restrict__ xn,
     const T *__restrict__ b, Geo g, Coef<T> cf, int kc, const int *__restrict__ stop,
-    SweepRes rs = SweepRes())
+    SweepRes rs = SweepRes(), SweepPro<T> pr = SweepPro<T>())
 {
   if (stop && *stop) return;
   if (RES && rs.q >= *rs.stopAt) return;
@@ -342,8 +355,16 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
    * ring (rows 1 and TJ+2, and positions 0 and RW-1 of the rows between) */
   constexpr int NR = 2, NRING = 2 * RW + 2 * TJ;
   static_assert(NRING <= NT, "ring tasks fit one round");
+  static_assert(PRO == 0 || (!ZX && !RES), "the prolongation goes into a plain sweep only");
   __shared__ T X[4][2 * HALF];
   __shared__ T Y[4][S::YS];
+  /* PRO: the coarse values under a tile (CW x CH cells of one coarse plane,
+   * indices clamped into the coarse interior), two planes of them; and two
+   * planes of S (below) at the X elements, each thread's own */
+  constexpr int CW = FT_I / 2 + 4, CH = TJ / 2 + 4, CSZ = CW * CH, NCS = (CSZ + NT - 1) / NT;
+  static_assert(4 * CSZ <= 8 * HALF, "the first coarse slabs fit the X planes");
+  __shared__ T CSl[PRO ? 2 * CSZ : 1];
+  __shared__ T SPw[PRO ? 2 : 1][PRO ? S::NXE : 1];
   const int tid = threadIdx.x, wave0 = tid & ~63;
   const int gxT = (g.ni + FT_I - 1) / FT_I, gyT = (g.nj + TJ - 1) / TJ;
   const int nTiles = gxT * gyT * ((g.nk + kc - 1) / kc);
@@ -358,15 +379,18 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
   const int ib = i0 - 2, jb = j0 - 2; /* global (i, j) of local (0, 0) */
 
   /* X loads: element e = (li, lj) row-major, clamped into the padded box
-   * (clamped entries are only read by updates whose results are not kept) */
+   * (clamped entries are only read by updates whose results are not kept);
+   * PRO: into the interior, a face ghost of the start being its interior
+   * neighbour's value */
   constexpr unsigned ES = sizeof(T);
+  constexpr int CLO = PRO ? 1 : 0;
   unsigned xOff[NX]; /* byte offsets within a plane */
   int xLds[NX];
 #pragma unroll
   for (int q = 0; q < NX; q++) {
     int e  = min(tid + NT * q, S::NXE - 1);
     int li = e % XW, lj = e / XW;
-    xOff[q] = (unsigned)(min(max(jb + lj, 0), nj + 1) * sj + min(max(ib + li, 0), ni + 1)) * ES;
+    xOff[q] = (unsigned)(min(max(jb + lj, CLO), nj + 1 - CLO) * sj + min(max(ib + li, CLO), ni + 1 - CLO)) * ES;
     xLds[q] = ((li + lj) & 1) * HALF + lj * RW + (li >> 1);
   }
   /* black task: position p in 1..32, row lj in 2..TJ+1; black column 2p + s,
@@ -423,20 +447,108 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
   const unsigned PB = (unsigned)sk * ES;
   auto soff = [&](int k) { return (unsigned)(k - kBase) * PB; };
 
+  /* ---- PRO: the coarse correction ----------------------------------------
+   * Fine plane k (clamped into 1..nk) lies in coarse plane K = (k + 1) / 2
+   * and leans towards K2 (K - 1 for odd k, K + 1 for even, clamped), so it
+   * takes P xc = a S_K + q S_K2 with S_P = a c1 + q c2 as in kProlongMarch:
+   * S_P is the bilinear part in (i, j) on coarse plane P. The planes it
+   * needs are hiC(k) - 1 and hiC(k), hiC(k) = min(k / 2 + 1, nkc), and each
+   * thread needs S only at its own X elements, so it keeps S of plane P in
+   * SPw[P & 1] at those elements (written and read by that thread alone:
+   * no barrier orders them) and makes the next plane's when storeX reaches
+   * an even plane (hiC grows by one; hi is the highest plane made). The new
+   * plane's S comes from its coarse slab in LDS (CSl[P & 1]), put there
+   * two plane steps before: an even step kk loads the slab of plane
+   * kk / 2 + 3 into registers, the odd step after it puts it into LDS, and
+   * the odd step two later makes its S. The first planes of S and the slabs
+   * the steps would have put before the first one come from the prologue.
+   * (S in registers instead of SPw took the 64 x 16 FP32 sweep from 61 to
+   * 85 VGPRs.) */
+  const int nkc = PRO ? pr.gc.nk : 1;
+  auto kClamp = [&](int k) { return min(max(k, 1), nk); };
+  auto hiC    = [&](int k) { return min((kClamp(k) >> 1) + 1, nkc); };
+  const int Ib = ((ib + 1) >> 1) - 1, Jb = ((jb + 1) >> 1) - 1; /* coarse (i, j) of slab (0, 0) */
+  const int pNeed  = hiC(k1 + 1);                /* the last coarse plane this chunk uses */
+  const int pFirst = max(hiC(k0 - 2) - 1, 1);   /* the first */
+  unsigned cOff[NCS]; /* byte offsets of this thread's slab cells within a coarse plane */
+#pragma unroll
+  for (int r = 0; r < NCS; r++) {
+    const int s = min(tid + NT * r, CSZ - 1);
+    const int I = min(max(Ib + s % CW, 1), pr.gc.ni), J = min(max(Jb + s / CW, 1), pr.gc.nj);
+    cOff[r] = PRO ? (unsigned)(J * pr.gc.sj + I) * ES : 0u;
+  }
+  /* based at the chunk's first coarse plane, as the fine arrays' are */
+  const __amdgpu_buffer_rsrc_t rXc = PRO ? planeRsrc(pr.xc, pr.gc, pFirst) : rXo;
+  const unsigned PBc = PRO ? (unsigned)pr.gc.sk * ES : 0u;
+  T slab[NCS]; /* a coarse slab in flight from an even plane step to the next */
+  auto loadSlab = [&](int P, T(&v)[NCS]) {
+#pragma unroll
+    for (int r = 0; r < NCS; r++) v[r] = ldPS((T *)nullptr, rXc, cOff[r], (unsigned)(P - pFirst) * PBc);
+  };
+  auto putSlab = [&](T *dst, const T(&v)[NCS]) {
+#pragma unroll
+    for (int r = 0; r < NCS; r++)
+      if (r < NCS - 1 || tid + NT * r < CSZ) dst[tid + NT * r] = v[r];
+  };
+  /* slab indices of each X element's coarse cells (I, J) and (I, J2), and
+   * I2 - I (+-1) */
+  int sO11[NX], sO12[NX], sDI[NX];
+#pragma unroll
+  for (int u = 0; u < NX; u++) {
+    const int e  = min(tid + NT * u, S::NXE - 1);
+    const int li = e % XW, lj = e / XW;
+    const int fi = min(max(ib + li, 1), ni), fj = min(max(jb + lj, 1), nj);
+    const int lI = ((fi + 1) >> 1) - Ib, lJ = ((fj + 1) >> 1) - Jb;
+    sO11[u] = lJ * CW + lI;
+    sO12[u] = (lJ + ((fj & 1) ? -1 : 1)) * CW + lI;
+    sDI[u]  = (fi & 1) ? -1 : 1;
+  }
+  auto makeS = [&](int P, const T *cs) { /* S_P at this thread's X elements from its slab cs */
+    const T a = (T)0.75, q = (T)0.25;
+#pragma unroll
+    for (int u = 0; u < NX; u++)
+      if (u < NX - 1 || tid + NT * u < S::NXE) {
+        const T c1 = a * cs[sO11[u]] + q * cs[sO11[u] + sDI[u]];
+        const T c2 = a * cs[sO12[u]] + q * cs[sO12[u] + sDI[u]];
+        SPw[P & 1][tid + NT * u] = a * c1 + q * c2;
+      }
+  };
+  int hi = 0; /* the highest plane of S made */
+  /* the slab ring's work at plane step kk (before its first barrier); the
+   * parity of kk is a compile-time argument, so a step holds no branch on it
+   * (a join of paths with and without a load in flight makes the compiler
+   * drain every outstanding load before the next use of one) */
+  auto proStep = [&](auto odd, int kk) {
+    if constexpr (!decltype(odd)::value) loadSlab(min(kk / 2 + 3, pNeed), slab);
+    else if ((kk + 5) / 2 <= pNeed) putSlab(CSl + (((kk + 5) / 2) & 1) * CSZ, slab);
+  };
+
   T xr[NX];
   auto fetchX = [&](int k) {
-    if (ZX) {
+    if (ZX || PRO == 2) {
 #pragma unroll
       for (int q = 0; q < NX; q++) xr[q] = (T)0;
       return;
     }
-    const unsigned so = soff(min(max(k, 0), nk + 1));
+    const unsigned so = soff(PRO ? kClamp(k) : min(max(k, 0), nk + 1));
 #pragma unroll
     for (int q = 0; q < NX; q++)
       xr[q] = q < NX - 1 || wave0 + NT * q < S::NXE ? ldPS((T *)nullptr, rXo, xOff[q], so) : (T)0;
   };
   auto storeX = [&](int k) {
     if (ZX) return;
+    if constexpr (PRO != 0) { /* SPw holds planes hiC(k) - 1 and hiC(k) */
+      const int kq = kClamp(k), K = (kq + 1) >> 1;
+      const int K2 = (kq & 1) ? max(K - 1, 1) : min(K + 1, nkc);
+      const T *s0 = SPw[K & 1], *s2 = SPw[K2 & 1];
+#pragma unroll
+      for (int q = 0; q < NX; q++)
+        if (q < NX - 1 || tid + NT * q < S::NXE) {
+          const T c = (T)0.75 * s0[tid + NT * q] + (T)0.25 * s2[tid + NT * q];
+          X[k & 3][xLds[q]] = PRO == 2 ? c : xr[q] + c;
+        }
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < NX; q++)
       if (q < NX - 1 || tid + NT * q < S::NXE) X[k & 3][xLds[q]] = xr[q];
@@ -529,7 +641,17 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
    * ring task), then red(kk) and black(kk-1). The pair of plane kk gives
    * red(kk) its b now and black(kk) its b at the next step (bbCarry). Loads
    * for planes past the chunk's last use (x: k1+1, b: k1) are skipped. */
-  auto iterate = [&](int kk, T(&xs)[NX], T &pa, T &pb, T &rg, T &bbCarry) {
+  auto iterate = [&](auto odd, int kk, T(&xs)[NX], T &pa, T &pb, T &rg, T &bbCarry) {
+    if constexpr (PRO != 0) {
+      proStep(odd, kk);
+      if constexpr (decltype(odd)::value) { /* kk + 1 is even: the window may move up */
+        const int h = hiC(kk + 1);
+        if (h > hi) {
+          makeS(h, CSl + (h & 1) * CSZ);
+          hi = h;
+        }
+      }
+    }
 #pragma unroll
     for (int q = 0; q < NX; q++) xr[q] = xs[q];
     storeX(kk + 1);
@@ -574,16 +696,96 @@ __global__ __launch_bounds__(32 * TJ, MINW) void kSweepRB(const T *__restrict__ 
   rgA = fetchRing(k0 - 1);
   fetchPair(k0, paB, pbB);
   rgB = fetchRing(k0);
+  if constexpr (PRO != 0) {
+    /* S of planes lo = pFirst and h0 = hiC(k0 - 2) for storeX(k0 - 2), of
+     * h1 = hiC(k0 - 1) for storeX(k0 - 1) (made after storeX(k0 - 2): it
+     * takes the SPw slot of lo), the slab of the next plane P1 = (k0 + 3) / 2
+     * in CSl[P1 & 1] (the first odd step makes its S) and, when k0 is even,
+     * the slab of P1 + 1 in registers for the first step (odd) to put: at
+     * most four slabs. Those of lo and h0 are staged in the X planes, free
+     * until the first storeX; h1's goes to CSl[h1 & 1] (k0 odd: the first
+     * step to write that buffer is k0, two barriers after its last read). */
+    const int h0 = hiC(k0 - 2), h1 = hiC(k0 - 1), P1 = min((k0 + 3) / 2, pNeed);
+    const int pLoad = min(P1 + ((k0 & 1) ? 0 : 1), pNeed);
+    T pro[4][NCS];
 #pragma unroll
-  for (int q = 0; q < NX; q++) xr[q] = xP[q];
-  storeX(k0 - 2 + 4);
+    for (int u = 0; u < 4; u++)
+      if (pFirst + u <= pLoad) loadSlab(pFirst + u, pro[u]);
+    T *tmp = &X[0][0];
 #pragma unroll
-  for (int q = 0; q < NX; q++) xr[q] = xQ[q];
-  storeX(k0 - 1);
-  iterate(k0 - 1, xA, paA, pbA, rgA, bbC);
-  for (int kk = k0; kk <= k1; kk += 2) {
-    iterate(kk, xB, paB, pbB, rgB, bbC);
-    if (kk + 1 <= k1) iterate(kk + 1, xA, paA, pbA, rgA, bbC);
+    for (int u = 0; u < 4; u++) {
+      const int P = pFirst + u;
+      if (P <= h0) putSlab(tmp + u * CSZ, pro[u]);
+      if ((P == h1 && h1 > h0) || (P == P1 && P1 > h1)) putSlab(CSl + (P & 1) * CSZ, pro[u]);
+      if (P == P1 + 1 && !(k0 & 1))
+#pragma unroll
+        for (int r = 0; r < NCS; r++) slab[r] = pro[u][r];
+    }
+    ldsBarrier();
+    makeS(pFirst, tmp);
+    if (h0 > pFirst) makeS(h0, tmp + (h0 - pFirst) * CSZ);
+    hi = h0;
+    ldsBarrier(); /* the reads of the staged slabs before the X planes are written */
+#pragma unroll
+    for (int q = 0; q < NX; q++) xr[q] = xP[q];
+    storeX(k0 - 2);
+    if (h1 > h0) {
+      makeS(h1, CSl + (h1 & 1) * CSZ);
+      hi = h1;
+    }
+#pragma unroll
+    for (int q = 0; q < NX; q++) xr[q] = xQ[q];
+    storeX(k0 - 1);
+  } else {
+#pragma unroll
+    for (int q = 0; q < NX; q++) xr[q] = xP[q];
+    storeX(k0 - 2);
+#pragma unroll
+    for (int q = 0; q < NX; q++) xr[q] = xQ[q];
+    storeX(k0 - 1);
+  }
+  using Odd  = std::integral_constant<bool, true>;
+  using Even = std::integral_constant<bool, false>;
+  if constexpr (PRO == 0) {
+    iterate(Odd(), k0 - 1, xA, paA, pbA, rgA, bbC);
+    for (int kk = k0; kk <= k1; kk += 2) {
+      iterate(Even(), kk, xB, paB, pbB, rgB, bbC);
+      if (kk + 1 <= k1) iterate(Odd(), kk + 1, xA, paA, pbA, rgA, bbC);
+    }
+  } else {
+    /* an even step loads a coarse slab and the odd step after it puts it
+     * into LDS, so every loop pass runs an even step first and the slab's
+     * registers are not carried round the loop (a loop-carried value still
+     * being loaded is copied at the loop's end, after a wait for every
+     * outstanding load). With k0 odd the first two steps run before the
+     * loop, and the register sets swap so that one loop serves both. */
+    int ks = k0;
+    if (k0 & 1) {
+      iterate(Even(), k0 - 1, xA, paA, pbA, rgA, bbC);
+      iterate(Odd(), k0, xB, paB, pbB, rgB, bbC); /* k0 < k1 */
+      ks = k0 + 1;
+#pragma unroll
+      for (int q = 0; q < NX; q++) {
+        const T t = xA[q];
+        xA[q]     = xB[q];
+        xB[q]     = t;
+      }
+      T t = paA;
+      paA = paB;
+      paB = t;
+      t   = pbA;
+      pbA = pbB;
+      pbB = t;
+      t   = rgA;
+      rgA = rgB;
+      rgB = t;
+    } else {
+      iterate(Odd(), k0 - 1, xA, paA, pbA, rgA, bbC);
+    }
+    for (int kk = ks; kk <= k1; kk += 2) {
+      iterate(Even(), kk, xB, paB, pbB, rgB, bbC);
+      if (kk + 1 <= k1) iterate(Odd(), kk + 1, xA, paA, pbA, rgA, bbC);
+    }
   }
   if constexpr (RES) {
     __shared__ double lsum[NT / 64];
@@ -645,6 +847,64 @@ static void launchSweepShape(const LevelT<T> &L, dim3 grd, int kc, bool timed, b
     else
       kSweepRB<T, TJ, MINW, false><<<grd, NT, 0, nusifStream()>>>(L.x, L.xAlt, L.b, L.g, L.cf, kc, gCycleStop);
   }
+}
+
+/* PRO 1: the sweep from L.x + P C.x; PRO 2: from P C.x (L.x not read) */
+template <typename T, int TJ, int MINW, int PRO>
+static void launchSweepProShape(const LevelT<T> &L, const LevelT<T> &C, dim3 grd, int kc, bool timed)
+{
+  const char *name = sizeof(T) == 8 ? (PRO == 2 ? "gmg_smooth64s_l0" : "gmg_smooth64p_l0")
+                                    : (PRO == 2 ? "gmg_smooth32s_l0" : "gmg_smooth32p_l0");
+  constexpr int NT = SweepRBShape<T, TJ>::NT;
+  SweepPro<T> pr;
+  pr.xc = C.x;
+  pr.gc = C.g;
+  if (timed)
+    NUSIF_TIMED(name, (kSweepRB<T, TJ, MINW, false, false, PRO><<<grd, NT, 0, nusifStream()>>>(
+                          L.x, L.xAlt, L.b, L.g, L.cf, kc, gCycleStop, SweepRes(), pr)));
+  else
+    kSweepRB<T, TJ, MINW, false, false, PRO><<<grd, NT, 0, nusifStream()>>>(
+        L.x, L.xAlt, L.b, L.g, L.cf, kc, gCycleStop, SweepRes(), pr);
+}
+
+/* Workgroups of the PRO sweep resident at once on the device (its LDS and
+ * registers allow fewer per CU than the plain sweep's: 3 against 4 for the
+ * 64 x 16 FP32 tile) */
+template <typename T, int TJ, int MINW> static long long proResident()
+{
+  static long long n = 0;
+  if (!n) {
+    int per = 0, dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kSweepRB<T, TJ, MINW, false, false, 1>,
+        SweepRBShape<T, TJ>::NT, 0));
+    n = (long long)(per > 0 ? per : 1) * prop.multiProcessorCount;
+  }
+  return n;
+}
+
+template <typename T, int PRO>
+static void launchSweepPro(const LevelT<T> &L, const LevelT<T> &C, SweepShape s, bool timed)
+{
+  /* the coarse level is the fine one halved (the fused sweep's slab of
+   * 36 x (TJ / 2 + 4) coarse cells per tile assumes it) */
+  if (C.g.ni * 2 != L.g.ni || C.g.nj * 2 != L.g.nj || C.g.nk * 2 != L.g.nk)
+    NUSIF_FATAL("fused prolongation: level %dx%dx%d is not twice %dx%dx%d", L.g.ni, L.g.nj, L.g.nk, C.g.ni,
+        C.g.nj, C.g.nk);
+  /* longer k-chunks where the plain sweep's tiles would take more than one
+   * round of resident workgroups: a second, partial round costs nearly as
+   * long as a full one (256^3 FP32: 768 chunks of 22 planes, not 1024 of 16) */
+  const long long res = s.tj == 16 ? proResident<T, 16, 4>() : (s.tj == 8 ? proResident<T, 8, 6>() : proResident<T, 4, 8>());
+  while (s.kc < FT_KC && sweepTiles(L.g, s.tj, s.kc) > res) s.kc++;
+  dim3 grd = fusedGrid(L.g, s);
+  if (s.tj == 16)
+    launchSweepProShape<T, 16, 4, PRO>(L, C, grd, s.kc, timed);
+  else if (s.tj == 8)
+    launchSweepProShape<T, 8, 6, PRO>(L, C, grd, s.kc, timed);
+  else
+    launchSweepProShape<T, 4, 8, PRO>(L, C, grd, s.kc, timed);
 }
 
 template <typename T> static void launchSweep(const LevelT<T> &L, SweepShape s, bool timed, bool zx)
@@ -1797,6 +2057,15 @@ template <typename T> static void vcycle(GmgState *G, LevelT<T> *L, int l, bool 
     kResidualRestrict<T><<<xcdGrid(cellGrid(C.g)), dim3(GTX, GTY), 0, nusifStream()>>>(
         L[l].x, L[l].b, L[l].g, L[l].cf, C.b, C.x, C.g, cellGrid(C.g).x, cellGrid(C.g).y, gCycleStop);
   vcycle(G, L, l + 1);
+  if (G->fuseProlong && G->nu2 > 0 && fusedLevel(L[l])) {
+    /* the prolongation inside the first post-smoothing sweep */
+    launchSweepPro<T, 1>(L[l], C, sweepShape<T>(L[l].g, G->tileRows), fine);
+    T *t      = L[l].x;
+    L[l].x    = L[l].xAlt;
+    L[l].xAlt = t;
+    smooth(G, L[l], G->nu2 - 1, fine, false);
+    return;
+  }
   dim3 pg((L[l].g.ni + GTX - 1) / GTX, (L[l].g.nj + GTY - 1) / GTY, (C.g.nk + PKC - 1) / PKC);
   if (fine)
     NUSIF_TIMED(sizeof(T) == 8 ? "gmg_prolong64_l0" : "gmg_prolong32_l0",
@@ -1843,6 +2112,7 @@ GmgState *gmgCreate(NusifSolverCtx *sc, const Parameter *prm)
   HIP_CHECK(hipHostMalloc((void **)&G->hInit, sizeof(CycleCtl), hipHostMallocDefault));
   for (int q = 0; q < CTL_RING; q++) HIP_CHECK(hipEventCreateWithFlags(&G->ev[q], hipEventDisableTiming));
   G->tileRows  = c->opt.mgTileRows;
+  G->fuseProlong = c->opt.mgFuseProlong;
   G->nu1       = prm->presmooth;
   G->nu2       = prm->postsmooth;
   G->maxCycles = c->opt.mgMaxCycles;

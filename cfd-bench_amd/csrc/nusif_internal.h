@@ -112,6 +112,7 @@ struct NusifOptions {
   int mgGraph;     /* gmg: replay each cycle from a captured HIP graph (1) or launch eagerly (0) */
   int mgGraphCache;/* gmg: captured cycle graphs kept at most (oldest evicted first) */
   int mgTileRows;  /* gmg: fused-sweep tile height, 0 automatic, 4, 8 or 16 */
+  int mgFuseProlong; /* gmg: prolongation inside the first post-smoothing sweep (1) or its own pass (0) */
   int mgSpeculate; /* gmg: enqueue cycles ahead of the convergence test (1) or one at a time (0) */
   int mgInner;     /* gmg: cycles before a solve's first convergence test (0: sized from the previous solve) */
   double lid;

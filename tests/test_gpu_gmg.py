@@ -183,6 +183,36 @@ def test_gpu_gmg_tail_cycle_equals_per_level_kernels(shape, levels, precision, g
         np.testing.assert_array_equal(o[0], out[0][0])
 
 
+@pytest.mark.parametrize("shape,levels,precision,post", [((128, 128, 64), 5, "mixed", 2),
+                                                        ((128, 128, 64), 5, "fp64", 2),
+                                                        ((96, 64, 32), 3, "mixed", 1),
+                                                        ((96, 64, 32), 3, "fp64", 3),
+                                                        ((64, 32, 64), 4, "mixed", 2)])
+def test_gpu_gmg_fused_prolongation_equals_separate_pass(shape, levels, precision, post, gpu_ok):
+    """The prolongation inside the first post-smoothing sweep (mg_fuse_prolong
+    1) against kProlongMarch followed by a plain sweep (mg_fuse_prolong 0):
+    the same p, bit for bit, and the same cycle count. The shapes give tiles
+    of 64 x 16, 64 x 8 and 64 x 4 cells, k-chunks of one to sixteen planes
+    and ragged tiles (96 columns); 1 to 3 post-smoothing sweeps."""
+    print("gmg fused prolongation", shape, levels, precision, post, flush=True)
+    ni, nj, nk = shape
+    prm = dict(G.KAT_PARAMS, imax=ni, jmax=nj, kmax=nk, xlength=ni / 32.0, ylength=nj / 32.0,
+               zlength=nk / 32.0, levels=levels, presmooth=2, postsmooth=post)
+    out = []
+    for fuse in (0, 1):
+        opts = {"solver": "gmg", "precision": precision, "mg_tol": 1e-9, "mg_project": 1,
+                "mg_fuse_prolong": fuse}
+        c = A.Case(lib("gpu_rb"), options=opts, **prm)
+        f = G.random_fields(c.shape, 11)
+        c.set_fields(p=np.zeros(c.shape), rhs=f["rhs"])
+        c.lib.solve(C.byref(c.s), c.d.p, c.d.rhs)
+        out.append((c.fields()["p"], c.lib.nusifSolverIterations(C.byref(c.s))))
+        c.close()
+    assert out[1][1] == out[0][1]
+    assert out[0][1] >= 2
+    np.testing.assert_array_equal(out[1][0], out[0][0])
+
+
 @pytest.mark.parametrize("precision,speculate,cache", [("fp64", 1, 64), ("mixed", 1, 64), ("fp64", 0, 2),
                                                      ("mixed", 0, 2), ("mixed", 1, 2)])
 def test_gpu_gmg_graph_replay_equals_eager_launches(precision, speculate, cache, gpu_ok):
