@@ -113,6 +113,10 @@ struct GmgState {
   int tail;     /* first level of the one-workgroup tail cycle, -1: none */
   int tileRows; /* fused-sweep tile height: 0 automatic, 4, 8 or 16 */
   int fuseProlong; /* prolongation inside the first post-smoothing sweep */
+  /* the next finest-level cycle starts from P setXc (level setGc, the
+   * finest halved) in its first sweep instead of from L32[0].x */
+  const float *setXc;
+  Geo setGc;
   double tol, omega;
   LevelT<double> L64[MAXLEV];
   LevelT<float> L32[MAXLEV];
@@ -2047,7 +2051,21 @@ template <typename T> static void vcycle(GmgState *G, LevelT<T> *L, int l, bool 
   /* the coarse levels start from x = 0 (zeroed by the restriction); the
    * finest one in mixed mode too (the error equation) unless the cycle
    * continues e, FP64 mode smooths p */
-  smooth(G, L[l], G->nu1, fine, l > 0 || (G->mixed && !cont));
+  bool fromSet = false;
+  if constexpr (std::is_same<T, float>::value)
+    if (fine && G->setXc) { /* the first sweep from P setXc: the prolongation's pass is not made */
+      LevelT<float> S = L[1];
+      S.x = const_cast<float *>(G->setXc);
+      S.g = G->setGc;
+      launchSweepPro<float, 2>(L[0], S, sweepShape<float>(L[0].g, G->tileRows), true);
+      float *t   = L[0].x;
+      L[0].x     = L[0].xAlt;
+      L[0].xAlt  = t;
+      G->setXc   = nullptr;
+      fromSet    = true;
+      smooth(G, L[l], G->nu1 - 1, fine, false);
+    }
+  if (!fromSet) smooth(G, L[l], G->nu1, fine, l > 0 || (G->mixed && !cont));
   LevelT<T> &C = L[l + 1];
   if (fine)
     NUSIF_TIMED(sizeof(T) == 8 ? "gmg_restrict64_l0" : "gmg_restrict32_l0",
@@ -2113,6 +2131,7 @@ GmgState *gmgCreate(NusifSolverCtx *sc, const Parameter *prm)
   for (int q = 0; q < CTL_RING; q++) HIP_CHECK(hipEventCreateWithFlags(&G->ev[q], hipEventDisableTiming));
   G->tileRows  = c->opt.mgTileRows;
   G->fuseProlong = c->opt.mgFuseProlong;
+  G->setXc       = nullptr;
   G->nu1       = prm->presmooth;
   G->nu2       = prm->postsmooth;
   G->maxCycles = c->opt.mgMaxCycles;
@@ -2561,7 +2580,16 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
       sc->wsTried++;
       sc->wsTaken += sc->warmUsed;
       if (sc->warmUsed) {
-        warmstartProlongCorrection(ws, L0.x);
+        /* the last step of the correction's prolongation goes into the
+         * first cycle's first sweep where that sweep is the fused one */
+        Geo g1;
+        const float *c1 = G->fuseProlong && G->nu1 > 0 && fusedLevel(L0) ? warmstartProlongToLevel1(ws, &g1) : nullptr;
+        if (c1 && g1.ni * 2 == L0.g.ni && g1.nj * 2 == L0.g.nj && g1.nk * 2 == L0.g.nk) {
+          G->setXc = c1;
+          G->setGc = g1;
+        } else {
+          warmstartProlongCorrection(ws, L0.x);
+        }
         fromGuess = true;
       }
       haveInitial = true;

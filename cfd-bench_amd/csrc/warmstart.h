@@ -33,6 +33,10 @@ bool warmstartPrepare(WarmStart *w, const double *p, const double *rhs, double i
 const float *warmstartCorrection(WarmStart *w);
 void warmstartForward(WarmStart *w);
 void warmstartProlongCorrection(WarmStart *w, float *fine);
+/* the correction prolongated from the network grid down to level 1 only
+ * (nothing to do when the network runs there); returns level 1's buffer and
+ * geometry (null when the network runs on the finest grid) */
+const float *warmstartProlongToLevel1(WarmStart *w, Geo *g1);
 void warmstartCoarseAcceptLaunch(WarmStart *w);
 void warmstartCaptureBegin(WarmStart *w, const double *p, const Geo &g0);
 void warmstartCaptureEnd(WarmStart *w, const double *p);

@@ -579,6 +579,14 @@ void warmstartProlongCorrection(WarmStart *w, float *fine)
     launchProlongSet32(l == 1 && fine ? fine : w->lv[l - 1].d, w->lv[l - 1].g, w->lv[l].d, w->lv[l].g);
 }
 
+const float *warmstartProlongToLevel1(WarmStart *w, Geo *g1)
+{
+  if (w->Ln < 1) return nullptr;
+  for (int l = w->Ln; l > 1; l--) launchProlongSet32(w->lv[l - 1].d, w->lv[l - 1].g, w->lv[l].d, w->lv[l].g);
+  *g1 = w->lv[1].g;
+  return w->lv[1].d;
+}
+
 /* r - A d on the network grid (7-point Neumann Laplacian with that grid's
  * spacings; d carries its face ghosts), FP32 */
 __global__ __launch_bounds__(256) void kCoarseDefect(const float *__restrict__ r, const float *__restrict__ d, Geo g,

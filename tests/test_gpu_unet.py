@@ -254,3 +254,36 @@ def test_gpu_unet_graph_replay_equals_eager(n, gpu_ok):
     print("guesses taken", out[1][3], "of 10")
     for k in A.FIELDS:
         np.testing.assert_array_equal(out[0][2][k], out[1][2][k], err_msg=k)
+
+
+@pytest.mark.parametrize("n", [128, 192])
+def test_gpu_warmstart_fused_prolongation_equals_separate_passes(n, gpu_ok):
+    """The warm start's correction prolongated inside the first sweep of the
+    first cycle, and the cycles' own prolongations inside their post-smoothing
+    (mg_fuse_prolong 1), against the separate passes (0): eight warm-started
+    steps, every field, the trace and the guesses taken bit for bit. 128^3:
+    the network on level 1 (64^3), its output read by the sweep directly;
+    192^3: the network on 48^3, one separate step to 96^3 first."""
+    weights = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "cfd-bench_amd",
+                           "weights", "unet_cavity.bin")
+    prm = dict(name="dcavity", re=1000.0, imax=n, jmax=n, kmax=n, tau=0.5, te=1e9, levels=5,
+               presmooth=2, postsmooth=2)
+    out = []
+    for fuse in (0, 1):
+        print("warm start fused prolongation", n, fuse, flush=True)
+        opts = {"solver": "gmg", "precision": "mixed", "mg_tol": 1e-6, "mg_omega": 1.3, "mg_inner": 0,
+                "warmstart": "unet", "unet_weights": weights, "mg_fuse_prolong": fuse}
+        c = A.Case(lib("gpu_rb"), options=opts, **prm)
+        tr = (C.c_double * 40)()
+        assert c.lib.nusifRunSteps(C.byref(c.d), C.byref(c.s), 5, tr) == 5
+        for _ in range(3):
+            c.step()
+        tried, taken = C.c_longlong(), C.c_longlong()
+        c.lib.nusifWarmstartCounts(C.byref(c.s), C.byref(tried), C.byref(taken))
+        out.append((np.array(tr[:20]), c.trace, c.fields(), taken.value))
+        c.close()
+    assert out[1][3] > 0, "no guess taken: the fused first sweep did not run"
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1] and out[0][3] == out[1][3]
+    for k in A.FIELDS:
+        np.testing.assert_array_equal(out[0][2][k], out[1][2][k], err_msg=k)
