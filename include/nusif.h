@@ -47,6 +47,10 @@ void nusifFreeSolver(Solver *s);
  *   mg_project    0 | 1   gmg: remove the rhs mean (all-Neumann compatibility)
  *   mg_tail       0 | 1   gmg: run the levels of <= 4096 cells in one workgroup
  *                         (1, default) or with one kernel per pass (0); same values
+ *   mg_fuse_prolong 0 | 1 gmg: each level's prolongation inside its first
+ *                         post-smoothing sweep, the warm start's last prolongation
+ *                         step inside the first sweep (1, default), or passes of
+ *                         their own (0); same values
  *   mg_graph      0 | 1   gmg: replay each cycle from a captured HIP graph (1) or
  *                         launch its kernels one by one (0, default); same values.
  *                         Not with warmstart unet (initSolver stops with an error)
