@@ -116,6 +116,7 @@ void nusifDefaultOptions(NusifOptions *o)
   o->warmstartSkip   = 0;
   o->fgRhs       = 1;
   o->rbFused     = 1;
+  o->bcFused     = 1;
   o->lid         = 1.0;
   o->uInflow     = 2.0;
   o->inflowNoise = 0.0;
@@ -255,6 +256,7 @@ extern "C" int nusifSetOption(Parameter *p, const char *key, const char *value)
   }
   if (k == "fg_rhs") return parseInt(v.c_str(), &o->fgRhs);
   if (k == "rb_fused") return parseInt(v.c_str(), &o->rbFused);
+  if (k == "bc_fused") return parseInt(v.c_str(), &o->bcFused);
   if (k == "mg_speculate") return parseInt(v.c_str(), &o->mgSpeculate);
   if (k == "mg_inner") {
     int r = parseInt(v.c_str(), &o->mgInner);

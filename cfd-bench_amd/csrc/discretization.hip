@@ -334,6 +334,337 @@ void launchSpecialBC(NusifCtx *c, const Discretization *d)
 }
 
 /* ------------------------------------------------------------------------ */
+/* The boundary work of a time step in one launch                           */
+/* ------------------------------------------------------------------------ */
+/* computeTimestep's maxima over the ghost shell, then setBoundaryConditions
+ * and setSpecialBoundaryCondition (src/main.c:61-63), after an adaptUV that
+ * nothing has overwritten since: one launch and the fold instead of the
+ * shell pass, the fold and four boundary launches.
+ *
+ * The reference applies the faces in the order TOP, BOTTOM, LEFT, RIGHT,
+ * FRONT, BACK (src/discretization.c:119-332), then the lid or the inflow.
+ * Every written cell here gets the value that order gives it:
+ * - A face reads a cell an earlier face writes only where a high face writes
+ *   its wall velocity inside the interior range (TOP v(i,nj,k), RIGHT
+ *   u(ni,j,k)): LEFT/RIGHT read v(iin,nj,k), FRONT/BACK read v(i,nj,kin)
+ *   and u(ni,j,kin). Those reads apply the earlier face's rule to the
+ *   original neighbour it copies (v(.,nj-1,.), u(ni-1,.,.)), which no face
+ *   writes.
+ * - A face reads the original of a cell a later face writes only at
+ *   u(ni,jin,k) (TOP/BOTTOM read it, RIGHT writes it) and w(.,.,nk)
+ *   (TOP/BOTTOM and LEFT/RIGHT read it, BACK writes it). Those ghost
+ *   updates move into the task that writes the cell, before its write:
+ *   TOP/BOTTOM's u ghost at i = ni into the LEFT/RIGHT task of row jin,
+ *   TOP/BOTTOM's and LEFT/RIGHT's w ghosts of plane nk into the FRONT/BACK
+ *   task of that column.
+ * - The lid writes u(i,nj+1,k) for i < ni, k < nk over TOP's ghost from
+ *   u(i,nj,k), which no face writes; the inflow writes u(0,j,k) over LEFT's
+ *   wall value. Neither cell is read by a face.
+ * - The maxima read each shell cell's old value in the task that writes it
+ *   (before the write); shell cells no task writes (the shell's edges and
+ *   corners, and the high ghosts u(ni+1,.,.), v(.,nj+1,.), w(.,.,nk+1)) are
+ *   read by the tasks of their faces or by edge tasks. A maximum does not
+ *   depend on the order it is taken in, so dt is the same bits.
+ * Tasks: TOP/BOTTOM per (i, k), LEFT/RIGHT per junction of two rows,
+ * FRONT/BACK per (i, j), one per edge or corner cell; each needs ni, nj,
+ * nk >= 3. */
+struct BoundaryStep {
+  int tTop, tBottom, tLeft, tRight, tFront, tBack;
+  int special; /* 0 none, 1 lid, 2 inflow */
+  double twoLid, inflow;
+  const double *profile;
+  int maxOn;                /* maxima (adaptive dt) */
+  const double *interior;   /* adaptUV's per-workgroup maxima */
+  int nInterior;
+  double *partials;         /* per workgroup */
+};
+
+__device__ inline bool bcActive(int t) { return t == NOSLIP || t == SLIP || t == OUTFLOW; }
+/* a ghost from its interior neighbour, as velocityFace forms it (a wall
+ * velocity is the cell next to it for OUTFLOW, else 0, the cell read only
+ * for OUTFLOW: a row end's extra cache line less on LEFT / RIGHT) */
+__device__ inline double bcGhost(int t, double x) { return t == OUTFLOW ? x : (t == NOSLIP ? -x : x); }
+
+__global__ __launch_bounds__(256) void kBoundaryStep(double *__restrict__ u, double *__restrict__ v,
+    double *__restrict__ w, Geo g, BoundaryStep a)
+{
+  __shared__ double lds[4];
+  const int ni = g.ni, nj = g.nj, nk = g.nk;
+  const int nTB = ni * nk, nLR = (nj + 1) * nk, nFB = ni * nj;
+  const int nEi = 4 * ni, nEj = 4 * nj, nEk = 4 * nk;
+  const int total = nTB + nLR + nFB + nEi + nEj + nEk + 8;
+  double mu = DBL_MIN, mv = DBL_MIN, mw = DBL_MIN;
+  auto mx   = [](double m, double x) { return m > x ? m : x; };
+  auto take = [&](long long c, bool tu, bool tv, bool tw) {
+    if (!a.maxOn) return;
+    if (tu) mu = mx(mu, fabs(u[c]));
+    if (tv) mv = mx(mv, fabs(v[c]));
+    if (tw) mw = mx(mw, fabs(w[c]));
+  };
+  /* Each task issues all its loads before its first store: a store to u, v
+   * or w may alias a later load of the same array as far as the compiler
+   * knows, so loads after a store wait for it (one memory latency per
+   * store-load turn); no cell a task stores is loaded after its store. */
+  const bool mOn = a.maxOn != 0;
+  auto amx = [&](double &m, double x) { m = mx(m, fabs(x)); };
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+    /* LEFT / RIGHT first: their row ends are the slow accesses, so their
+     * workgroups start first. A task takes the junction of rows J and J + 1
+     * of plane k: RIGHT's cells of row J and LEFT's of row J + 1, which lie
+     * next to each other in memory (the end of a row, then the start of the
+     * next). */
+    int t = q;
+    if (t < nLR) {
+      const int J = t % (nj + 1), k = 1 + t / (nj + 1);
+      const bool hasR = J >= 1, hasL = J + 1 <= nj, wk = k < nk;
+      const int jR = J, jL = J + 1, tR = a.tRight, tL = a.tLeft;
+      /* RIGHT's row: TOP / BOTTOM's u ghost at i = ni moves here (RIGHT
+       * writes u(ni, jin, k)); v(ni, nj, k) as TOP left it */
+      const bool gMove = hasR && (jR == nj || jR == 1);
+      const int tG = jR == nj ? a.tTop : a.tBottom;
+      const long long cR = g.at(ni + 1, jR, k), cG = g.at(ni, jR == nj ? nj + 1 : 0, k), cL = g.at(0, jL, k);
+      double mRu = 0.0, mRv = 0.0, mRw = 0.0, mGu = 0.0, gSrc = 0.0, vR = 0.0, wR = 0.0, uR = 0.0;
+      double mLu = 0.0, mLv = 0.0, mLw = 0.0, vL = 0.0, wL = 0.0, uL = 0.0;
+      if (hasR) {
+        if (mOn) {
+          mRu = u[cR];
+          mRv = v[cR];
+          if (wk) mRw = w[cR];
+          if (gMove) mGu = u[cG];
+        }
+        if (gMove) gSrc = u[g.at(ni, jR, k)];
+        if (bcActive(tR)) {
+          vR = bcActive(a.tTop) && jR == nj ? (a.tTop == OUTFLOW ? v[g.at(ni, nj - 1, k)] : 0.0) : v[g.at(ni, jR, k)];
+          if (wk) wR = w[g.at(ni, jR, k)];
+          if (tR == OUTFLOW) uR = u[g.at(ni - 1, jR, k)];
+        }
+      }
+      if (hasL) {
+        if (mOn) {
+          mLu = u[cL];
+          mLv = v[cL];
+          if (wk) mLw = w[cL];
+        }
+        if (bcActive(tL)) {
+          vL = bcActive(a.tTop) && jL == nj ? (a.tTop == OUTFLOW ? v[g.at(1, nj - 1, k)] : 0.0) : v[g.at(1, jL, k)];
+          if (wk) wL = w[g.at(1, jL, k)];
+          if (tL == OUTFLOW) uL = u[g.at(1, jL, k)];
+        }
+      }
+      if (mOn) {
+        if (hasR) {
+          amx(mu, mRu);
+          amx(mv, mRv);
+          if (wk) amx(mw, mRw);
+          if (gMove) amx(mu, mGu);
+        }
+        if (hasL) {
+          amx(mu, mLu);
+          amx(mv, mLv);
+          if (wk) amx(mw, mLw);
+        }
+      }
+      if (hasR) {
+        if (gMove && bcActive(tG)) u[cG] = bcGhost(tG, gSrc);
+        if (bcActive(tR)) {
+          v[cR] = bcGhost(tR, vR);
+          if (wk) w[cR] = bcGhost(tR, wR);
+          u[g.at(ni, jR, k)] = tR == OUTFLOW ? uR : 0.0;
+        }
+      }
+      if (hasL) {
+        if (bcActive(tL)) {
+          v[cL] = bcGhost(tL, vL);
+          if (wk) w[cL] = bcGhost(tL, wL);
+          u[cL] = tL == OUTFLOW ? uL : 0.0;
+        }
+        if (a.special == 2) u[cL] = a.profile ? a.profile[(long long)k * (nj + 2) + jL] : a.inflow;
+      }
+      continue;
+    }
+    t -= nLR;
+    if (t < nTB) { /* TOP and BOTTOM at (i, k) */
+      const int i = 1 + t % ni, k = 1 + t / ni;
+      const bool iu = i < ni, wk = k < nk, lid = a.special == 1 && iu && wk;
+      const int tT = a.tTop, tB = a.tBottom;
+      const long long c0 = g.at(i, 0, k), c1 = g.at(i, nj + 1, k);
+      double m0u = 0.0, m0v = 0.0, m0w = 0.0, m1u = 0.0, m1v = 0.0, m1w = 0.0;
+      if (mOn) {
+        if (iu) {
+          m0u = u[c0];
+          m1u = u[c1];
+        }
+        m0v = v[c0];
+        m1v = v[c1];
+        if (wk) {
+          m0w = w[c0];
+          m1w = w[c1];
+        }
+      }
+      /* TOP reads row nj (v: nj - 1), BOTTOM row 1 */
+      const double uN = iu && (bcActive(tT) || lid) ? u[g.at(i, nj, k)] : 0.0;
+      const double uS = iu && bcActive(tB) ? u[g.at(i, 1, k)] : 0.0;
+      const double vN = tT == OUTFLOW ? v[g.at(i, nj - 1, k)] : 0.0;
+      const double vS = tB == OUTFLOW ? v[g.at(i, 1, k)] : 0.0;
+      const double wN = wk && bcActive(tT) ? w[g.at(i, nj, k)] : 0.0;
+      const double wS = wk && bcActive(tB) ? w[g.at(i, 1, k)] : 0.0;
+      if (mOn) {
+        if (iu) {
+          amx(mu, m0u);
+          amx(mu, m1u);
+        }
+        amx(mv, m0v);
+        amx(mv, m1v);
+        if (wk) {
+          amx(mw, m0w);
+          amx(mw, m1w);
+        }
+      }
+      if (bcActive(tT)) {
+        if (iu) u[c1] = bcGhost(tT, uN);
+        v[g.at(i, nj, k)] = vN;
+        if (wk) w[c1] = bcGhost(tT, wN);
+      }
+      if (bcActive(tB)) {
+        if (iu) u[c0] = bcGhost(tB, uS);
+        v[c0] = vS;
+        if (wk) w[c0] = bcGhost(tB, wS);
+      }
+      if (lid) u[c1] = a.twoLid - uN;
+      continue;
+    }
+    t -= nTB;
+    if (t < nFB) { /* FRONT and BACK at (i, j) */
+      const int i = 1 + t % ni, j = 1 + t / ni;
+      const int tF = a.tFront, tK = a.tBack;
+      const long long c0 = g.at(i, j, 0), c1 = g.at(i, j, nk + 1);
+      /* TOP / BOTTOM's and LEFT / RIGHT's w ghosts of plane nk move here
+       * (BACK writes w(i, j, nk)); u(ni, j, kin) as RIGHT left it, v(i, nj,
+       * kin) as TOP left it */
+      const bool gTB = j == nj || j == 1, gLR = i == ni || i == 1;
+      const int tGTB = j == nj ? a.tTop : a.tBottom, tGLR = i == ni ? a.tRight : a.tLeft;
+      const long long cGTB = g.at(i, j == nj ? nj + 1 : 0, nk), cGLR = g.at(i == ni ? ni + 1 : 0, j, nk);
+      const bool rightW = bcActive(a.tRight) && i == ni, topW = bcActive(a.tTop) && j == nj;
+      double m0u = 0.0, m0v = 0.0, m0w = 0.0, m1u = 0.0, m1v = 0.0, m1w = 0.0, mGt = 0.0, mGl = 0.0;
+      if (mOn) {
+        m0u = u[c0];
+        m0v = v[c0];
+        m0w = w[c0];
+        m1u = u[c1];
+        m1v = v[c1];
+        m1w = w[c1];
+        if (gTB) mGt = w[cGTB];
+        if (gLR) mGl = w[cGLR];
+      }
+      const double wNk = (gTB && bcActive(tGTB)) || (gLR && bcActive(tGLR)) ? w[g.at(i, j, nk)] : 0.0;
+      auto srcU = [&](int kin) {
+        return rightW ? (a.tRight == OUTFLOW ? u[g.at(ni - 1, j, kin)] : 0.0) : u[g.at(i, j, kin)];
+      };
+      auto srcV = [&](int kin) {
+        return topW ? (a.tTop == OUTFLOW ? v[g.at(i, nj - 1, kin)] : 0.0) : v[g.at(i, j, kin)];
+      };
+      const double uF = bcActive(tF) ? srcU(1) : 0.0, vF = bcActive(tF) ? srcV(1) : 0.0;
+      const double wF = tF == OUTFLOW ? w[g.at(i, j, 1)] : 0.0;
+      const double uK = bcActive(tK) ? srcU(nk) : 0.0, vK = bcActive(tK) ? srcV(nk) : 0.0;
+      const double wK = tK == OUTFLOW ? w[g.at(i, j, nk - 1)] : 0.0;
+      if (mOn) {
+        amx(mu, m0u);
+        amx(mv, m0v);
+        amx(mw, m0w);
+        amx(mu, m1u);
+        amx(mv, m1v);
+        amx(mw, m1w);
+        if (gTB) amx(mw, mGt);
+        if (gLR) amx(mw, mGl);
+      }
+      if (gTB && bcActive(tGTB)) w[cGTB] = bcGhost(tGTB, wNk);
+      if (gLR && bcActive(tGLR)) w[cGLR] = bcGhost(tGLR, wNk);
+      if (bcActive(tF)) {
+        u[c0] = bcGhost(tF, uF);
+        v[c0] = bcGhost(tF, vF);
+        w[c0] = wF;
+      }
+      if (bcActive(tK)) {
+        u[c1] = bcGhost(tK, uK);
+        v[c1] = bcGhost(tK, vK);
+        w[g.at(i, j, nk)] = wK;
+      }
+      continue;
+    }
+    t -= nFB;
+    /* edge and corner cells: maxima only */
+    long long c;
+    if (t < nEi) {
+      const int i = 1 + t % ni, e = t / ni;
+      c = g.at(i, (e & 1) ? nj + 1 : 0, (e & 2) ? nk + 1 : 0);
+    } else if ((t -= nEi) < nEj) {
+      const int j = 1 + t % nj, e = t / nj;
+      c = g.at((e & 1) ? ni + 1 : 0, j, (e & 2) ? nk + 1 : 0);
+    } else if ((t -= nEj) < nEk) {
+      const int k = 1 + t % nk, e = t / nk;
+      c = g.at((e & 1) ? ni + 1 : 0, (e & 2) ? nj + 1 : 0, k);
+    } else {
+      t -= nEk;
+      c = g.at((t & 1) ? ni + 1 : 0, (t & 2) ? nj + 1 : 0, (t & 4) ? nk + 1 : 0);
+    }
+    take(c, true, true, true);
+  }
+  if (!a.maxOn) return;
+  /* this workgroup's share of adaptUV's maxima */
+  const int per  = (a.nInterior + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int qEnd = min(a.nInterior, ((int)blockIdx.x + 1) * per);
+  for (int q = (int)blockIdx.x * per + (int)threadIdx.x; q < qEnd; q += (int)blockDim.x) {
+    mu = mx(mu, a.interior[3 * q]);
+    mv = mx(mv, a.interior[3 * q + 1]);
+    mw = mx(mw, a.interior[3 * q + 2]);
+  }
+  mu = blockMax(mu, lds);
+  mv = blockMax(mv, lds);
+  mw = blockMax(mw, lds);
+  if (threadIdx.x == 0) {
+    a.partials[3 * blockIdx.x + 0] = mu;
+    a.partials[3 * blockIdx.x + 1] = mv;
+    a.partials[3 * blockIdx.x + 2] = mw;
+  }
+}
+
+bool boundaryStepFits(const NusifCtx *c) { return c->geo.ni >= 3 && c->geo.nj >= 3 && c->geo.nk >= 3; }
+
+/* computeTimestep (when d is non-null and adaptive, from the shell and
+ * adaptUV's maxima: c->cflValid must hold), setBoundaryConditions and
+ * setSpecialBoundaryCondition as one launch and the fold */
+void launchBoundaryStep(NusifCtx *c, const Discretization *d, bool withMax)
+{
+  const Geo &g = c->geo;
+  BoundaryStep a;
+  a.tTop = d->bcTop;
+  a.tBottom = d->bcBottom;
+  a.tLeft = d->bcLeft;
+  a.tRight = d->bcRight;
+  a.tFront = d->bcFront;
+  a.tBack = d->bcBack;
+  a.special = strcmp(d->problem, "dcavity") == 0 ? 1 : (strcmp(d->problem, "canal") == 0 ? 2 : 0);
+  a.twoLid = 2.0 * c->opt.lid;
+  a.inflow = c->opt.uInflow;
+  a.profile = c->inflow;
+  a.maxOn = withMax ? 1 : 0;
+  a.interior = c->cflPartials;
+  a.nInterior = c->cflInteriorBlocks;
+  a.partials = c->cflPartials + 3LL * c->cflInteriorBlocks;
+  if (withMax && !c->cflValid) NUSIF_FATAL("boundary step: the maxima need the adaptUV partials");
+  const long long tasks = (long long)g.ni * g.nk + (long long)(g.nj + 1) * g.nk + (long long)g.ni * g.nj +
+                          4LL * (g.ni + g.nj + g.nk) + 8;
+  long long nb = (tasks + 255) / 256;
+  if (nb > SHELL_BLOCKS) nb = SHELL_BLOCKS;
+  kBoundaryStep<<<(unsigned)nb, 256, 0, nusifStream()>>>(c->u, c->v, c->w, g, a);
+  if (withMax) {
+    DtArgs da = {1, d->dtBound, d->grid.dx, d->grid.dy, d->grid.dz, d->tau};
+    kFoldMax3<<<1, 1024, 0, nusifStream()>>>(a.partials, (int)nb, c->scal, c->scalHostDev, da);
+  }
+  c->cflValid = false; /* wall faces of the interior range rewritten */
+}
+
+/* ------------------------------------------------------------------------ */
 /* computeFG (src/discretization.c:455-652)                                 */
 /* ------------------------------------------------------------------------ */
 /* c*((a1*b1) - (a0*b0)) + gamma*c*(|a1|*d1 + |a0|*d0), c = inv*0.25, in the

@@ -193,7 +193,7 @@ static const KeySpec kKeys[] = {
 /* Keys this library adds; matched exactly. None of the reference keys is a
  * prefix of any of them, so the reference's own keys never fire on them. */
 static const char *kExtraKeys[] = {"nsteps", "solver", "precision", "mg_tol", "mg_maxcycles",
-    "mg_omega", "mg_project", "mg_tail", "mg_fuse_prolong", "mg_graph", "mg_tile_rows", "mg_speculate", "mg_inner", "lid", "u_inflow", "inflow_noise", "seed",
+    "mg_omega", "mg_project", "mg_tail", "mg_fuse_prolong", "bc_fused", "mg_graph", "mg_tile_rows", "mg_speculate", "mg_inner", "lid", "u_inflow", "inflow_noise", "seed",
     "warmstart", "unet_weights", "unet_extent", "mg_graph_cache", "verbose", "unet_accept_level", "unet_graph", "warmstart_skip"};
 
 void readParameter(Parameter *param, const char *filename)

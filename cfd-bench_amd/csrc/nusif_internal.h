@@ -129,6 +129,7 @@ struct NusifOptions {
                         * this many cycles (0: never skipped) */
   int fgRhs;       /* computeFG also forms rhs (1, default), computeRHS then skips its pass */
   int rbFused;     /* rb, mg: one launch per sweep where the grid allows (1, default) */
+  int bcFused;     /* nusifRunSteps: the step's maxima and boundary passes in one launch (1, default) */
   int verbose;     /* 0 quiet; 1 the reference's -DVERBOSE output (configuration at
                     * initDiscretization, one line per solve); 2 also -DPROFILING's
                     * solver-time line after each red-black solve */
@@ -253,16 +254,19 @@ void nusifTimerEnd(KernelTimer *t);
     if (tm_) nusifTimerEnd(tm_);                                                      \
   } while (0)
 
-/* Launchers (step.hip) */
+/* Launchers (discretization.hip) */
 /* CFL maxima into scal (and its pinned mirror); with d, also dt (the
  * reference's arithmetic, src/discretization.c:431-453) into scal->dt */
 void launchAbsMax3(NusifCtx *c, const Discretization *d = nullptr);
 void launchVelocityBC(NusifCtx *c, const Discretization *d);
 void launchSpecialBC(NusifCtx *c, const Discretization *d);
+/* launchAbsMax3(c, d) (withMax: needs c->cflValid), launchVelocityBC and
+ * launchSpecialBC as one launch and the maxima's fold; the same values.
+ * Needs ni, nj, nk >= 3 (boundaryStepFits). */
+bool boundaryStepFits(const NusifCtx *c);
+void launchBoundaryStep(NusifCtx *c, const Discretization *d, bool withMax);
 void launchNormalize(NusifCtx *c);
 void launchAdaptUV(NusifCtx *c, const Discretization *d);
-void launchVelocityBC(NusifCtx *c, const Discretization *d);
-void launchSpecialBC(NusifCtx *c, const Discretization *d);
 void launchComputeFG(NusifCtx *c, const Discretization *d);
 /* gmg.hip kernels the warm start also runs: r0 = rhs - A p restricted to
  * the next level with rhs (returns the partial count; partials hold r0^2

@@ -234,13 +234,19 @@ extern "C" int nusifRunSteps(Discretization *d, Solver *s, int nsteps, double *t
    * solve's first read-back has already waited for. */
   const bool devDt = d->tau > 0.0;
   for (int q = 0; q < nsteps; q++) {
+    /* the shell's maxima, the faces and the lid or inflow in one launch
+     * where adaptUV's maxima are there (every step after the first) */
+    const bool fused = c->opt.bcFused && boundaryStepFits(c) && (!devDt || c->cflValid);
+    if (fused) launchBoundaryStep(c, d, devDt);
+    else if (devDt) launchAbsMax3(c, d);
     if (devDt) {
-      launchAbsMax3(c, d);
       HIP_CHECK(hipEventRecord(c->dtEvent, nusifStream()));
       c->dtOnDevice = true;
     }
-    setBoundaryConditions(d);
-    setSpecialBoundaryCondition(d);
+    if (!fused) {
+      setBoundaryConditions(d);
+      setSpecialBoundaryCondition(d);
+    }
     computeFG(d);
     computeRHS(d);
     if (c->nt % 100 == 0) normalizePressure(d);

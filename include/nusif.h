@@ -101,6 +101,10 @@ void nusifFreeSolver(Solver *s);
  *                         in (1, default; computeRHS recomputes when f, g, h,
  *                         rhs or dt changed in between); 0: computeRHS runs
  *                         its own pass. Same values either way
+ *   bc_fused      0 | 1   nusifRunSteps: each step's CFL maxima (after the
+ *                         first step), velocity boundary conditions and lid
+ *                         or inflow in one launch and a fold (1, default) or
+ *                         the reference's calls one by one (0); same values
  *   rb_fused      0 | 1   rb and mg solvers: each sweep (both colour passes
  *                         and the ghost copy) in one launch where the grid
  *                         allows (imax >= 32, kmax >= 4; 1, default); 0: a

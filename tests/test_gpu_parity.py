@@ -121,6 +121,38 @@ def test_gpu_run_steps_matches_step_calls(gpu_ok):
     b.close()
 
 
+@pytest.mark.parametrize("bcs,name,shape,extra", [
+    ("noslip", "dcavity", (24, 20, 16), {}),
+    ("mixed", "dcavity", (23, 9, 7), {}),
+    ("mixed2", "canal", (16, 5, 12), {"inflow_noise": "0.01"}),
+    ("mixed", "canal", (3, 3, 3), {}),
+    ("noslip", "canal", (24, 8, 8), {}),
+])
+def test_gpu_fused_boundary_step_equals_reference_calls(bcs, name, shape, extra, gpu_ok):
+    """nusifRunSteps with each step's CFL maxima over the ghost shell, the six
+    velocity faces and the lid or inflow in one launch (bc_fused 1) against
+    computeTimestep, setBoundaryConditions and setSpecialBoundaryCondition
+    one by one (bc_fused 0): eight steps, the trace (t, dt, res) and every
+    field bit for bit. The BC sets mix NOSLIP, SLIP, OUTFLOW and PERIODIC
+    faces (the fused kernel moves the ghost updates whose source a later
+    face overwrites into that face's task, and applies an earlier face's wall
+    rule where a later face reads the wall velocity); 3 x 3 x 3 is the
+    smallest grid it takes."""
+    print("fused boundary step", bcs, name, shape, flush=True)
+    prm = dict(G.KAT_PARAMS, **G.BC_SETS[bcs], name=name, imax=shape[0], jmax=shape[1], kmax=shape[2])
+    out = []
+    for fused in (0, 1):
+        c = A.Case(lib("gpu_rb"), options=dict(extra, bc_fused=str(fused)), **prm)
+        c.set_fields(**G.random_fields(c.shape, 5))
+        tr = (C.c_double * (4 * 8))()
+        assert c.lib.nusifRunSteps(C.byref(c.d), C.byref(c.s), 8, tr) == 8
+        out.append((np.array(tr), c.fields()))
+        c.close()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    for k in A.FIELDS:
+        np.testing.assert_array_equal(out[0][1][k], out[1][1][k], err_msg=k)
+
+
 @pytest.mark.parametrize("shape", [(32, 32, 32), (70, 37, 9), (48, 20, 4)])
 def test_gpu_rb_fused_iteration_matches_colour_passes(gpu_ok, shape):
     """One launch per SOR iteration (rb_fused 1, the default) against one

@@ -248,7 +248,7 @@ def test_set_option_accepts_documented_keys():
     block = doc[doc.index("Options beyond the reference"):doc.index("int nusifSetOption")]
     keys = set(re.findall(r"^ \*   ([a-z_0-9]+)\s", block, flags=re.M))
     valid = {"nsteps": "7", "solver": "gmg", "precision": "mixed", "mg_tol": "1e-8",
-             "mg_maxcycles": "50", "mg_omega": "1.3", "mg_project": "1", "mg_tail": "0", "mg_fuse_prolong": "0",
+             "mg_maxcycles": "50", "mg_omega": "1.3", "mg_project": "1", "mg_tail": "0", "mg_fuse_prolong": "0", "bc_fused": "0",
              "mg_graph": "0", "mg_tile_rows": "8", "mg_inner": "0", "mg_speculate": "0",
              "lid": "1.5", "u_inflow": "1.0", "inflow_noise": "0.01", "seed": "3",
              "warmstart": "none", "unet_weights": "/tmp/w.bin", "unet_extent": "32", "fg_rhs": "0",
