@@ -105,6 +105,7 @@ struct GmgState {
   CycleCtl *hRingDev; /* their device address (written by kFoldCycle) */
   CycleCtl *hInit;
   hipEvent_t ev[CTL_RING];
+  hipEvent_t evInput; /* after the warm start's input pass (its norms) */
   DctTables dct;
   double *tab64;
   float *tab32;
@@ -2129,6 +2130,7 @@ GmgState *gmgCreate(NusifSolverCtx *sc, const Parameter *prm)
   HIP_CHECK(hipHostGetDevicePointer((void **)&G->hRingDev, G->hRing, 0));
   HIP_CHECK(hipHostMalloc((void **)&G->hInit, sizeof(CycleCtl), hipHostMallocDefault));
   for (int q = 0; q < CTL_RING; q++) HIP_CHECK(hipEventCreateWithFlags(&G->ev[q], hipEventDisableTiming));
+  HIP_CHECK(hipEventCreateWithFlags(&G->evInput, hipEventDisableTiming));
   G->tileRows  = c->opt.mgTileRows;
   G->fuseProlong = c->opt.mgFuseProlong;
   G->setXc       = nullptr;
@@ -2228,6 +2230,7 @@ void gmgFree(GmgState *G)
   HIP_CHECK(hipHostFree(G->hRing));
   HIP_CHECK(hipHostFree(G->hInit));
   for (int q = 0; q < CTL_RING; q++) HIP_CHECK(hipEventDestroy(G->ev[q]));
+  HIP_CHECK(hipEventDestroy(G->evInput));
   for (CycleGraph &e : G->graphs) HIP_CHECK(hipGraphExecDestroy(e.exec));
   G->graphs.clear();
   if (G->tab64) HIP_CHECK(hipFree(G->tab64));
@@ -2553,6 +2556,7 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
   bool haveInitial = false; /* mixed: residual of c->p already in L0.b, its norms in rnInit/nb */
   double rnInit = 0.0;
   bool fromGuess = false; /* mixed: the first cycle starts from the network's correction in L0.x */
+  bool decisionPending = false; /* mixed: the device's acceptance decision is read after the cycles */
   if (ws && G->mixed && warmstartNet(ws) && warmstartLevel(ws) > 0) {
     /* Mixed precision: the network's correction is the starting value of
      * the FP32 error equation A e = r0, not a new FP64 iterate. The input
@@ -2565,36 +2569,38 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
     LevelT<float> &L0 = G->L32[0];
     if (warmstartPrepare(ws, p, rhs, G->cf0.idx2, G->cf0.idy2, G->cf0.idz2, L0.b,
             l0NeedsZero(G) ? L0.x : nullptr, c->scal->norm2, c->scalHostDev->norm2)) {
+      /* the host waits for the input pass's norms only; the network, the
+       * acceptance decision and the first cycles are queued behind them.
+       * The decision is made on the device: a rejected correction is zeroed
+       * there, so both branches start the first cycle from the prolongated
+       * correction (e = 0 when rejected, the same values as the solve from
+       * the previous pressure). */
+      HIP_CHECK(hipEventRecord(G->evInput, nusifStream()));
       warmstartCaptureBegin(ws, p, g);
       warmstartForward(ws);
       warmstartCoarseAcceptLaunch(ws);
-      const double *wn = warmstartNormsAsync(ws);
-      double nb0       = 0.0;
-      double rn0       = readNorms(c, &nb0); /* one read-back: also the folds above */
-      double rc0 = sqrt(warmstartReferenceNorm2(ws)), rcg = sqrt(wn[2]);
+      warmstartDecideLaunch(ws, c->opt.warmstart == 2);
+      /* the last step of the correction's prolongation goes into the
+       * first cycle's first sweep where that sweep is the fused one */
+      Geo g1;
+      const float *c1 = G->fuseProlong && G->nu1 > 0 && fusedLevel(L0) ? warmstartProlongToLevel1(ws, &g1) : nullptr;
+      if (c1 && g1.ni * 2 == L0.g.ni && g1.nj * 2 == L0.g.nj && g1.nk * 2 == L0.g.nk) {
+        G->setXc = c1;
+        G->setGc = g1;
+      } else {
+        warmstartProlongCorrection(ws, L0.x);
+      }
+      fromGuess = true;
+      HIP_CHECK(hipEventSynchronize(G->evInput));
+      const double nb0 = sqrt(c->scalHost->norm2[1]);
+      const double rn0 = sqrt(c->scalHost->norm2[0]);
       sc->wsRes[0] = rn0;
       sc->wsRes[1] = -1.0; /* the guess's fine-grid residual is not formed */
-      sc->wsRes[2] = rc0;
-      sc->wsRes[3] = rcg;
-      sc->warmUsed = (c->opt.warmstart == 2 || rcg < rc0) ? 1 : 0;
       sc->wsTried++;
-      sc->wsTaken += sc->warmUsed;
-      if (sc->warmUsed) {
-        /* the last step of the correction's prolongation goes into the
-         * first cycle's first sweep where that sweep is the fused one */
-        Geo g1;
-        const float *c1 = G->fuseProlong && G->nu1 > 0 && fusedLevel(L0) ? warmstartProlongToLevel1(ws, &g1) : nullptr;
-        if (c1 && g1.ni * 2 == L0.g.ni && g1.nj * 2 == L0.g.nj && g1.nk * 2 == L0.g.nk) {
-          G->setXc = c1;
-          G->setGc = g1;
-        } else {
-          warmstartProlongCorrection(ws, L0.x);
-        }
-        fromGuess = true;
-      }
-      haveInitial = true;
-      rnInit      = rn0;
-      nb          = nb0;
+      decisionPending = true;
+      haveInitial     = true;
+      rnInit          = rn0;
+      nb              = nb0;
     }
   }
   if (ws && !haveInitial) {
@@ -2695,6 +2701,18 @@ double gmgSolve(NusifSolverCtx *sc, double *p, const double *rhsIn)
       c->pAlt    = tp;
     });
     p = c->p;
+    if (decisionPending) {
+      /* the cycles ended with a host synchronisation; without one (no
+       * cycle) the stream is drained here, and a prolongated correction no
+       * cycle used is dropped */
+      if (cycles == 0) HIP_CHECK(hipStreamSynchronize(nusifStream()));
+      G->setXc         = nullptr;
+      const double *wn = warmstartNormsAsync(ws);
+      sc->wsRes[2]     = sqrt(warmstartReferenceNorm2(ws));
+      sc->wsRes[3]     = sqrt(wn[2]);
+      sc->warmUsed     = warmstartDecision(ws);
+      sc->wsTaken += sc->warmUsed;
+    }
   }
   if (ws) warmstartCaptureEnd(ws, c->p);
   /* every path above leaves c->p with its face ghosts written: the last

@@ -38,6 +38,11 @@ void warmstartProlongCorrection(WarmStart *w, float *fine);
  * geometry (null when the network runs on the finest grid) */
 const float *warmstartProlongToLevel1(WarmStart *w, Geo *g1);
 void warmstartCoarseAcceptLaunch(WarmStart *w);
+/* the acceptance decision made on the device after warmstartCoarseAcceptLaunch
+ * (the correction zeroed when rejected); warmstartDecision reads it once the
+ * stream has run it */
+void warmstartDecideLaunch(WarmStart *w, int force);
+int warmstartDecision(const WarmStart *w);
 void warmstartCaptureBegin(WarmStart *w, const double *p, const Geo &g0);
 void warmstartCaptureEnd(WarmStart *w, const double *p);
 size_t warmstartSamples(const WarmStart *w);

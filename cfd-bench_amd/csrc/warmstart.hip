@@ -623,6 +623,39 @@ void warmstartCoarseAcceptLaunch(WarmStart *w)
   }
 }
 
+/* The acceptance decision on the device: the guess is taken when its
+ * restricted sum of squares (scal[2]) is below the previous pressure's
+ * (scal[ref]), or always with force; 1.0 or 0.0 into scal[5] and its host
+ * mirror. A rejected guess's correction on the network grid is zeroed, so
+ * the solve that follows starts from e = 0 on either branch without the
+ * host waiting for the network. */
+__global__ void kAcceptDecide(double *scal, double *scalHost, int ref, int force)
+{
+  if (threadIdx.x == 0) {
+    const double u = force || scal[2] < scal[ref] ? 1.0 : 0.0;
+    scal[5]        = u;
+    scalHost[5]    = u;
+  }
+}
+
+__global__ __launch_bounds__(256) void kZeroRejected(float *__restrict__ d, long long n, const double *scal)
+{
+  if (scal[5] != 0.0) return;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
+    d[q] = 0.f;
+}
+
+void warmstartDecideLaunch(WarmStart *w, int force)
+{
+  hipStream_t st   = nusifStream();
+  const WsLevel &N = w->lv[w->Ln];
+  kAcceptDecide<<<1, 64, 0, st>>>(w->scal, w->scalHostDev, w->Lc > w->Ln ? 4 : 0, force);
+  const long long n = (long long)N.g.n;
+  kZeroRejected<<<(unsigned)std::min<long long>((n + 255) / 256, 1024), 256, 0, st>>>(N.d, n, w->scal);
+}
+
+int warmstartDecision(const WarmStart *w) { return w->scalHost[5] != 0.0 ? 1 : 0; }
+
 float *warmstartResidualBuffer(WarmStart *w) { return w->lv[0].r; }
 
 /* Sum of squares of R r on the network level, R = repeated 8-child
